@@ -1,0 +1,212 @@
+"""Seeded synthetic workloads for the realignment path (no datasets exist here).
+
+Each workload is a list of chunk-level read/reference pairs in run-length
+(RLE) space, the shape poa_realign (polisher.c:656-703) sees per chunk:
+  ref_sym   chunk reference, RLE symbols (run length << 8 | base)
+  read_sym  read clipped to the chunk, RLE symbols
+  anchors   strictly increasing (x, y, expansion) RLE match pairs of the
+            true alignment (the CIGAR-derived anchors of
+            htsIntegration.c:347-650, expansion = band half-width W)
+  strand    1 forward / 0 reverse (selects the state machine)
+
+Reads are simulated in reference orientation (as BAM stores them) with
+per-base error events sub:ins:del = 4:3:3; 70% of insertions repeat the
+current base (a +-1 homopolymer run-length change), deletions fall on the
+base where they are drawn.
+
+Workloads (sizes follow the benchmark configurations):
+  config0  5 kb uniform-ACGT reference, 20 full-length reads, 10% error, W=32
+  config1  400 kb reference (GC 45%, runs ~ Geometric(0.45)), 40 chunks of
+           10 kb with 500 bp boundaries, 60x log-normal reads (mean 8 kb,
+           sd 6 kb, clipped to 0.5-40 kb), 9% error, W=10
+  config2  5 Mb reference (GC 50%, runs ~ Geometric(0.5), 1% tandem
+           repeats), 100 kb chunks with 5 kb boundaries, 50x reads (mean
+           20 kb, sd 15 kb, max 100 kb), 8% error, W=10
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import List
+
+import numpy as np
+
+MAX_REPEAT_EXCLUSIVE = 51
+_COMP = np.array([3, 2, 1, 0], dtype=np.uint8)
+
+
+@dataclasses.dataclass
+class Pair:
+    ref_sym: np.ndarray
+    read_sym: np.ndarray
+    anchors: np.ndarray  # (n, 3) int64
+    strand: int
+    chunk: int = 0
+    read: int = 0
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    pairs: List[Pair]
+    W: int
+    ref_len: int = 0
+    chunks: int = 1
+
+
+def _ref_uniform(rng, n):
+    return rng.integers(0, 4, size=n, dtype=np.uint8)
+
+
+def _ref_runs(rng, n, gc, p_geom, tandem_frac=0.0):
+    """Reference built run by run: base ~ GC composition redrawn if equal to
+    the previous run's base, run length ~ Geometric(p) >= 1."""
+    pa = (1.0 - gc) / 2.0
+    probs = np.array([pa, gc / 2.0, gc / 2.0, pa])
+    nruns = int(n / (1.0 / p_geom) * 1.3) + 16
+    bases = rng.choice(4, size=nruns, p=probs).astype(np.uint8)
+    # redraw equal neighbours until none remain
+    for _ in range(64):
+        eq = np.flatnonzero(bases[1:] == bases[:-1]) + 1
+        if eq.size == 0:
+            break
+        bases[eq] = rng.choice(4, size=eq.size, p=probs)
+    runs = rng.geometric(p_geom, size=nruns)
+    seq = np.repeat(bases, runs)[:n]
+    if tandem_frac > 0:
+        total = 0
+        while total < tandem_frac * n:
+            k = int(rng.integers(2, 7))
+            copies = int(rng.integers(10, 41))
+            unit = rng.integers(0, 4, size=k, dtype=np.uint8)
+            rep = np.tile(unit, copies)
+            pos = int(rng.integers(0, max(1, n - rep.size)))
+            seq[pos:pos + rep.size] = rep[: max(0, min(rep.size, n - pos))]
+            total += rep.size
+    return seq
+
+
+def _mutate(rng, ref, err):
+    """Per-base error events. Returns (read bases, read position of each
+    reference base or -1 when deleted)."""
+    n = ref.size
+    u = rng.random(n)
+    ev = np.zeros(n, dtype=np.int8)  # 0 copy, 1 sub, 2 ins, 3 del
+    ev[u < err * 0.4] = 1
+    ev[(u >= err * 0.4) & (u < err * 0.7)] = 2
+    ev[(u >= err * 0.7) & (u < err)] = 3
+    outlen = np.where(ev == 3, 0, np.where(ev == 2, 2, 1))
+    ends = np.cumsum(outlen)
+    starts = ends - outlen
+    read = np.empty(int(ends[-1]) if n else 0, dtype=np.uint8)
+    keep = outlen > 0
+    first = ref.copy()
+    sub = ev == 1
+    first[sub] = (ref[sub] + rng.integers(1, 4, size=int(sub.sum()), dtype=np.uint8)) % 4
+    read[starts[keep]] = first[keep]
+    ins = ev == 2
+    homo = rng.random(n) < 0.7
+    ins_base = np.where(homo, ref, rng.integers(0, 4, size=n, dtype=np.uint8))
+    read[starts[ins] + 1] = ins_base[ins]
+    pos = np.where(keep, starts, -1)
+    return read, pos
+
+
+def _rle(bases):
+    if bases.size == 0:
+        return bases, np.zeros(0, np.int64), np.zeros(0, np.int64)
+    new = np.r_[True, bases[1:] != bases[:-1]]
+    starts = np.flatnonzero(new)
+    runs = np.diff(np.r_[starts, bases.size])
+    run_id = np.cumsum(new) - 1
+    return bases[starts], runs, run_id
+
+
+def _symbols(rbases, runs):
+    rl = np.minimum(runs, MAX_REPEAT_EXCLUSIVE - 1).astype(np.uint32)
+    return ((rl << 8) | rbases.astype(np.uint32)).astype(np.uint16)
+
+
+def _pair(ref_seg, read_seg, ref_idx, read_idx, W, strand, chunk, read_id):
+    """RLE both segments and derive strictly increasing RLE anchors from the
+    aligned base pairs (ref_idx[k], read_idx[k])."""
+    rb, rr, rid = _rle(ref_seg)
+    qb, qr, qid = _rle(read_seg)
+    X = rid[ref_idx]
+    Y = qid[read_idx]
+    if X.size:
+        keep = np.r_[True, (X[1:] > X[:-1]) & (Y[1:] > Y[:-1])]
+        X, Y = X[keep], Y[keep]
+    anchors = np.stack([X, Y, np.full(X.size, W, dtype=np.int64)], axis=1).astype(np.int64)
+    return Pair(_symbols(rb, rr), _symbols(qb, qr), anchors, strand, chunk, read_id)
+
+
+def config0(n_reads=20, ref_len=5000, seed=0, err=0.10, W=32, strands=(1,)):
+    rng = np.random.default_rng(seed)
+    ref = _ref_uniform(rng, ref_len)
+    pairs = []
+    for r in range(n_reads):
+        read, pos = _mutate(rng, ref, err)
+        al = np.flatnonzero(pos >= 0)
+        strand = strands[r % len(strands)]
+        pairs.append(_pair(ref, read, al, pos[al], W, strand, 0, r))
+    return Workload("config0", pairs, W, ref_len, 1)
+
+
+def _lognormal_lengths(rng, n, mean, sd, lo, hi):
+    s2 = np.log(1.0 + (sd / mean) ** 2)
+    mu = np.log(mean) - s2 / 2.0
+    L = rng.lognormal(mu, np.sqrt(s2), size=n)
+    return np.clip(L, lo, hi).astype(np.int64)
+
+
+def chunked(name, seed, ref_len, gc, p_geom, tandem, chunk_size, boundary, depth, mean, sd, lo, hi, err, W,
+            max_chunks=None):
+    rng = np.random.default_rng(seed)
+    ref = _ref_runs(rng, ref_len, gc, p_geom, tandem)
+    nreads = int(round(depth * ref_len / mean))
+    lens = np.minimum(_lognormal_lengths(rng, nreads, mean, sd, lo, hi), ref_len)
+    starts = rng.integers(0, np.maximum(1, ref_len - lens + 1))
+    strands = rng.integers(0, 2, size=nreads)
+    nchunks = (ref_len + chunk_size - 1) // chunk_size
+    if max_chunks is not None:
+        nchunks = min(nchunks, max_chunks)
+    chunk_reads = [[] for _ in range(nchunks)]
+    for r in range(nreads):
+        s, e = int(starts[r]), int(starts[r] + lens[r])
+        c0 = max(0, (s - boundary) // chunk_size)
+        c1 = min(nchunks - 1, (e - 1 + boundary) // chunk_size)
+        if c0 > c1:
+            continue
+        read, pos = _mutate(rng, ref[s:e], err)
+        for c in range(c0, c1 + 1):
+            chunk_reads[c].append((r, s, read, pos))
+    pairs = []
+    for c in range(nchunks):
+        cs = max(0, c * chunk_size - boundary)
+        ce = min(ref_len, (c + 1) * chunk_size + boundary)
+        ref_seg = ref[cs:ce]
+        for (r, s, read, pos) in chunk_reads[c]:
+            # clip the read to the reference bases it aligns inside the chunk
+            lo_i, hi_i = max(cs - s, 0), min(ce - s, pos.size)
+            if hi_i <= lo_i:
+                continue
+            p = pos[lo_i:hi_i]
+            al = np.flatnonzero(p >= 0)
+            if al.size < 2:
+                continue
+            q0, q1 = int(p[al[0]]), int(p[al[-1]])
+            read_seg = read[q0:q1 + 1]
+            ref_idx = al + (lo_i + s - cs)
+            read_idx = p[al] - q0
+            pairs.append(_pair(ref_seg, read_seg, ref_idx, read_idx, W, int(strands[r]), c, r))
+    return Workload(name, pairs, W, ref_len, nchunks)
+
+
+def config1(seed=1, max_chunks=None):
+    return chunked("config1", seed, 400_000, 0.45, 0.45, 0.0, 10_000, 500, 60, 8000, 6000, 500, 40_000, 0.09, 10,
+                   max_chunks)
+
+
+def config2(seed=2, max_chunks=None):
+    return chunked("config2", seed, 5_000_000, 0.50, 0.5, 0.01, 100_000, 5_000, 50, 20_000, 15_000, 500, 100_000,
+                   0.08, 10, max_chunks)

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PARAMS = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.json")
+PARAMS_NO_RLE = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.no_rle.json")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+
+
+@pytest.fixture(scope="session")
+def params_path():
+    return PARAMS
