@@ -1,0 +1,157 @@
+"""Benchmark: batched pair-HMM forward-backward + posteriors on one MI355X.
+
+Workload (the batched-uneven-pairs configuration): 40 chunks of 10 kb (500 bp
+boundaries) from a seeded 400 kb reference (GC 45%, runs ~ Geometric(0.45)),
+60x log-normal reads (mean 8 kb, sd 6 kb, 0.5-40 kb), 9% error, RLE symbols,
+anchors = true RLE match pairs with expansion 10, allParams.np.json
+3-state RLE model.  One step = one POA realignment iteration: every
+read/chunk pair (~5.7k) through forward, backward and posterior extraction
+in one persistent kernel launch, inputs resident in HBM.
+
+Prints one JSON line (rank 0, N=1)."""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair-HMM GCUPS + reads/s: batched FP32 vs plain, posterior err <=1e-4"
+FLOPS_PER_CELL = 42          # forward 18 + backward 18 + posterior 6 (FMA = 2)
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
+PARAMS = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--cpu-pairs", type=int, default=1200)
+    ap.add_argument("--err-pairs", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (chunks shard across GPUs "
+              "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from marginpolish_amd import crop, pairhmm, sim
+
+    t0 = time.time()
+    w = sim.config1()
+    pairs = [(p.ref_sym, p.read_sym, p.anchors, p.strand) for p in w.pairs]
+    log("workload: %d pairs built in %.1f s" % (len(pairs), time.time() - t0))
+    pp = pairhmm.PolishParams(PARAMS)
+    eng = pairhmm.default_engine()
+
+    # reads/s: one POA iteration end to end (host prep + upload + kernel + result handoff)
+    t1 = time.time()
+    b = pairhmm.Batch(pp.machines, pairs, pp.p, crop_reference=True, engine=eng)
+    t_prep = time.time() - t1
+    b.run()
+    res = b.fetch()
+    t_iter = time.time() - t1
+    reads_per_s = len(pairs) / t_iter
+    cells = b.cells
+    log("iteration: prep %.3f s, total %.3f s, %d cells" % (t_prep, t_iter, cells))
+
+    for _ in range(args.warmup):
+        b.run()
+    kms = []
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        kms.append(b.kernel_ms)
+    wall = time.perf_counter() - t2
+    ms_per_step = wall * 1e3 / args.steps
+    kernel_ms = float(np.mean(kms))
+    gcups = cells / (ms_per_step * 1e-3) / 1e9
+    kernel_gcups = cells / (kernel_ms * 1e-3) / 1e9
+    log("batched: %.3f ms/step wall, %.3f ms kernel, %.1f GCUPS" % (ms_per_step, kernel_ms, gcups))
+
+    # plain per-read kernel (one launch per pair) on a sample of pairs
+    order = np.argsort([-len(p[0]) for p in pairs])
+    samp = [pairs[i] for i in order[:: max(1, len(pairs) // args.plain_sample)][: args.plain_sample]]
+    bp = pairhmm.Batch(pp.machines, samp, pp.p, crop_reference=True, engine=eng)
+    bp.run_plain()
+    plain_gcups = bp.cells / (bp.kernel_ms * 1e-3) / 1e9
+    bs = pairhmm.Batch(pp.machines, samp, pp.p, crop_reference=True, engine=eng)
+    bs.run()
+    log("plain: %d pairs, %.3f ms, %.2f GCUPS" % (len(samp), bp.kernel_ms, plain_gcups))
+
+    out = {
+        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config1: 40 chunks x 10 kb (+-500 bp), 400 kb ref, 60x reads mean 8 kb, 9% error, "
+                               "RLE, anchor expansion 10, allParams.np.json, one POA realign iteration",
+                   "pairs": len(pairs), "cells": int(cells), "band_half_width": 10},
+        "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
+        "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),
+        "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
+        "plain_sample_pairs": len(samp),
+    }
+    achieved = FLOPS_PER_CELL * cells / (kernel_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_kernel_config1.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic}
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        from tests.parity import as_rows, compare
+        op = O.PolishParams(PARAMS)
+        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        ncores = max(1, min(ncores, 16))
+        rng = np.random.default_rng(0)
+        pick = rng.choice(len(pairs), size=min(args.cpu_pairs, len(pairs)), replace=False)
+        cpu_jobs = []
+        for i in pick:
+            sx, sy, an, _ = crop.crop_pair(*pairs[i][:3])
+            cpu_jobs.append((sx, sy, an, pairs[i][3]))
+        t3 = time.perf_counter()
+        ccells, _ = O.batch_cells((op.reverse, op.forward), cpu_jobs, op.p, O.LOGADD_POLY, ncores)
+        tc = time.perf_counter() - t3
+        one = cpu_jobs[: max(1, len(cpu_jobs) // 10)]
+        t4 = time.perf_counter()
+        c1, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_POLY, 1)
+        t1c = time.perf_counter() - t4
+        log("cpu: %d pairs on %d threads %.2f s; %d pairs on 1 thread %.2f s" % (len(cpu_jobs), ncores, tc,
+                                                                             len(one), t1c))
+        out["cpu_baseline"] = {"value": round(ccells / tc / 1e9, 5), "unit": "GCUPS", "cores": ncores,
+                               "kind": "port",
+                               "sample": "%d of %d config1 pairs (%d cells), reference algorithm restated in C "
+                                         "(FP64 log space, polynomial logAdd), OpenMP over pairs"
+                                         % (len(cpu_jobs), len(pairs), ccells),
+                               "single_core_gcups": round(c1 / t1c / 1e9, 5)}
+        # max-abs posterior error vs the exact FP64 oracle on a sample
+        worst = 0.0
+        for i in pick[: args.err_pairs]:
+            sx, sy, an, off = crop.crop_pair(*pairs[i][:3])
+            om, ox, oy, _, _ = O.posteriors(op.forward if pairs[i][3] else op.reverse, sx, sy, an, op.p)
+            gm, gx, gy, _ = res[i]
+            for g, o in ((gm, om), (gx, ox), (gy, oy)):
+                o = o.copy()
+                o[:, 1] += off
+                try:
+                    worst = max(worst, compare(as_rows(g), o, op.p.threshold, 1e-4)[0])
+                except AssertionError:
+                    worst = 1.0  # an entry far above threshold on one side only
+        out["max_abs_posterior_err"] = worst
+        out["posterior_err_sample_pairs"] = int(min(args.err_pairs, len(pick)))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

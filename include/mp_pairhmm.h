@@ -1,0 +1,174 @@
+/*
+ * mp_pairhmm.h -- C ABI of the MI355X pair-HMM realignment engine.
+ *
+ * Drop-in for the hot path of MarginPolish's read realignment: the banded
+ * pair-HMM forward-backward with run-length-aware emissions and posterior
+ * match / gap-X / gap-Y extraction.  Plain C types only (pointers, sizes,
+ * POD structs); no torch or HIP types cross this boundary.
+ *
+ * Coordinates follow the reference: sX is the reference (x), sY the read (y);
+ * symbols are uint16 (run length << 8 | base, base in {A,C,G,T,N} = 0..4,
+ * stateMachine.c:116-146); anchors are (x, y, expansion) triples in sequence
+ * coordinates with strictly increasing x and y and even expansion
+ * (pairwiseAligner.c:120-173).  Output tuples are (weight, x, y) with
+ * weight = floor(min(p,1) * 1e7) for posteriors p >= threshold and
+ * coordinates minus one, exactly as addPosteriorProb (pairwiseAligner.c:586-595);
+ * gap-X tuples may have y == -1 and gap-Y tuples x == -1.
+ *
+ * Errors never abort: every entry returns an mp_status and
+ * mp_last_error() gives a thread-local message.
+ */
+#ifndef MP_PAIRHMM_H
+#define MP_PAIRHMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MP_OK = 0,
+    MP_EINVAL = 1,        /* bad argument (null pointer, negative length)          */
+    MP_EPARAMS = 2,       /* params JSON: unreadable, unknown key, missing section */
+    MP_EANCHORS = 3,      /* anchors not strictly increasing / out of range / odd  */
+    MP_EHIP = 4,          /* HIP runtime error                                     */
+    MP_ENOMEM = 5,        /* host or device allocation failed                      */
+    MP_EUNSUPPORTED = 6,  /* band wider than the kernel supports                   */
+    MP_ENODEVICE = 7      /* no gfx950 device visible                              */
+} mp_status;
+
+/* Output lists of one pair. */
+enum { MP_LIST_MATCH = 0, MP_LIST_GAPX = 1, MP_LIST_GAPY = 2 };
+
+/* What a batch computes per pair. */
+enum {
+    MP_MODE_POSTERIORS = 0,   /* match + gapX + gapY posteriors (getAlignedPairsWithIndelsUsingAnchors) */
+    MP_MODE_MATCH_ONLY = 1    /* match posteriors only (getAlignedPairsUsingAnchors)                     */
+};
+
+/* Batch flags. */
+enum {
+    MP_BATCH_CROP_REFERENCE = 1  /* crop sX per pair as poa_realign's caller does (polisher.c:605-654) */
+};
+
+typedef struct mp_model mp_model;          /* both strands' state machines + tables */
+typedef struct mp_engine mp_engine;        /* one device, one stream, reusable buffers */
+typedef struct mp_batch mp_batch;          /* a device-resident batch of pairs */
+
+/* (weight, x, y): the stIntTuple of the reference lists. */
+typedef struct {
+    int64_t weight;
+    int32_t x;
+    int32_t y;
+} mp_aligned_pair;
+
+/* PairwiseAlignmentParameters (pairwiseAligner.h:28-49), the fields the path reads. */
+typedef struct {
+    double threshold;                  /* 0.01 */
+    int64_t minDiagsBetweenTraceBack;  /* 10000 in the shipped params */
+    int64_t traceBackDiagonals;        /* 40 */
+    int64_t diagonalExpansion;         /* 10 */
+    int64_t splitMatrixBiggerThanThis; /* 250e6 */
+    int32_t dynamicAnchorExpansion;    /* 1 */
+} mp_band_params;
+
+/* One read/reference pair, the arguments of
+ * getAlignedPairsWithIndelsUsingAnchors (pairwiseAligner.h:75-77). */
+typedef struct {
+    const uint16_t *sX;
+    int64_t lX;
+    const uint16_t *sY;
+    int64_t lY;
+    const int64_t *anchors;  /* nAnchors (x, y, expansion) triples; may be NULL when nAnchors == 0 */
+    int64_t nAnchors;
+    int32_t strand;          /* 1: stateMachineForForwardStrandRead, 0: ...ReverseStrandRead */
+    int32_t raggedLeft;
+    int32_t raggedRight;
+} mp_pair;
+
+const char *mp_last_error(void);
+const char *mp_version(void);
+
+/* ---- model: replaces params_readParams / polishParams_jsonParse
+ *      (parser.c:154-408, 503-528) and hmm_getStateMachine
+ *      (stateMachine.c:676-689) for the keys on this path.  Same JSON schema;
+ *      an unknown key in the polish section is an error as in parser.c:374. */
+mp_status mp_model_load_json(const char *path, mp_model **out);
+/* stateMachine3_constructNucleotide(threeState) (stateMachine.c:611-620):
+ * default transitions and nucleotide emissions, no run-length term. */
+mp_status mp_model_default_nucleotide(mp_model **out);
+mp_status mp_model_band_params(const mp_model *m, mp_band_params *out);
+/* Resolved natural-log tables of one strand (for inspection and tests):
+ * trans9 = {matchContinue, matchFromGapX, matchFromGapY, gapOpenX, gapOpenY,
+ *           gapExtendX, gapExtendY, gapSwitchToX, gapSwitchToY}. */
+mp_status mp_model_tables(const mp_model *m, int32_t strand, double *trans9, double *match16, double *gapX4,
+                          double *gapY4, int32_t *rle);
+/* repeatSubMatrix_getLogProb (polisher.c:1756-1775): log10 P(observed | underlying, base, strand). */
+double mp_model_repeat_log10(const mp_model *m, int32_t base, int32_t strand, int32_t observed, int32_t underlying);
+int32_t mp_model_max_repeat_count(const mp_model *m);  /* maximumRepeatLength (51) */
+void mp_model_free(mp_model *m);
+
+/* ---- host geometry (no device needed) ---- */
+/* band_construct / band_constructDynamic (pairwiseAligner.c:120-226): xmyL and
+ * xmyR of every anti-diagonal 0..lX+lY (arrays of lX+lY+1). */
+mp_status mp_band_construct(const int64_t *anchors, int64_t nAnchors, int64_t lX, int64_t lY, int64_t expansion,
+                            int32_t dynamic, int32_t *xmyL, int32_t *xmyR);
+/* getSplitPoints (pairwiseAligner.c:899-926): writes up to cap (x1, y1, x2, y2)
+ * tuples, returns the number of split blocks (or -1 on bad arguments). */
+int64_t mp_split_points(const int64_t *anchors, int64_t nAnchors, int64_t lX, int64_t lY, int64_t maxArea,
+                        int32_t raggedLeft, int32_t raggedRight, int64_t *out, int64_t cap);
+/* Traceback windows of getPosteriorProbsWithBanding (pairwiseAligner.c:715-783)
+ * for a band: up to cap (tracebackDiagonal, tracedBackTo, tracedBackFrom)
+ * triples; returns the number of windows. */
+int64_t mp_traceback_windows(const int32_t *xmyL, const int32_t *xmyR, int64_t lXlY, const mp_band_params *p,
+                             int32_t *out, int64_t cap);
+
+/* ---- engine ---- */
+mp_status mp_engine_create(int32_t device, mp_engine **out);
+void mp_engine_free(mp_engine *e);
+/* Compute units of the engine's device (for launch sizing reports). */
+int32_t mp_engine_cu_count(const mp_engine *e);
+
+/* ---- per-pair entry: getAlignedPairsWithIndelsUsingAnchors /
+ *      getAlignedPairsUsingAnchors (pairwiseAligner.c:1088-1130), including
+ *      the large-gap split (getSplitPoints, 875-991).  One kernel launch per
+ *      call.  Lists are written into caller-owned arrays; *count receives the
+ *      list length; when it exceeds *cap nothing is truncated silently: the
+ *      call returns MP_OK with counts set and the arrays filled only up to cap,
+ *      so the caller can retry with larger arrays.  logProb receives the
+ *      forward log-likelihood of the last window. */
+mp_status mp_pairhmm_posteriors(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,
+                                int32_t mode, mp_aligned_pair *matches, int64_t matchCap, int64_t *matchCount,
+                                mp_aligned_pair *gapX, int64_t gapXCap, int64_t *gapXCount,
+                                mp_aligned_pair *gapY, int64_t gapYCap, int64_t *gapYCount, double *logProb);
+
+/* ---- batched entry: every pair of one POA iteration (all chunks) in one
+ *      persistent kernel launch.  poa_realign (polisher.c:656-703) loops per
+ *      read; the batch replaces that loop. ---- */
+
+/* Host preparation + upload: split points, cropping (flag), band geometry,
+ * traceback windows, band-area job ordering, output sizing. */
+mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
+                          const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out);
+/* One persistent launch over every pair (outputs stay on the device). */
+mp_status mp_batch_run(mp_batch *b);
+/* The same work as one kernel launch per pair (the plain per-read path). */
+mp_status mp_batch_run_plain(mp_batch *b);
+/* Device time of the last run's pair-HMM kernel launch(es), milliseconds (HIP events on the engine's stream). */
+double mp_batch_kernel_ms(const mp_batch *b);
+int64_t mp_batch_cells(const mp_batch *b);          /* band cells over all pairs (integer band geometry) */
+int64_t mp_batch_npairs(const mp_batch *b);
+/* Copy results to the host.  counts: npairs*3 list lengths; logProb: npairs
+ * (may be NULL).  Lists are reported per pair in the reference order and, with
+ * MP_BATCH_CROP_REFERENCE, in uncropped reference coordinates. */
+mp_status mp_batch_fetch_counts(mp_batch *b, int64_t *counts, double *logProb);
+/* Copy list `list` of every pair, concatenated in pair order (sizes from
+ * mp_batch_fetch_counts).  out must hold sum(counts[:, list]) entries. */
+mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_pair *out);
+void mp_batch_free(mp_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

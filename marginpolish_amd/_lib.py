@@ -1,0 +1,106 @@
+"""ctypes binding of libmp_pairhmm.so (the C ABI in include/mp_pairhmm.h).
+
+The HIP library is the only implementation of the path: when it is missing
+or no gfx950 device is present, every call raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmp_pairhmm.so")
+
+MP_OK = 0
+STATUS = {0: "MP_OK", 1: "MP_EINVAL", 2: "MP_EPARAMS", 3: "MP_EANCHORS", 4: "MP_EHIP", 5: "MP_ENOMEM",
+          6: "MP_EUNSUPPORTED", 7: "MP_ENODEVICE"}
+
+LIST_MATCH, LIST_GAPX, LIST_GAPY = 0, 1, 2
+MODE_POSTERIORS, MODE_MATCH_ONLY = 0, 1
+BATCH_CROP_REFERENCE = 1
+
+
+class MPError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class BandParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("minDiagsBetweenTraceBack", C.c_int64),
+                ("traceBackDiagonals", C.c_int64), ("diagonalExpansion", C.c_int64),
+                ("splitMatrixBiggerThanThis", C.c_int64), ("dynamicAnchorExpansion", C.c_int32)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("sX", C.POINTER(C.c_uint16)), ("lX", C.c_int64), ("sY", C.POINTER(C.c_uint16)),
+                ("lY", C.c_int64), ("anchors", C.POINTER(C.c_int64)), ("nAnchors", C.c_int64),
+                ("strand", C.c_int32), ("raggedLeft", C.c_int32), ("raggedRight", C.c_int32)]
+
+
+class AlignedPair(C.Structure):
+    _fields_ = [("weight", C.c_int64), ("x", C.c_int32), ("y", C.c_int32)]
+
+
+ALIGNED_PAIR_DTYPE = [("weight", "<i8"), ("x", "<i4"), ("y", "<i4")]
+
+EXPORTS = [
+    "mp_last_error", "mp_version", "mp_model_load_json", "mp_model_default_nucleotide", "mp_model_band_params",
+    "mp_model_tables", "mp_model_repeat_log10", "mp_model_max_repeat_count", "mp_model_free", "mp_engine_create",
+    "mp_engine_free", "mp_engine_cu_count", "mp_pairhmm_posteriors", "mp_batch_create", "mp_batch_run",
+    "mp_batch_run_plain", "mp_batch_kernel_ms", "mp_batch_cells", "mp_batch_npairs", "mp_batch_fetch_counts",
+    "mp_batch_fetch_list", "mp_batch_free", "mp_band_construct", "mp_split_points", "mp_traceback_windows",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MPError(7, "libmp_pairhmm.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i64, i32, dbl = C.c_int64, C.c_int32, C.c_double
+    P = C.POINTER
+    sig = {
+        "mp_last_error": (C.c_char_p, []),
+        "mp_version": (C.c_char_p, []),
+        "mp_model_load_json": (C.c_int, [C.c_char_p, P(vp)]),
+        "mp_model_default_nucleotide": (C.c_int, [P(vp)]),
+        "mp_model_band_params": (C.c_int, [vp, P(BandParams)]),
+        "mp_model_tables": (C.c_int, [vp, i32, P(dbl), P(dbl), P(dbl), P(dbl), P(i32)]),
+        "mp_model_repeat_log10": (dbl, [vp, i32, i32, i32, i32]),
+        "mp_model_max_repeat_count": (i32, [vp]),
+        "mp_model_free": (None, [vp]),
+        "mp_engine_create": (C.c_int, [i32, P(vp)]),
+        "mp_engine_free": (None, [vp]),
+        "mp_engine_cu_count": (i32, [vp]),
+        "mp_pairhmm_posteriors": (C.c_int, [vp, vp, P(Pair), P(BandParams), i32, P(AlignedPair), i64, P(i64),
+                                            P(AlignedPair), i64, P(i64), P(AlignedPair), i64, P(i64), P(dbl)]),
+        "mp_batch_create": (C.c_int, [vp, vp, P(Pair), i64, P(BandParams), i32, i32, P(vp)]),
+        "mp_batch_run": (C.c_int, [vp]),
+        "mp_batch_run_plain": (C.c_int, [vp]),
+        "mp_batch_kernel_ms": (dbl, [vp]),
+        "mp_batch_cells": (i64, [vp]),
+        "mp_batch_npairs": (i64, [vp]),
+        "mp_batch_fetch_counts": (C.c_int, [vp, P(i64), P(dbl)]),
+        "mp_batch_fetch_list": (C.c_int, [vp, i32, P(AlignedPair)]),
+        "mp_batch_free": (None, [vp]),
+        "mp_band_construct": (C.c_int, [P(i64), i64, i64, i64, i64, i32, P(i32), P(i32)]),
+        "mp_split_points": (i64, [P(i64), i64, i64, i64, i64, i32, i32, P(i64), i64]),
+        "mp_traceback_windows": (i64, [P(i32), P(i32), i64, P(BandParams), P(i32), i64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != MP_OK:
+        raise MPError(status, lib().mp_last_error().decode(errors="replace"))

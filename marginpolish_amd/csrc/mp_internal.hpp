@@ -1,0 +1,61 @@
+// mp_internal.hpp -- host-side structures shared by the model loader, the
+// batch planner and the HIP driver.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mp_pairhmm.h"
+
+namespace mp {
+
+constexpr int kMaxRepeat = 51;            // MAXIMUM_REPEAT_LENGTH, margin.h:123
+constexpr int kRepeatEntries = 5 * kMaxRepeat * kMaxRepeat;
+
+// Transition slots, natural log, in the order of StateMachine3 (stateMachine.c:490-500).
+enum Trans { kMatchContinue, kMatchFromGapX, kMatchFromGapY, kGapOpenX, kGapOpenY,
+             kGapExtendX, kGapExtendY, kGapSwitchToX, kGapSwitchToY };
+
+struct Machine {
+    double trans[9];
+    double match[16], gapX[4], gapY[4];
+    int rle = 0;
+    int strand = 1;
+};
+
+void set_error(const std::string &msg);
+
+}  // namespace mp
+
+struct mp_model {
+    mp::Machine mach[2];          // [0] reverse strand, [1] forward strand
+    std::vector<double> repeatLog10;  // [5][51][51] log10, [base][underlying][observed]
+    mp_band_params p;
+};
+
+namespace mp {
+
+// Geometry of one sub-problem (after large-gap splitting).
+struct Band {
+    std::vector<int32_t> L, R;    // xmyL / xmyR per anti-diagonal 0..D
+};
+
+struct Window {
+    int32_t tb, lo, from;
+};
+
+// band_constructDynamic / band_construct semantics (pairwiseAligner.c:120-226).
+// Returns false when a diagonal is invalid (bad anchors).
+bool build_band(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY, int64_t expansion, bool dynamic,
+                int32_t *L, int32_t *R);
+// Traceback windows of getPosteriorProbsWithBanding (pairwiseAligner.c:715-783).
+void build_windows(const int32_t *L, const int32_t *R, int64_t D, const mp_band_params &p,
+                   std::vector<Window> &out);
+// getSplitPoints (pairwiseAligner.c:899-926): (x1, y1, x2, y2) tuples.
+void split_points(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY, int64_t maxArea, bool raggedL,
+                  bool raggedR, std::vector<int64_t> &out);
+// Anchor validity: strictly increasing, inside the sequences, even expansion.
+bool anchors_valid(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY);
+
+}  // namespace mp

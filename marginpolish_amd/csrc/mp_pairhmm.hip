@@ -1,0 +1,1021 @@
+// mp_pairhmm.hip -- banded pair-HMM forward-backward + posterior extraction
+// for gfx950 (MI355X), and the C-ABI batch driver around it.
+//
+// Reference algorithm: getPosteriorProbsWithBanding (pairwiseAligner.c:687-808)
+// with stateMachine3_cellCalculate (stateMachine.c:559-581), RLE emissions
+// (stateMachine.c:702-736) and posterior emission (pairwiseAligner.c:586-664).
+//
+// Kernel design
+//  * One band anti-diagonal per step; one lane per band cell.  A job (one
+//    read/reference sub-problem) owns a "row" of SEG lanes (SEG = 16, 32 or
+//    64, the smallest power of two >= the job's widest anti-diagonal), so a
+//    64-lane wavefront runs 64/SEG independent jobs.
+//  * Forward and backward are the same step in probability space:
+//      forward : p = e (.) in,  q = T^t p,  value = p, send q
+//      backward: p = in,        q = T p,    value = q, send e (.) q
+//    with the three neighbour inputs (gap-X from x-1 / x+1, gap-Y from y-1 /
+//    y+1 on the previous step, match from two steps back) exchanged with
+//    ds_bpermute inside the row.  A row switches between forward and backward
+//    by selects, so rows of one wave in different phases do not diverge.
+//  * FP32 scaled arithmetic: every K anti-diagonals the row maximum is folded
+//    to [0.5,1) by an exact power-of-two rescale; the exponents are integers,
+//    so the scale bookkeeping is exact.  Per traceback window the total is
+//    sum(F(tb) . end) (the cut total of the reference is constant across the
+//    window in exact arithmetic); log-likelihoods are formed in FP64.
+//  * Forward values of the current traceback window live in an HBM ring
+//    (3 floats per lane per anti-diagonal) and are read back by the backward
+//    sweep for the posteriors; posteriors >= threshold are compacted per row
+//    with a ballot/popcount prefix and written as (weight, x-1, y-1) tuples.
+//  * Persistent launch: one workgroup per CU, every row pulls jobs from a
+//    per-class atomic queue ordered by band area (largest first).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mp_internal.hpp"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device-side layout
+// ---------------------------------------------------------------------------
+
+// Per-strand emission table, copied to LDS: E[5][5] match (N rows/cols =
+// 1/16), GX[5], GY[5] (N = 1/4), then the run-length factor
+// RL[5][51][51] = exp(2.3025 * log10 P(observed | underlying)) by x base.
+constexpr int kOffE = 0, kOffGX = 25, kOffGY = 30, kOffRL = 40;
+constexpr int kTabStride = 13056;  // >= 40 + 5*51*51, multiple of 64
+constexpr int kRingStride = 192;   // floats per ring entry: 3 states x 64 lanes
+constexpr int kNumClasses = 3;     // SEG 16, 32, 64
+constexpr int kBlock = 512;
+
+struct DevJob {
+    int64_t symX, symY, bandOff, outOff;
+    int32_t lX, lY, winOff, nWin;
+    int32_t cap[3];
+    int32_t flags;  // bit0 strand, bit1 ragged left, bit2 ragged right
+};
+
+struct DevWin {
+    int32_t tb, lo, from, pad;
+};
+
+struct KParams {
+    const DevJob *jobs;
+    const uint16_t *sym;
+    const int2 *band;
+    const DevWin *win;
+    int4 *out;
+    int32_t *counts;
+    double *logProb;
+    float *ring;
+    int32_t *ringS;
+    const float *tables;
+    int32_t *queue;  // kNumClasses counters
+    int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
+    int32_t ringLen;
+    int32_t withIndels;
+    int32_t rle;
+    int32_t rescaleMask;
+    float threshold;
+    float T[9];          // probability transitions, [from*3 + to], M=0 X=1 Y=2
+    float startV[2][3];  // [ragged][state]
+    float endV[2][3];
+};
+
+__device__ __forceinline__ float bperm(float v, int srcLane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(srcLane << 2, __float_as_int(v)));
+}
+
+template <int SEG>
+__device__ __forceinline__ float row_max(float v) {
+#pragma unroll
+    for (int o = SEG / 2; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int SEG>
+__device__ __forceinline__ float row_sum(float v) {
+#pragma unroll
+    for (int o = SEG / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+enum { PH_FETCH = 0, PH_FWD = 1, PH_BWD = 2, PH_DONE = 3 };
+
+template <int SEG>
+__device__ void run_class(const KParams &kp, const float *lds, int cls) {
+    const int lane = threadIdx.x & 63;
+    const int k = lane & (SEG - 1);
+    const int rowBase = lane & ~(SEG - 1);
+    const int rowInWave = lane / SEG;
+    const size_t waveSlot = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    float *ringW = kp.ring + waveSlot * (size_t)kp.ringLen * kRingStride;
+    int32_t *ringSW = kp.ringS + waveSlot * (size_t)kp.ringLen * 4;
+    const uint64_t rowMask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << rowBase;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int R = kp.ringLen;
+
+    // Row state (identical in every lane of a row).
+    int phase = PH_FETCH;
+    int job = 0, d = 0, dir = 1, rpos = 0, win = 0, nWin = 0;
+    int tb = 0, lo = 0, from = 0, D = 0, lXj = 0, lYj = 0;
+    int64_t symX = 0, symY = 0, bandOff = 0, outOff = 0;
+    int capM = 0, capX = 0, capY = 0, flags = 0, winOff = 0;
+    int cntM = 0, cntX = 0, cntY = 0;
+    int S = 0, SFtb = 0;
+    float totalInv = 0.f;
+    bool init = false;
+    // Neighbour bands (previous step, two steps back) and exchanged values.
+    int Lm1 = 1, Rm1 = -1, Lm2 = 1, Rm2 = -1;
+    float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
+    // Forward state parked while the backward sweep of a window runs.
+    int fLm1 = 1, fRm1 = -1, fLm2 = 1, fRm2 = -1, fS = 0, fRpos = 0;
+    float fX1 = 0.f, fY1 = 0.f, fM1 = 0.f, fM2 = 0.f;
+    // Row coefficient matrix: forward T^t, backward T.
+    float c00, c01, c02, c10, c11, c12, c20, c21, c22;
+    c00 = c01 = c02 = c10 = c11 = c12 = c20 = c21 = c22 = 0.f;
+
+    for (;;) {
+        if (phase == PH_FETCH) {
+            int j = 0;
+            if (k == 0) j = atomicAdd(&kp.queue[cls], 1);
+            j = __shfl(j, rowBase, 64);
+            job = kp.jobBegin[cls] + j;
+            if (job >= kp.jobEnd[cls]) {
+                phase = PH_DONE;
+            } else {
+                const DevJob jb = kp.jobs[job];
+                symX = jb.symX; symY = jb.symY; bandOff = jb.bandOff; outOff = jb.outOff;
+                lXj = jb.lX; lYj = jb.lY; D = lXj + lYj;
+                winOff = jb.winOff; nWin = jb.nWin;
+                capM = jb.cap[0]; capX = jb.cap[1]; capY = jb.cap[2];
+                flags = jb.flags;
+                const DevWin w0 = kp.win[winOff];
+                win = 0; tb = w0.tb; lo = w0.lo; from = w0.from;
+                cntM = cntX = cntY = 0;
+                d = 0; dir = 1; rpos = 0; S = 0; init = true;
+                Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
+                oX1 = oY1 = oM1 = oM2 = 0.f;
+                // forward: q = T^t p  (q_t = sum_s T[s][t] p_s)
+                c00 = kp.T[0]; c01 = kp.T[3]; c02 = kp.T[6];
+                c10 = kp.T[1]; c11 = kp.T[4]; c12 = kp.T[7];
+                c20 = kp.T[2]; c21 = kp.T[5]; c22 = kp.T[8];
+                phase = PH_FWD;
+            }
+        }
+        if (__all(phase == PH_DONE)) break;
+        if (phase != PH_FWD && phase != PH_BWD) continue;
+
+        const bool fwd = phase == PH_FWD;
+        const int2 bd = kp.band[bandOff + d];
+        const int L0 = bd.x, R0 = bd.y;
+        const int xmy = L0 + 2 * k;
+        const bool act = xmy <= R0;
+        const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
+        const bool strand = flags & 1;
+
+        // Symbols and emissions of this cell (x = 0 or y = 0 read as N; those
+        // emissions are never paired with an in-band neighbour).
+        unsigned sx = 4u, sy = 4u;
+        if (act && x > 0) sx = kp.sym[symX + x - 1];
+        if (act && y > 0) sy = kp.sym[symY + y - 1];
+        const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
+        const float *tab = lds + (strand ? kTabStride : 0);
+        const float eX = tab[kOffGX + xb];
+        const float eY = tab[kOffGY + yb];
+        float eM = tab[kOffE + xb * 5 + yb];
+        if (kp.rle) {
+            const unsigned rx = min(sx >> 8, 50u), ry = min(sy >> 8, 50u);
+            eM *= tab[kOffRL + xb * 2601 + rx * 51 + ry];
+        }
+
+        // Neighbour inputs.
+        const int xX = xmy - dir, xY = xmy + dir;
+        const bool vX = act && xX >= Lm1 && xX <= Rm1;
+        const bool vY = act && xY >= Lm1 && xY <= Rm1;
+        const bool vM = act && xmy >= Lm2 && xmy <= Rm2;
+        const int sX_ = rowBase + (vX ? ((xX - Lm1) >> 1) : k);
+        const int sY_ = rowBase + (vY ? ((xY - Lm1) >> 1) : k);
+        const int sM_ = rowBase + (vM ? ((xmy - Lm2) >> 1) : k);
+        float inX = bperm(oX1, sX_);
+        float inY = bperm(oY1, sY_);
+        float inM = bperm(oM2, sM_);
+        inX = vX ? inX : 0.f;
+        inY = vY ? inY : 0.f;
+        inM = vM ? inM : 0.f;
+
+        float p0, p1, p2;
+        if (fwd) {
+            p0 = eM * inM; p1 = eX * inX; p2 = eY * inY;
+            if (init) {
+                const int rg = (flags >> 1) & 1;
+                p0 = kp.startV[rg][0]; p1 = kp.startV[rg][1]; p2 = kp.startV[rg][2];
+            }
+        } else {
+            p0 = inM; p1 = inX; p2 = inY;
+        }
+        float q0 = c00 * p0 + c01 * p1 + c02 * p2;
+        float q1 = c10 * p0 + c11 * p1 + c12 * p2;
+        float q2 = c20 * p0 + c21 * p1 + c22 * p2;
+        const bool atEnd = tb == D;
+        if (!fwd && init) {
+            const int rg = (atEnd && ((flags >> 2) & 1)) ? 1 : 0;
+            q0 = kp.endV[rg][0]; q1 = kp.endV[rg][1]; q2 = kp.endV[rg][2];
+        }
+        float v0 = fwd ? p0 : q0, v1 = fwd ? p1 : q1, v2 = fwd ? p2 : q2;
+        float o0 = fwd ? q0 : eM * q0, o1 = fwd ? q1 : eX * q1, o2 = fwd ? q2 : eY * q2;
+        if (!act) v0 = v1 = v2 = o0 = o1 = o2 = 0.f;
+
+        // Exact power-of-two rescale every K anti-diagonals.
+        if ((d & kp.rescaleMask) == 0) {
+            float m = row_max<SEG>(fmaxf(v0, fmaxf(v1, v2)));
+            int e2 = 0;
+            if (m > 0.f) e2 = __builtin_amdgcn_frexp_expf(m);
+            v0 = ldexpf(v0, -e2); v1 = ldexpf(v1, -e2); v2 = ldexpf(v2, -e2);
+            o0 = ldexpf(o0, -e2); o1 = ldexpf(o1, -e2); o2 = ldexpf(o2, -e2);
+            oM1 = ldexpf(oM1, -e2);
+            S += e2;
+        }
+
+        float *ring = ringW + (size_t)rpos * kRingStride + lane;
+        if (fwd) {
+            ring[0] = v0; ring[64] = v1; ring[128] = v2;
+            if (k == 0) ringSW[rpos * 4 + rowInWave] = S;
+        } else if (d <= from && d > lo) {
+            // Posteriors of this anti-diagonal.
+            const float f0 = ring[0], f1 = ring[64], f2 = ring[128];
+            const int SFd = ringSW[rpos * 4 + rowInWave];
+            const float sc = ldexpf(totalInv, SFd + S - SFtb);
+            const float pM = f0 * v0 * sc, pX = f1 * v1 * sc, pY = f2 * v2 * sc;
+            const float thr = kp.threshold;
+            const bool okM = act && x > 0 && y > 0 && pM >= thr;
+            const bool okX = kp.withIndels && act && x > 0 && pX >= thr;
+            const bool okY = kp.withIndels && act && y > 0 && pY >= thr;
+            const uint64_t bM = __ballot(okM) & rowMask;
+            const uint64_t bX = __ballot(okX) & rowMask;
+            const uint64_t bY = __ballot(okY) & rowMask;
+            if (okM) {
+                const int idx = cntM + __popcll(bM & below);
+                if (idx < capM) {
+                    const long long w = (long long)floorf(fminf(pM, 1.f) * 1e7f);
+                    kp.out[outOff + idx] = make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
+                }
+            }
+            if (okX) {
+                const int idx = cntX + __popcll(bX & below);
+                if (idx < capX) {
+                    const long long w = (long long)floorf(fminf(pX, 1.f) * 1e7f);
+                    kp.out[outOff + capM + idx] = make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
+                }
+            }
+            if (okY) {
+                const int idx = cntY + __popcll(bY & below);
+                if (idx < capY) {
+                    const long long w = (long long)floorf(fminf(pY, 1.f) * 1e7f);
+                    kp.out[outOff + capM + capX + idx] =
+                        make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
+                }
+            }
+            cntM += __popcll(bM);
+            cntX += __popcll(bX);
+            cntY += __popcll(bY);
+        }
+
+        // Shift the exchange registers and bands.
+        oM2 = oM1; oX1 = o1; oY1 = o2; oM1 = o0;
+        Lm2 = Lm1; Rm2 = Rm1; Lm1 = L0; Rm1 = R0;
+        init = false;
+
+        if (fwd) {
+            if (d == tb) {
+                // Window total sum(F(tb) . end) and the switch to the backward sweep.
+                const int rg = (atEnd && ((flags >> 2) & 1)) ? 1 : 0;
+                const float t = row_sum<SEG>(v0 * kp.endV[rg][0] + v1 * kp.endV[rg][1] + v2 * kp.endV[rg][2]);
+                SFtb = S;
+                totalInv = t > 0.f ? 1.f / t : 0.f;
+                if (atEnd && k == 0) kp.logProb[job] = log((double)t) + (double)SFtb * 0.69314718055994530942;
+                fX1 = oX1; fY1 = oY1; fM1 = oM1; fM2 = oM2;
+                fLm1 = Lm1; fRm1 = Rm1; fLm2 = Lm2; fRm2 = Rm2; fS = S; fRpos = rpos;
+                oX1 = oY1 = oM1 = oM2 = 0.f;
+                Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
+                S = 0; dir = -1; init = true;
+                // backward: q = T p
+                c00 = kp.T[0]; c01 = kp.T[1]; c02 = kp.T[2];
+                c10 = kp.T[3]; c11 = kp.T[4]; c12 = kp.T[5];
+                c20 = kp.T[6]; c21 = kp.T[7]; c22 = kp.T[8];
+                phase = PH_BWD;
+            } else {
+                d += 1;
+                rpos = rpos + 1 == R ? 0 : rpos + 1;
+            }
+        } else {
+            if (d <= lo + 1) {
+                if (win + 1 == nWin) {
+                    if (k == 0) {
+                        kp.counts[3 * job + 0] = cntM;
+                        kp.counts[3 * job + 1] = cntX;
+                        kp.counts[3 * job + 2] = cntY;
+                    }
+                    phase = PH_FETCH;
+                } else {
+                    win += 1;
+                    const DevWin w1 = kp.win[winOff + win];
+                    d = tb + 1;
+                    tb = w1.tb; lo = w1.lo; from = w1.from;
+                    oX1 = fX1; oY1 = fY1; oM1 = fM1; oM2 = fM2;
+                    Lm1 = fLm1; Rm1 = fRm1; Lm2 = fLm2; Rm2 = fRm2; S = fS;
+                    rpos = fRpos + 1 == R ? 0 : fRpos + 1;
+                    dir = 1;
+                    c00 = kp.T[0]; c01 = kp.T[3]; c02 = kp.T[6];
+                    c10 = kp.T[1]; c11 = kp.T[4]; c12 = kp.T[7];
+                    c20 = kp.T[2]; c21 = kp.T[5]; c22 = kp.T[8];
+                    phase = PH_FWD;
+                }
+            } else {
+                d -= 1;
+                rpos = rpos == 0 ? R - 1 : rpos - 1;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void pairhmm_kernel(KParams kp) {
+    extern __shared__ float lds[];
+    for (int i = threadIdx.x; i < 2 * kTabStride; i += blockDim.x) lds[i] = kp.tables[i];
+    __syncthreads();
+    run_class<64>(kp, lds, 2);
+    run_class<32>(kp, lds, 1);
+    run_class<16>(kp, lds, 0);
+}
+
+__global__ void fill_i32(int32_t *p, int32_t v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+#define HIP_TRY(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            mp::set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call); \
+            return MP_EHIP;                                                               \
+        }                                                                                 \
+    } while (0)
+
+struct HostJob {
+    int64_t pair;
+    int64_t xOff, yOff;  // split offsets inside the (cropped) pair
+    int64_t symX, symY;
+    int32_t lX, lY;
+    int32_t flags;
+    int64_t bandOff;
+    int32_t winOff, nWin;
+    int32_t wmax;
+    int32_t cls;
+    int64_t cells;
+    int32_t cap[3];
+    int64_t outOff;
+    int32_t ringNeed;
+};
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    mp_status alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count == 0) return MP_OK;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            mp::set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            return MP_ENOMEM;
+        }
+        return MP_OK;
+    }
+    mp_status upload(const std::vector<T> &v, hipStream_t s) {
+        mp_status st = alloc(v.size());
+        if (st != MP_OK || v.empty()) return st;
+        HIP_TRY(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        return MP_OK;
+    }
+};
+
+}  // namespace
+
+struct mp_engine {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct mp_batch {
+    mp_engine *eng = nullptr;
+    int32_t mode = 0;
+    int32_t flags = 0;
+    int64_t npairs = 0;
+    std::vector<HostJob> jobs;         // in device order
+    std::vector<int64_t> pairCrop;     // reference crop offset per pair
+    std::vector<int64_t> pairFirstJob; // jobs of a pair are consecutive in split order in `order`
+    std::vector<int32_t> order;        // pair-major list of device job indices (split order)
+    std::vector<int32_t> pairNJobs;
+    int64_t cells = 0;
+    int32_t ringLen = 1;
+    int32_t classBegin[kNumClasses] = {0, 0, 0}, classEnd[kNumClasses] = {0, 0, 0};
+    KParams kp;
+    DevBuf<DevJob> dJobs;
+    DevBuf<uint16_t> dSym;
+    DevBuf<int2> dBand;
+    DevBuf<DevWin> dWin;
+    DevBuf<int4> dOut;
+    DevBuf<int32_t> dCounts;
+    DevBuf<double> dLogProb;
+    DevBuf<float> dRing;
+    DevBuf<int32_t> dRingS;
+    DevBuf<float> dTables;
+    DevBuf<int32_t> dQueue;
+    int64_t outTotal = 0;
+    int32_t gridRows = 0;
+    double kernelMs = 0.0;
+    bool ran = false;
+    std::vector<int32_t> hCounts;  // fetched
+};
+
+namespace {
+
+mp_status launch(mp_batch *b, int grid, int block, const KParams &kp) {
+    size_t lds = 2 * kTabStride * sizeof(float);
+    hipLaunchKernelGGL(pairhmm_kernel, dim3(grid), dim3(block), lds, b->eng->stream, kp);
+    HIP_TRY(hipGetLastError());
+    return MP_OK;
+}
+
+mp_status reset_queue(mp_batch *b, int32_t *q, int n) {
+    hipLaunchKernelGGL(fill_i32, dim3((n + 255) / 256), dim3(256), 0, b->eng->stream, q, 0, n);
+    HIP_TRY(hipGetLastError());
+    return MP_OK;
+}
+
+void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
+    tab.assign(2 * kTabStride, 0.f);
+    for (int s = 0; s < 2; s++) {
+        const mp::Machine &mc = m->mach[s];
+        float *t = tab.data() + s * kTabStride;
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+                double lp = (i >= 4 || j >= 4) ? -2.772588722 : mc.match[i * 4 + j];
+                t[kOffE + i * 5 + j] = (float)std::exp(lp);
+            }
+        for (int i = 0; i < 5; i++) {
+            t[kOffGX + i] = (float)std::exp(i >= 4 ? -1.386294361 : mc.gapX[i]);
+            t[kOffGY + i] = (float)std::exp(i >= 4 ? -1.386294361 : mc.gapY[i]);
+        }
+        for (int b = 0; b < 5; b++) {
+            int bb = b == 4 ? 0 : b;
+            int row = s ? bb : 3 - bb;
+            for (int u = 0; u < mp::kMaxRepeat; u++)
+                for (int o = 0; o < mp::kMaxRepeat; o++) {
+                    double l10 = m->repeatLog10[(size_t)row * mp::kMaxRepeat * mp::kMaxRepeat +
+                                                (size_t)u * mp::kMaxRepeat + o];
+                    t[kOffRL + b * 2601 + u * 51 + o] = (float)std::exp(2.3025 * l10);
+                }
+        }
+    }
+    const mp::Machine &f = m->mach[1];
+    auto P = [&](int i) { return (float)std::exp(f.trans[i]); };
+    // T[from*3 + to]
+    kp.T[0] = P(mp::kMatchContinue); kp.T[1] = P(mp::kGapOpenX);     kp.T[2] = P(mp::kGapOpenY);
+    kp.T[3] = P(mp::kMatchFromGapX); kp.T[4] = P(mp::kGapExtendX);   kp.T[5] = P(mp::kGapSwitchToY);
+    kp.T[6] = P(mp::kMatchFromGapY); kp.T[7] = P(mp::kGapSwitchToX); kp.T[8] = P(mp::kGapExtendY);
+    kp.startV[0][0] = 1.f; kp.startV[0][1] = 0.f; kp.startV[0][2] = 0.f;
+    kp.startV[1][0] = 0.f; kp.startV[1][1] = 1.f; kp.startV[1][2] = 1.f;
+    kp.endV[0][0] = P(mp::kMatchContinue);
+    kp.endV[0][1] = P(mp::kMatchFromGapX);
+    kp.endV[0][2] = P(mp::kMatchFromGapY);
+    kp.endV[1][0] = (float)std::exp((f.trans[mp::kGapOpenX] + f.trans[mp::kGapOpenY]) / 2.0);
+    kp.endV[1][1] = P(mp::kGapExtendX);
+    kp.endV[1][2] = P(mp::kGapExtendY);
+    kp.rle = f.rle;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
+    if (!out) {
+        mp::set_error("mp_engine_create: null argument");
+        return MP_EINVAL;
+    }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) {
+        mp::set_error("no HIP device " + std::to_string(device) + " visible");
+        return MP_ENODEVICE;
+    }
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+        mp::set_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        return MP_ENODEVICE;
+    }
+    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                2 * kTabStride * sizeof(float)));
+    mp_engine *e = new mp_engine();
+    e->device = device;
+    e->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        delete e;
+        mp::set_error("cannot create HIP stream/events");
+        return MP_EHIP;
+    }
+    *out = e;
+    return MP_OK;
+}
+
+extern "C" void mp_engine_free(mp_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" int32_t mp_engine_cu_count(const mp_engine *e) { return e ? e->cus : 0; }
+
+extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
+                                     const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out) {
+    if (!e || !m || !p || !out || (npairs > 0 && !pairs) || npairs < 0) {
+        mp::set_error("mp_batch_create: null or negative argument");
+        return MP_EINVAL;
+    }
+    if (mode != MP_MODE_POSTERIORS && mode != MP_MODE_MATCH_ONLY) {
+        mp::set_error("mp_batch_create: unknown mode");
+        return MP_EINVAL;
+    }
+    if (p->traceBackDiagonals < 1 || p->diagonalExpansion < 0 || (p->diagonalExpansion & 1) ||
+        p->minDiagsBetweenTraceBack < 2 || p->traceBackDiagonals + 1 >= p->minDiagsBetweenTraceBack) {
+        mp::set_error("mp_batch_create: band parameters violate the reference prerequisites "
+                      "(pairwiseAligner.c:692-696)");
+        return MP_EINVAL;
+    }
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(e->device));
+    mp_batch *b = new mp_batch();
+    b->eng = e;
+    b->mode = mode;
+    b->flags = flags;
+    b->npairs = npairs;
+    b->pairCrop.assign(npairs, 0);
+    b->pairNJobs.assign(npairs, 0);
+    b->pairFirstJob.assign(npairs, 0);
+
+    // ---- 1. symbols, cropping, split points -> jobs (pair order) ----
+    std::vector<uint16_t> sym;
+    std::vector<HostJob> jobs;
+    std::vector<std::vector<int64_t>> jobAnchors;
+    std::vector<int64_t> sp, anc;
+    for (int64_t i = 0; i < npairs; i++) {
+        const mp_pair &pr = pairs[i];
+        if (pr.lX < 0 || pr.lY < 0 || (pr.lX > 0 && !pr.sX) || (pr.lY > 0 && !pr.sY) ||
+            (pr.nAnchors > 0 && !pr.anchors) || pr.nAnchors < 0 || pr.lX > INT32_MAX / 2 || pr.lY > INT32_MAX / 2) {
+            delete b;
+            mp::set_error("pair " + std::to_string(i) + ": bad sequence or anchor arguments");
+            return MP_EINVAL;
+        }
+        if (!mp::anchors_valid(pr.anchors, pr.nAnchors, pr.lX, pr.lY)) {
+            delete b;
+            mp::set_error("pair " + std::to_string(i) + ": anchors must be strictly increasing, inside the "
+                          "sequences, with even non-negative expansion");
+            return MP_EANCHORS;
+        }
+        int64_t first = 0, end = pr.lX;
+        if ((flags & MP_BATCH_CROP_REFERENCE) && pr.nAnchors > 0) {
+            const int64_t *a0 = pr.anchors, *al = pr.anchors + 3 * (pr.nAnchors - 1);
+            first = std::max<int64_t>(a0[0] - a0[1], 0);
+            end = std::min<int64_t>(1 + al[0] + (pr.lY - al[1]), pr.lX);
+        }
+        b->pairCrop[i] = first;
+        const int64_t lX = end - first, lY = pr.lY;
+        anc.assign(pr.anchors ? pr.anchors : nullptr, pr.anchors ? pr.anchors + 3 * pr.nAnchors : nullptr);
+        for (int64_t a = 0; a < pr.nAnchors; a++) anc[3 * a] -= first;
+        const int64_t baseX = (int64_t)sym.size();
+        sym.insert(sym.end(), pr.sX + first, pr.sX + end);
+        const int64_t baseY = (int64_t)sym.size();
+        sym.insert(sym.end(), pr.sY, pr.sY + lY);
+        mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
+                         pr.raggedRight != 0, sp);
+        const int64_t ns = (int64_t)sp.size() / 4;
+        b->pairFirstJob[i] = (int64_t)jobs.size();
+        int64_t j = 0;
+        for (int64_t s = 0; s < ns; s++) {
+            const int64_t x1 = sp[4 * s], y1 = sp[4 * s + 1], x2 = sp[4 * s + 2], y2 = sp[4 * s + 3];
+            std::vector<int64_t> sub;
+            while (j < pr.nAnchors) {
+                const int64_t ax = anc[3 * j], ay = anc[3 * j + 1];
+                if (ax + ay >= x2 + y2) break;
+                sub.push_back(ax - x1);
+                sub.push_back(ay - y1);
+                sub.push_back(anc[3 * j + 2]);
+                j++;
+            }
+            HostJob hj{};
+            hj.pair = i;
+            hj.xOff = x1;
+            hj.yOff = y1;
+            hj.symX = baseX + x1;
+            hj.symY = baseY + y1;
+            hj.lX = (int32_t)(x2 - x1);
+            hj.lY = (int32_t)(y2 - y1);
+            const bool rl = pr.raggedLeft || s > 0, rr = pr.raggedRight || s < ns - 1;
+            hj.flags = (pr.strand ? 1 : 0) | (rl ? 2 : 0) | (rr ? 4 : 0);
+            jobs.push_back(hj);
+            jobAnchors.push_back(std::move(sub));
+        }
+        b->pairNJobs[i] = (int32_t)ns;
+    }
+
+    // ---- 2. bands, windows, classes (parallel over jobs) ----
+    const int64_t nj = (int64_t)jobs.size();
+    std::vector<int64_t> bandOff(nj + 1, 0);
+    for (int64_t j = 0; j < nj; j++) bandOff[j + 1] = bandOff[j] + (jobs[j].lX + jobs[j].lY + 1);
+    std::vector<int2> band((size_t)bandOff[nj]);
+    std::vector<std::vector<mp::Window>> wins(nj);
+    std::atomic<int> err{0};
+    {
+        const int nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::atomic<int64_t> next{0};
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&]() {
+                std::vector<int32_t> L, R;
+                for (;;) {
+                    int64_t j = next.fetch_add(1);
+                    if (j >= nj) break;
+                    HostJob &hj = jobs[j];
+                    const int64_t D = hj.lX + hj.lY;
+                    L.resize(D + 1);
+                    R.resize(D + 1);
+                    const std::vector<int64_t> &a = jobAnchors[j];
+                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion,
+                                        p->dynamicAnchorExpansion != 0, L.data(), R.data())) {
+                        err = 1;
+                        continue;
+                    }
+                    int32_t wmax = 0;
+                    int64_t cells = 0;
+                    for (int64_t d = 0; d <= D; d++) {
+                        int32_t w = (R[d] - L[d]) / 2 + 1;
+                        wmax = std::max(wmax, w);
+                        cells += w;
+                        band[bandOff[j] + d] = make_int2(L[d], R[d]);
+                    }
+                    hj.wmax = wmax;
+                    hj.cells = D > 0 ? cells : 0;
+                    hj.cls = wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3));
+                    hj.bandOff = bandOff[j];
+                    if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, wins[j]);
+                    int32_t need = 1;
+                    for (auto &w : wins[j]) need = std::max(need, w.tb - w.lo + 2);
+                    hj.ringNeed = need;
+                }
+            });
+        for (auto &t : th) t.join();
+    }
+    if (err) {
+        delete b;
+        mp::set_error("invalid band (anchors inconsistent with the sequences)");
+        return MP_EANCHORS;
+    }
+    for (int64_t j = 0; j < nj; j++) {
+        if (jobs[j].cls == 3) {
+            delete b;
+            mp::set_error("pair " + std::to_string(jobs[j].pair) + ": band width " + std::to_string(jobs[j].wmax) +
+                          " exceeds the 64-cell kernel limit");
+            return MP_EUNSUPPORTED;
+        }
+    }
+
+    // ---- 3. device order: by class, then band area descending ----
+    std::vector<int32_t> ord;
+    for (int64_t j = 0; j < nj; j++)
+        if (jobs[j].lX + jobs[j].lY > 0) ord.push_back((int32_t)j);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
+        if (jobs[a].cls != jobs[c].cls) return jobs[a].cls < jobs[c].cls;
+        return jobs[a].cells > jobs[c].cells;
+    });
+    std::vector<int32_t> devIndex(nj, -1);
+    std::vector<DevJob> dj(ord.size());
+    std::vector<DevWin> dw;
+    int64_t outOff = 0;
+    int32_t ringLen = 2;
+    for (size_t q = 0; q < ord.size(); q++) {
+        HostJob &hj = jobs[ord[q]];
+        devIndex[ord[q]] = (int32_t)q;
+        const int64_t D = hj.lX + hj.lY;
+        hj.cap[0] = (int32_t)std::min<int64_t>(2 * D + 64, INT32_MAX / 4);
+        hj.cap[1] = mode == MP_MODE_POSTERIORS ? (int32_t)std::min<int64_t>(D + 64, INT32_MAX / 4) : 0;
+        hj.cap[2] = hj.cap[1];
+        hj.outOff = outOff;
+        outOff += (int64_t)hj.cap[0] + hj.cap[1] + hj.cap[2];
+        hj.winOff = (int32_t)dw.size();
+        hj.nWin = (int32_t)wins[ord[q]].size();
+        for (auto &w : wins[ord[q]]) dw.push_back(DevWin{w.tb, w.lo, w.from, 0});
+        ringLen = std::max(ringLen, hj.ringNeed);
+        DevJob &d = dj[q];
+        d.symX = hj.symX; d.symY = hj.symY; d.bandOff = hj.bandOff; d.outOff = hj.outOff;
+        d.lX = hj.lX; d.lY = hj.lY; d.winOff = hj.winOff; d.nWin = hj.nWin;
+        d.cap[0] = hj.cap[0]; d.cap[1] = hj.cap[1]; d.cap[2] = hj.cap[2];
+        d.flags = hj.flags;
+        b->cells += hj.cells;
+    }
+    for (int c = 0; c < kNumClasses; c++) {
+        b->classBegin[c] = (int32_t)ord.size();
+        b->classEnd[c] = (int32_t)ord.size();
+    }
+    for (int c = kNumClasses - 1; c >= 0; c--)
+        for (size_t q = 0; q < ord.size(); q++)
+            if (jobs[ord[q]].cls == c) {
+                b->classBegin[c] = std::min<int32_t>(b->classBegin[c], (int32_t)q);
+                b->classEnd[c] = (int32_t)q + 1;
+            }
+    for (int c = 0; c < kNumClasses; c++)
+        if (b->classBegin[c] > b->classEnd[c]) b->classBegin[c] = b->classEnd[c];
+    // pair-major job list in split order (device indices; -1 for empty jobs)
+    b->order.resize(nj);
+    for (int64_t j = 0; j < nj; j++) b->order[j] = devIndex[j];
+    b->jobs.resize(ord.size());
+    for (size_t q = 0; q < ord.size(); q++) b->jobs[q] = jobs[ord[q]];
+    b->ringLen = ringLen;
+    b->outTotal = outOff;
+
+    // ---- 4. upload ----
+    hipStream_t s = e->stream;
+    std::vector<float> tab;
+    KParams &kp = b->kp;
+    std::memset(&kp, 0, sizeof(kp));
+    build_tables(m, tab, kp);
+    mp_status st;
+    if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
+        (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) || (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) ||
+        (st = b->dCounts.alloc(std::max<size_t>(3 * dj.size(), 1))) ||
+        (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dQueue.alloc(kNumClasses))) {
+        delete b;
+        return st;
+    }
+    // Ring: one slice per resident wavefront.
+    b->gridRows = e->cus;
+    const size_t waves = (size_t)e->cus * (kBlock / 64);
+    if ((st = b->dRing.alloc(waves * (size_t)ringLen * kRingStride)) ||
+        (st = b->dRingS.alloc(waves * (size_t)ringLen * 4))) {
+        delete b;
+        return st;
+    }
+    kp.jobs = b->dJobs.p;
+    kp.sym = b->dSym.p;
+    kp.band = b->dBand.p;
+    kp.win = b->dWin.p;
+    kp.out = b->dOut.p;
+    kp.counts = b->dCounts.p;
+    kp.logProb = b->dLogProb.p;
+    kp.ring = b->dRing.p;
+    kp.ringS = b->dRingS.p;
+    kp.tables = b->dTables.p;
+    kp.queue = b->dQueue.p;
+    kp.ringLen = ringLen;
+    kp.withIndels = mode == MP_MODE_POSTERIORS;
+    kp.rescaleMask = 3;
+    kp.threshold = (float)p->threshold;
+    for (int c = 0; c < kNumClasses; c++) {
+        kp.jobBegin[c] = b->classBegin[c];
+        kp.jobEnd[c] = b->classEnd[c];
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = b;
+    return MP_OK;
+}
+
+static mp_status batch_run_impl(mp_batch *b, bool plain) {
+    mp_engine *e = b->eng;
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    const int32_t nj = (int32_t)b->jobs.size();
+    if (nj == 0) {
+        b->kernelMs = 0.0;
+        b->ran = true;
+        return MP_OK;
+    }
+    mp_status st;
+    float ms = 0.f;
+    if (!plain) {
+        if ((st = reset_queue(b, b->dQueue.p, kNumClasses))) return st;
+        HIP_TRY(hipEventRecord(e->ev0, s));
+        if ((st = launch(b, e->cus, kBlock, b->kp))) return st;
+        HIP_TRY(hipEventRecord(e->ev1, s));
+        HIP_TRY(hipEventSynchronize(e->ev1));
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    } else {
+        // one launch (one wavefront) per job, in device order
+        DevBuf<int32_t> q;
+        if ((st = q.alloc((size_t)nj * kNumClasses))) return st;
+        if ((st = reset_queue(b, q.p, nj * kNumClasses))) return st;
+        HIP_TRY(hipEventRecord(e->ev0, s));
+        for (int32_t j = 0; j < nj; j++) {
+            KParams kp = b->kp;
+            for (int c = 0; c < kNumClasses; c++) kp.jobBegin[c] = kp.jobEnd[c] = j;
+            const int c = b->jobs[j].cls;
+            kp.jobEnd[c] = j + 1;
+            kp.queue = q.p + (size_t)j * kNumClasses;
+            if ((st = launch(b, 1, 64, kp))) return st;
+        }
+        HIP_TRY(hipEventRecord(e->ev1, s));
+        HIP_TRY(hipEventSynchronize(e->ev1));
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    }
+    b->kernelMs = ms;
+    // Overflowing output lists: size them exactly and run the batch again.
+    b->hCounts.resize((size_t)nj * 3);
+    HIP_TRY(hipMemcpy(b->hCounts.data(), b->dCounts.p, sizeof(int32_t) * 3 * nj, hipMemcpyDeviceToHost));
+    bool overflow = false;
+    for (int32_t j = 0; j < nj && !overflow; j++)
+        for (int l = 0; l < 3; l++)
+            if (b->hCounts[3 * j + l] > b->jobs[j].cap[l]) overflow = true;
+    if (overflow) {
+        std::vector<DevJob> dj(nj);
+        HIP_TRY(hipMemcpy(dj.data(), b->dJobs.p, sizeof(DevJob) * nj, hipMemcpyDeviceToHost));
+        int64_t off = 0;
+        for (int32_t j = 0; j < nj; j++) {
+            for (int l = 0; l < 3; l++) {
+                b->jobs[j].cap[l] = std::max(b->jobs[j].cap[l], b->hCounts[3 * j + l]);
+                dj[j].cap[l] = b->jobs[j].cap[l];
+            }
+            b->jobs[j].outOff = off;
+            dj[j].outOff = off;
+            off += (int64_t)dj[j].cap[0] + dj[j].cap[1] + dj[j].cap[2];
+        }
+        HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
+        if ((st = b->dOut.alloc(std::max<int64_t>(off, 1)))) return st;
+        b->kp.out = b->dOut.p;
+        b->outTotal = off;
+        return batch_run_impl(b, plain);
+    }
+    b->ran = true;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_run(mp_batch *b) {
+    if (!b) {
+        mp::set_error("mp_batch_run: null batch");
+        return MP_EINVAL;
+    }
+    return batch_run_impl(b, false);
+}
+
+extern "C" mp_status mp_batch_run_plain(mp_batch *b) {
+    if (!b) {
+        mp::set_error("mp_batch_run_plain: null batch");
+        return MP_EINVAL;
+    }
+    return batch_run_impl(b, true);
+}
+
+extern "C" double mp_batch_kernel_ms(const mp_batch *b) { return b ? b->kernelMs : 0.0; }
+extern "C" int64_t mp_batch_cells(const mp_batch *b) { return b ? b->cells : 0; }
+extern "C" int64_t mp_batch_npairs(const mp_batch *b) { return b ? b->npairs : 0; }
+
+extern "C" mp_status mp_batch_fetch_counts(mp_batch *b, int64_t *counts, double *logProb) {
+    if (!b || !counts) {
+        mp::set_error("mp_batch_fetch_counts: null argument");
+        return MP_EINVAL;
+    }
+    if (!b->ran) {
+        mp::set_error("mp_batch_fetch_counts: batch has not run");
+        return MP_EINVAL;
+    }
+    const int32_t nj = (int32_t)b->jobs.size();
+    std::vector<double> lp(std::max(nj, 1));
+    if (nj) {
+        HIP_TRY(hipSetDevice(b->eng->device));
+        HIP_TRY(hipMemcpy(lp.data(), b->dLogProb.p, sizeof(double) * nj, hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < b->npairs; i++) {
+        int64_t c[3] = {0, 0, 0};
+        double l = 0.0;
+        for (int32_t s = 0; s < b->pairNJobs[i]; s++) {
+            int32_t q = b->order[b->pairFirstJob[i] + s];
+            if (q < 0) continue;
+            for (int k = 0; k < 3; k++) c[k] += b->hCounts[3 * q + k];
+            l += lp[q];
+        }
+        for (int k = 0; k < 3; k++) counts[3 * i + k] = c[k];
+        if (logProb) logProb[i] = l;
+    }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_pair *out) {
+    if (!b || list < 0 || list > 2) {
+        mp::set_error("mp_batch_fetch_list: bad argument");
+        return MP_EINVAL;
+    }
+    if (!b->ran) {
+        mp::set_error("mp_batch_fetch_list: batch has not run");
+        return MP_EINVAL;
+    }
+    const int32_t nj = (int32_t)b->jobs.size();
+    if (nj == 0) return MP_OK;
+    std::vector<int4> host((size_t)b->outTotal);
+    HIP_TRY(hipSetDevice(b->eng->device));
+    HIP_TRY(hipMemcpy(host.data(), b->dOut.p, sizeof(int4) * b->outTotal, hipMemcpyDeviceToHost));
+    int64_t w = 0;
+    for (int64_t i = 0; i < b->npairs; i++) {
+        const int64_t crop = b->pairCrop[i];
+        for (int32_t s = 0; s < b->pairNJobs[i]; s++) {
+            const int32_t q = b->order[b->pairFirstJob[i] + s];
+            if (q < 0) continue;
+            const HostJob &hj = b->jobs[q];
+            int64_t base = hj.outOff;
+            for (int l = 0; l < list; l++) base += hj.cap[l];
+            const int32_t n = b->hCounts[3 * q + list];
+            // the reference appends each split's list reversed (pairwiseAligner.c:1072-1074)
+            for (int32_t t = n - 1; t >= 0; t--) {
+                const int4 v = host[base + t];
+                mp_aligned_pair &o = out[w++];
+                o.weight = (int64_t)(uint32_t)v.x | ((int64_t)v.y << 32);
+                o.x = (int32_t)(v.z + hj.xOff + crop);
+                o.y = (int32_t)(v.w + hj.yOff);
+            }
+        }
+    }
+    return MP_OK;
+}
+
+extern "C" void mp_batch_free(mp_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->eng->device);
+    delete b;
+}
+
+extern "C" mp_status mp_pairhmm_posteriors(mp_engine *e, const mp_model *m, const mp_pair *pair,
+                                           const mp_band_params *p, int32_t mode, mp_aligned_pair *matches,
+                                           int64_t matchCap, int64_t *matchCount, mp_aligned_pair *gapX,
+                                           int64_t gapXCap, int64_t *gapXCount, mp_aligned_pair *gapY,
+                                           int64_t gapYCap, int64_t *gapYCount, double *logProb) {
+    if (!pair || !matchCount) {
+        mp::set_error("mp_pairhmm_posteriors: null argument");
+        return MP_EINVAL;
+    }
+    mp_batch *b = nullptr;
+    mp_status st = mp_batch_create(e, m, pair, 1, p, mode, 0, &b);
+    if (st) return st;
+    if ((st = mp_batch_run_plain(b))) {
+        mp_batch_free(b);
+        return st;
+    }
+    int64_t counts[3];
+    double lp = 0.0;
+    if ((st = mp_batch_fetch_counts(b, counts, &lp))) {
+        mp_batch_free(b);
+        return st;
+    }
+    mp_aligned_pair *dst[3] = {matches, gapX, gapY};
+    int64_t caps[3] = {matchCap, gapXCap, gapYCap};
+    int64_t *cnt[3] = {matchCount, gapXCount, gapYCount};
+    for (int l = 0; l < 3; l++) {
+        if (cnt[l]) *cnt[l] = counts[l];
+        if (!dst[l] || counts[l] == 0) continue;
+        std::vector<mp_aligned_pair> tmp(counts[l]);
+        if ((st = mp_batch_fetch_list(b, l, tmp.data()))) {
+            mp_batch_free(b);
+            return st;
+        }
+        std::memcpy(dst[l], tmp.data(), sizeof(mp_aligned_pair) * std::min(counts[l], caps[l]));
+    }
+    if (logProb) *logProb = lp;
+    mp_batch_free(b);
+    return MP_OK;
+}

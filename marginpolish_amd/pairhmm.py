@@ -1,0 +1,285 @@
+"""Reference-shaped Python interface of the MI355X pair-HMM engine.
+
+Mirrors the entry points of impl/pairwiseAligner.c and the realign caller of
+impl/polisher.c, with the same argument meaning:
+
+  params_readParams(path)                        parser.c:503-528
+  getAlignedPairsWithIndelsUsingAnchors(...)     pairwiseAligner.c:1105-1130
+  getAlignedPairsUsingAnchors(...)               pairwiseAligner.c:1088-1103
+  getAlignedPairs / getAlignedPairsWithIndels    pairwiseAligner.c:1132-1149
+  getAlignedPairsWithIndelsCroppingReference     polisher.c:605-654
+  realign_all(...)                               the per-read loop of poa_realign
+                                                 (polisher.c:669-700), batched
+
+Lists are numpy structured arrays with fields (weight, x, y), in the
+reference's output order.  Every call runs on the GPU through
+libmp_pairhmm.so; errors raise MPError (the reference aborts instead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALIGNED_PAIR_DTYPE, MPError, check, lib  # noqa: F401
+
+
+@dataclass
+class PairwiseAlignmentParameters:
+    """pairwiseAligner.h:28-49 (fields on this path), defaults of
+    pairwiseAlignmentBandingParameters_construct (pairwiseAligner.c:999-1011)."""
+    threshold: float = 0.01
+    minDiagsBetweenTraceBack: int = 1000
+    traceBackDiagonals: int = 40
+    diagonalExpansion: int = 20
+    splitMatrixBiggerThanThis: int = 3000 * 3000
+    dynamicAnchorExpansion: bool = False
+
+    def c(self):
+        return _lib.BandParams(self.threshold, self.minDiagsBetweenTraceBack, self.traceBackDiagonals,
+                               self.diagonalExpansion, self.splitMatrixBiggerThanThis,
+                               int(bool(self.dynamicAnchorExpansion)))
+
+
+class StateMachines:
+    """Both strands' 3-state machines and tables (an mp_model)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_json(cls, path):
+        h = C.c_void_p()
+        check(lib().mp_model_load_json(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def default_nucleotide(cls):
+        """stateMachine3_constructNucleotide(threeState)."""
+        h = C.c_void_p()
+        check(lib().mp_model_default_nucleotide(C.byref(h)))
+        return cls(h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def band_params(self):
+        bp = _lib.BandParams()
+        check(lib().mp_model_band_params(self._h, C.byref(bp)))
+        return PairwiseAlignmentParameters(bp.threshold, bp.minDiagsBetweenTraceBack, bp.traceBackDiagonals,
+                                           bp.diagonalExpansion, bp.splitMatrixBiggerThanThis,
+                                           bool(bp.dynamicAnchorExpansion))
+
+    def tables(self, strand):
+        t = (C.c_double * 9)()
+        m = (C.c_double * 16)()
+        gx = (C.c_double * 4)()
+        gy = (C.c_double * 4)()
+        rle = C.c_int32()
+        check(lib().mp_model_tables(self._h, int(strand), t, m, gx, gy, C.byref(rle)))
+        return dict(trans=list(t), match=list(m), gapX=list(gx), gapY=list(gy), rle=bool(rle.value))
+
+    def repeat_log10(self, base, strand, observed, underlying):
+        return lib().mp_model_repeat_log10(self._h, base, strand, observed, underlying)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.mp_model_free(self._h)
+            self._h = None
+
+
+class PolishParams:
+    """params_readParams for the realignment path: the read/reference state
+    machines (forward + reverse strand) and the pairwise alignment params."""
+
+    def __init__(self, path):
+        self.machines = StateMachines.from_json(path)
+        self.p = self.machines.band_params()
+        self.useRepeatCountsInAlignment = self.machines.tables(1)["rle"]
+        self.maximumRepeatLength = lib().mp_model_max_repeat_count(self.machines.handle)
+
+
+def params_readParams(path):
+    return PolishParams(path)
+
+
+class Engine:
+    """One gfx950 device and its stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().mp_engine_create(int(device), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def cu_count(self):
+        return lib().mp_engine_cu_count(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mp_engine_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_engine = None
+
+
+def default_engine():
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+def _as_u16(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint16))
+
+
+def _as_anchors(a):
+    if a is None:
+        return np.zeros((0, 3), dtype=np.int64)
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.int64).reshape(-1, 3))
+    return arr
+
+
+class Batch:
+    """A device-resident batch of read/reference pairs (mp_batch)."""
+
+    def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None):
+        """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight])."""
+        self.engine = engine or default_engine()
+        self._keep = []
+        n = len(pairs)
+        arr = (_lib.Pair * max(n, 1))()
+        for i, pr in enumerate(pairs):
+            sx, sy, an, strand = pr[0], pr[1], pr[2], pr[3]
+            rl = pr[4] if len(pr) > 4 else False
+            rr = pr[5] if len(pr) > 5 else False
+            sx, sy, an = _as_u16(sx), _as_u16(sy), _as_anchors(an)
+            self._keep += [sx, sy, an]
+            arr[i] = _lib.Pair(sx.ctypes.data_as(C.POINTER(C.c_uint16)), len(sx),
+                               sy.ctypes.data_as(C.POINTER(C.c_uint16)), len(sy),
+                               an.ctypes.data_as(C.POINTER(C.c_int64)), an.shape[0], int(strand), int(rl), int(rr))
+        h = C.c_void_p()
+        pc = p.c()
+        check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
+                                    _lib.BATCH_CROP_REFERENCE if crop_reference else 0, C.byref(h)))
+        self._h = h
+        self.npairs = n
+        self.mode = mode
+
+    def run(self):
+        check(lib().mp_batch_run(self._h))
+        return self
+
+    def run_plain(self):
+        check(lib().mp_batch_run_plain(self._h))
+        return self
+
+    @property
+    def kernel_ms(self):
+        return lib().mp_batch_kernel_ms(self._h)
+
+    @property
+    def cells(self):
+        return lib().mp_batch_cells(self._h)
+
+    def counts(self):
+        c = np.zeros((max(self.npairs, 1), 3), dtype=np.int64)
+        lp = np.zeros(max(self.npairs, 1), dtype=np.float64)
+        check(lib().mp_batch_fetch_counts(self._h, c.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          lp.ctypes.data_as(C.POINTER(C.c_double))))
+        return c[: self.npairs], lp[: self.npairs]
+
+    def fetch(self):
+        """Per pair: (matches, gapX, gapY, logProb)."""
+        counts, lp = self.counts()
+        lists = []
+        for li in range(3):
+            tot = int(counts[:, li].sum())
+            buf = np.zeros(max(tot, 1), dtype=ALIGNED_PAIR_DTYPE)
+            if tot:
+                check(lib().mp_batch_fetch_list(self._h, li, buf.ctypes.data_as(C.POINTER(_lib.AlignedPair))))
+            lists.append(np.split(buf[:tot], np.cumsum(counts[:, li])[:-1]) if self.npairs else [])
+        return [(lists[0][i], lists[1][i], lists[2][i], float(lp[i])) for i in range(self.npairs)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mp_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _single(machines, strand, sX, sY, anchors, p, ragged_left, ragged_right, mode, engine=None):
+    b = Batch(machines, [(sX, sY, anchors, strand, ragged_left, ragged_right)], p, mode=mode, engine=engine)
+    b.run_plain()
+    m, gx, gy, lp = b.fetch()[0]
+    b.close()
+    return m, gx, gy, lp
+
+
+def getAlignedPairsWithIndelsUsingAnchors(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd=False,
+                                          alignmentHasRaggedRightEnd=False, engine=None):
+    """Returns (alignedPairs, gapXPairs, gapYPairs)."""
+    m, gx, gy, _ = _single(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd,
+                           alignmentHasRaggedRightEnd, _lib.MODE_POSTERIORS, engine)
+    return m, gx, gy
+
+
+def getAlignedPairsUsingAnchors(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd=False,
+                                alignmentHasRaggedRightEnd=False, engine=None):
+    m, _, _, _ = _single(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd,
+                         alignmentHasRaggedRightEnd, _lib.MODE_MATCH_ONLY, engine)
+    return m
+
+
+def getAlignedPairs(machines, strand, sX, sY, p, alignmentHasRaggedLeftEnd=False, alignmentHasRaggedRightEnd=False,
+                    engine=None):
+    return getAlignedPairsUsingAnchors(machines, strand, sX, sY, None, p, alignmentHasRaggedLeftEnd,
+                                       alignmentHasRaggedRightEnd, engine)
+
+
+def getAlignedPairsWithIndels(machines, strand, sX, sY, p, alignmentHasRaggedLeftEnd=False,
+                              alignmentHasRaggedRightEnd=False, engine=None):
+    return getAlignedPairsWithIndelsUsingAnchors(machines, strand, sX, sY, None, p, alignmentHasRaggedLeftEnd,
+                                                 alignmentHasRaggedRightEnd, engine)
+
+
+def getAlignedPairsWithIndelsCroppingReference(params, ref_sym, read_sym, read_strand, anchors, engine=None):
+    """polisher.c:605-654 for one read: returns (matches, inserts, deletes) in
+    uncropped reference coordinates (gap-X -> deletes, gap-Y -> inserts)."""
+    b = Batch(params.machines, [(ref_sym, read_sym, anchors, read_strand)], params.p, crop_reference=True,
+              engine=engine)
+    b.run_plain()
+    m, gx, gy, _ = b.fetch()[0]
+    b.close()
+    return m, gy, gx
+
+
+def realign_all(params, pairs, engine=None):
+    """The per-read loop of poa_realign over every read of every chunk, as one
+    batched launch.  pairs: (ref_sym, read_sym, anchors, strand) per read.
+    Returns per read (matches, inserts, deletes)."""
+    b = Batch(params.machines, pairs, params.p, crop_reference=True, engine=engine)
+    b.run()
+    out = [(m, gy, gx) for (m, gx, gy, _) in b.fetch()]
+    b.close()
+    return out

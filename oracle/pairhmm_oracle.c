@@ -418,7 +418,7 @@ struct or_result {
     List lists[3];        /* match, gapX, gapY */
     double expect[9];     /* transition expectations T[from*3+to] */
     double likelihood;    /* per-diagonal accumulated total (expectation hack) */
-    double totalLogProb;  /* last window's total of the last sub-problem */
+    double totalLogProb;  /* sum over sub-problems of the last window's total (forward log-likelihood) */
     int64_t cells;        /* band cells of all sub-problems */
     int status;
 };
@@ -534,7 +534,7 @@ static int banded_fb(const Ctx *c, const int64_t *anchors, int64_t n, const uint
                 if (d2 <= from) {
                     if (count++ % 10 == 0) {
                         total = diag_total(c, &dp, sX, sY, d2);
-                        if (count == 1) res->totalLogProb = total;
+                        if (count == 1 && atEnd) res->totalLogProb += total;
                     }
                     diag_posterior(c, &dp, sX, sY, d2, total, p->threshold, kind, sub, res);
                     if (d2 < from || atEnd) dp.hasF[d2] = 0;

@@ -1,0 +1,258 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bar (FP32 scaled arithmetic vs FP64): every posterior within 1e-4 absolute of
+O_exact (exact log-sum-exp, same bands, windows and splits); an entry may be
+present on one side only when its posterior is within 1e-4 of the threshold.
+Output order equals the reference order for the entries both sides hold.
+"""
+import os
+import random
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import brute
+from oracle import oracle as O
+from tests.conftest import PARAMS
+from tests.parity import as_map, as_rows, compare, order_consistent
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ph():
+    from marginpolish_amd import pairhmm
+    return pairhmm
+
+
+@pytest.fixture(scope="module")
+def params(ph):
+    return ph.PolishParams(PARAMS), O.PolishParams(PARAMS)
+
+
+def _bp(ph, op):
+    return ph.PairwiseAlignmentParameters(op.threshold, op.minDiagsBetweenTraceBack, op.traceBackDiagonals,
+                                          op.diagonalExpansion, op.splitMatrixBiggerThanThis,
+                                          op.dynamicAnchorExpansion)
+
+
+def test_known_answer_agcg(ph):
+    say("case: AGCG/AGTTCG default machine, threshold 0.2")
+    m = ph.StateMachines.default_nucleotide()
+    p = ph.PairwiseAlignmentParameters(threshold=0.2, diagonalExpansion=2)
+    mm = ph.getAlignedPairsUsingAnchors(m, 1, O.symbols("AGCG"), O.symbols("AGTTCG"), None, p)
+    assert {(int(r["x"]), int(r["y"])) for r in mm} == {(0, 0), (1, 1), (2, 4), (3, 5)}
+
+
+def _rand_syms(rng, n, rle):
+    bases = "".join(rng.choice("ACGT") for _ in range(n))
+    if not rle:
+        return O.symbols(bases)
+    return O.rle_symbols(bases, [rng.choice([1, 1, 2, 3, 4, 7, 12]) for _ in range(n)])
+
+
+def test_brute_force_tiny(ph, params):
+    """All cell-states (threshold 0) vs exact path enumeration, 3 machines x 4
+    ragged combinations, in one batched launch."""
+    pp, op = params
+    dm = ph.StateMachines.default_nucleotide()
+    odm = O.default_nucleotide_model()
+    rng = random.Random(101)
+    p = ph.PairwiseAlignmentParameters(threshold=0.0, diagonalExpansion=20, dynamicAnchorExpansion=True)
+    for mach, models in ((dm, (odm, odm)), (pp.machines, (op.reverse, op.forward))):
+        cases = []
+        for i in range(40):
+            rl, rr, strand = bool(i & 1), bool(i & 2), (i >> 2) & 1
+            lX, lY = rng.randint(1, 5), rng.randint(1, 5)
+            om = models[strand]
+            cases.append((_rand_syms(rng, lX, om.rle), _rand_syms(rng, lY, om.rle), None, strand, rl, rr))
+        say("case: brute force, rle=%s, %d pairs" % (models[1].rle, len(cases)))
+        b = ph.Batch(mach, cases, p)
+        b.run()
+        res = b.fetch()
+        for (sx, sy, _, strand, rl, rr), (gm, gx, gy, lp) in zip(cases, res):
+            total, post = brute.enumerate_paths(models[strand], sx, sy, rl, rr)
+            assert abs(lp - total) < 1e-4 * max(1.0, abs(total))
+            for st, lst in ((0, gm), (1, gx), (2, gy)):
+                got = np.zeros_like(post[st])
+                for r in lst:
+                    got[r["x"] + 1, r["y"] + 1] = r["weight"] / 1e7
+                want = np.minimum(post[st], 1.0)
+                if st == 0:
+                    want[0, :] = 0; want[:, 0] = 0
+                elif st == 1:
+                    want[0, :] = 0
+                else:
+                    want[:, 0] = 0
+                assert np.max(np.abs(got - want)) <= TOL + 1e-7, (st, np.max(np.abs(got - want)))
+
+
+def _oracle_jobs(op, jobs, oparams, mode=O.LOGADD_EXACT):
+    out = []
+    for (sx, sy, an, strand, *rg) in jobs:
+        rl = rg[0] if rg else False
+        rr = rg[1] if len(rg) > 1 else False
+        m = op.forward if strand else op.reverse
+        out.append(O.posteriors(m, sx, sy, an, oparams, rl, rr, mode=mode))
+    return out
+
+
+def _check(gres, ores, thr, check_order=True):
+    worst = 0.0
+    for (gm, gx, gy, glp), (om, ox, oy, olp, _) in zip(gres, ores):
+        for g, o in ((gm, om), (gx, ox), (gy, oy)):
+            g = as_rows(g)
+            err, ncom, _ = compare(g, o, thr, TOL)
+            assert err <= TOL + 1e-7, err
+            worst = max(worst, err)
+            if check_order:
+                assert order_consistent(g, o)
+        assert abs(glp - olp) <= 1e-3 * max(1.0, abs(olp)), (glp, olp)
+    return worst
+
+
+def test_config0_parity(ph, params):
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(strands=(1, 0))
+    jobs = []
+    for pr in w.pairs:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    say("case: config0 20 reads x 5 kb, W=32, both strands")
+    b = ph.Batch(pp.machines, jobs, pp.p)
+    b.run()
+    g = b.fetch()
+    worst = _check(g, _oracle_jobs(op, jobs, op.p), op.p.threshold)
+    say("config0 max-abs posterior err vs O_exact: %.3g" % worst)
+    # O_faithful differs from O_exact by the reference's own logAdd error
+    of = _oracle_jobs(op, jobs[:3], op.p, O.LOGADD_POLY)
+    errs = [compare(as_rows(gg[0]), oo[0], op.p.threshold, 1.0)[0] for gg, oo in zip(g[:3], of)]
+    say("config0 max-abs posterior diff vs O_faithful: %.3g" % max(errs))
+
+
+def test_multi_window_parity(ph, params):
+    """Several traceback windows per pair (minDiagsBetweenTraceBack 300)."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(n_reads=6, ref_len=2000, seed=7, W=10, strands=(1, 0))
+    jobs = []
+    for pr in w.pairs:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=300, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    say("case: multi-window, 6 pairs")
+    b = ph.Batch(pp.machines, jobs, _bp(ph, opar))
+    b.run()
+    _check(b.fetch(), _oracle_jobs(op, jobs, opar), opar.threshold)
+
+
+def test_split_and_ragged_parity(ph, params):
+    """Large-gap splits (tiny splitMatrixBiggerThanThis) with ragged ends."""
+    pp, op = params
+    rng = random.Random(5)
+    jobs = []
+    for i in range(8):
+        n = rng.randint(60, 200)
+        sx = _rand_syms(rng, n, True)
+        sy = sx.copy()
+        keep = [j for j in range(n) if rng.random() > 0.05]
+        sy = sx[keep]
+        anchors = []
+        for k in range(0, len(keep), rng.randint(15, 40)):
+            if not anchors or (keep[k] > anchors[-1][0] and k > anchors[-1][1]):
+                anchors.append((keep[k], k, 6))
+        jobs.append((sx, sy, np.array(anchors, np.int64), i & 1, bool(i & 2), bool(i & 4)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=6,
+                    splitMatrixBiggerThanThis=900, dynamicAnchorExpansion=True)
+    say("case: splits + ragged, 8 pairs")
+    b = ph.Batch(pp.machines, jobs, _bp(ph, opar))
+    b.run()
+    _check(b.fetch(), _oracle_jobs(op, jobs, opar), opar.threshold)
+
+
+@pytest.fixture(scope="module")
+def config1(params):
+    from marginpolish_amd import sim
+    return sim.config1()
+
+
+def test_config1_subset_realign(ph, params, config1):
+    """poa_realign caller semantics (reference cropping, gapX->deletes,
+    gapY->inserts) on 300 chunk-level pairs of config 1, one batched launch."""
+    from marginpolish_amd import crop
+    pp, op = params
+    rng = random.Random(3)
+    sel = rng.sample(range(len(config1.pairs)), 300)
+    pairs = [(config1.pairs[i].ref_sym, config1.pairs[i].read_sym, config1.pairs[i].anchors,
+              config1.pairs[i].strand) for i in sel]
+    say("case: config1 subset 300 pairs, realign_all")
+    res = ph.realign_all(pp, pairs)
+    worst = 0.0
+    for (rs, qs, an, strand), (m, ins, dels) in zip(pairs, res):
+        sx, sy, a2, off = crop.crop_pair(rs, qs, an)
+        om, ox, oy, olp, _ = O.posteriors(op.forward if strand else op.reverse, sx, sy, a2, op.p)
+        for g, o in ((m, om), (dels, ox), (ins, oy)):
+            o = o.copy()
+            o[:, 1] += off
+            err, _, _ = compare(as_rows(g), o, op.p.threshold, TOL)
+            worst = max(worst, err)
+            assert err <= TOL + 1e-7
+    say("config1 subset max-abs posterior err vs O_exact: %.3g" % worst)
+
+
+def test_batch_equals_plain_and_match_only(ph, params, config1):
+    from marginpolish_amd import crop
+    pp, op = params
+    jobs = []
+    for pr in config1.pairs[:40]:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    say("case: batch vs plain, 40 pairs")
+    b1 = ph.Batch(pp.machines, jobs, pp.p).run()
+    b2 = ph.Batch(pp.machines, jobs, pp.p).run_plain()
+    b3 = ph.Batch(pp.machines, jobs, pp.p, mode=1).run()
+    for r1, r2, r3 in zip(b1.fetch(), b2.fetch(), b3.fetch()):
+        for l in range(3):
+            assert np.array_equal(r1[l], r2[l])
+        assert np.array_equal(r1[0], r3[0]) and len(r3[1]) == 0 and len(r3[2]) == 0
+        assert r1[3] == r2[3]
+
+
+def test_config1_full_batch_properties(ph, params, config1):
+    """Every pair of config 1 in one launch; size-independent checks."""
+    pp, op = params
+    pairs = [(p.ref_sym, p.read_sym, p.anchors, p.strand) for p in config1.pairs]
+    say("case: config1 full batch %d pairs" % len(pairs))
+    b = ph.Batch(pp.machines, pairs, pp.p, crop_reference=True)
+    b.run()
+    counts, lp = b.counts()
+    assert np.all(np.isfinite(lp)) and np.all(lp < 0)
+    res = b.fetch()
+    for (rs, qs, an, _), (m, gx, gy, _) in zip(pairs[::50], res[::50]):
+        mx = np.zeros(len(rs) + 1)
+        for arr in (m, gx):
+            assert np.all(arr["weight"] > 0) and np.all(arr["weight"] <= 10 ** 7)
+            assert np.all(arr["x"] < len(rs)) and np.all(arr["y"] < len(qs))
+            np.add.at(mx, arr["x"], arr["weight"])
+        assert np.all(mx <= 1e7 * (1 + 1e-3) + 16)
+        assert len(as_map(as_rows(m))) == len(m)
+
+
+def test_wide_band_rejected_loudly(ph):
+    m = ph.StateMachines.default_nucleotide()
+    p = ph.PairwiseAlignmentParameters()
+    rng = random.Random(1)
+    sx = _rand_syms(rng, 150, False)
+    with pytest.raises(ph.MPError) as ei:
+        ph.getAlignedPairs(m, 1, sx, sx, p)
+    assert ei.value.status == 6
