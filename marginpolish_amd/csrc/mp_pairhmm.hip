@@ -7,25 +7,30 @@
 //
 // Kernel design
 //  * One band anti-diagonal per step; one lane per band cell.  A job (one
-//    read/reference sub-problem) owns a "row" of SEG lanes (SEG = 16, 32 or
-//    64, the smallest power of two >= the job's widest anti-diagonal), so a
-//    64-lane wavefront runs 64/SEG independent jobs.
-//  * Forward and backward are the same step in probability space:
-//      forward : p = e (.) in,  q = T^t p,  value = p, send q
-//      backward: p = in,        q = T p,    value = q, send e (.) q
-//    with the three neighbour inputs (gap-X from x-1 / x+1, gap-Y from y-1 /
-//    y+1 on the previous step, match from two steps back) exchanged with
-//    ds_bpermute inside the row.  A row switches between forward and backward
-//    by selects, so rows of one wave in different phases do not diverge.
-//  * FP32 scaled arithmetic: every K anti-diagonals the row maximum is folded
-//    to [0.5,1) by an exact power-of-two rescale; the exponents are integers,
-//    so the scale bookkeeping is exact.  Per traceback window the total is
-//    sum(F(tb) . end) (the cut total of the reference is constant across the
-//    window in exact arithmetic); log-likelihoods are formed in FP64.
+//    read/reference sub-problem) owns a "row" of seg lanes (16, 32 or 64, the
+//    smallest power of two >= the job's widest anti-diagonal), so a 64-lane
+//    wavefront runs 64/seg independent jobs.
+//  * Anti-diagonals are walked in grid-aligned macro steps of 8 (one "cell"
+//    of the grid d = 8g..8g+7), forward ascending and backward descending.
+//    The 8 steps are unrolled and every per-step operand (packed band entry,
+//    symbols, forward values read back by the backward sweep) is a statically
+//    indexed register loaded one macro step ahead (the band two), so no wait
+//    inside a macro step covers a memory operation of that step.
+//  * Probability space: forward f = e (.) in, sends q = T^t f; backward
+//    B = T in, sends u = e (.) B.  The three neighbour inputs (gap-X from
+//    xmy-1 / xmy+1 on the previous step, gap-Y from xmy+1 / xmy-1, match from
+//    two steps back) are one-lane DPP wave shifts when the band edge moved by
+//    one (the common case), ds_bpermute with a range check otherwise.
+//  * FP32 scaled arithmetic: at the start of every grid cell the row maximum
+//    is folded to [0.5,1) by an exact power-of-two rescale (integer exponent,
+//    exact bookkeeping).  Per traceback window the total is sum(F(tb) . end)
+//    (the reference's cut total is constant across a window in exact
+//    arithmetic); log-likelihoods are formed in FP64.
 //  * Forward values of the current traceback window live in an HBM ring
-//    (3 floats per lane per anti-diagonal) and are read back by the backward
-//    sweep for the posteriors; posteriors >= threshold are compacted per row
-//    with a ballot/popcount prefix and written as (weight, x-1, y-1) tuples.
+//    (float4 per lane per anti-diagonal: 3 states + the cell's exponent) and
+//    are read back by the backward sweep; posteriors >= threshold are
+//    compacted per row with ballot/popcount prefixes and written as
+//    (weight, x-1, y-1) tuples in the reference's per-window emission order.
 //  * Persistent launch: one workgroup per CU, every row pulls jobs from a
 //    per-class atomic queue ordered by band area (largest first).
 #include <hip/hip_runtime.h>
@@ -53,9 +58,11 @@ namespace {
 // RL[5][51][51] = exp(2.3025 * log10 P(observed | underlying)) by x base.
 constexpr int kOffE = 0, kOffGX = 25, kOffGY = 30, kOffRL = 40;
 constexpr int kTabStride = 13056;  // >= 40 + 5*51*51, multiple of 64
-constexpr int kRingStride = 192;   // floats per ring entry: 3 states x 64 lanes
+constexpr int kRingStride = 256;   // floats per ring diagonal per wave: float4 x 64 lanes
 constexpr int kNumClasses = 3;     // SEG 16, 32, 64
+constexpr int kSymPad = 128;       // N symbols around the symbol pool
 constexpr int kBlock = 512;
+constexpr int kK = 8;              // anti-diagonals per macro step (grid cell)
 
 struct DevJob {
     int64_t symX, symY, bandOff, outOff;
@@ -71,20 +78,18 @@ struct DevWin {
 struct KParams {
     const DevJob *jobs;
     const uint16_t *sym;
-    const int2 *band;
+    const int *band;  // packed (xmyL << 8) | (width - 1), cells of kK padded
     const DevWin *win;
     int4 *out;
     int32_t *counts;
     double *logProb;
     float *ring;
-    int32_t *ringS;
     const float *tables;
     int32_t *queue;  // kNumClasses counters
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;
     int32_t withIndels;
     int32_t rle;
-    int32_t rescaleMask;
     float threshold;
     float T[9];          // probability transitions, [from*3 + to], M=0 X=1 Y=2
     float startV[2][3];  // [ragged][state]
@@ -95,255 +100,355 @@ __device__ __forceinline__ float bperm(float v, int srcLane) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(srcLane << 2, __float_as_int(v)));
 }
 
-template <int SEG>
-__device__ __forceinline__ float row_max(float v) {
-#pragma unroll
-    for (int o = SEG / 2; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-template <int SEG>
-__device__ __forceinline__ float row_sum(float v) {
-#pragma unroll
-    for (int o = SEG / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 enum { PH_FETCH = 0, PH_FWD = 1, PH_BWD = 2, PH_DONE = 3 };
 
-template <int SEG>
-__device__ void run_class(const KParams &kp, const float *lds, int cls) {
-    const int lane = threadIdx.x & 63;
-    const int k = lane & (SEG - 1);
-    const int rowBase = lane & ~(SEG - 1);
-    const int rowInWave = lane / SEG;
-    const size_t waveSlot = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    float *ringW = kp.ring + waveSlot * (size_t)kp.ringLen * kRingStride;
-    int32_t *ringSW = kp.ringS + waveSlot * (size_t)kp.ringLen * 4;
-    const uint64_t rowMask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << rowBase;
-    const uint64_t below = (1ull << lane) - 1ull;
-    const int R = kp.ringLen;
+__device__ __forceinline__ float shr1(float v) {  // lane i <- lane i-1 (0 into lane 0)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float shl1(float v) {  // lane i <- lane i+1 (0 into lane 63)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
 
-    // Row state (identical in every lane of a row).
+__device__ __forceinline__ float row_max(float v, int seg) {
+    for (int o = seg >> 1; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float row_sum(float v, int seg) {
+    for (int o = seg >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Band entries are packed as (xmyL << 8) | (width - 1).
+__device__ __forceinline__ int band_L(int b) { return b >> 8; }
+__device__ __forceinline__ int band_R(int b) { return (b >> 8) + 2 * (b & 255); }
+
+// Neighbour input of a cell from the exchange register v of a previous step:
+// the source lane is k + s.  Fast path (|s| <= 1): one DPP shift; the lanes
+// that would read across a row boundary read 0.  General path: ds_bpermute
+// with an explicit band-range check.
+struct Xchg {
+    int k, seg, rowBase;
+    __device__ __forceinline__ float fast(float v, int s) const {
+        const float l = shr1(v), r = shl1(v);
+        const float a = (k == 0) ? 0.f : l;
+        const float b = (k == seg - 1) ? 0.f : r;
+        return s == 0 ? v : (s < 0 ? a : b);
+    }
+    __device__ __forceinline__ float slow(float v, int srcXmy, int Lp, int Rp, bool act) const {
+        const bool ok = act && srcXmy >= Lp && srcXmy <= Rp;
+        const float r = bperm(v, rowBase + (ok ? ((srcXmy - Lp) >> 1) : k));
+        return ok ? r : 0.f;
+    }
+};
+
+// One job per row of seg lanes (16, 32 or 64); the anti-diagonals are walked
+// in grid-aligned macro steps of kK (8).  A forward macro step visits the 8
+// diagonals of one grid cell in ascending order, a backward one in descending
+// order; both bodies are unrolled so every per-step operand (band, symbols,
+// ring entries) is a statically indexed register loaded one macro step ahead
+// (the band two).
+__device__ void run_class(const KParams &kp, const float *lds, int cls, int seg) {
+    constexpr int K = kK;
+    const int lane = threadIdx.x & 63;
+    const int k = lane & (seg - 1);
+    const int rowBase = lane & ~(seg - 1);
+    const int rowInWave = lane / seg;
+    const size_t waveSlot = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int R = kp.ringLen;  // multiple of K
+    float4 *ringRow = reinterpret_cast<float4 *>(kp.ring) + waveSlot * (size_t)R * 64 + (size_t)rowInWave * R * seg + k;
+    const uint64_t rowMask = (seg == 64 ? ~0ull : ((1ull << seg) - 1ull)) << rowBase;
+    const uint64_t belowInRow = rowMask & ((1ull << lane) - 1ull);
+    const Xchg xc{k, seg, rowBase};
+
     int phase = PH_FETCH;
-    int job = 0, d = 0, dir = 1, rpos = 0, win = 0, nWin = 0;
-    int tb = 0, lo = 0, from = 0, D = 0, lXj = 0, lYj = 0;
+    int job = 0, d = 0, gs = 0, win = 0, nWin = 0;
+    int tb = 0, lo = 0, from = 0, D = 0, lastCell = 0;
     int64_t symX = 0, symY = 0, bandOff = 0, outOff = 0;
     int capM = 0, capX = 0, capY = 0, flags = 0, winOff = 0;
     int cntM = 0, cntX = 0, cntY = 0;
     int S = 0, SFtb = 0;
     float totalInv = 0.f;
-    bool init = false;
-    // Neighbour bands (previous step, two steps back) and exchanged values.
+    bool init = false, reload = true;
+    float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
     int Lm1 = 1, Rm1 = -1, Lm2 = 1, Rm2 = -1;
     float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
-    // Forward state parked while the backward sweep of a window runs.
-    int fLm1 = 1, fRm1 = -1, fLm2 = 1, fRm2 = -1, fS = 0, fRpos = 0;
+    int fLm1 = 1, fRm1 = -1, fLm2 = 1, fRm2 = -1, fS = 0, fTb = 0;
     float fX1 = 0.f, fY1 = 0.f, fM1 = 0.f, fM2 = 0.f;
-    // Row coefficient matrix: forward T^t, backward T.
-    float c00, c01, c02, c10, c11, c12, c20, c21, c22;
-    c00 = c01 = c02 = c10 = c11 = c12 = c20 = c21 = c22 = 0.f;
+    const float T00 = kp.T[0], T01 = kp.T[1], T02 = kp.T[2], T10 = kp.T[3], T11 = kp.T[4], T12 = kp.T[5],
+                T20 = kp.T[6], T21 = kp.T[7], T22 = kp.T[8];
+    int bC[K], bN[K];
+    unsigned sxC[K], syC[K];
+    float4 rC[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        bC[j] = bN[j] = 0;
+        sxC[j] = syC[j] = 4u;
+        rC[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 
     for (;;) {
         if (phase == PH_FETCH) {
-            int j = 0;
-            if (k == 0) j = atomicAdd(&kp.queue[cls], 1);
-            j = __shfl(j, rowBase, 64);
-            job = kp.jobBegin[cls] + j;
+            int jn = 0;
+            if (k == 0) jn = atomicAdd(&kp.queue[cls], 1);
+            jn = __shfl(jn, rowBase, 64);
+            job = kp.jobBegin[cls] + jn;
             if (job >= kp.jobEnd[cls]) {
                 phase = PH_DONE;
             } else {
                 const DevJob jb = kp.jobs[job];
                 symX = jb.symX; symY = jb.symY; bandOff = jb.bandOff; outOff = jb.outOff;
-                lXj = jb.lX; lYj = jb.lY; D = lXj + lYj;
+                D = jb.lX + jb.lY;
+                lastCell = D / K;
                 winOff = jb.winOff; nWin = jb.nWin;
                 capM = jb.cap[0]; capX = jb.cap[1]; capY = jb.cap[2];
                 flags = jb.flags;
                 const DevWin w0 = kp.win[winOff];
                 win = 0; tb = w0.tb; lo = w0.lo; from = w0.from;
                 cntM = cntX = cntY = 0;
-                d = 0; dir = 1; rpos = 0; S = 0; init = true;
+                d = 0; gs = 0; S = 0; init = true; reload = true;
                 Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
                 oX1 = oY1 = oM1 = oM2 = 0.f;
-                // forward: q = T^t p  (q_t = sum_s T[s][t] p_s)
-                c00 = kp.T[0]; c01 = kp.T[3]; c02 = kp.T[6];
-                c10 = kp.T[1]; c11 = kp.T[4]; c12 = kp.T[7];
-                c20 = kp.T[2]; c21 = kp.T[5]; c22 = kp.T[8];
+                const bool rg = (flags >> 1) & 1;
+                iv0 = rg ? kp.startV[1][0] : kp.startV[0][0];
+                iv1 = rg ? kp.startV[1][1] : kp.startV[0][1];
+                iv2 = rg ? kp.startV[1][2] : kp.startV[0][2];
                 phase = PH_FWD;
             }
         }
         if (__all(phase == PH_DONE)) break;
         if (phase != PH_FWD && phase != PH_BWD) continue;
-
         const bool fwd = phase == PH_FWD;
-        const int2 bd = kp.band[bandOff + d];
-        const int L0 = bd.x, R0 = bd.y;
-        const int xmy = L0 + 2 * k;
-        const bool act = xmy <= R0;
-        const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
-        const bool strand = flags & 1;
+        const int step = fwd ? K : -K;
+        const int cell = gs / K;
+        const int *bandJ = kp.band + bandOff;
 
-        // Symbols and emissions of this cell (x = 0 or y = 0 read as N; those
-        // emissions are never paired with an in-band neighbour).
-        unsigned sx = 4u, sy = 4u;
-        if (act && x > 0) sx = kp.sym[symX + x - 1];
-        if (act && y > 0) sy = kp.sym[symY + y - 1];
-        const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
-        const float *tab = lds + (strand ? kTabStride : 0);
-        const float eX = tab[kOffGX + xb];
-        const float eY = tab[kOffGY + yb];
-        float eM = tab[kOffE + xb * 5 + yb];
-        if (kp.rle) {
-            const unsigned rx = min(sx >> 8, 50u), ry = min(sy >> 8, 50u);
-            eM *= tab[kOffRL + xb * 2601 + rx * 51 + ry];
-        }
-
-        // Neighbour inputs.
-        const int xX = xmy - dir, xY = xmy + dir;
-        const bool vX = act && xX >= Lm1 && xX <= Rm1;
-        const bool vY = act && xY >= Lm1 && xY <= Rm1;
-        const bool vM = act && xmy >= Lm2 && xmy <= Rm2;
-        const int sX_ = rowBase + (vX ? ((xX - Lm1) >> 1) : k);
-        const int sY_ = rowBase + (vY ? ((xY - Lm1) >> 1) : k);
-        const int sM_ = rowBase + (vM ? ((xmy - Lm2) >> 1) : k);
-        float inX = bperm(oX1, sX_);
-        float inY = bperm(oY1, sY_);
-        float inM = bperm(oM2, sM_);
-        inX = vX ? inX : 0.f;
-        inY = vY ? inY : 0.f;
-        inM = vM ? inM : 0.f;
-
-        float p0, p1, p2;
-        if (fwd) {
-            p0 = eM * inM; p1 = eX * inX; p2 = eY * inY;
-            if (init) {
-                const int rg = (flags >> 1) & 1;
-                p0 = kp.startV[rg][0]; p1 = kp.startV[rg][1]; p2 = kp.startV[rg][2];
+        if (reload) {
+            const int cN = min(max(cell + (fwd ? 1 : -1), 0), lastCell);
+            const int4 *b4 = reinterpret_cast<const int4 *>(bandJ + (size_t)cell * K);
+            const int4 *n4 = reinterpret_cast<const int4 *>(bandJ + (size_t)cN * K);
+            const int4 c0 = b4[0], c1 = b4[1], n0 = n4[0], n1 = n4[1];
+            bC[0] = c0.x; bC[1] = c0.y; bC[2] = c0.z; bC[3] = c0.w; bC[4] = c1.x; bC[5] = c1.y; bC[6] = c1.z; bC[7] = c1.w;
+            bN[0] = n0.x; bN[1] = n0.y; bN[2] = n0.z; bN[3] = n0.w; bN[4] = n1.x; bN[5] = n1.y; bN[6] = n1.z; bN[7] = n1.w;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int dj = gs + j, xmy = band_L(bC[j]) + 2 * k;
+                sxC[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
+                syC[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
             }
-        } else {
-            p0 = inM; p1 = inX; p2 = inY;
+            if (!fwd) {
+                const float4 *rb = ringRow + (size_t)((gs % R) * seg);
+#pragma unroll
+                for (int j = 0; j < K; j++) rC[j] = rb[(size_t)j * seg];
+            }
+            reload = false;
         }
-        float q0 = c00 * p0 + c01 * p1 + c02 * p2;
-        float q1 = c10 * p0 + c11 * p1 + c12 * p2;
-        float q2 = c20 * p0 + c21 * p1 + c22 * p2;
+
+        // Prefetch: band of the cell two macro steps ahead, symbols and (backward)
+        // ring entries of the next cell.
+        int bN2[K];
+        unsigned sxN[K], syN[K];
+        float4 rN[K];
+        {
+            const int c2 = min(max(cell + 2 * (fwd ? 1 : -1), 0), lastCell);
+            const int4 *n4 = reinterpret_cast<const int4 *>(bandJ + (size_t)c2 * K);
+            const int4 n0 = n4[0], n1 = n4[1];
+            bN2[0] = n0.x; bN2[1] = n0.y; bN2[2] = n0.z; bN2[3] = n0.w;
+            bN2[4] = n1.x; bN2[5] = n1.y; bN2[6] = n1.z; bN2[7] = n1.w;
+            const int c1 = min(max(cell + (fwd ? 1 : -1), 0), lastCell);
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int dj = c1 * K + j, xmy = band_L(bN[j]) + 2 * k;
+                sxN[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
+                syN[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
+            }
+            if (!fwd) {
+                const float4 *rb = ringRow + (size_t)(((c1 * K) % R) * seg);
+#pragma unroll
+                for (int j = 0; j < K; j++) rN[j] = rb[(size_t)j * seg];
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) rN[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+
+        const float *tab = lds + ((flags & 1) ? kTabStride : 0);
         const bool atEnd = tb == D;
-        if (!fwd && init) {
-            const int rg = (atEnd && ((flags >> 2) & 1)) ? 1 : 0;
-            q0 = kp.endV[rg][0]; q1 = kp.endV[rg][1]; q2 = kp.endV[rg][2];
-        }
-        float v0 = fwd ? p0 : q0, v1 = fwd ? p1 : q1, v2 = fwd ? p2 : q2;
-        float o0 = fwd ? q0 : eM * q0, o1 = fwd ? q1 : eX * q1, o2 = fwd ? q2 : eY * q2;
-        if (!act) v0 = v1 = v2 = o0 = o1 = o2 = 0.f;
-
-        // Exact power-of-two rescale every K anti-diagonals.
-        if ((d & kp.rescaleMask) == 0) {
-            float m = row_max<SEG>(fmaxf(v0, fmaxf(v1, v2)));
-            int e2 = 0;
-            if (m > 0.f) e2 = __builtin_amdgcn_frexp_expf(m);
-            v0 = ldexpf(v0, -e2); v1 = ldexpf(v1, -e2); v2 = ldexpf(v2, -e2);
-            o0 = ldexpf(o0, -e2); o1 = ldexpf(o1, -e2); o2 = ldexpf(o2, -e2);
-            oM1 = ldexpf(oM1, -e2);
-            S += e2;
-        }
-
-        float *ring = ringW + (size_t)rpos * kRingStride + lane;
-        if (fwd) {
-            ring[0] = v0; ring[64] = v1; ring[128] = v2;
-            if (k == 0) ringSW[rpos * 4 + rowInWave] = S;
-        } else if (d <= from && d > lo) {
-            // Posteriors of this anti-diagonal.
-            const float f0 = ring[0], f1 = ring[64], f2 = ring[128];
-            const int SFd = ringSW[rpos * 4 + rowInWave];
-            const float sc = ldexpf(totalInv, SFd + S - SFtb);
-            const float pM = f0 * v0 * sc, pX = f1 * v1 * sc, pY = f2 * v2 * sc;
-            const float thr = kp.threshold;
-            const bool okM = act && x > 0 && y > 0 && pM >= thr;
-            const bool okX = kp.withIndels && act && x > 0 && pX >= thr;
-            const bool okY = kp.withIndels && act && y > 0 && pY >= thr;
-            const uint64_t bM = __ballot(okM) & rowMask;
-            const uint64_t bX = __ballot(okX) & rowMask;
-            const uint64_t bY = __ballot(okY) & rowMask;
-            if (okM) {
-                const int idx = cntM + __popcll(bM & below);
-                if (idx < capM) {
-                    const long long w = (long long)floorf(fminf(pM, 1.f) * 1e7f);
-                    kp.out[outOff + idx] = make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
-                }
-            }
-            if (okX) {
-                const int idx = cntX + __popcll(bX & below);
-                if (idx < capX) {
-                    const long long w = (long long)floorf(fminf(pX, 1.f) * 1e7f);
-                    kp.out[outOff + capM + idx] = make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
-                }
-            }
-            if (okY) {
-                const int idx = cntY + __popcll(bY & below);
-                if (idx < capY) {
-                    const long long w = (long long)floorf(fminf(pY, 1.f) * 1e7f);
-                    kp.out[outOff + capM + capX + idx] =
-                        make_int4((int)(w & 0xffffffff), (int)(w >> 32), x - 1, y - 1);
-                }
-            }
-            cntM += __popcll(bM);
-            cntX += __popcll(bX);
-            cntY += __popcll(bY);
-        }
-
-        // Shift the exchange registers and bands.
-        oM2 = oM1; oX1 = o1; oY1 = o2; oM1 = o0;
-        Lm2 = Lm1; Rm2 = Rm1; Lm1 = L0; Rm1 = R0;
-        init = false;
+        const bool ragEnd = atEnd && ((flags >> 2) & 1);
+        bool switchBwd = false, winDone = false;
+        float sv0 = 0.f, sv1 = 0.f, sv2 = 0.f;
+        float4 *ringCell = ringRow + (size_t)((gs % R) * seg);
 
         if (fwd) {
-            if (d == tb) {
-                // Window total sum(F(tb) . end) and the switch to the backward sweep.
-                const int rg = (atEnd && ((flags >> 2) & 1)) ? 1 : 0;
-                const float t = row_sum<SEG>(v0 * kp.endV[rg][0] + v1 * kp.endV[rg][1] + v2 * kp.endV[rg][2]);
-                SFtb = S;
-                totalInv = t > 0.f ? 1.f / t : 0.f;
-                if (atEnd && k == 0) kp.logProb[job] = log((double)t) + (double)SFtb * 0.69314718055994530942;
-                fX1 = oX1; fY1 = oY1; fM1 = oM1; fM2 = oM2;
-                fLm1 = Lm1; fRm1 = Rm1; fLm2 = Lm2; fRm2 = Rm2; fS = S; fRpos = rpos;
-                oX1 = oY1 = oM1 = oM2 = 0.f;
-                Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
-                S = 0; dir = -1; init = true;
-                // backward: q = T p
-                c00 = kp.T[0]; c01 = kp.T[1]; c02 = kp.T[2];
-                c10 = kp.T[3]; c11 = kp.T[4]; c12 = kp.T[5];
-                c20 = kp.T[6]; c21 = kp.T[7]; c22 = kp.T[8];
-                phase = PH_BWD;
-            } else {
-                d += 1;
-                rpos = rpos + 1 == R ? 0 : rpos + 1;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int dj = gs + j;
+                if (dj >= d && dj <= tb) {
+                    const int L0 = band_L(bC[j]), R0 = band_R(bC[j]);
+                    const int xmy = L0 + 2 * k;
+                    const bool act = xmy <= R0;
+                    const unsigned sx = sxC[j], sy = syC[j];
+                    const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
+                    const float eX = tab[kOffGX + xb], eY = tab[kOffGY + yb];
+                    float eM = tab[kOffE + xb * 5 + yb];
+                    if (kp.rle) eM *= tab[kOffRL + xb * 2601 + min(sx >> 8, 50u) * 51 + min(sy >> 8, 50u)];
+                    const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
+                    float inX, inY, inM;
+                    if ((dL1 == 1 || dL1 == -1) && dL2 >= -2 && dL2 <= 2) {
+                        inX = xc.fast(oX1, (dL1 - 1) >> 1);
+                        inY = xc.fast(oY1, (dL1 + 1) >> 1);
+                        inM = xc.fast(oM2, dL2 >> 1);
+                    } else {
+                        inX = xc.slow(oX1, xmy - 1, Lm1, Rm1, act);
+                        inY = xc.slow(oY1, xmy + 1, Lm1, Rm1, act);
+                        inM = xc.slow(oM2, xmy, Lm2, Rm2, act);
+                    }
+                    float p0 = eM * inM, p1 = eX * inX, p2 = eY * inY;
+                    if (j == 0 && init) { p0 = iv0; p1 = iv1; p2 = iv2; }
+                    if (!act) p0 = p1 = p2 = 0.f;
+                    if (j == 0 && dj == d) {
+                        // exact power-of-two rescale at the start of each grid cell
+                        const float m = row_max(fmaxf(p0, fmaxf(p1, p2)), seg);
+                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                        p0 = ldexpf(p0, -e2); p1 = ldexpf(p1, -e2); p2 = ldexpf(p2, -e2);
+                        oM1 = ldexpf(oM1, -e2);
+                        S += e2;
+                    }
+                    // send q = T^t p
+                    const float q0 = T00 * p0 + T10 * p1 + T20 * p2;
+                    const float q1 = T01 * p0 + T11 * p1 + T21 * p2;
+                    const float q2 = T02 * p0 + T12 * p1 + T22 * p2;
+                    ringCell[(size_t)j * seg] = make_float4(p0, p1, p2, __int_as_float(S));
+                    if (dj == tb) {
+                        sv0 = p0; sv1 = p1; sv2 = p2;
+                        switchBwd = true;
+                    }
+                    oM2 = oM1; oX1 = q1; oY1 = q2; oM1 = q0;
+                    Lm2 = Lm1; Rm2 = Rm1; Lm1 = L0; Rm1 = R0;
+                }
             }
         } else {
-            if (d <= lo + 1) {
-                if (win + 1 == nWin) {
-                    if (k == 0) {
-                        kp.counts[3 * job + 0] = cntM;
-                        kp.counts[3 * job + 1] = cntX;
-                        kp.counts[3 * job + 2] = cntY;
+#pragma unroll
+            for (int j = K - 1; j >= 0; j--) {
+                const int dj = gs + j;
+                if (dj <= d && dj > lo) {
+                    const int L0 = band_L(bC[j]), R0 = band_R(bC[j]);
+                    const int xmy = L0 + 2 * k;
+                    const bool act = xmy <= R0;
+                    const int x = (dj + xmy) >> 1, y = (dj - xmy) >> 1;
+                    const unsigned sx = sxC[j], sy = syC[j];
+                    const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
+                    const float eX = tab[kOffGX + xb], eY = tab[kOffGY + yb];
+                    float eM = tab[kOffE + xb * 5 + yb];
+                    if (kp.rle) eM *= tab[kOffRL + xb * 2601 + min(sx >> 8, 50u) * 51 + min(sy >> 8, 50u)];
+                    // successors: X at (d+1, xmy+1), Y at (d+1, xmy-1), M at (d+2, xmy)
+                    const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
+                    float inX, inY, inM;
+                    if ((dL1 == 1 || dL1 == -1) && dL2 >= -2 && dL2 <= 2) {
+                        inX = xc.fast(oX1, (dL1 + 1) >> 1);
+                        inY = xc.fast(oY1, (dL1 - 1) >> 1);
+                        inM = xc.fast(oM2, dL2 >> 1);
+                    } else {
+                        inX = xc.slow(oX1, xmy + 1, Lm1, Rm1, act);
+                        inY = xc.slow(oY1, xmy - 1, Lm1, Rm1, act);
+                        inM = xc.slow(oM2, xmy, Lm2, Rm2, act);
                     }
-                    phase = PH_FETCH;
-                } else {
-                    win += 1;
-                    const DevWin w1 = kp.win[winOff + win];
-                    d = tb + 1;
-                    tb = w1.tb; lo = w1.lo; from = w1.from;
-                    oX1 = fX1; oY1 = fY1; oM1 = fM1; oM2 = fM2;
-                    Lm1 = fLm1; Rm1 = fRm1; Lm2 = fLm2; Rm2 = fRm2; S = fS;
-                    rpos = fRpos + 1 == R ? 0 : fRpos + 1;
-                    dir = 1;
-                    c00 = kp.T[0]; c01 = kp.T[3]; c02 = kp.T[6];
-                    c10 = kp.T[1]; c11 = kp.T[4]; c12 = kp.T[7];
-                    c20 = kp.T[2]; c21 = kp.T[5]; c22 = kp.T[8];
-                    phase = PH_FWD;
+                    // B = T in  (B_s = sum_t T[s][t] in_t)
+                    float q0 = T00 * inM + T01 * inX + T02 * inY;
+                    float q1 = T10 * inM + T11 * inX + T12 * inY;
+                    float q2 = T20 * inM + T21 * inX + T22 * inY;
+                    if (init && dj == d) { q0 = iv0; q1 = iv1; q2 = iv2; }
+                    if (!act) q0 = q1 = q2 = 0.f;
+                    if (j == K - 1 && dj == d && !init) {
+                        const float m = row_max(fmaxf(q0, fmaxf(q1, q2)), seg);
+                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                        q0 = ldexpf(q0, -e2); q1 = ldexpf(q1, -e2); q2 = ldexpf(q2, -e2);
+                        oM1 = ldexpf(oM1, -e2);
+                        S += e2;
+                    }
+                    if (dj <= from) {
+                        const float4 f = rC[j];
+                        const float sc = ldexpf(totalInv, __float_as_int(f.w) + S - SFtb);
+                        const float pM = f.x * q0 * sc, pX = f.y * q1 * sc, pY = f.z * q2 * sc;
+                        const float thr = kp.threshold;
+                        const bool okM = act && x > 0 && y > 0 && pM >= thr;
+                        const bool okX = kp.withIndels && act && x > 0 && pX >= thr;
+                        const bool okY = kp.withIndels && act && y > 0 && pY >= thr;
+                        const uint64_t bM = __ballot(okM), bX = __ballot(okX), bY = __ballot(okY);
+                        if (okM) {
+                            const int idx = cntM + __popcll(bM & belowInRow);
+                            if (idx < capM)
+                                kp.out[outOff + idx] = make_int4((int)floorf(fminf(pM, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        if (okX) {
+                            const int idx = cntX + __popcll(bX & belowInRow);
+                            if (idx < capX)
+                                kp.out[outOff + capM + idx] =
+                                    make_int4((int)floorf(fminf(pX, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        if (okY) {
+                            const int idx = cntY + __popcll(bY & belowInRow);
+                            if (idx < capY)
+                                kp.out[outOff + capM + capX + idx] =
+                                    make_int4((int)floorf(fminf(pY, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        cntM += __popcll(bM & rowMask);
+                        cntX += __popcll(bX & rowMask);
+                        cntY += __popcll(bY & rowMask);
+                    }
+                    if (dj == lo + 1) winDone = true;
+                    // send u = e (.) B
+                    oM2 = oM1; oX1 = eX * q1; oY1 = eY * q2; oM1 = eM * q0;
+                    Lm2 = Lm1; Rm2 = Rm1; Lm1 = L0; Rm1 = R0;
                 }
-            } else {
-                d -= 1;
-                rpos = rpos == 0 ? R - 1 : rpos - 1;
             }
+        }
+        init = false;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            bC[j] = bN[j]; bN[j] = bN2[j];
+            sxC[j] = sxN[j]; syC[j] = syN[j];
+            rC[j] = rN[j];
+        }
+
+        if (switchBwd) {
+            // Window total sum(F(tb) . end); the backward sweep starts at tb.
+            const float e0 = ragEnd ? kp.endV[1][0] : kp.endV[0][0];
+            const float e1 = ragEnd ? kp.endV[1][1] : kp.endV[0][1];
+            const float e2 = ragEnd ? kp.endV[1][2] : kp.endV[0][2];
+            const float t = row_sum(sv0 * e0 + sv1 * e1 + sv2 * e2, seg);
+            SFtb = S;
+            totalInv = t > 0.f ? 1.f / t : 0.f;
+            if (atEnd && k == 0) kp.logProb[job] = log((double)t) + (double)SFtb * 0.69314718055994530942;
+            iv0 = e0; iv1 = e1; iv2 = e2;
+            fX1 = oX1; fY1 = oY1; fM1 = oM1; fM2 = oM2;
+            fLm1 = Lm1; fRm1 = Rm1; fLm2 = Lm2; fRm2 = Rm2; fS = S; fTb = tb;
+            oX1 = oY1 = oM1 = oM2 = 0.f;
+            Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
+            S = 0; init = true; reload = true;
+            d = tb; gs = tb & ~(K - 1);
+            phase = PH_BWD;
+        } else if (winDone) {
+            if (win + 1 == nWin) {
+                if (k == 0) {
+                    kp.counts[3 * job + 0] = cntM;
+                    kp.counts[3 * job + 1] = cntX;
+                    kp.counts[3 * job + 2] = cntY;
+                }
+                phase = PH_FETCH;
+            } else {
+                win += 1;
+                const DevWin w1 = kp.win[winOff + win];
+                d = fTb + 1; gs = d & ~(K - 1);
+                tb = w1.tb; lo = w1.lo; from = w1.from;
+                oX1 = fX1; oY1 = fY1; oM1 = fM1; oM2 = fM2;
+                Lm1 = fLm1; Rm1 = fRm1; Lm2 = fLm2; Rm2 = fRm2; S = fS;
+                reload = true;
+                phase = PH_FWD;
+            }
+        } else {
+            gs += step;
+            d = fwd ? gs : gs + K - 1;
         }
     }
 }
@@ -352,9 +457,7 @@ __global__ __launch_bounds__(kBlock) void pairhmm_kernel(KParams kp) {
     extern __shared__ float lds[];
     for (int i = threadIdx.x; i < 2 * kTabStride; i += blockDim.x) lds[i] = kp.tables[i];
     __syncthreads();
-    run_class<64>(kp, lds, 2);
-    run_class<32>(kp, lds, 1);
-    run_class<16>(kp, lds, 0);
+    for (int cls = kNumClasses - 1; cls >= 0; cls--) run_class(kp, lds, cls, 16 << cls);
 }
 
 __global__ void fill_i32(int32_t *p, int32_t v, int n) {
@@ -444,7 +547,7 @@ struct mp_batch {
     KParams kp;
     DevBuf<DevJob> dJobs;
     DevBuf<uint16_t> dSym;
-    DevBuf<int2> dBand;
+    DevBuf<int32_t> dBand;
     DevBuf<DevWin> dWin;
     DevBuf<int4> dOut;
     DevBuf<int32_t> dCounts;
@@ -595,7 +698,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->pairFirstJob.assign(npairs, 0);
 
     // ---- 1. symbols, cropping, split points -> jobs (pair order) ----
-    std::vector<uint16_t> sym;
+    // Symbol pool with N padding at both ends (the kernel reads up to one
+    // row width beyond a sequence without clamping).
+    std::vector<uint16_t> sym(kSymPad, 4);
     std::vector<HostJob> jobs;
     std::vector<std::vector<int64_t>> jobAnchors;
     std::vector<int64_t> sp, anc;
@@ -659,11 +764,14 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         b->pairNJobs[i] = (int32_t)ns;
     }
 
+    sym.resize(sym.size() + kSymPad, 4);
+
     // ---- 2. bands, windows, classes (parallel over jobs) ----
     const int64_t nj = (int64_t)jobs.size();
     std::vector<int64_t> bandOff(nj + 1, 0);
-    for (int64_t j = 0; j < nj; j++) bandOff[j + 1] = bandOff[j] + (jobs[j].lX + jobs[j].lY + 1);
-    std::vector<int2> band((size_t)bandOff[nj]);
+    // Each job's band padded to whole cells of kK anti-diagonals (16-byte aligned).
+    for (int64_t j = 0; j < nj; j++) bandOff[j + 1] = bandOff[j] + ((jobs[j].lX + jobs[j].lY) / kK + 1) * kK;
+    std::vector<int32_t> band((size_t)bandOff[nj]);
     std::vector<std::vector<mp::Window>> wins(nj);
     std::atomic<int> err{0};
     {
@@ -692,16 +800,18 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                         int32_t w = (R[d] - L[d]) / 2 + 1;
                         wmax = std::max(wmax, w);
                         cells += w;
-                        band[bandOff[j] + d] = make_int2(L[d], R[d]);
+                        band[bandOff[j] + d] = (int32_t)((uint32_t)L[d] << 8) | (std::min(w, 256) - 1);
                     }
+                    for (int64_t d = D + 1; d < bandOff[j + 1] - bandOff[j]; d++) band[bandOff[j] + d] = band[bandOff[j] + D];
                     hj.wmax = wmax;
                     hj.cells = D > 0 ? cells : 0;
                     hj.cls = wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3));
                     hj.bandOff = bandOff[j];
                     if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, wins[j]);
+                    // live forward entries: (lo, tb] of a window; whole cells plus margin
                     int32_t need = 1;
-                    for (auto &w : wins[j]) need = std::max(need, w.tb - w.lo + 2);
-                    hj.ringNeed = need;
+                    for (auto &w : wins[j]) need = std::max(need, w.tb - w.lo + 2 * kK);
+                    hj.ringNeed = (need + kK - 1) / kK * kK;
                 }
             });
         for (auto &t : th) t.join();
@@ -732,7 +842,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     std::vector<DevJob> dj(ord.size());
     std::vector<DevWin> dw;
     int64_t outOff = 0;
-    int32_t ringLen = 2;
+    int32_t ringLen = 4 * kK;
     for (size_t q = 0; q < ord.size(); q++) {
         HostJob &hj = jobs[ord[q]];
         devIndex[ord[q]] = (int32_t)q;
@@ -790,8 +900,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // Ring: one slice per resident wavefront.
     b->gridRows = e->cus;
     const size_t waves = (size_t)e->cus * (kBlock / 64);
-    if ((st = b->dRing.alloc(waves * (size_t)ringLen * kRingStride)) ||
-        (st = b->dRingS.alloc(waves * (size_t)ringLen * 4))) {
+    if ((st = b->dRing.alloc(waves * (size_t)ringLen * kRingStride))) {
         delete b;
         return st;
     }
@@ -803,12 +912,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.counts = b->dCounts.p;
     kp.logProb = b->dLogProb.p;
     kp.ring = b->dRing.p;
-    kp.ringS = b->dRingS.p;
     kp.tables = b->dTables.p;
     kp.queue = b->dQueue.p;
     kp.ringLen = ringLen;
     kp.withIndels = mode == MP_MODE_POSTERIORS;
-    kp.rescaleMask = 3;
     kp.threshold = (float)p->threshold;
     for (int c = 0; c < kNumClasses; c++) {
         kp.jobBegin[c] = b->classBegin[c];
