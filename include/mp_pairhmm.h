@@ -44,7 +44,10 @@ enum { MP_LIST_MATCH = 0, MP_LIST_GAPX = 1, MP_LIST_GAPY = 2 };
 /* What a batch computes per pair. */
 enum {
     MP_MODE_POSTERIORS = 0,   /* match + gapX + gapY posteriors (getAlignedPairsWithIndelsUsingAnchors) */
-    MP_MODE_MATCH_ONLY = 1    /* match posteriors only (getAlignedPairsUsingAnchors)                     */
+    MP_MODE_MATCH_ONLY = 1,   /* match posteriors only (getAlignedPairsUsingAnchors)                     */
+    MP_MODE_FORWARD_ONLY = 2, /* forward log-likelihood only (computeForwardProbability): static band of
+                                 diagonalExpansion, no large-gap split, total at the last anti-diagonal   */
+    MP_MODE_EXPECTATIONS = 3  /* expected transition counts + likelihood (getExpectationsUsingAnchors)   */
 };
 
 /* Batch flags. */
@@ -166,7 +169,19 @@ mp_status mp_batch_fetch_counts(mp_batch *b, int64_t *counts, double *logProb);
 /* Copy list `list` of every pair, concatenated in pair order (sizes from
  * mp_batch_fetch_counts).  out must hold sum(counts[:, list]) entries. */
 mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_pair *out);
+/* Expectation mode: per pair the 9 expected transition counts T[from*3+to]
+ * (M=0, X=1, Y=2) summed over the posterior diagonals of every window and
+ * split, and the reference's per-diagonal likelihood sum (pairwiseAligner.c:674). */
+mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, double *likelihood);
 void mp_batch_free(mp_batch *b);
+
+/* computeForwardProbability (pairwiseAligner.c:813-865) for one pair. */
+mp_status mp_pairhmm_forward_log_prob(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,
+                                      double *logProb);
+/* getExpectationsUsingAnchors (pairwiseAligner.c:1151-1156) for one pair:
+ * trans9 receives the expected transition counts, likelihood the hmm->likelihood increment. */
+mp_status mp_pairhmm_expectations(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,
+                                  double *trans9, double *likelihood);
 
 #ifdef __cplusplus
 }

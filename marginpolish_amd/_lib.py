@@ -16,7 +16,7 @@ STATUS = {0: "MP_OK", 1: "MP_EINVAL", 2: "MP_EPARAMS", 3: "MP_EANCHORS", 4: "MP_
           6: "MP_EUNSUPPORTED", 7: "MP_ENODEVICE"}
 
 LIST_MATCH, LIST_GAPX, LIST_GAPY = 0, 1, 2
-MODE_POSTERIORS, MODE_MATCH_ONLY = 0, 1
+MODE_POSTERIORS, MODE_MATCH_ONLY, MODE_FORWARD_ONLY, MODE_EXPECTATIONS = 0, 1, 2, 3
 BATCH_CROP_REFERENCE = 1
 
 
@@ -50,6 +50,7 @@ EXPORTS = [
     "mp_engine_free", "mp_engine_cu_count", "mp_pairhmm_posteriors", "mp_batch_create", "mp_batch_run",
     "mp_batch_run_plain", "mp_batch_kernel_ms", "mp_batch_cells", "mp_batch_npairs", "mp_batch_fetch_counts",
     "mp_batch_fetch_list", "mp_batch_free", "mp_band_construct", "mp_split_points", "mp_traceback_windows",
+    "mp_batch_fetch_expectations", "mp_pairhmm_forward_log_prob", "mp_pairhmm_expectations",
 ]
 
 _lib = None
@@ -92,6 +93,9 @@ def lib():
         "mp_band_construct": (C.c_int, [P(i64), i64, i64, i64, i64, i32, P(i32), P(i32)]),
         "mp_split_points": (i64, [P(i64), i64, i64, i64, i64, i32, i32, P(i64), i64]),
         "mp_traceback_windows": (i64, [P(i32), P(i32), i64, P(BandParams), P(i32), i64]),
+        "mp_batch_fetch_expectations": (C.c_int, [vp, P(dbl), P(dbl)]),
+        "mp_pairhmm_forward_log_prob": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl)]),
+        "mp_pairhmm_expectations": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl), P(dbl)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -89,6 +89,8 @@ struct KParams {
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;
     int32_t withIndels;
+    int32_t mode;        // MP_MODE_*
+    double *expect;      // per job: 9 expected transition counts + likelihood (expectation mode)
     int32_t rle;
     float threshold;
     float T[9];          // probability transitions, [from*3 + to], M=0 X=1 Y=2
@@ -169,11 +171,15 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
     int cntM = 0, cntX = 0, cntY = 0;
     int S = 0, SFtb = 0;
     float totalInv = 0.f;
+    double likeAcc = 0.0;
     bool init = false, reload = true;
     float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
     int Lm1 = 1, Rm1 = -1, Lm2 = 1, Rm2 = -1;
     float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
     int fLm1 = 1, fRm1 = -1, fLm2 = 1, fRm2 = -1, fS = 0, fTb = 0;
+    const bool expMode = kp.mode == MP_MODE_EXPECTATIONS, fwdOnly = kp.mode == MP_MODE_FORWARD_ONLY;
+    const bool postMode = kp.mode == MP_MODE_POSTERIORS || kp.mode == MP_MODE_MATCH_ONLY;
+    const int stopOff = expMode ? 0 : 1;  // backward sweep ends at lo + stopOff
     float fX1 = 0.f, fY1 = 0.f, fM1 = 0.f, fM2 = 0.f;
     const float T00 = kp.T[0], T01 = kp.T[1], T02 = kp.T[2], T10 = kp.T[3], T11 = kp.T[4], T12 = kp.T[5],
                 T20 = kp.T[6], T21 = kp.T[7], T22 = kp.T[8];
@@ -206,6 +212,7 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
                 const DevWin w0 = kp.win[winOff];
                 win = 0; tb = w0.tb; lo = w0.lo; from = w0.from;
                 cntM = cntX = cntY = 0;
+                likeAcc = 0.0;
                 d = 0; gs = 0; S = 0; init = true; reload = true;
                 Lm1 = 1; Rm1 = -1; Lm2 = 1; Rm2 = -1;
                 oX1 = oY1 = oM1 = oM2 = 0.f;
@@ -277,6 +284,7 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
         const bool ragEnd = atEnd && ((flags >> 2) & 1);
         bool switchBwd = false, winDone = false;
         float sv0 = 0.f, sv1 = 0.f, sv2 = 0.f;
+        float e00 = 0.f, e01 = 0.f, e02 = 0.f, e10 = 0.f, e11 = 0.f, e12 = 0.f, e20 = 0.f, e21 = 0.f, e22 = 0.f;
         float4 *ringCell = ringRow + (size_t)((gs % R) * seg);
 
         if (fwd) {
@@ -318,7 +326,7 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
                     const float q0 = T00 * p0 + T10 * p1 + T20 * p2;
                     const float q1 = T01 * p0 + T11 * p1 + T21 * p2;
                     const float q2 = T02 * p0 + T12 * p1 + T22 * p2;
-                    ringCell[(size_t)j * seg] = make_float4(p0, p1, p2, __int_as_float(S));
+                    if (!fwdOnly) ringCell[(size_t)j * seg] = make_float4(p0, p1, p2, __int_as_float(S));
                     if (dj == tb) {
                         sv0 = p0; sv1 = p1; sv2 = p2;
                         switchBwd = true;
@@ -331,11 +339,12 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
 #pragma unroll
             for (int j = K - 1; j >= 0; j--) {
                 const int dj = gs + j;
-                if (dj <= d && dj > lo) {
+                if (dj <= d && dj >= lo + stopOff) {
                     const int L0 = band_L(bC[j]), R0 = band_R(bC[j]);
                     const int xmy = L0 + 2 * k;
                     const bool act = xmy <= R0;
                     const int x = (dj + xmy) >> 1, y = (dj - xmy) >> 1;
+                    const int Sin = S;
                     const unsigned sx = sxC[j], sy = syC[j];
                     const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
                     const float eX = tab[kOffGX + xb], eY = tab[kOffGY + yb];
@@ -366,7 +375,19 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
                         oM1 = ldexpf(oM1, -e2);
                         S += e2;
                     }
-                    if (dj <= from) {
+                    if (expMode && act) {
+                        // expected transition counts, pushed from this cell (the predecessor) into
+                        // successors on posterior diagonals (diagonalCalculationExpectations)
+                        const float4 f = rC[j];
+                        const float sc = ldexpf(totalInv, __float_as_int(f.w) + Sin - SFtb);
+                        const bool okXY = dj + 1 > lo && dj + 1 <= from, okM = dj + 2 > lo && dj + 2 <= from;
+                        const float tX = okXY ? inX : 0.f, tY = okXY ? inY : 0.f, tM = okM ? inM : 0.f;
+                        const float a0 = f.x * sc, a1 = f.y * sc, a2 = f.z * sc;
+                        e00 += a0 * (T00 * tM); e01 += a0 * (T01 * tX); e02 += a0 * (T02 * tY);
+                        e10 += a1 * (T10 * tM); e11 += a1 * (T11 * tX); e12 += a1 * (T12 * tY);
+                        e20 += a2 * (T20 * tM); e21 += a2 * (T21 * tX); e22 += a2 * (T22 * tY);
+                    }
+                    if (postMode && dj <= from) {
                         const float4 f = rC[j];
                         const float sc = ldexpf(totalInv, __float_as_int(f.w) + S - SFtb);
                         const float pM = f.x * q0 * sc, pX = f.y * q1 * sc, pY = f.z * q2 * sc;
@@ -396,11 +417,18 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
                         cntX += __popcll(bX & rowMask);
                         cntY += __popcll(bY & rowMask);
                     }
-                    if (dj == lo + 1) winDone = true;
+                    if (dj == lo + stopOff) winDone = true;
                     // send u = e (.) B
                     oM2 = oM1; oX1 = eX * q1; oY1 = eY * q2; oM1 = eM * q0;
                     Lm2 = Lm1; Rm2 = Rm1; Lm1 = L0; Rm1 = R0;
                 }
+            }
+        }
+        if (expMode && !fwd) {
+            const float ev[9] = {e00, e01, e02, e10, e11, e12, e20, e21, e22};
+            for (int i = 0; i < 9; i++) {
+                const float v = row_sum(ev[i], seg);
+                if (k == 0 && v != 0.f) atomicAdd(&kp.expect[(size_t)job * 10 + i], (double)v);
             }
         }
         init = false;
@@ -419,7 +447,10 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
             const float t = row_sum(sv0 * e0 + sv1 * e1 + sv2 * e2, seg);
             SFtb = S;
             totalInv = t > 0.f ? 1.f / t : 0.f;
-            if (atEnd && k == 0) kp.logProb[job] = log((double)t) + (double)SFtb * 0.69314718055994530942;
+            const double lt = log((double)t) + (double)SFtb * 0.69314718055994530942;
+            if (atEnd && k == 0) kp.logProb[job] = lt;
+            // getExpectations adds the window total once per posterior diagonal (pairwiseAligner.c:674)
+            likeAcc += (double)(from - lo) * lt;
             iv0 = e0; iv1 = e1; iv2 = e2;
             fX1 = oX1; fY1 = oY1; fM1 = oM1; fM2 = oM2;
             fLm1 = Lm1; fRm1 = Rm1; fLm2 = Lm2; fRm2 = Rm2; fS = S; fTb = tb;
@@ -428,12 +459,17 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
             S = 0; init = true; reload = true;
             d = tb; gs = tb & ~(K - 1);
             phase = PH_BWD;
+            if (fwdOnly) {
+                if (k == 0) kp.counts[3 * job + 0] = kp.counts[3 * job + 1] = kp.counts[3 * job + 2] = 0;
+                phase = PH_FETCH;
+            }
         } else if (winDone) {
             if (win + 1 == nWin) {
                 if (k == 0) {
                     kp.counts[3 * job + 0] = cntM;
                     kp.counts[3 * job + 1] = cntX;
                     kp.counts[3 * job + 2] = cntY;
+                    if (expMode) kp.expect[(size_t)job * 10 + 9] = likeAcc;
                 }
                 phase = PH_FETCH;
             } else {
@@ -556,6 +592,7 @@ struct mp_batch {
     DevBuf<int32_t> dRingS;
     DevBuf<float> dTables;
     DevBuf<int32_t> dQueue;
+    DevBuf<double> dExpect;
     int64_t outTotal = 0;
     int32_t gridRows = 0;
     double kernelMs = 0.0;
@@ -676,7 +713,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("mp_batch_create: null or negative argument");
         return MP_EINVAL;
     }
-    if (mode != MP_MODE_POSTERIORS && mode != MP_MODE_MATCH_ONLY) {
+    if (mode < MP_MODE_POSTERIORS || mode > MP_MODE_EXPECTATIONS) {
         mp::set_error("mp_batch_create: unknown mode");
         return MP_EINVAL;
     }
@@ -732,8 +769,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         sym.insert(sym.end(), pr.sX + first, pr.sX + end);
         const int64_t baseY = (int64_t)sym.size();
         sym.insert(sym.end(), pr.sY, pr.sY + lY);
-        mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
-                         pr.raggedRight != 0, sp);
+        if (mode == MP_MODE_FORWARD_ONLY) {  // computeForwardProbability does not split
+            sp.assign({0, 0, lX, lY});
+        } else {
+            mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
+                             pr.raggedRight != 0, sp);
+        }
         const int64_t ns = (int64_t)sp.size() / 4;
         b->pairFirstJob[i] = (int64_t)jobs.size();
         int64_t j = 0;
@@ -789,8 +830,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     L.resize(D + 1);
                     R.resize(D + 1);
                     const std::vector<int64_t> &a = jobAnchors[j];
-                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion,
-                                        p->dynamicAnchorExpansion != 0, L.data(), R.data())) {
+                    const bool dyn = mode != MP_MODE_FORWARD_ONLY && p->dynamicAnchorExpansion != 0;
+                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion, dyn,
+                                        L.data(), R.data())) {
                         err = 1;
                         continue;
                     }
@@ -807,7 +849,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     hj.cells = D > 0 ? cells : 0;
                     hj.cls = wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3));
                     hj.bandOff = bandOff[j];
-                    if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, wins[j]);
+                    if (D > 0 && mode == MP_MODE_FORWARD_ONLY) wins[j].assign(1, mp::Window{(int32_t)D, 0, (int32_t)D});
+                    else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, wins[j]);
                     // live forward entries: (lo, tb] of a window; whole cells plus margin
                     int32_t need = 1;
                     for (auto &w : wins[j]) need = std::max(need, w.tb - w.lo + 2 * kK);
@@ -847,7 +890,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         HostJob &hj = jobs[ord[q]];
         devIndex[ord[q]] = (int32_t)q;
         const int64_t D = hj.lX + hj.lY;
-        hj.cap[0] = (int32_t)std::min<int64_t>(2 * D + 64, INT32_MAX / 4);
+        hj.cap[0] = mode <= MP_MODE_MATCH_ONLY ? (int32_t)std::min<int64_t>(2 * D + 64, INT32_MAX / 4) : 0;
         hj.cap[1] = mode == MP_MODE_POSTERIORS ? (int32_t)std::min<int64_t>(D + 64, INT32_MAX / 4) : 0;
         hj.cap[2] = hj.cap[1];
         hj.outOff = outOff;
@@ -855,7 +898,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         hj.winOff = (int32_t)dw.size();
         hj.nWin = (int32_t)wins[ord[q]].size();
         for (auto &w : wins[ord[q]]) dw.push_back(DevWin{w.tb, w.lo, w.from, 0});
-        ringLen = std::max(ringLen, hj.ringNeed);
+        if (mode != MP_MODE_FORWARD_ONLY) ringLen = std::max(ringLen, hj.ringNeed);
         DevJob &d = dj[q];
         d.symX = hj.symX; d.symY = hj.symY; d.bandOff = hj.bandOff; d.outOff = hj.outOff;
         d.lX = hj.lX; d.lY = hj.lY; d.winOff = hj.winOff; d.nWin = hj.nWin;
@@ -893,7 +936,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) || (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) ||
         (st = b->dCounts.alloc(std::max<size_t>(3 * dj.size(), 1))) ||
-        (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dQueue.alloc(kNumClasses))) {
+        (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dQueue.alloc(kNumClasses)) ||
+        (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1))) {
         delete b;
         return st;
     }
@@ -916,6 +960,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.queue = b->dQueue.p;
     kp.ringLen = ringLen;
     kp.withIndels = mode == MP_MODE_POSTERIORS;
+    kp.mode = mode;
+    kp.expect = b->dExpect.p;
     kp.threshold = (float)p->threshold;
     for (int c = 0; c < kNumClasses; c++) {
         kp.jobBegin[c] = b->classBegin[c];
@@ -938,6 +984,8 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     }
     mp_status st;
     float ms = 0.f;
+    if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_queue(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
+        return st;
     if (!plain) {
         if ((st = reset_queue(b, b->dQueue.p, kNumClasses))) return st;
         HIP_TRY(hipEventRecord(e->ev0, s));
@@ -1079,6 +1127,63 @@ extern "C" mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_p
         }
     }
     return MP_OK;
+}
+
+extern "C" mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, double *likelihood) {
+    if (!b || !trans) {
+        mp::set_error("mp_batch_fetch_expectations: null argument");
+        return MP_EINVAL;
+    }
+    if (!b->ran || b->mode != MP_MODE_EXPECTATIONS) {
+        mp::set_error("mp_batch_fetch_expectations: batch has not run in expectation mode");
+        return MP_EINVAL;
+    }
+    const int32_t nj = (int32_t)b->jobs.size();
+    std::vector<double> ex((size_t)std::max(nj, 1) * 10, 0.0);
+    if (nj) {
+        HIP_TRY(hipSetDevice(b->eng->device));
+        HIP_TRY(hipMemcpy(ex.data(), b->dExpect.p, sizeof(double) * 10 * nj, hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < b->npairs; i++) {
+        double t[10] = {0};
+        for (int32_t s = 0; s < b->pairNJobs[i]; s++) {
+            const int32_t q = b->order[b->pairFirstJob[i] + s];
+            if (q < 0) continue;
+            for (int k = 0; k < 10; k++) t[k] += ex[(size_t)q * 10 + k];
+        }
+        for (int k = 0; k < 9; k++) trans[9 * i + k] = t[k];
+        if (likelihood) likelihood[i] = t[9];
+    }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_pairhmm_forward_log_prob(mp_engine *e, const mp_model *m, const mp_pair *pair,
+                                                 const mp_band_params *p, double *logProb) {
+    if (!pair || !logProb) {
+        mp::set_error("mp_pairhmm_forward_log_prob: null argument");
+        return MP_EINVAL;
+    }
+    mp_batch *b = nullptr;
+    mp_status st = mp_batch_create(e, m, pair, 1, p, MP_MODE_FORWARD_ONLY, 0, &b);
+    if (st) return st;
+    int64_t counts[3];
+    if (!(st = mp_batch_run_plain(b))) st = mp_batch_fetch_counts(b, counts, logProb);
+    mp_batch_free(b);
+    return st;
+}
+
+extern "C" mp_status mp_pairhmm_expectations(mp_engine *e, const mp_model *m, const mp_pair *pair,
+                                             const mp_band_params *p, double *trans9, double *likelihood) {
+    if (!pair || !trans9) {
+        mp::set_error("mp_pairhmm_expectations: null argument");
+        return MP_EINVAL;
+    }
+    mp_batch *b = nullptr;
+    mp_status st = mp_batch_create(e, m, pair, 1, p, MP_MODE_EXPECTATIONS, 0, &b);
+    if (st) return st;
+    if (!(st = mp_batch_run_plain(b))) st = mp_batch_fetch_expectations(b, trans9, likelihood);
+    mp_batch_free(b);
+    return st;
 }
 
 extern "C" void mp_batch_free(mp_batch *b) {

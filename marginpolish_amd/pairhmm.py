@@ -7,6 +7,8 @@ impl/polisher.c, with the same argument meaning:
   getAlignedPairsWithIndelsUsingAnchors(...)     pairwiseAligner.c:1105-1130
   getAlignedPairsUsingAnchors(...)               pairwiseAligner.c:1088-1103
   getAlignedPairs / getAlignedPairsWithIndels    pairwiseAligner.c:1132-1149
+  computeForwardProbability(...)                 pairwiseAligner.c:813-865
+  getExpectationsUsingAnchors / getExpectations  pairwiseAligner.c:1151-1164
   getAlignedPairsWithIndelsCroppingReference     polisher.c:605-654
   realign_all(...)                               the per-read loop of poa_realign
                                                  (polisher.c:669-700), batched
@@ -216,6 +218,15 @@ class Batch:
             lists.append(np.split(buf[:tot], np.cumsum(counts[:, li])[:-1]) if self.npairs else [])
         return [(lists[0][i], lists[1][i], lists[2][i], float(lp[i])) for i in range(self.npairs)]
 
+    def expectations(self):
+        """Expectation mode: (npairs, 9) expected transition counts T[from*3+to]
+        and the per-pair likelihood increment."""
+        t = np.zeros((max(self.npairs, 1), 9), dtype=np.float64)
+        lk = np.zeros(max(self.npairs, 1), dtype=np.float64)
+        check(lib().mp_batch_fetch_expectations(self._h, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                                lk.ctypes.data_as(C.POINTER(C.c_double))))
+        return t[: self.npairs], lk[: self.npairs]
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mp_batch_free(self._h)
@@ -261,6 +272,34 @@ def getAlignedPairsWithIndels(machines, strand, sX, sY, p, alignmentHasRaggedLef
                               alignmentHasRaggedRightEnd=False, engine=None):
     return getAlignedPairsWithIndelsUsingAnchors(machines, strand, sX, sY, None, p, alignmentHasRaggedLeftEnd,
                                                  alignmentHasRaggedRightEnd, engine)
+
+
+def computeForwardProbability(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd=False,
+                              alignmentHasRaggedRightEnd=False, engine=None):
+    """Forward log-likelihood with the static band of p.diagonalExpansion."""
+    b = Batch(machines, [(sX, sY, anchorPairs, strand, alignmentHasRaggedLeftEnd, alignmentHasRaggedRightEnd)], p,
+              mode=_lib.MODE_FORWARD_ONLY, engine=engine)
+    b.run_plain()
+    _, lp = b.counts()
+    b.close()
+    return float(lp[0])
+
+
+def getExpectationsUsingAnchors(machines, strand, sX, sY, anchorPairs, p, alignmentHasRaggedLeftEnd=False,
+                                alignmentHasRaggedRightEnd=False, engine=None):
+    """Expected transition counts (9, [from*3+to]) and the likelihood increment."""
+    b = Batch(machines, [(sX, sY, anchorPairs, strand, alignmentHasRaggedLeftEnd, alignmentHasRaggedRightEnd)], p,
+              mode=_lib.MODE_EXPECTATIONS, engine=engine)
+    b.run_plain()
+    t, lk = b.expectations()
+    b.close()
+    return t[0], float(lk[0])
+
+
+def getExpectations(machines, strand, sX, sY, p, alignmentHasRaggedLeftEnd=False, alignmentHasRaggedRightEnd=False,
+                    engine=None):
+    return getExpectationsUsingAnchors(machines, strand, sX, sY, None, p, alignmentHasRaggedLeftEnd,
+                                       alignmentHasRaggedRightEnd, engine)
 
 
 def getAlignedPairsWithIndelsCroppingReference(params, ref_sym, read_sym, read_strand, anchors, engine=None):

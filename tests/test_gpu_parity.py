@@ -256,3 +256,77 @@ def test_wide_band_rejected_loudly(ph):
     with pytest.raises(ph.MPError) as ei:
         ph.getAlignedPairs(m, 1, sx, sx, p)
     assert ei.value.status == 6
+
+
+def test_forward_log_prob_parity(ph, params):
+    """computeForwardProbability: static band of diagonalExpansion, no split,
+    vs O_exact and vs exact enumeration on tiny pairs."""
+    pp, op = params
+    rng = random.Random(31)
+    dm = ph.StateMachines.default_nucleotide()
+    odm = O.default_nucleotide_model()
+    p = ph.PairwiseAlignmentParameters(diagonalExpansion=4)
+    opar = O.Params(diagonalExpansion=4)
+    cases = []
+    for i in range(24):
+        lX, lY = rng.randint(1, 5), rng.randint(1, 5)
+        cases.append((_rand_syms(rng, lX, False), _rand_syms(rng, lY, False), None, 1, bool(i & 1), bool(i & 2)))
+    say("case: forward-only tiny x24")
+    b = ph.Batch(dm, cases, p, mode=2)
+    b.run()
+    _, lps = b.counts()
+    for (sx, sy, _, _, rl, rr), lp in zip(cases, lps):
+        total, _ = brute.enumerate_paths(odm, sx, sy, rl, rr)
+        assert abs(lp - total) < 1e-4 * max(1.0, abs(total)), (lp, total)
+    from marginpolish_amd import crop, sim
+    w = sim.config0(n_reads=6, ref_len=1500, seed=9, W=8, strands=(1, 0))
+    jobs = []
+    for pr in w.pairs:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    say("case: forward-only config0-like x6")
+    p2 = ph.PairwiseAlignmentParameters(diagonalExpansion=8, dynamicAnchorExpansion=True)
+    o2 = O.Params(diagonalExpansion=8, dynamicAnchorExpansion=True)
+    b = ph.Batch(pp.machines, jobs, p2, mode=2)
+    b.run()
+    _, lps = b.counts()
+    for (sx, sy, an, strand), lp in zip(jobs, lps):
+        want = O.forward_prob(op.forward if strand else op.reverse, sx, sy, an, o2)
+        assert abs(lp - want) < 1e-5 * abs(want), (lp, want)
+    one = ph.computeForwardProbability(pp.machines, jobs[0][3], jobs[0][0], jobs[0][1], jobs[0][2], p2)
+    assert abs(one - lps[0]) < 1e-9 * abs(lps[0])
+
+
+def test_expectations_parity(ph, params):
+    """getExpectationsUsingAnchors: expected transition counts and the
+    per-diagonal likelihood sum vs O_exact (multi-window, splits, ragged)."""
+    pp, op = params
+    from marginpolish_amd import crop, sim
+    w = sim.config0(n_reads=6, ref_len=1200, seed=13, W=10, strands=(1, 0))
+    jobs = []
+    for i, pr in enumerate(w.pairs):
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand, bool(i & 1), bool(i & 2)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=400, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    say("case: expectations x6 (multi-window)")
+    b = ph.Batch(pp.machines, jobs, _bp(ph, opar), mode=3)
+    b.run()
+    t, lk = b.expectations()
+    for (sx, sy, an, strand, rl, rr), tg, lg in zip(jobs, t, lk):
+        te, le = O.expectations(op.forward if strand else op.reverse, sx, sy, an, opar, rl, rr)
+        assert np.allclose(tg, te, rtol=1e-4, atol=1e-3), (tg, te)
+        assert abs(lg - le) < 1e-6 * abs(le), (lg, le)
+    # tiny pairs vs the oracle (pinned by enumeration in the oracle tests)
+    dm = ph.StateMachines.default_nucleotide()
+    odm = O.default_nucleotide_model()
+    rng = random.Random(8)
+    cases = [(_rand_syms(rng, rng.randint(1, 6), False), _rand_syms(rng, rng.randint(1, 6), False), None, 1)
+             for _ in range(20)]
+    p = ph.PairwiseAlignmentParameters(threshold=0.0, diagonalExpansion=20)
+    b = ph.Batch(dm, cases, p, mode=3)
+    b.run()
+    t, _ = b.expectations()
+    for (sx, sy, _, _), tg in zip(cases, t):
+        te, _ = O.expectations(odm, sx, sy, [], O.Params(threshold=0.0, diagonalExpansion=20))
+        assert np.allclose(tg, te, rtol=1e-4, atol=1e-5), (tg, te)
