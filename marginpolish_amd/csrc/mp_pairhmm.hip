@@ -61,6 +61,7 @@ constexpr int kTabStride = 13056;  // >= 40 + 5*51*51, multiple of 64
 constexpr int kRingStride = 256;   // floats per ring diagonal per wave: float4 x 64 lanes
 constexpr int kNumClasses = 3;     // SEG 16, 32, 64
 constexpr int kSymPad = 128;       // N symbols around the symbol pool
+constexpr int kWideMaxC = 32;      // wide kernel: up to 64 * 32 = 2048 cells per anti-diagonal
 constexpr int kBlock = 512;
 constexpr int kK = 8;              // anti-diagonals per macro step (grid cell)
 
@@ -78,14 +79,16 @@ struct DevWin {
 struct KParams {
     const DevJob *jobs;
     const uint16_t *sym;
-    const int *band;  // packed (xmyL << 8) | (width - 1), cells of kK padded
+    const int *band;  // packed (xmyL << 12) | (width - 1), cells of kK padded
     const DevWin *win;
     int4 *out;
     int32_t *counts;
     double *logProb;
     float *ring;
     const float *tables;
-    int32_t *queue;  // kNumClasses counters
+    int32_t *queue;  // kNumClasses + 1 counters (the last for wide jobs)
+    float *ringWide;
+    int32_t ringWideLen, wideC, wideBegin, wideEnd;
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;
     int32_t withIndels;
@@ -121,9 +124,9 @@ __device__ __forceinline__ float row_sum(float v, int seg) {
     return v;
 }
 
-// Band entries are packed as (xmyL << 8) | (width - 1).
-__device__ __forceinline__ int band_L(int b) { return b >> 8; }
-__device__ __forceinline__ int band_R(int b) { return (b >> 8) + 2 * (b & 255); }
+// Band entries are packed as (xmyL << 12) | (width - 1).
+__device__ __forceinline__ int band_L(int b) { return b >> 12; }
+__device__ __forceinline__ int band_R(int b) { return (b >> 12) + 2 * (b & 4095); }
 
 // Neighbour input of a cell from the exchange register v of a previous step:
 // the source lane is k + s.  Fast path (|s| <= 1): one DPP shift; the lanes
@@ -489,6 +492,275 @@ __device__ void run_class(const KParams &kp, const float *lds, int cls, int seg)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide bands (more than 64 cells on some anti-diagonal: anchor-free calls,
+// large band half-widths).  One job per wavefront; cell i = lane + 64c for
+// c < C.  The (X, Y, M) outputs of the last three anti-diagonals live in LDS
+// and are read by band coordinate; one anti-diagonal per step.  Scaling is
+// lazy: the exponent chosen at a grid-cell start comes from the previous
+// anti-diagonal's maximum, and the two-back match input is corrected by the
+// same exponent on the next step.
+// ---------------------------------------------------------------------------
+
+struct WideCtx {
+    const KParams &kp;
+    float *xb;  // [3 slots][3 states][W]
+    int W, C, lane;
+    int64_t symX, symY, bandOff;
+    const float *tab;
+    __device__ float *slot(int d, int st) const { return xb + ((size_t)(((d % 3) + 3) % 3) * 3 + st) * W; }
+    __device__ void band(int d, int &L, int &R) const {
+        const int b = kp.band[bandOff + d];
+        L = band_L(b);
+        R = band_R(b);
+    }
+};
+
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void pairhmm_wide_kernel(KParams kp) {
+    extern __shared__ float xbuf[];
+    const int lane = threadIdx.x;
+    const int C = kp.wideC, W = 64 * C;
+    float4 *ring = reinterpret_cast<float4 *>(kp.ringWide) + (size_t)blockIdx.x * kp.ringWideLen * W;
+    const int R = kp.ringWideLen;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const bool expMode = kp.mode == MP_MODE_EXPECTATIONS, fwdOnly = kp.mode == MP_MODE_FORWARD_ONLY;
+    const bool postMode = kp.mode <= MP_MODE_MATCH_ONLY;
+    const int stopOff = expMode ? 0 : 1;
+    const float T[9] = {kp.T[0], kp.T[1], kp.T[2], kp.T[3], kp.T[4], kp.T[5], kp.T[6], kp.T[7], kp.T[8]};
+
+    for (;;) {
+        int jn = 0;
+        if (lane == 0) jn = atomicAdd(&kp.queue[3], 1);
+        jn = __shfl(jn, 0, 64);
+        const int job = kp.wideBegin + jn;
+        if (job >= kp.wideEnd) break;
+        const DevJob jb = kp.jobs[job];
+        const int D = jb.lX + jb.lY;
+        const bool rgL = (jb.flags >> 1) & 1, rgR = (jb.flags >> 2) & 1;
+        WideCtx cx{kp, xbuf, W, C, lane, jb.symX, jb.symY, jb.bandOff, kp.tables + ((jb.flags & 1) ? kTabStride : 0)};
+        int cntM = 0, cntX = 0, cntY = 0;
+        double likeAcc = 0.0;
+        double eacc[9];
+        for (int i = 0; i < 9; i++) eacc[i] = 0.0;
+        int S = 0;          // forward exponent
+        float lastMax = 0.75f;
+        int pendM = 0;      // exponent correction of the two-back match input
+        int dStart = 0;     // first forward anti-diagonal of this window
+        for (int i = lane; i < 9 * W; i += 64) xbuf[i] = 0.f;
+        __syncthreads();
+        for (int win = 0; win < jb.nWin; win++) {
+            const DevWin wn = kp.win[jb.winOff + win];
+            const bool atEnd = wn.tb == D;
+            // ---- forward from dStart to tb ----
+            for (int d = dStart; d <= wn.tb; d++) {
+                int e2 = 0;
+                if (d > 0 && (d & (kK - 1)) == 0) {
+                    const float m = wave_max(lastMax);
+                    e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                }
+                const float scl = ldexpf(1.f, -e2), sclM = ldexpf(1.f, -e2 - pendM);
+                int L0, R0, L1 = 1, R1 = -1, L2 = 1, R2 = -1;
+                cx.band(d, L0, R0);
+                if (d >= 1) cx.band(d - 1, L1, R1);
+                if (d >= 2) cx.band(d - 2, L2, R2);
+                float mx = 0.f;
+                for (int c = 0; c < C; c++) {
+                    const int i = lane + 64 * c, xmy = L0 + 2 * i;
+                    const bool act = xmy <= R0;
+                    const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
+                    float p0 = 0.f, p1 = 0.f, p2 = 0.f;
+                    if (act) {
+                        const unsigned sx = x > 0 ? kp.sym[jb.symX + x - 1] : 4u, sy = y > 0 ? kp.sym[jb.symY + y - 1] : 4u;
+                        const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
+                        float eM = cx.tab[kOffE + xb * 5 + yb];
+                        if (kp.rle) eM *= cx.tab[kOffRL + xb * 2601 + min(sx >> 8, 50u) * 51 + min(sy >> 8, 50u)];
+                        if (d == 0) {
+                            p0 = rgL ? kp.startV[1][0] : kp.startV[0][0];
+                            p1 = rgL ? kp.startV[1][1] : kp.startV[0][1];
+                            p2 = rgL ? kp.startV[1][2] : kp.startV[0][2];
+                        } else {
+                            const int sX = xmy - 1, sY = xmy + 1;
+                            const float inX = (sX >= L1 && sX <= R1) ? cx.slot(d - 1, 1)[(sX - L1) >> 1] : 0.f;
+                            const float inY = (sY >= L1 && sY <= R1) ? cx.slot(d - 1, 2)[(sY - L1) >> 1] : 0.f;
+                            const float inM = (xmy >= L2 && xmy <= R2) ? cx.slot(d - 2, 0)[(xmy - L2) >> 1] : 0.f;
+                            p0 = eM * inM * sclM;
+                            p1 = cx.tab[kOffGX + xb] * inX * scl;
+                            p2 = cx.tab[kOffGY + yb] * inY * scl;
+                        }
+                    }
+                    mx = fmaxf(mx, fmaxf(p0, fmaxf(p1, p2)));
+                    cx.slot(d, 0)[i] = T[0] * p0 + T[3] * p1 + T[6] * p2;
+                    cx.slot(d, 1)[i] = T[1] * p0 + T[4] * p1 + T[7] * p2;
+                    cx.slot(d, 2)[i] = T[2] * p0 + T[5] * p1 + T[8] * p2;
+                    if (!fwdOnly) ring[(size_t)(d % R) * W + i] = make_float4(p0, p1, p2, __int_as_float(S + e2));
+                    if (d == wn.tb) {
+                        const float e0 = rgR && atEnd ? kp.endV[1][0] : kp.endV[0][0];
+                        const float e1 = rgR && atEnd ? kp.endV[1][1] : kp.endV[0][1];
+                        const float e2v = rgR && atEnd ? kp.endV[1][2] : kp.endV[0][2];
+                        lastMax = p0 * e0 + p1 * e1 + p2 * e2v + (c == 0 ? 0.f : lastMax);  // partial total
+                    }
+                }
+                S += e2;
+                pendM = e2;
+                if (d != wn.tb) lastMax = mx;
+                __syncthreads();
+            }
+            const float t = wave_sum(lastMax);
+            const int SFtb = S;
+            const float totalInv = t > 0.f ? 1.f / t : 0.f;
+            const double lt = log((double)t) + (double)SFtb * 0.69314718055994530942;
+            if (atEnd && lane == 0) kp.logProb[job] = lt;
+            likeAcc += (double)(wn.from - wn.lo) * lt;
+            if (fwdOnly) break;
+            // ---- backward from tb down to lo + stopOff ----
+            for (int i = lane; i < 9 * W; i += 64) xbuf[i] = 0.f;
+            __syncthreads();
+            int SB = 0, pendMB = 0;
+            float lastMaxB = 0.75f;
+            for (int d = wn.tb; d >= wn.lo + stopOff; d--) {
+                int e2 = 0;
+                if (d < wn.tb && ((d + 1) & (kK - 1)) == 0) {
+                    const float m = wave_max(lastMaxB);
+                    e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                }
+                const float scl = ldexpf(1.f, -e2);
+                int L0, R0, L1 = 1, R1 = -1, L2 = 1, R2 = -1;
+                cx.band(d, L0, R0);
+                if (d + 1 <= wn.tb) cx.band(d + 1, L1, R1);
+                if (d + 2 <= wn.tb) cx.band(d + 2, L2, R2);
+                float mx = 0.f;
+                float ep[9];
+                for (int q = 0; q < 9; q++) ep[q] = 0.f;
+                for (int c = 0; c < C; c++) {
+                    const int i = lane + 64 * c, xmy = L0 + 2 * i;
+                    const bool act = xmy <= R0;
+                    const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
+                    float q0 = 0.f, q1 = 0.f, q2 = 0.f, eX = 0.f, eY = 0.f, eM = 0.f, inX = 0.f, inY = 0.f, inM = 0.f;
+                    if (act) {
+                        const unsigned sx = x > 0 ? kp.sym[jb.symX + x - 1] : 4u, sy = y > 0 ? kp.sym[jb.symY + y - 1] : 4u;
+                        const unsigned xb = min(sx & 255u, 4u), yb = min(sy & 255u, 4u);
+                        eX = cx.tab[kOffGX + xb];
+                        eY = cx.tab[kOffGY + yb];
+                        eM = cx.tab[kOffE + xb * 5 + yb];
+                        if (kp.rle) eM *= cx.tab[kOffRL + xb * 2601 + min(sx >> 8, 50u) * 51 + min(sy >> 8, 50u)];
+                        if (d == wn.tb) {
+                            const bool rg = atEnd && rgR;
+                            q0 = rg ? kp.endV[1][0] : kp.endV[0][0];
+                            q1 = rg ? kp.endV[1][1] : kp.endV[0][1];
+                            q2 = rg ? kp.endV[1][2] : kp.endV[0][2];
+                        } else {
+                            const int sX = xmy + 1, sY = xmy - 1;
+                            inX = (sX >= L1 && sX <= R1) ? cx.slot(d + 1, 1)[(sX - L1) >> 1] : 0.f;
+                            inY = (sY >= L1 && sY <= R1) ? cx.slot(d + 1, 2)[(sY - L1) >> 1] : 0.f;
+                            inM = (xmy >= L2 && xmy <= R2) ? cx.slot(d + 2, 0)[(xmy - L2) >> 1] * ldexpf(1.f, -pendMB) : 0.f;
+                            q0 = (T[0] * inM + T[1] * inX + T[2] * inY) * scl;
+                            q1 = (T[3] * inM + T[4] * inX + T[5] * inY) * scl;
+                            q2 = (T[6] * inM + T[7] * inX + T[8] * inY) * scl;
+                        }
+                    }
+                    mx = fmaxf(mx, fmaxf(q0, fmaxf(q1, q2)));
+                    const float4 f = act ? ring[(size_t)(d % R) * W + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (expMode && act) {
+                        const float sc = ldexpf(totalInv, __float_as_int(f.w) + SB - SFtb);
+                        const bool okXY = d + 1 > wn.lo && d + 1 <= wn.from, okM = d + 2 > wn.lo && d + 2 <= wn.from;
+                        const float tX = okXY ? inX : 0.f, tY = okXY ? inY : 0.f, tM = okM ? inM : 0.f;
+                        const float a0 = f.x * sc, a1 = f.y * sc, a2 = f.z * sc;
+                        ep[0] += a0 * T[0] * tM; ep[1] += a0 * T[1] * tX; ep[2] += a0 * T[2] * tY;
+                        ep[3] += a1 * T[3] * tM; ep[4] += a1 * T[4] * tX; ep[5] += a1 * T[5] * tY;
+                        ep[6] += a2 * T[6] * tM; ep[7] += a2 * T[7] * tX; ep[8] += a2 * T[8] * tY;
+                    }
+                    if (postMode && d <= wn.from) {
+                        const float sc = ldexpf(totalInv, __float_as_int(f.w) + SB + e2 - SFtb);
+                        const float pM = f.x * q0 * sc, pX = f.y * q1 * sc, pY = f.z * q2 * sc;
+                        const float thr = kp.threshold;
+                        const bool okM = act && x > 0 && y > 0 && pM >= thr;
+                        const bool okX = kp.withIndels && act && x > 0 && pX >= thr;
+                        const bool okY = kp.withIndels && act && y > 0 && pY >= thr;
+                        const uint64_t bM = __ballot(okM), bX = __ballot(okX), bY = __ballot(okY);
+                        if (okM) {
+                            const int idx = cntM + __popcll(bM & below);
+                            if (idx < jb.cap[0])
+                                kp.out[jb.outOff + idx] = make_int4((int)floorf(fminf(pM, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        if (okX) {
+                            const int idx = cntX + __popcll(bX & below);
+                            if (idx < jb.cap[1])
+                                kp.out[jb.outOff + jb.cap[0] + idx] =
+                                    make_int4((int)floorf(fminf(pX, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        if (okY) {
+                            const int idx = cntY + __popcll(bY & below);
+                            if (idx < jb.cap[2])
+                                kp.out[jb.outOff + jb.cap[0] + jb.cap[1] + idx] =
+                                    make_int4((int)floorf(fminf(pY, 1.f) * 1e7f), 0, x - 1, y - 1);
+                        }
+                        cntM += __popcll(bM);
+                        cntX += __popcll(bX);
+                        cntY += __popcll(bY);
+                    }
+                    cx.slot(d, 0)[i] = eM * q0;
+                    cx.slot(d, 1)[i] = eX * q1;
+                    cx.slot(d, 2)[i] = eY * q2;
+                }
+                if (expMode)
+                    for (int q = 0; q < 9; q++) eacc[q] += (double)ep[q];
+                SB += e2;
+                pendMB = e2;
+                lastMaxB = mx;
+                __syncthreads();
+            }
+            // ---- resume the forward sweep after tb: rebuild its exchange state from the ring ----
+            if (win + 1 < jb.nWin) {
+                const int tb = wn.tb;
+                int L0, R0, L1, R1;
+                cx.band(tb, L0, R0);
+                cx.band(tb - 1, L1, R1);
+                for (int c = 0; c < C; c++) {
+                    const int i = lane + 64 * c;
+                    const float4 f = ring[(size_t)(tb % R) * W + i];
+                    const float4 g = ring[(size_t)((tb - 1) % R) * W + i];
+                    const bool a0 = L0 + 2 * i <= R0, a1 = L1 + 2 * i <= R1;
+                    const float sg = ldexpf(1.f, __float_as_int(g.w) - __float_as_int(f.w));
+                    cx.slot(tb, 0)[i] = a0 ? T[0] * f.x + T[3] * f.y + T[6] * f.z : 0.f;
+                    cx.slot(tb, 1)[i] = a0 ? T[1] * f.x + T[4] * f.y + T[7] * f.z : 0.f;
+                    cx.slot(tb, 2)[i] = a0 ? T[2] * f.x + T[5] * f.y + T[8] * f.z : 0.f;
+                    cx.slot(tb - 1, 0)[i] = a1 ? (T[0] * g.x + T[3] * g.y + T[6] * g.z) * sg : 0.f;
+                }
+                __syncthreads();
+                dStart = tb + 1;
+                pendM = 0;
+                lastMax = 0.75f;
+            }
+        }
+        if (lane == 0) {
+            kp.counts[3 * job + 0] = cntM;
+            kp.counts[3 * job + 1] = cntX;
+            kp.counts[3 * job + 2] = cntY;
+        }
+        if (expMode) {
+            for (int q = 0; q < 9; q++) {
+                const double v = wave_sum_d(eacc[q]);
+                if (lane == 0) kp.expect[(size_t)job * 10 + q] = v;
+            }
+            if (lane == 0) kp.expect[(size_t)job * 10 + 9] = likeAcc;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void pairhmm_kernel(KParams kp) {
     extern __shared__ float lds[];
     for (int i = threadIdx.x; i < 2 * kTabStride; i += blockDim.x) lds[i] = kp.tables[i];
@@ -593,6 +865,8 @@ struct mp_batch {
     DevBuf<float> dTables;
     DevBuf<int32_t> dQueue;
     DevBuf<double> dExpect;
+    DevBuf<float> dRingWide;
+    int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
     int64_t outTotal = 0;
     int32_t gridRows = 0;
     double kernelMs = 0.0;
@@ -605,6 +879,13 @@ namespace {
 mp_status launch(mp_batch *b, int grid, int block, const KParams &kp) {
     size_t lds = 2 * kTabStride * sizeof(float);
     hipLaunchKernelGGL(pairhmm_kernel, dim3(grid), dim3(block), lds, b->eng->stream, kp);
+    HIP_TRY(hipGetLastError());
+    return MP_OK;
+}
+
+mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
+    size_t lds = (size_t)9 * 64 * kp.wideC * sizeof(float);
+    hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), lds, b->eng->stream, kp);
     HIP_TRY(hipGetLastError());
     return MP_OK;
 }
@@ -683,6 +964,8 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
     }
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 2 * kTabStride * sizeof(float)));
+    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                9 * 64 * kWideMaxC * sizeof(float)));
     mp_engine *e = new mp_engine();
     e->device = device;
     e->cus = prop.multiProcessorCount;
@@ -842,7 +1125,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                         int32_t w = (R[d] - L[d]) / 2 + 1;
                         wmax = std::max(wmax, w);
                         cells += w;
-                        band[bandOff[j] + d] = (int32_t)((uint32_t)L[d] << 8) | (std::min(w, 256) - 1);
+                        band[bandOff[j] + d] = (int32_t)((uint32_t)L[d] << 12) | (std::min(w, 4096) - 1);
                     }
                     for (int64_t d = D + 1; d < bandOff[j + 1] - bandOff[j]; d++) band[bandOff[j] + d] = band[bandOff[j] + D];
                     hj.wmax = wmax;
@@ -865,10 +1148,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         return MP_EANCHORS;
     }
     for (int64_t j = 0; j < nj; j++) {
-        if (jobs[j].cls == 3) {
+        if (jobs[j].wmax > 64 * kWideMaxC) {
             delete b;
             mp::set_error("pair " + std::to_string(jobs[j].pair) + ": band width " + std::to_string(jobs[j].wmax) +
-                          " exceeds the 64-cell kernel limit");
+                          " exceeds the " + std::to_string(64 * kWideMaxC) + "-cell kernel limit");
             return MP_EUNSUPPORTED;
         }
     }
@@ -898,7 +1181,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         hj.winOff = (int32_t)dw.size();
         hj.nWin = (int32_t)wins[ord[q]].size();
         for (auto &w : wins[ord[q]]) dw.push_back(DevWin{w.tb, w.lo, w.from, 0});
-        if (mode != MP_MODE_FORWARD_ONLY) ringLen = std::max(ringLen, hj.ringNeed);
+        if (mode != MP_MODE_FORWARD_ONLY && hj.cls < 3) ringLen = std::max(ringLen, hj.ringNeed);
+        if (hj.cls == 3) {
+            b->wideC = std::max(b->wideC, (hj.wmax + 63) / 64);
+            if (mode != MP_MODE_FORWARD_ONLY) b->ringWideLen = std::max(b->ringWideLen, hj.ringNeed);
+        }
         DevJob &d = dj[q];
         d.symX = hj.symX; d.symY = hj.symY; d.bandOff = hj.bandOff; d.outOff = hj.outOff;
         d.lX = hj.lX; d.lY = hj.lY; d.winOff = hj.winOff; d.nWin = hj.nWin;
@@ -918,6 +1205,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
             }
     for (int c = 0; c < kNumClasses; c++)
         if (b->classBegin[c] > b->classEnd[c]) b->classBegin[c] = b->classEnd[c];
+    b->wideBegin = b->wideEnd = (int32_t)ord.size();
+    for (size_t q = 0; q < ord.size(); q++)
+        if (jobs[ord[q]].cls == 3) {
+            b->wideBegin = std::min<int32_t>(b->wideBegin, (int32_t)q);
+            b->wideEnd = (int32_t)q + 1;
+        }
     // pair-major job list in split order (device indices; -1 for empty jobs)
     b->order.resize(nj);
     for (int64_t j = 0; j < nj; j++) b->order[j] = devIndex[j];
@@ -936,7 +1229,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) || (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) ||
         (st = b->dCounts.alloc(std::max<size_t>(3 * dj.size(), 1))) ||
-        (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dQueue.alloc(kNumClasses)) ||
+        (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dQueue.alloc(kNumClasses + 1)) ||
         (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1))) {
         delete b;
         return st;
@@ -948,6 +1241,16 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         delete b;
         return st;
     }
+    if (b->wideEnd > b->wideBegin) {
+        // wide jobs: one wavefront each, ring of float4 per cell; bounded to ~48 GB
+        const size_t perWave = (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * sizeof(float4);
+        const size_t cap = std::max<size_t>(1, ((size_t)48 << 30) / perWave);
+        b->wideWaves = (int32_t)std::min<size_t>({(size_t)(b->wideEnd - b->wideBegin), (size_t)e->cus * 4, cap});
+        if ((st = b->dRingWide.alloc((size_t)b->wideWaves * perWave / sizeof(float)))) {
+            delete b;
+            return st;
+        }
+    }
     kp.jobs = b->dJobs.p;
     kp.sym = b->dSym.p;
     kp.band = b->dBand.p;
@@ -956,6 +1259,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.counts = b->dCounts.p;
     kp.logProb = b->dLogProb.p;
     kp.ring = b->dRing.p;
+    kp.ringWide = b->dRingWide.p;
+    kp.ringWideLen = std::max(b->ringWideLen, 1);
+    kp.wideC = std::max(b->wideC, 1);
+    kp.wideBegin = b->wideBegin;
+    kp.wideEnd = b->wideEnd;
     kp.tables = b->dTables.p;
     kp.queue = b->dQueue.p;
     kp.ringLen = ringLen;
@@ -986,26 +1294,36 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     float ms = 0.f;
     if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_queue(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
         return st;
+    const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
+                           b->classEnd[2] > b->classBegin[2];
     if (!plain) {
-        if ((st = reset_queue(b, b->dQueue.p, kNumClasses))) return st;
+        if ((st = reset_queue(b, b->dQueue.p, kNumClasses + 1))) return st;
         HIP_TRY(hipEventRecord(e->ev0, s));
-        if ((st = launch(b, e->cus, kBlock, b->kp))) return st;
+        if (anyNarrow && (st = launch(b, e->cus, kBlock, b->kp))) return st;
+        if (b->wideEnd > b->wideBegin && (st = launch_wide(b, b->wideWaves, b->kp))) return st;
         HIP_TRY(hipEventRecord(e->ev1, s));
         HIP_TRY(hipEventSynchronize(e->ev1));
         HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     } else {
         // one launch (one wavefront) per job, in device order
         DevBuf<int32_t> q;
-        if ((st = q.alloc((size_t)nj * kNumClasses))) return st;
-        if ((st = reset_queue(b, q.p, nj * kNumClasses))) return st;
+        const int nq = kNumClasses + 1;
+        if ((st = q.alloc((size_t)nj * nq))) return st;
+        if ((st = reset_queue(b, q.p, nj * nq))) return st;
         HIP_TRY(hipEventRecord(e->ev0, s));
         for (int32_t j = 0; j < nj; j++) {
             KParams kp = b->kp;
             for (int c = 0; c < kNumClasses; c++) kp.jobBegin[c] = kp.jobEnd[c] = j;
+            kp.wideBegin = kp.wideEnd = j;
             const int c = b->jobs[j].cls;
-            kp.jobEnd[c] = j + 1;
-            kp.queue = q.p + (size_t)j * kNumClasses;
-            if ((st = launch(b, 1, 64, kp))) return st;
+            kp.queue = q.p + (size_t)j * nq;
+            if (c < 3) {
+                kp.jobEnd[c] = j + 1;
+                if ((st = launch(b, 1, 64, kp))) return st;
+            } else {
+                kp.wideEnd = j + 1;
+                if ((st = launch_wide(b, 1, kp))) return st;
+            }
         }
         HIP_TRY(hipEventRecord(e->ev1, s));
         HIP_TRY(hipEventSynchronize(e->ev1));

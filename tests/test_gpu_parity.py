@@ -248,14 +248,86 @@ def test_config1_full_batch_properties(ph, params, config1):
         assert len(as_map(as_rows(m))) == len(m)
 
 
-def test_wide_band_rejected_loudly(ph):
+def test_too_wide_band_rejected_loudly(ph):
     m = ph.StateMachines.default_nucleotide()
     p = ph.PairwiseAlignmentParameters()
     rng = random.Random(1)
-    sx = _rand_syms(rng, 150, False)
+    sx = _rand_syms(rng, 2500, False)
     with pytest.raises(ph.MPError) as ei:
         ph.getAlignedPairs(m, 1, sx, sx, p)
     assert ei.value.status == 6
+
+
+def _read_fasta(path):
+    seqs, name = [], None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith(">"):
+            name = line[1:]
+            seqs.append([name, ""])
+        elif line:
+            seqs[-1][1] += line
+    return seqs
+
+
+def test_wide_band_anchor_free_ecoli(ph, params):
+    """Anchor-free realignment (getAlignedPairsWithIndels, RL cap 49 as in
+    poa_realign's anchor-free branch, polisher.c:676-683) of real ONT reads to
+    their 100 bp truth windows: full-rectangle bands of 50-110 cells, run by
+    the wide-band kernel, vs O_exact."""
+    import glob
+    pp, op = params
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ecoli_100bp")
+    jobs = []
+    for ref in sorted(glob.glob(os.path.join(root, "*.ref.fasta"))):
+        truth = _read_fasta(ref)[0][1]
+        rb, rr = O.rle_encode(truth)
+        sx = O.rle_symbols(rb, rr, 50)
+        for name, read in _read_fasta(ref.replace(".ref.fasta", ".fasta"))[:12]:
+            qb, qr = O.rle_encode(read)
+            jobs.append((sx, O.rle_symbols(qb, qr, 50), None, 1 if name.endswith("_F") else 0))
+    say("case: ecoli anchor-free x%d (wide kernel)" % len(jobs))
+    b = ph.Batch(pp.machines, jobs, pp.p)
+    b.run()
+    worst = _check(b.fetch(), _oracle_jobs(op, jobs, op.p), op.p.threshold)
+    say("ecoli anchor-free max-abs posterior err: %.3g" % worst)
+
+
+def test_wide_band_windows_modes(ph, params):
+    """Wide bands (W=80) with several traceback windows, splits and ragged
+    ends, in posterior, forward-only and expectation modes, plus batched vs
+    plain identity."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(n_reads=4, ref_len=1500, seed=17, W=80, strands=(1, 0))
+    jobs = []
+    for i, pr in enumerate(w.pairs):
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an[::7], pr.strand, bool(i & 1), bool(i & 2)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=500, traceBackDiagonals=40, diagonalExpansion=80,
+                    splitMatrixBiggerThanThis=40000, dynamicAnchorExpansion=True)
+    say("case: wide windows/splits x4")
+    b = ph.Batch(pp.machines, jobs, _bp(ph, opar))
+    b.run()
+    g = b.fetch()
+    _check(g, _oracle_jobs(op, jobs, opar), opar.threshold)
+    b2 = ph.Batch(pp.machines, jobs, _bp(ph, opar)).run_plain()
+    for r1, r2 in zip(g, b2.fetch()):
+        for l in range(3):
+            assert np.array_equal(r1[l], r2[l])
+    be = ph.Batch(pp.machines, jobs, _bp(ph, opar), mode=3)
+    be.run()
+    t, lk = be.expectations()
+    for (sx, sy, an, strand, rl, rr), tg, lg in zip(jobs, t, lk):
+        te, le = O.expectations(op.forward if strand else op.reverse, sx, sy, an, opar, rl, rr)
+        assert np.allclose(tg, te, rtol=1e-4, atol=1e-3), (tg, te)
+        assert abs(lg - le) < 1e-6 * abs(le)
+    bf = ph.Batch(pp.machines, jobs, _bp(ph, opar), mode=2)
+    bf.run()
+    _, lps = bf.counts()
+    for (sx, sy, an, strand, rl, rr), lp in zip(jobs, lps):
+        want = O.forward_prob(op.forward if strand else op.reverse, sx, sy, an, opar, rl, rr)
+        assert abs(lp - want) < 1e-5 * abs(want), (lp, want)
 
 
 def test_forward_log_prob_parity(ph, params):
