@@ -1,0 +1,224 @@
+// mp_device.hpp -- device-side layout shared by the pair-HMM kernels
+// (gfx950): job/window descriptors, the kernel parameter block, the symbol
+// encoding that turns an emission into four LDS reads, band entry packing,
+// and wavefront helpers (DPP shifts and row reductions).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mp_pairhmm.h"
+
+namespace mpk {
+
+// ---------------------------------------------------------------------------
+// Emission table (floats, copied to LDS).  Both strands share the run-length
+// table: the strand only changes which of its four base rows a reference
+// base selects (polisher.c:1756-1765: row = strand ? base : 3 - base, N -> A).
+//   GX[strand][5], GY[strand][5], E[strand][5][5], RL[4][51][51]
+// ---------------------------------------------------------------------------
+constexpr int kTabGX = 0, kTabGY = 10, kTabE = 20, kTabRL = 72;
+constexpr int kRL = 51;                        // run lengths 0..50
+constexpr int kTabFloats = kTabRL + 4 * kRL * kRL;  // 10476
+// Every byte offset an encoded symbol pair can form stays below this bound,
+// so the LDS allocation is padded to it (no read can leave the allocation).
+constexpr int kTabReach = kTabRL * 4 + 65536;
+constexpr uint32_t kRLLastByte = (4 * kRL * kRL - 1) * 4;
+
+// Symbol encoding (uint32 per sequence position, built on the host per pair
+// and strand).  Reference side (x):
+//   bits  0-15  byte offset of the run-length row: (row * 51 + rl_x) * 51 * 4
+//   bits 16-23  byte offset of the match row:       (strand * 25 + base_x * 5) * 4
+//   bits 24-31  byte offset of the gap-X entry:     (strand * 5 + base_x) * 4
+// Read side (y):
+//   bits  0-15  rl_y * 4
+//   bits 16-23  base_y * 4
+//   bits 24-31  byte offset of the gap-Y entry:     (strand * 5 + base_y) * 4
+// so x + y carries the run-length and match-table offsets in one add.
+__host__ __device__ inline uint32_t enc_x(uint32_t sym, int strand) {
+    const uint32_t base = (sym & 255u) < 4u ? (sym & 255u) : 4u;
+    const uint32_t rl = (sym >> 8) < 50u ? (sym >> 8) : 50u;
+    const uint32_t row = (base & 3u) ^ (strand ? 0u : 3u);
+    return ((row * kRL + rl) * kRL * 4u) | (((uint32_t)strand * 25u + base * 5u) * 4u << 16) |
+           (((uint32_t)strand * 5u + base) * 4u << 24);
+}
+__host__ __device__ inline uint32_t enc_y(uint32_t sym, int strand) {
+    const uint32_t base = (sym & 255u) < 4u ? (sym & 255u) : 4u;
+    const uint32_t rl = (sym >> 8) < 50u ? (sym >> 8) : 50u;
+    return (rl * 4u) | ((base * 4u) << 16) | ((((uint32_t)strand * 5u + base) * 4u) << 24);
+}
+
+struct Emis {
+    float eX, eY, eM;
+};
+
+__device__ __forceinline__ Emis emissions(const float *tab, uint32_t sx, uint32_t sy) {
+    const char *t = reinterpret_cast<const char *>(tab);
+    const uint32_t s = sx + sy;
+    // The clamp keeps cells outside the band (which may read any pool entry)
+    // inside the table; in-band cells never reach it.
+    const float rl = *reinterpret_cast<const float *>(t + kTabRL * 4 + min(s & 0xffffu, kRLLastByte));
+    const float e = *reinterpret_cast<const float *>(t + kTabE * 4 + ((s >> 16) & 0xffu));
+    const float ex = *reinterpret_cast<const float *>(t + kTabGX * 4 + (sx >> 24));
+    const float ey = *reinterpret_cast<const float *>(t + kTabGY * 4 + (sy >> 24));
+    return Emis{ex, ey, e * rl};
+}
+
+// ---------------------------------------------------------------------------
+// Band entries, one int32 per anti-diagonal: bits 31-11 xmyL (signed),
+// bits 10-0 width - 1 (wide jobs).  Narrow jobs (width <= 64) use bits 5-0
+// for width - 1, bit 6 = "this 8-diagonal cell needs the general exchange
+// going forward", bit 7 = the same going backward.
+// ---------------------------------------------------------------------------
+constexpr int kBandShift = 11;
+constexpr int kSlowF = 1 << 6, kSlowB = 1 << 7;
+constexpr int kMaxDiagonals = 1 << 20;  // per job (record coordinates use 20 bits)
+__host__ __device__ inline int band_L(int b) { return b >> kBandShift; }
+__host__ __device__ inline int band_wm1_narrow(int b) { return b & 63; }
+__host__ __device__ inline int band_wm1_wide(int b) { return b & 2047; }
+
+// Posterior record: (pM, pX, pY) as float bits and (d << 11) | cell index.
+__host__ __device__ inline uint32_t rec_code(int d, int cell) { return ((uint32_t)d << 11) | (uint32_t)cell; }
+
+constexpr int kNumClasses = 3;  // rows of 16, 32, 64 lanes
+constexpr int kK = 8;           // anti-diagonals per macro step (grid cell)
+constexpr int kSymPad = 128;    // padding entries around the symbol pool
+constexpr int kWideMaxC = 32;   // wide kernel: up to 64 * 32 cells per anti-diagonal
+constexpr int kMaxPairs = 8;    // forward/backward wavefront pairs per workgroup
+
+struct DevJob {
+    int64_t symX, symY, bandOff, outOff;
+    int32_t lX, lY, winOff, nWin;
+    int32_t cap[2];  // record capacity: forward side, backward side
+    int32_t flags;   // bit0 strand, bit1 ragged left, bit2 ragged right
+    int32_t pad;
+};
+
+// One traceback window (pairwiseAligner.c:715-783): forward to tb, backward
+// from tb to lo+1, posteriors on (lo, from]; cx = crossing diagonal of the
+// concurrent forward/backward sweeps.
+struct DevWin {
+    int32_t tb, lo, from, cx;
+};
+
+struct KParams {
+    const DevJob *jobs;
+    const uint32_t *sym;
+    const int *band;
+    const DevWin *win;
+    int4 *out;           // posterior records
+    int32_t *counts;     // per job: records (forward side, backward side)
+    int32_t *winEnd;     // per window: cumulative records at window end (forward side, backward side)
+    double *logProb;     // per job
+    float *ring;         // forward/backward values kept for the other sweep
+    int32_t *ringExp;    // per 8-diagonal cell exponents of those values
+    const float *tables;
+    int32_t *queue;      // kNumClasses + 1 counters (the last for wide jobs)
+    double *expect;      // per job: 9 expected transition counts + likelihood
+    int32_t *error;      // set when a wavefront gives up waiting for its partner (never in a correct run)
+    float *ringWide;
+    int32_t ringWideLen, wideC, wideBegin, wideEnd;
+    int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
+    int32_t ringLen;     // anti-diagonals per ring (multiple of kK)
+    int32_t withIndels;
+    int32_t mode;
+    float threshold;
+    float T[9];          // probability transitions, [from * 3 + to], M = 0, X = 1, Y = 2
+    float startV[2][3];  // [ragged][state]
+    float endV[2][3];
+};
+
+// ---------------------------------------------------------------------------
+// Wavefront helpers
+// ---------------------------------------------------------------------------
+template <int CTRL, bool BOUND = true>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, BOUND));
+}
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+constexpr int kRowShr1 = 0x111, kRowShl1 = 0x101, kWaveShr1 = 0x138, kWaveShl1 = 0x130;
+constexpr int kQuadSwap1 = 0xB1, kQuadSwap2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
+constexpr int kRowBcast0 = 0x150, kRowBcast15 = 0x15F;
+
+// Lane i <- lane i-1 / i+1 inside rows of SEG lanes; 0 where the source lane
+// is outside the row.
+template <int SEG>
+__device__ __forceinline__ float row_shr(float v, int k) {
+    if constexpr (SEG == 16) {
+        return dpp<kRowShr1>(v);
+    } else if constexpr (SEG == 64) {
+        return dpp<kWaveShr1>(v);
+    } else {
+        const float r = dpp<kWaveShr1>(v);
+        return k == 0 ? 0.f : r;
+    }
+}
+template <int SEG>
+__device__ __forceinline__ float row_shl(float v, int k) {
+    if constexpr (SEG == 16) {
+        return dpp<kRowShl1>(v);
+    } else if constexpr (SEG == 64) {
+        return dpp<kWaveShl1>(v);
+    } else {
+        const float r = dpp<kWaveShl1>(v);
+        return k == SEG - 1 ? 0.f : r;
+    }
+}
+
+// Reductions over a row of SEG lanes; every lane of the row receives the
+// same, bitwise identical result.
+template <int SEG>
+__device__ __forceinline__ float rmax(float v) {
+    v = fmaxf(v, dpp<kQuadSwap1, false>(v));
+    v = fmaxf(v, dpp<kQuadSwap2, false>(v));
+    v = fmaxf(v, dpp<kRowHalfMirror, false>(v));
+    v = fmaxf(v, dpp<kRowMirror, false>(v));
+    if constexpr (SEG >= 32) v = fmaxf(v, __shfl_xor(v, 16, 64));
+    if constexpr (SEG >= 64) v = fmaxf(v, __shfl_xor(v, 32, 64));
+    return v;
+}
+template <int SEG>
+__device__ __forceinline__ float rsum(float v) {
+    v += dpp<kQuadSwap1, false>(v);
+    v += dpp<kQuadSwap2, false>(v);
+    v += dpp<kRowHalfMirror, false>(v);
+    v += dpp<kRowMirror, false>(v);
+    if constexpr (SEG >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (SEG >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+// Value of lane `src` of the row (row-uniform broadcast).
+template <int SEG>
+__device__ __forceinline__ int rbcast(int v, int rowBase, int src) {
+    if constexpr (SEG == 16) {
+        return src == 0 ? dppi<kRowBcast0>(v) : dppi<kRowBcast15>(v);
+    } else {
+        return __shfl(v, rowBase + src, 64);
+    }
+}
+
+__device__ __forceinline__ float bperm(float v, int srcLane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(srcLane << 2, __float_as_int(v)));
+}
+
+// Neighbour input from the previous step's register v: the cell at xmy reads
+// the source diagonal's cell srcXmy (band [Lp, Rp]); 0 outside it.
+__device__ __forceinline__ float gather(float v, int srcXmy, int Lp, int Rp, bool act, int rowBase, int k) {
+    const bool ok = act && srcXmy >= Lp && srcXmy <= Rp;
+    const float r = bperm(v, rowBase + (ok ? ((srcXmy - Lp) >> 1) : k));
+    return ok ? r : 0.f;
+}
+
+// Consecutive idle iterations (every row of the wavefront waiting, each with
+// an s_sleep) after which a wavefront reports an error and leaves: about two
+// seconds, far beyond any legitimate wait for the partner wavefront.
+constexpr int kSpinLimit = 1 << 24;
+
+// s_waitcnt vmcnt(0): every vector memory operation of this wave has completed.
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+}  // namespace mpk
