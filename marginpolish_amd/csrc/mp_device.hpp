@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "mp_pairhmm.h"
 
@@ -66,16 +67,18 @@ __device__ __forceinline__ Emis emissions(const float *tab, uint32_t sx, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// Band entries, one int32 per anti-diagonal: bits 31-11 xmyL (signed),
-// bits 10-0 width - 1 (wide jobs).  Narrow jobs (width <= 64) use bits 5-0
-// for width - 1, bit 6 = "this 8-diagonal cell needs the general exchange
-// going forward", bit 7 = the same going backward.
+// Band entries, one int32 per anti-diagonal d: bits 31-11 xL = x of the
+// band's first cell (signed; xmyL = 2 xL - d), bits 10-0 width - 1 (wide
+// jobs).  Narrow jobs (width <= 64) use bits 7-0 for width - 1, bit 8 = "this
+// 8-diagonal cell needs the general neighbour exchange going forward",
+// bit 9 = the same going backward.  Cell k of diagonal d is (xL + k, d - xL - k).
 // ---------------------------------------------------------------------------
 constexpr int kBandShift = 11;
-constexpr int kSlowF = 1 << 6, kSlowB = 1 << 7;
+constexpr int kSlowF = 1 << 8, kSlowB = 1 << 9;
 constexpr int kMaxDiagonals = 1 << 20;  // per job (record coordinates use 20 bits)
-__host__ __device__ inline int band_L(int b) { return b >> kBandShift; }
-__host__ __device__ inline int band_wm1_narrow(int b) { return b & 63; }
+constexpr int kEmptyBand = INT32_MIN;   // entry of a missing anti-diagonal: no cell matches it
+__host__ __device__ inline int band_xL(int b) { return b >> kBandShift; }
+__host__ __device__ inline int band_wm1_narrow(int b) { return b & 255; }
 __host__ __device__ inline int band_wm1_wide(int b) { return b & 2047; }
 
 // Posterior record: (pM, pX, pY) as float bits and (d << 11) | cell index.
@@ -110,7 +113,7 @@ struct KParams {
     int4 *out;           // posterior records
     int32_t *counts;     // per job: records (forward side, backward side)
     int32_t *winEnd;     // per window: cumulative records at window end (forward side, backward side)
-    double *logProb;     // per job
+    int2 *logTotal;      // per job: total likelihood as (FP32 t, exponent S): log = ln t + S ln 2
     float *ring;         // forward/backward values kept for the other sweep
     int32_t *ringExp;    // per 8-diagonal cell exponents of those values
     const float *tables;
@@ -132,6 +135,14 @@ struct KParams {
 // ---------------------------------------------------------------------------
 // Wavefront helpers
 // ---------------------------------------------------------------------------
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 template <int CTRL, bool BOUND = true>
 __device__ __forceinline__ float dpp(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, BOUND));
@@ -181,6 +192,18 @@ __device__ __forceinline__ float rmax(float v) {
     if constexpr (SEG >= 64) v = fmaxf(v, __shfl_xor(v, 32, 64));
     return v;
 }
+// Row maximum of non-negative floats through their bit patterns (same order).
+template <int SEG>
+__device__ __forceinline__ float rmaxpos(float f) {
+    uint32_t v = __float_as_uint(f);
+    v = max(v, (uint32_t)dppi<kQuadSwap1>((int)v));
+    v = max(v, (uint32_t)dppi<kQuadSwap2>((int)v));
+    v = max(v, (uint32_t)dppi<kRowHalfMirror>((int)v));
+    v = max(v, (uint32_t)dppi<kRowMirror>((int)v));
+    if constexpr (SEG >= 32) v = max(v, (uint32_t)__shfl_xor((int)v, 16, 64));
+    if constexpr (SEG >= 64) v = max(v, (uint32_t)__shfl_xor((int)v, 32, 64));
+    return __uint_as_float(v);
+}
 template <int SEG>
 __device__ __forceinline__ float rsum(float v) {
     v += dpp<kQuadSwap1, false>(v);
@@ -205,11 +228,11 @@ __device__ __forceinline__ float bperm(float v, int srcLane) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(srcLane << 2, __float_as_int(v)));
 }
 
-// Neighbour input from the previous step's register v: the cell at xmy reads
-// the source diagonal's cell srcXmy (band [Lp, Rp]); 0 outside it.
-__device__ __forceinline__ float gather(float v, int srcXmy, int Lp, int Rp, bool act, int rowBase, int k) {
-    const bool ok = act && srcXmy >= Lp && srcXmy <= Rp;
-    const float r = bperm(v, rowBase + (ok ? ((srcXmy - Lp) >> 1) : k));
+// Neighbour input from another step's register v: this cell reads cell
+// `src` of the source anti-diagonal (cells 0..srcWm1); 0 outside it.
+__device__ __forceinline__ float gather(float v, int src, int srcWm1, bool act, int rowBase, int k) {
+    const bool ok = act && src >= 0 && src <= srcWm1;
+    const float r = bperm(v, rowBase + (ok ? src : k));
     return ok ? r : 0.f;
 }
 

@@ -131,7 +131,7 @@ struct mp_batch {
     DevBuf<int4> dOut;
     DevBuf<int32_t> dCounts;
     DevBuf<int32_t> dWinEnd;
-    DevBuf<double> dLogProb;
+    DevBuf<int2> dLogTotal;
     DevBuf<float> dRing;
     DevBuf<int32_t> dRingExp;
     DevBuf<float> dTables;
@@ -402,12 +402,22 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         b->pairNJobs[i] = (int32_t)ns;
     }
     sym.resize(sym.size() + kSymPad, enc_x(4, 0));
+    if (sym.size() >= ((size_t)1 << 30)) {
+        delete b;
+        mp::set_error("batch too large: more than 2^30 symbols (split it into several batches)");
+        return MP_EUNSUPPORTED;
+    }
 
     // ---- 2. bands, windows, classes (parallel over jobs) ----
     const int64_t nj = (int64_t)jobs.size();
     std::vector<int64_t> bandOff(nj + 1, 0);
     // Each job's band padded to whole cells of kK anti-diagonals plus one extra cell.
     for (int64_t j = 0; j < nj; j++) bandOff[j + 1] = bandOff[j] + ((jobs[j].lX + jobs[j].lY) / kK + 2) * kK;
+    if (bandOff[nj] >= ((int64_t)1 << 30)) {
+        delete b;
+        mp::set_error("batch too large: more than 2^30 anti-diagonals (split it into several batches)");
+        return MP_EUNSUPPORTED;
+    }
     std::vector<int32_t> &band = b->bandHost;
     band.assign((size_t)bandOff[nj], 0);
     std::vector<std::vector<DevWin>> wins(nj);
@@ -418,7 +428,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         std::atomic<int64_t> next{0};
         for (int t = 0; t < nth; t++)
             th.emplace_back([&]() {
-                std::vector<int32_t> L, R;
+                std::vector<int32_t> L, R, XL;
                 std::vector<mp::Window> w;
                 for (;;) {
                     int64_t j = next.fetch_add(1);
@@ -447,19 +457,23 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     hj.bandOff = bandOff[j];
                     int32_t *bj = band.data() + bandOff[j];
                     const bool narrow = hj.cls < 3;
+                    // xL = x of the band's first cell: (d + xmyL) / 2
+                    XL.resize(D + 1);
+                    for (int64_t d = 0; d <= D; d++) XL[d] = (int32_t)((d + L[d]) / 2);
                     for (int64_t d = 0; d <= D; d++) {
                         const int32_t wd = (R[d] - L[d]) / 2 + 1;
-                        bj[d] = (int32_t)((uint32_t)L[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
+                        bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
                     }
                     const int64_t padEnd = bandOff[j + 1] - bandOff[j];
                     for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
                     if (narrow) {
                         // cells whose neighbour exchange is not a one-lane shift
+                        auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
                         for (int64_t d = 0; d <= D; d++) {
-                            const bool fastF = d == 0 || (std::abs(L[d] - L[d - 1]) == 1 &&
-                                                          (d < 2 || std::abs(L[d] - L[d - 2]) <= 2));
-                            const bool fastB = d == D || (std::abs(L[d] - L[d + 1]) == 1 &&
-                                                          (d + 2 > D || std::abs(L[d] - L[d + 2]) <= 2));
+                            const bool fastF =
+                                d == 0 || (in(XL[d] - XL[d - 1], 0, 1) && (d < 2 || in(XL[d] - XL[d - 2], 0, 2)));
+                            const bool fastB =
+                                d == D || (in(XL[d] - XL[d + 1], -1, 0) && (d + 2 > D || in(XL[d] - XL[d + 2], -2, 0)));
                             const int64_t c0 = d & ~(int64_t)(kK - 1);
                             if (!fastF)
                                 for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowF;
@@ -580,7 +594,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
-        (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogProb.alloc(std::max<size_t>(dj.size(), 1))) ||
+        (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
         (st = b->dQueue.alloc(kNumClasses + 1)) || (st = b->dError.alloc(1)) ||
         (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1))) {
         delete b;
@@ -593,7 +607,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     if (anyNarrow) {
         const size_t waves = (size_t)b->narrowBlocks * 8;
         if (split) {
-            if ((st = b->dRing.alloc(waves * (size_t)ringLen * 64 * 3)) ||
+            if ((st = b->dRing.alloc(waves * (size_t)(ringLen / kK) * (kRingCellBytes / 4))) ||
                 (st = b->dRingExp.alloc(waves * (size_t)(ringLen / kK) * 4))) {
                 delete b;
                 return st;
@@ -622,7 +636,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.out = b->dOut.p;
     kp.counts = b->dCounts.p;
     kp.winEnd = b->dWinEnd.p;
-    kp.logProb = b->dLogProb.p;
+    kp.logTotal = b->dLogTotal.p;
     kp.ring = b->dRing.p;
     kp.ringExp = b->dRingExp.p;
     kp.ringWide = b->dRingWide.p;
@@ -773,12 +787,18 @@ static mp_status materialize(mp_batch *b) {
     }
     b->pairLogProb.assign(np, 0.0);
     std::vector<double> lp(std::max(nj, 1), 0.0);
+    std::vector<int2> lt(std::max(nj, 1));
     std::vector<int32_t> wend(2 * std::max<size_t>(b->winHost.size(), 1), 0);
     std::vector<int4> rec;
     std::vector<int64_t> dense(2 * (size_t)std::max(nj, 1) + 1, 0);
     if (nj > 0) {
         HIP_TRY(hipSetDevice(b->eng->device));
-        HIP_TRY(hipMemcpy(lp.data(), b->dLogProb.p, sizeof(double) * nj, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(lt.data(), b->dLogTotal.p, sizeof(int2) * nj, hipMemcpyDeviceToHost));
+        for (int32_t j = 0; j < nj; j++) {
+            float t;
+            std::memcpy(&t, &lt[j].x, sizeof(float));
+            lp[j] = std::log((double)t) + (double)lt[j].y * 0.69314718055994530942;
+        }
         const bool post = b->mode <= MP_MODE_MATCH_ONLY;
         if (post) {
             if (!b->winHost.empty())
@@ -828,8 +848,7 @@ static mp_status materialize(mp_batch *b) {
             auto emitRec = [&](const int4 &r) {
                 const uint32_t code = (uint32_t)r.w;
                 const int d = (int)(code >> 11), cellIdx = (int)(code & 2047u);
-                const int xmy = band_L(bj[d]) + 2 * cellIdx;
-                const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
+                const int x = band_xL(bj[d]) + cellIdx, y = d - x;
                 float pv[3];
                 std::memcpy(pv, &r, sizeof(pv));
                 const bool ok[3] = {pv[0] >= thr && x > 0 && y > 0, withIndels && pv[1] >= thr && x > 0,

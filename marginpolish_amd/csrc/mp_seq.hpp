@@ -7,7 +7,7 @@
 // (pairwiseAligner.c:347-369, 666-677, 1151-1164).
 #pragma once
 
-#include "mp_device.hpp"
+#include "mp_split.hpp"
 
 namespace mpk {
 
@@ -21,9 +21,9 @@ enum { Q_FETCH = 0, Q_FWD = 1, Q_BWD = 2, Q_DONE = 3 };
 // traceback window are kept in an HBM ring (float4: 3 states + the cell's
 // exponent) for the backward sweep.
 template <int SEG>
-__device__ void seq_class(const KParams &kp, const float *tab, int cls) {
+__device__ __forceinline__ void seq_class(const KParams &kp, const float *tab, int cls) {
     constexpr int K = kK;
-    constexpr int kEmpty = INT32_MIN;
+    constexpr int kEmpty = kEmptyBand;
     const int lane = threadIdx.x & 63;
     const int k = lane & (SEG - 1);
     const int rowBase = lane & ~(SEG - 1);
@@ -45,9 +45,9 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
     bool init = false, reload = true;
     float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
     float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
-    int Lm1 = band_L(kEmpty), Lm2 = band_L(kEmpty), bm1 = kEmpty, bm2 = kEmpty;
+    int bm1 = kEmpty, bm2 = kEmpty;
     float fX1 = 0.f, fY1 = 0.f, fM1 = 0.f, fM2 = 0.f;
-    int fLm1 = 0, fLm2 = 0, fbm1 = kEmpty, fbm2 = kEmpty, fS = 0, fTb = 0;
+    int fbm1 = kEmpty, fbm2 = kEmpty, fS = 0, fTb = 0;
     int bC[K], bN[K];
     uint32_t sxC[K], syC[K];
     float4 rC[K];
@@ -89,7 +89,6 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                 reload = true;
                 oX1 = oY1 = oM1 = oM2 = 0.f;
                 bm1 = bm2 = kEmpty;
-                Lm1 = Lm2 = band_L(kEmpty);
                 const bool rg = (flags >> 1) & 1;
                 iv0 = rg ? kp.startV[1][0] : kp.startV[0][0];
                 iv1 = rg ? kp.startV[1][1] : kp.startV[0][1];
@@ -116,9 +115,9 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
             const float4 *rb = ringRow + (size_t)((gs % R) * SEG);
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                const int dj = gs + j, xmy = band_L(bC[j]) + 2 * k;
-                sxC[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
-                syC[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
+                const int dj = gs + j, x = band_xL(bC[j]) + k;
+                sxC[j] = kp.sym[symX + x - 1];
+                syC[j] = kp.sym[symY + (dj - x) - 1];
                 rC[j] = rb[(size_t)j * SEG];
             }
             reload = false;
@@ -134,9 +133,9 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
             const float4 *rb = ringRow + (size_t)(((cN * K) % R) * SEG);
 #pragma unroll
             for (int j = 0; j < K; j++) {
-                const int dj = cN * K + j, xmy = band_L(bN[j]) + 2 * k;
-                sxN[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
-                syN[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
+                const int dj = cN * K + j, x = band_xL(bN[j]) + k;
+                sxN[j] = kp.sym[symX + x - 1];
+                syN[j] = kp.sym[symY + (dj - x) - 1];
                 rN[j] = rb[(size_t)j * SEG];
             }
         }
@@ -155,25 +154,12 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                 const int dj = gs + j;
                 if (dj >= d && dj <= tb) {
                     const int b = bC[j];
-                    const int L0 = band_L(b);
+                    const int xL = band_xL(b);
                     const bool act = k <= band_wm1_narrow(b);
                     const Emis e = emissions(tab, sxC[j], syC[j]);
                     float inX, inY, inM;
-                    if (!slow) {
-                        const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
-                        const float xs = row_shr<SEG>(oX1, k), ys = row_shl<SEG>(oY1, k);
-                        const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
-                        inX = dL1 < 0 ? xs : oX1;
-                        inY = dL1 > 0 ? ys : oY1;
-                        inM = dL2 < 0 ? ml : (dL2 > 0 ? mr : oM2);
-                    } else {
-                        const int xmy = L0 + 2 * k;
-                        const int Rm1 = bm1 == kEmpty ? Lm1 - 2 : Lm1 + 2 * band_wm1_narrow(bm1);
-                        const int Rm2 = bm2 == kEmpty ? Lm2 - 2 : Lm2 + 2 * band_wm1_narrow(bm2);
-                        inX = gather(oX1, xmy - 1, Lm1, Rm1, act, rowBase, k);
-                        inY = gather(oY1, xmy + 1, Lm1, Rm1, act, rowBase, k);
-                        inM = gather(oM2, xmy, Lm2, Rm2, act, rowBase, k);
-                    }
+                    if (!slow) fwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+                    else fwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
                     float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
                     if (j == 0 && init) {
                         p0 = iv0;
@@ -182,17 +168,17 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                     }
                     if (!act) p0 = p1 = p2 = 0.f;
                     if (j == 0 && dj == d) {
-                        const float m = rmax<SEG>(fmaxf(p0, fmaxf(p1, p2)));
-                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                        const float m = rmaxpos<SEG>(fmaxf(p0, fmaxf(p1, p2)));
+                        const int e2 = __builtin_amdgcn_frexp_expf(m);
                         p0 = ldexpf(p0, -e2);
                         p1 = ldexpf(p1, -e2);
                         p2 = ldexpf(p2, -e2);
                         oM1 = ldexpf(oM1, -e2);
                         S += e2;
                     }
-                    const float q0 = T00 * p0 + T10 * p1 + T20 * p2;
-                    const float q1 = T01 * p0 + T11 * p1 + T21 * p2;
-                    const float q2 = T02 * p0 + T12 * p1 + T22 * p2;
+                    const float q0 = fmaf(T20, p2, fmaf(T10, p1, T00 * p0));
+                    const float q1 = fmaf(T21, p2, fmaf(T11, p1, T01 * p0));
+                    const float q2 = fmaf(T22, p2, fmaf(T12, p1, T02 * p0));
                     if (expMode) ringCell[(size_t)j * SEG] = make_float4(p0, p1, p2, __int_as_float(S));
                     if (dj == tb) {
                         sv0 = p0;
@@ -204,9 +190,7 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                     oX1 = q1;
                     oY1 = q2;
                     oM1 = q0;
-                    Lm2 = Lm1;
                     bm2 = bm1;
-                    Lm1 = L0;
                     bm1 = b;
                 }
             }
@@ -216,29 +200,16 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                 const int dj = gs + j;
                 if (dj <= d && dj >= lo) {
                     const int b = bC[j];
-                    const int L0 = band_L(b);
+                    const int xL = band_xL(b);
                     const bool act = k <= band_wm1_narrow(b);
                     const int Sin = S;
                     const Emis e = emissions(tab, sxC[j], syC[j]);
                     float inX, inY, inM;
-                    if (!slow) {
-                        const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
-                        const float xs = row_shl<SEG>(oX1, k), ys = row_shr<SEG>(oY1, k);
-                        const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
-                        inX = dL1 > 0 ? xs : oX1;
-                        inY = dL1 < 0 ? ys : oY1;
-                        inM = dL2 < 0 ? ml : (dL2 > 0 ? mr : oM2);
-                    } else {
-                        const int xmy = L0 + 2 * k;
-                        const int Rm1 = bm1 == kEmpty ? Lm1 - 2 : Lm1 + 2 * band_wm1_narrow(bm1);
-                        const int Rm2 = bm2 == kEmpty ? Lm2 - 2 : Lm2 + 2 * band_wm1_narrow(bm2);
-                        inX = gather(oX1, xmy + 1, Lm1, Rm1, act, rowBase, k);
-                        inY = gather(oY1, xmy - 1, Lm1, Rm1, act, rowBase, k);
-                        inM = gather(oM2, xmy, Lm2, Rm2, act, rowBase, k);
-                    }
-                    float q0 = T00 * inM + T01 * inX + T02 * inY;
-                    float q1 = T10 * inM + T11 * inX + T12 * inY;
-                    float q2 = T20 * inM + T21 * inX + T22 * inY;
+                    if (!slow) bwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+                    else bwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
+                    float q0 = fmaf(T02, inY, fmaf(T01, inX, T00 * inM));
+                    float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
+                    float q2 = fmaf(T22, inY, fmaf(T21, inX, T20 * inM));
                     if (init && dj == d) {
                         q0 = iv0;
                         q1 = iv1;
@@ -246,8 +217,8 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                     }
                     if (!act) q0 = q1 = q2 = 0.f;
                     if (j == K - 1 && dj == d && !init) {
-                        const float m = rmax<SEG>(fmaxf(q0, fmaxf(q1, q2)));
-                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
+                        const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
+                        const int e2 = __builtin_amdgcn_frexp_expf(m);
                         q0 = ldexpf(q0, -e2);
                         q1 = ldexpf(q1, -e2);
                         q2 = ldexpf(q2, -e2);
@@ -271,9 +242,7 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                     oX1 = e.eX * q1;
                     oY1 = e.eY * q2;
                     oM1 = e.eM * q0;
-                    Lm2 = Lm1;
                     bm2 = bm1;
-                    Lm1 = L0;
                     bm1 = b;
                 }
             }
@@ -302,7 +271,7 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
             SFtb = S;
             totalInv = t > 0.f ? 1.f / t : 0.f;
             const double lt = log((double)t) + (double)SFtb * 0.69314718055994530942;
-            if (atEnd && k == 0) kp.logProb[job] = lt;
+            if (atEnd && k == 0) kp.logTotal[job] = make_int2(__float_as_int(t), SFtb);
             // getExpectations adds the window total once per posterior diagonal (pairwiseAligner.c:674)
             likeAcc += (double)(from - lo) * lt;
             if (!expMode) {
@@ -313,10 +282,9 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                 iv1 = e1;
                 iv2 = e2;
                 fX1 = oX1; fY1 = oY1; fM1 = oM1; fM2 = oM2;
-                fLm1 = Lm1; fLm2 = Lm2; fbm1 = bm1; fbm2 = bm2; fS = S; fTb = tb;
+                fbm1 = bm1; fbm2 = bm2; fS = S; fTb = tb;
                 oX1 = oY1 = oM1 = oM2 = 0.f;
                 bm1 = bm2 = kEmpty;
-                Lm1 = Lm2 = band_L(kEmpty);
                 S = 0;
                 init = true;
                 reload = true;
@@ -340,7 +308,7 @@ __device__ void seq_class(const KParams &kp, const float *tab, int cls) {
                 lo = w1.lo;
                 from = w1.from;
                 oX1 = fX1; oY1 = fY1; oM1 = fM1; oM2 = fM2;
-                Lm1 = fLm1; Lm2 = fLm2; bm1 = fbm1; bm2 = fbm2; S = fS;
+                bm1 = fbm1; bm2 = fbm2; S = fS;
                 reload = true;
                 phase = Q_FWD;
             }

@@ -21,6 +21,12 @@
 // forward values; both continue their recursions.  A window's critical path
 // is one sweep instead of two.
 //
+// Anti-diagonals are walked in grid-aligned macro steps of 8 (one "cell" of
+// the grid d = 8g..8g+7); every per-step operand (band entry, encoded
+// symbols, partner values) is a register loaded one macro step ahead (the
+// band two).  Macro steps that cover all 8 anti-diagonals of every running
+// row with one-lane neighbour shifts (the common case) run a branch-free body.
+//
 // Output: one record per cell with any posterior >= threshold:
 // (pM, pX, pY, (d << 11) | cell).  The forward side writes each anti-diagonal's
 // cells in descending order, the backward side in ascending order, so that
@@ -42,6 +48,9 @@ struct SplitShared {
 constexpr int kSplitLdsBytes =
     ((kTabFloats * 4 + (int)sizeof(SplitShared)) > kTabReach ? (kTabFloats * 4 + (int)sizeof(SplitShared))
                                                               : kTabReach);
+// Ring geometry: [wave slot][cell][64 lanes][8 anti-diagonals] of 3 floats.
+constexpr int kRingLaneBytes = kK * 12;              // 96
+constexpr int kRingCellBytes = 64 * kRingLaneBytes;  // 6144
 
 enum { S_FETCH = 0, S_RUN = 1, S_XWAIT = 2, S_JWAIT = 3, S_EXIT = 4 };
 
@@ -56,29 +65,89 @@ struct F3 {
     float x, y, z;
 };
 
+// Neighbour inputs.  Cell k of anti-diagonal d is (x, y) = (xL + k, d - xL - k).
+// Forward: X from (x-1, y) and Y from (x, y-1) on d-1, M from (x-1, y-1) on d-2.
+// Backward: X from (x+1, y) and Y from (x, y+1) on d+1, M from (x+1, y+1) on d+2.
+// xL1 / xL2 are the band starts of those diagonals; the fast forms assume
+// the one-lane shifts that hold on every cell flagged fast by the host.
 template <int SEG>
-__device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh, int cls) {
+__device__ __forceinline__ void fwd_in_fast(int xL, int xL1, int xL2, float oX1, float oY1, float oM2, int k,
+                                            float &inX, float &inY, float &inM) {
+    const bool same1 = xL == xL1;  // X from lane k-1, Y from lane k; else X from k, Y from k+1
+    const int dm = xL - xL2;       // M from lane k-1 (0), k (1), k+1 (2)
+    const float xs = row_shr<SEG>(oX1, k), ys = row_shl<SEG>(oY1, k);
+    const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
+    inX = same1 ? xs : oX1;
+    inY = same1 ? oY1 : ys;
+    inM = dm == 0 ? ml : (dm == 2 ? mr : oM2);
+}
+__device__ __forceinline__ void fwd_in_slow(int xL, int b1, int b2, float oX1, float oY1, float oM2, bool act,
+                                            int rowBase, int k, float &inX, float &inY, float &inM) {
+    const int xL1 = band_xL(b1), xL2 = band_xL(b2);
+    inX = gather(oX1, k + xL - xL1 - 1, band_wm1_narrow(b1), act, rowBase, k);
+    inY = gather(oY1, k + xL - xL1, band_wm1_narrow(b1), act, rowBase, k);
+    inM = gather(oM2, k + xL - xL2 - 1, band_wm1_narrow(b2), act, rowBase, k);
+}
+template <int SEG>
+__device__ __forceinline__ void bwd_in_fast(int xL, int xL1, int xL2, float oX1, float oY1, float oM2, int k,
+                                            float &inX, float &inY, float &inM) {
+    const bool same1 = xL == xL1;  // X from lane k+1, Y from lane k; else X from k, Y from k-1
+    const int dm = xL - xL2;       // M from lane k-1 (-2), k (-1), k+1 (0)
+    const float xs = row_shl<SEG>(oX1, k), ys = row_shr<SEG>(oY1, k);
+    const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
+    inX = same1 ? xs : oX1;
+    inY = same1 ? oY1 : ys;
+    inM = dm == -2 ? ml : (dm == 0 ? mr : oM2);
+}
+__device__ __forceinline__ void bwd_in_slow(int xL, int b1, int b2, float oX1, float oY1, float oM2, bool act,
+                                            int rowBase, int k, float &inX, float &inY, float &inM) {
+    const int xL1 = band_xL(b1), xL2 = band_xL(b2);
+    inX = gather(oX1, k + xL - xL1 + 1, band_wm1_narrow(b1), act, rowBase, k);
+    inY = gather(oY1, k + xL - xL1, band_wm1_narrow(b1), act, rowBase, k);
+    inM = gather(oM2, k + xL - xL2 + 1, band_wm1_narrow(b2), act, rowBase, k);
+}
+
+template <int SEG>
+__device__ __forceinline__ void split_class(const KParams &kp, const float *tab, SplitShared *sh, int cls) {
     constexpr int K = kK;
-    constexpr int kEmpty = INT32_MIN;  // band entry of a missing anti-diagonal (no cell can match)
+    // Every parameter is read once here: the loop's acquire loads would otherwise
+    // force the compiler to re-read the parameter block after each of them.
+    const DevJob *const gJobs = kp.jobs;
+    const DevWin *const gWin = kp.win;
+    int4 *const gOut = kp.out;
+    int32_t *const gCounts = kp.counts;
+    int32_t *const gWinEnd = kp.winEnd;
+    int2 *const gLogTotal = kp.logTotal;
+    int32_t *const gQueue = kp.queue + cls;
+    int32_t *const gError = kp.error;
+    const int gJobBegin = kp.jobBegin[cls], gJobEnd = kp.jobEnd[cls];
+    const float sv00 = kp.startV[0][0], sv01 = kp.startV[0][1], sv02 = kp.startV[0][2];
+    const float sv10 = kp.startV[1][0], sv11 = kp.startV[1][1], sv12 = kp.startV[1][2];
+    const float ev00 = kp.endV[0][0], ev01 = kp.endV[0][1], ev02 = kp.endV[0][2];
+    const float ev10 = kp.endV[1][0], ev11 = kp.endV[1][1], ev12 = kp.endV[1][2];
     const int lane = threadIdx.x & 63;
     const int k = lane & (SEG - 1);
     const int row = lane / SEG;
     const int rowBase = lane & ~(SEG - 1);
-    const int wib = threadIdx.x >> 6;
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int pr = wib >> 1;
     const bool bwd = (wib & 1) != 0;  // wave-uniform role
     const size_t slot = (size_t)blockIdx.x * (blockDim.x >> 6) + wib;
-    const int R = kp.ringLen;
-    const int RC = R / K;
-    // ring: [slot][R anti-diagonals][64 lanes] of 3 floats; exponents: [slot][R/8 cells][4 rows]
-    F3 *myRing = reinterpret_cast<F3 *>(kp.ring) + slot * (size_t)R * 64 + lane;
-    const F3 *peerRing = reinterpret_cast<const F3 *>(kp.ring) + (slot ^ 1) * (size_t)R * 64 + lane;
-    int *myExp = kp.ringExp + slot * (size_t)RC * 4 + row;
-    const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4 + row;
+    const int RC = kp.ringLen / K;
+    char *myRing = reinterpret_cast<char *>(kp.ring) + slot * (size_t)RC * kRingCellBytes;
+    const char *peerRing = reinterpret_cast<const char *>(kp.ring) + (slot ^ 1) * (size_t)RC * kRingCellBytes;
+    int *myExp = kp.ringExp + slot * (size_t)RC * 4;
+    const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4;
+    const uint32_t laneRing = (uint32_t)lane * kRingLaneBytes;
+    const char *symB = reinterpret_cast<const char *>(kp.sym);
+    const char *bandB = reinterpret_cast<const char *>(kp.band);
     const float T00 = kp.T[0], T01 = kp.T[1], T02 = kp.T[2], T10 = kp.T[3], T11 = kp.T[4], T12 = kp.T[5],
                 T20 = kp.T[6], T21 = kp.T[7], T22 = kp.T[8];
-    const float thr = kp.threshold;
-    const bool withIndels = kp.withIndels != 0;
+    // wave-uniform thresholds (scalar registers): a record needs pM >= thr or,
+    // with indels, pX or pY >= thr
+    const float thr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kp.threshold)));
+    const float thrXY = __int_as_float(
+        __builtin_amdgcn_readfirstlane(kp.withIndels ? __float_as_int(kp.threshold) : 0x7f800000));
     int *xflagMine = &sh->xflag[cls][pr][bwd ? 1 : 0][row];
     int *xflagPeer = &sh->xflag[cls][pr][bwd ? 0 : 1][row];
     int *jobSeqP = &sh->jobSeq[cls][pr][row];
@@ -88,14 +157,15 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
     int phase = bwd ? S_JWAIT : S_FETCH;
     int jn = 0, xn = 0;
     int job = -1, D = 0, lastCell = 0, nWin = 0, winOff = 0, jflags = 0, cap = 0, win = 0;
-    int64_t symX = 0, symY = 0, bandOff = 0, outBase = 0;
-    int tb = 0, lo = 0, from = 0, cx = 0;
-    int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, rpos = 0, cpos = 0;
+    uint32_t xrow = 0, yrow = 0, bandRow = 0;  // byte offsets: x/y symbols of cell k, the job's band
+    int64_t outBase = 0;
+    int tb = 0, lo = 0, from = 0, cx = 0, postFrom = -1;
+    int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, cpos = 0;
     bool post = false, init = false, reload = true;
     float totalInv = 0.f;
     float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
     float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
-    int Lm1 = 0, Lm2 = 0, bm1 = kEmpty, bm2 = kEmpty;
+    int bm1 = kEmptyBand, bm2 = kEmptyBand;  // band entries of the previous two processed diagonals
     int bC[K], bN[K];
     uint32_t sxC[K], syC[K];
     F3 rC[K];
@@ -107,13 +177,33 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         rC[j] = F3{0.f, 0.f, 0.f};
     }
 
+    auto bandCell = [&](int c, int *dst) {
+        const int4 *p = reinterpret_cast<const int4 *>(bandB + bandRow + (uint32_t)c * 32u);
+        const int4 a = p[0], b = p[1];
+        dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+        dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
+    };
+    auto symCell = [&](int c, const int *bb, uint32_t *sx, uint32_t *sy) {
+        const uint32_t yg = yrow + (uint32_t)c * (K * 4);
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int xL = band_xL(bb[j]);
+            sx[j] = *reinterpret_cast<const uint32_t *>(symB + (xrow + ((uint32_t)xL << 2)));
+            sy[j] = *reinterpret_cast<const uint32_t *>(symB + (yg - ((uint32_t)xL << 2)) + j * 4);
+        }
+    };
+    auto peerCell = [&](int cp, F3 *r) {
+        const float4 *p = reinterpret_cast<const float4 *>(peerRing + ((uint32_t)cp * kRingCellBytes + laneRing));
+        const float4 a = p[0], b = p[1], c = p[2], e = p[3], f = p[4], g = p[5];
+        r[0] = F3{a.x, a.y, a.z}; r[1] = F3{a.w, b.x, b.y}; r[2] = F3{b.z, b.w, c.x}; r[3] = F3{c.y, c.z, c.w};
+        r[4] = F3{e.x, e.y, e.z}; r[5] = F3{e.w, f.x, f.y}; r[6] = F3{f.z, f.w, g.x}; r[7] = F3{g.y, g.z, g.w};
+    };
     auto resetExchange = [&]() {
         oX1 = oY1 = oM1 = oM2 = 0.f;
-        bm1 = bm2 = kEmpty;
-        Lm1 = Lm2 = band_L(kEmpty);
+        bm1 = bm2 = kEmptyBand;
     };
     auto loadWin = [&]() {
-        const DevWin w = kp.win[winOff + win];
+        const DevWin w = gWin[winOff + win];
         tb = w.tb;
         lo = w.lo;
         from = w.from;
@@ -125,18 +215,19 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         S = 0;
         init = true;
         post = false;
+        postFrom = -1;
         reload = true;
         resetExchange();
         const bool rg = tb == D && ((jflags >> 2) & 1);
-        iv0 = rg ? kp.endV[1][0] : kp.endV[0][0];
-        iv1 = rg ? kp.endV[1][1] : kp.endV[0][1];
-        iv2 = rg ? kp.endV[1][2] : kp.endV[0][2];
+        iv0 = rg ? ev10 : ev00;
+        iv1 = rg ? ev11 : ev01;
+        iv2 = rg ? ev12 : ev02;
     };
     auto setupJob = [&]() {
-        const DevJob jb = kp.jobs[job];
-        symX = jb.symX;
-        symY = jb.symY;
-        bandOff = jb.bandOff;
+        const DevJob jb = gJobs[job];
+        xrow = (uint32_t)(jb.symX - 1 + k) * 4u;
+        yrow = (uint32_t)(jb.symY - 1 - k) * 4u;
+        bandRow = (uint32_t)jb.bandOff * 4u;
         D = jb.lX + jb.lY;
         lastCell = D / K + 1;  // bands are padded by one extra cell
         winOff = jb.winOff;
@@ -153,12 +244,13 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
             S = 0;
             init = true;
             post = false;
+            postFrom = -1;
             reload = true;
             resetExchange();
             const bool rg = (jflags >> 1) & 1;
-            iv0 = rg ? kp.startV[1][0] : kp.startV[0][0];
-            iv1 = rg ? kp.startV[1][1] : kp.startV[0][1];
-            iv2 = rg ? kp.startV[1][2] : kp.startV[0][2];
+            iv0 = rg ? sv10 : sv00;
+            iv1 = rg ? sv11 : sv01;
+            iv2 = rg ? sv12 : sv02;
         } else {
             startBwdWindow();
         }
@@ -169,16 +261,15 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
     auto publishCrossing = [&]() {
         float v0, v1, v2, v3;
         if (!bwd) {
-            const int *bandJ = kp.band + bandOff;
+            const int *bandJ = reinterpret_cast<const int *>(bandB + bandRow);
             const int b1 = bandJ[cx + 1], b2 = bandJ[cx + 2];
-            const int L1 = band_L(b1), L2 = band_L(b2);
+            const int xL1 = band_xL(b1), xL2 = band_xL(b2);
             const bool a1 = k <= band_wm1_narrow(b1), a2 = k <= band_wm1_narrow(b2) && cx + 2 <= D;
-            const int Rm1 = bm1 == kEmpty ? Lm1 - 2 : Lm1 + 2 * band_wm1_narrow(bm1);
-            const int Rm2 = bm2 == kEmpty ? Lm2 - 2 : Lm2 + 2 * band_wm1_narrow(bm2);
-            v0 = gather(oX1, L1 + 2 * k - 1, Lm1, Rm1, a1, rowBase, k);
-            v1 = gather(oY1, L1 + 2 * k + 1, Lm1, Rm1, a1, rowBase, k);
-            v2 = gather(oM2, L1 + 2 * k, Lm2, Rm2, a1, rowBase, k);
-            v3 = gather(oM1, L2 + 2 * k, Lm1, Rm1, a2, rowBase, k);
+            const int xc = band_xL(bm1), xcm = band_xL(bm2), wc = band_wm1_narrow(bm1), wcm = band_wm1_narrow(bm2);
+            v0 = gather(oX1, k + xL1 - xc - 1, wc, a1, rowBase, k);
+            v1 = gather(oY1, k + xL1 - xc, wc, a1, rowBase, k);
+            v2 = gather(oM2, k + xL1 - xcm - 1, wcm, a1, rowBase, k);
+            v3 = gather(oM1, k + xL2 - xc - 1, wc, a2, rowBase, k);
         } else {
             v0 = oX1;
             v1 = oY1;
@@ -196,10 +287,10 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         phase = S_XWAIT;
     };
     auto bwdWindowEnd = [&]() {
-        if (k == 0) kp.winEnd[2 * (winOff + win) + 1] = cnt;
+        if (k == 0) gWinEnd[2 * (winOff + win) + 1] = cnt;
         if (win + 1 == nWin) {
             if (k == 0) {
-                kp.counts[2 * job + 1] = cnt;
+                gCounts[2 * job + 1] = cnt;
                 lds_release(doneBP, jn);
             }
             phase = S_JWAIT;
@@ -215,10 +306,10 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         // ---------------- per-row protocol ----------------
         if (phase == S_FETCH) {  // forward rows only
             int jnew = 0;
-            if (k == 0) jnew = atomicAdd(&kp.queue[cls], 1);
+            if (k == 0) jnew = atomicAdd(gQueue, 1);
             jnew = rbcast<SEG>(jnew, rowBase, 0);
-            job = kp.jobBegin[cls] + jnew;
-            if (job >= kp.jobEnd[cls]) job = -1;
+            job = gJobBegin + jnew;
+            if (job >= gJobEnd) job = -1;
             if (k == 0) {
                 *jobIdP = job;
                 lds_release(jobSeqP, jn + 1);
@@ -244,10 +335,10 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
                                           sh->xv[pr][sl][0][3][lane] * sh->xv[pr][sl][1][3][lane]);
                 Stot = sh->xexp[pr][sl][0][row] + sh->xexp[pr][sl][1][row];
                 totalInv = t > 0.f ? 1.f / t : 0.f;
-                if (!bwd && tb == D && k == 0)
-                    kp.logProb[job] = log((double)t) + (double)Stot * 0.69314718055994530942;
+                if (!bwd && tb == D && k == 0) gLogTotal[job] = make_int2(__float_as_int(t), Stot);
                 xn += 1;
                 post = true;
+                postFrom = from;
                 reload = true;
                 phase = S_RUN;
                 if (!bwd) {
@@ -262,8 +353,8 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         if (__all(phase == S_EXIT)) break;
         const bool running = phase == S_RUN;
         if (!__any(running)) {
-            if (++idle > kSpinLimit || __hip_atomic_load(kp.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                if (lane == 0) atomicExch(kp.error, 1);
+            if (++idle > kSpinLimit || __hip_atomic_load(gError, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                if (lane == 0) atomicExch(gError, 1);
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -271,230 +362,216 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         }
         idle = 0;
         if (!running) continue;
+        const int stop = bwd ? (post ? lo + 1 : cx + 1) : (post ? tb : cx);
         // a macro step that cannot reach the row's stop diagonal means a broken
         // invariant: report it instead of walking off the band
-        {
-            const int stopNow = bwd ? (post ? lo + 1 : cx + 1) : (post ? tb : cx);
-            if (bwd ? (gs + K - 1 < stopNow || gs < 0) : (gs > stopNow)) {
-                atomicExch(kp.error, 2);
-                phase = S_EXIT;
-                continue;
-            }
+        if (bwd ? (gs + K - 1 < stop || gs < 0) : (gs > stop)) {
+            atomicExch(gError, 2);
+            phase = S_EXIT;
+            continue;
         }
 
         // ---------------- one macro step (8 anti-diagonals of one grid cell) ----------------
         const int cell = gs / K;
-        const int *bandJ = kp.band + bandOff;
         const int cN = bwd ? max(cell - 1, 0) : min(cell + 1, lastCell);
         const int cN2 = bwd ? max(cell - 2, 0) : min(cell + 2, lastCell);
         if (reload) {
-            rpos = gs % R;
             cpos = cell % RC;
-            const int4 *b4 = reinterpret_cast<const int4 *>(bandJ + (size_t)cell * K);
-            const int4 *n4 = reinterpret_cast<const int4 *>(bandJ + (size_t)cN * K);
-            const int4 c0 = b4[0], c1 = b4[1], n0 = n4[0], n1 = n4[1];
-            bC[0] = c0.x; bC[1] = c0.y; bC[2] = c0.z; bC[3] = c0.w;
-            bC[4] = c1.x; bC[5] = c1.y; bC[6] = c1.z; bC[7] = c1.w;
-            bN[0] = n0.x; bN[1] = n0.y; bN[2] = n0.z; bN[3] = n0.w;
-            bN[4] = n1.x; bN[5] = n1.y; bN[6] = n1.z; bN[7] = n1.w;
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int dj = gs + j, xmy = band_L(bC[j]) + 2 * k;
-                sxC[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
-                syC[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
-                rC[j] = peerRing[(size_t)(rpos + j) * 64];
-            }
-            eC = peerExp[cpos * 4];
+            bandCell(cell, bC);
+            bandCell(cN, bN);
+            symCell(cell, bC, sxC, syC);
+            peerCell(cpos, rC);
+            eC = peerExp[cpos * 4 + row];
             reload = false;
         }
-        // prefetch: band two cells ahead; symbols, partner values and partner
-        // exponent of the next cell
-        const int rposN = bwd ? (rpos == 0 ? R - K : rpos - K) : (rpos + K == R ? 0 : rpos + K);
         const int cposN = bwd ? (cpos == 0 ? RC - 1 : cpos - 1) : (cpos + 1 == RC ? 0 : cpos + 1);
         int bN2[K];
-        uint32_t sxN[K], syN[K];
-        F3 rN[K];
-        int eN;
-        {
-            const int4 *n4 = reinterpret_cast<const int4 *>(bandJ + (size_t)cN2 * K);
-            const int4 n0 = n4[0], n1 = n4[1];
-            bN2[0] = n0.x; bN2[1] = n0.y; bN2[2] = n0.z; bN2[3] = n0.w;
-            bN2[4] = n1.x; bN2[5] = n1.y; bN2[6] = n1.z; bN2[7] = n1.w;
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int dj = cN * K + j, xmy = band_L(bN[j]) + 2 * k;
-                sxN[j] = kp.sym[symX + ((dj + xmy) >> 1) - 1];
-                syN[j] = kp.sym[symY + ((dj - xmy) >> 1) - 1];
-                rN[j] = peerRing[(size_t)(rposN + j) * 64];
-            }
-            eN = peerExp[cposN * 4];
-        }
+        bandCell(cN2, bN2);
+        const int eN = peerExp[cposN * 4 + row];
+        // rolling prefetch: after step j consumes its symbols and partner values,
+        // the same registers are reloaded for step j of the next cell
+        const uint32_t ygN = yrow + (uint32_t)cN * (K * 4);
+        const uint32_t peerN = (uint32_t)cposN * kRingCellBytes + laneRing;
+        // Loads are never predicated (a predicated load would be copied into its
+        // register with a wait for every load in flight): cells outside the next
+        // band read the symbol pool's first entry, and rows not emitting
+        // posteriors read one fixed ring slot, both cache hits.
+        auto prefetchStep = [&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const int xLn = band_xL(bN[j]);
+            const bool in = k <= band_wm1_narrow(bN[j]);
+            const uint32_t xa = in ? xrow + ((uint32_t)xLn << 2) : 0u;
+            const uint32_t ya = in ? ygN - ((uint32_t)xLn << 2) + j * 4 : 0u;
+            const uint32_t pa = (in & post) ? peerN + j * 12 : 0u;
+            sxC[j] = *reinterpret_cast<const uint32_t *>(symB + xa);
+            syC[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
+            rC[j] = *reinterpret_cast<const F3 *>(peerRing + pa);
+        };
 
-        const bool slow = __any((bC[0] & (bwd ? kSlowB : kSlowF)) != 0);
-        F3 *ringCell = myRing + (size_t)rpos * 64;
+        const bool slowRow = (bC[0] & (bwd ? kSlowB : kSlowF)) != 0;
+        const bool fullRow =
+            !init && !slowRow && (bwd ? (d == gs + K - 1 && stop <= gs) : (d == gs && stop >= gs + K - 1));
+        const bool full = __all(fullRow);
+        const bool anyPost = __any(post);
+        const bool slowWave = __any(slowRow);
+        const uint32_t cbase = (uint32_t)cpos * kRingCellBytes + laneRing;
+        int *expSlot = myExp + cpos * 4 + row;
         bool reached = false;
 
+        // One step.  FULL: every running row processes all 8 anti-diagonals with
+        // fast shifts (no per-step conditions); POST: some row emits posteriors.
+        auto fwdStep = [&](auto J, auto FULLc, auto POSTc, float &sc) {
+            constexpr int j = decltype(J)::value;
+            constexpr bool FULL = decltype(FULLc)::value, POST = decltype(POSTc)::value;
+            const int dj = gs + j;
+            if (!FULL && !(dj >= d && dj <= stop)) {
+                prefetchStep(J);
+                return;
+            }
+            const int b = bC[j];
+            const int xL = band_xL(b);
+            const bool act = k <= band_wm1_narrow(b);
+            const Emis e = emissions(tab, sxC[j], syC[j]);
+            float inX, inY, inM;
+            if (FULL || !slowWave) fwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+            else fwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
+            float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
+            if (!FULL && j == 0 && init) {
+                p0 = iv0;
+                p1 = iv1;
+                p2 = iv2;
+            }
+            p0 = act ? p0 : 0.f;
+            p1 = act ? p1 : 0.f;
+            p2 = act ? p2 : 0.f;
+            if (j == 0 && (FULL || dj == d)) {
+                // exact power-of-two rescale at the start of every grid cell
+                const float m = rmaxpos<SEG>(fmaxf(p0, fmaxf(p1, p2)));
+                const int e2 = __builtin_amdgcn_frexp_expf(m);
+                p0 = ldexpf(p0, -e2);
+                p1 = ldexpf(p1, -e2);
+                p2 = ldexpf(p2, -e2);
+                oM1 = ldexpf(oM1, -e2);
+                S += e2;
+                *expSlot = S;
+            }
+            // kept for the backward row (before the crossing) and for the next
+            // window's lower posterior diagonals (above `from`)
+            if (act & (!post | (dj > from))) *reinterpret_cast<F3 *>(myRing + (cbase + j * 12)) = F3{p0, p1, p2};
+            if constexpr (POST) {
+                if (!FULL || j == 0) sc = ldexpf(totalInv, S + eC - Stot);
+                const F3 pb = rC[j];
+                const float pM = p0 * pb.x * sc, pX = p1 * pb.y * sc, pY = p2 * pb.z * sc;
+                const float pXY = __uint_as_float(max(__float_as_uint(pX), __float_as_uint(pY)));
+                const bool emit = act & (dj <= postFrom) & ((pM >= thr) | (pXY >= thrXY));
+                const uint64_t m = __ballot(emit);
+                const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const int first = rbcast<SEG>(below, rowBase, 0);
+                const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
+                const int idx = cnt + (last - below - 1);  // descending cell order
+                if (emit && idx < cap)
+                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                                      (int)rec_code(dj, k));
+                cnt += last - first;
+            }
+            if (!FULL && dj == stop) reached = true;
+            prefetchStep(J);
+            const float q0 = fmaf(T20, p2, fmaf(T10, p1, T00 * p0));
+            const float q1 = fmaf(T21, p2, fmaf(T11, p1, T01 * p0));
+            const float q2 = fmaf(T22, p2, fmaf(T12, p1, T02 * p0));
+            oM2 = oM1;
+            oX1 = q1;
+            oY1 = q2;
+            oM1 = q0;
+            bm2 = bm1;
+            bm1 = b;
+        };
+        auto bwdStep = [&](auto J, auto FULLc, auto POSTc, float &sc) {
+            constexpr int j = K - 1 - decltype(J)::value;
+            constexpr bool FULL = decltype(FULLc)::value, POST = decltype(POSTc)::value;
+            const int dj = gs + j;
+            if (!FULL && !(dj <= d && dj >= stop)) {
+                prefetchStep(std::integral_constant<int, j>{});
+                return;
+            }
+            const int b = bC[j];
+            const int xL = band_xL(b);
+            const bool act = k <= band_wm1_narrow(b);
+            const Emis e = emissions(tab, sxC[j], syC[j]);
+            float inX, inY, inM;
+            if (FULL || !slowWave) bwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+            else bwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
+            // B = T in
+            float q0 = fmaf(T02, inY, fmaf(T01, inX, T00 * inM));
+            float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
+            float q2 = fmaf(T22, inY, fmaf(T21, inX, T20 * inM));
+            if (!FULL && init && dj == d) {
+                q0 = iv0;
+                q1 = iv1;
+                q2 = iv2;
+            }
+            q0 = act ? q0 : 0.f;
+            q1 = act ? q1 : 0.f;
+            q2 = act ? q2 : 0.f;
+            if (j == K - 1 && (FULL || (dj == d && !init))) {
+                const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
+                const int e2 = __builtin_amdgcn_frexp_expf(m);
+                q0 = ldexpf(q0, -e2);
+                q1 = ldexpf(q1, -e2);
+                q2 = ldexpf(q2, -e2);
+                oM1 = ldexpf(oM1, -e2);
+                S += e2;
+            }
+            // the cell's exponent is final after its first processed diagonal
+            if (FULL ? j == K - 1 : dj == d) *expSlot = S;
+            if (act & !post) *reinterpret_cast<F3 *>(myRing + (cbase + j * 12)) = F3{q0, q1, q2};
+            if constexpr (POST) {
+                if (!FULL || j == K - 1) sc = ldexpf(totalInv, eC + S - Stot);
+                const F3 pf = rC[j];
+                const float pM = pf.x * q0 * sc, pX = pf.y * q1 * sc, pY = pf.z * q2 * sc;
+                const float pXY = __uint_as_float(max(__float_as_uint(pX), __float_as_uint(pY)));
+                const bool emit = act & (dj <= postFrom) & ((pM >= thr) | (pXY >= thrXY));
+                const uint64_t m = __ballot(emit);
+                const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const int first = rbcast<SEG>(below, rowBase, 0);
+                const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
+                const int idx = cnt + (below - first);  // ascending cell order
+                if (emit && idx < cap)
+                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                                      (int)rec_code(dj, k));
+                cnt += last - first;
+            }
+            if (!FULL && dj == stop) reached = true;
+            prefetchStep(std::integral_constant<int, j>{});
+            // send u = e (.) B
+            oM2 = oM1;
+            oX1 = e.eX * q1;
+            oY1 = e.eY * q2;
+            oM1 = e.eM * q0;
+            bm2 = bm1;
+            bm1 = b;
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        float sc = 0.f;
         if (!bwd) {
-            const int stop = post ? tb : cx;
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int dj = gs + j;
-                if (dj >= d && dj <= stop) {
-                    const int b = bC[j];
-                    const int L0 = band_L(b);
-                    const bool act = k <= band_wm1_narrow(b);
-                    const Emis e = emissions(tab, sxC[j], syC[j]);
-                    float inX, inY, inM;
-                    if (!slow) {
-                        const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
-                        const float xs = row_shr<SEG>(oX1, k), ys = row_shl<SEG>(oY1, k);
-                        const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
-                        inX = dL1 < 0 ? xs : oX1;
-                        inY = dL1 > 0 ? ys : oY1;
-                        inM = dL2 < 0 ? ml : (dL2 > 0 ? mr : oM2);
-                    } else {
-                        const int xmy = L0 + 2 * k;
-                        const int Rm1 = bm1 == kEmpty ? Lm1 - 2 : Lm1 + 2 * band_wm1_narrow(bm1);
-                        const int Rm2 = bm2 == kEmpty ? Lm2 - 2 : Lm2 + 2 * band_wm1_narrow(bm2);
-                        inX = gather(oX1, xmy - 1, Lm1, Rm1, act, rowBase, k);
-                        inY = gather(oY1, xmy + 1, Lm1, Rm1, act, rowBase, k);
-                        inM = gather(oM2, xmy, Lm2, Rm2, act, rowBase, k);
-                    }
-                    float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
-                    if (j == 0 && init) {
-                        p0 = iv0;
-                        p1 = iv1;
-                        p2 = iv2;
-                    }
-                    if (!act) p0 = p1 = p2 = 0.f;
-                    if (j == 0 && dj == d) {
-                        // exact power-of-two rescale at the start of every grid cell
-                        const float m = rmax<SEG>(fmaxf(p0, fmaxf(p1, p2)));
-                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
-                        p0 = ldexpf(p0, -e2);
-                        p1 = ldexpf(p1, -e2);
-                        p2 = ldexpf(p2, -e2);
-                        oM1 = ldexpf(oM1, -e2);
-                        S += e2;
-                        if (k == 0) myExp[cpos * 4] = S;
-                    }
-                    ringCell[(size_t)j * 64] = F3{p0, p1, p2};
-                    if (post && dj <= from) {
-                        const F3 bb = rC[j];
-                        const float sc = ldexpf(totalInv, S + eC - Stot);
-                        const float pM = p0 * bb.x * sc, pX = p1 * bb.y * sc, pY = p2 * bb.z * sc;
-                        const bool emit = act && (pM >= thr || (withIndels && (pX >= thr || pY >= thr)));
-                        const uint64_t m = __ballot(emit);
-                        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        const int first = rbcast<SEG>(below, rowBase, 0);
-                        const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
-                        const int idx = cnt + (last - below - 1);  // descending cell order
-                        if (emit && idx < cap)
-                            kp.out[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX),
-                                                              __float_as_int(pY), (int)rec_code(dj, k));
-                        cnt += last - first;
-                    }
-                    if (dj == stop) reached = true;
-                    const float q0 = T00 * p0 + T10 * p1 + T20 * p2;
-                    const float q1 = T01 * p0 + T11 * p1 + T21 * p2;
-                    const float q2 = T02 * p0 + T12 * p1 + T22 * p2;
-                    oM2 = oM1;
-                    oX1 = q1;
-                    oY1 = q2;
-                    oM1 = q0;
-                    Lm2 = Lm1;
-                    bm2 = bm1;
-                    Lm1 = L0;
-                    bm1 = b;
-                }
-            }
+            if (full && anyPost) static_for<0, K>([&](auto J) { fwdStep(J, T_{}, T_{}, sc); });
+            else if (full) static_for<0, K>([&](auto J) { fwdStep(J, T_{}, F_{}, sc); });
+            else static_for<0, K>([&](auto J) { fwdStep(J, F_{}, T_{}, sc); });
+            if (full) reached = stop == gs + K - 1;
         } else {
-            const int stop = post ? lo + 1 : cx + 1;
-#pragma unroll
-            for (int j = K - 1; j >= 0; j--) {
-                const int dj = gs + j;
-                if (dj <= d && dj >= stop) {
-                    const int b = bC[j];
-                    const int L0 = band_L(b);
-                    const bool act = k <= band_wm1_narrow(b);
-                    const Emis e = emissions(tab, sxC[j], syC[j]);
-                    float inX, inY, inM;
-                    // successors: X at (d+1, xmy+1), Y at (d+1, xmy-1), M at (d+2, xmy)
-                    if (!slow) {
-                        const int dL1 = L0 - Lm1, dL2 = L0 - Lm2;
-                        const float xs = row_shl<SEG>(oX1, k), ys = row_shr<SEG>(oY1, k);
-                        const float ml = row_shr<SEG>(oM2, k), mr = row_shl<SEG>(oM2, k);
-                        inX = dL1 > 0 ? xs : oX1;
-                        inY = dL1 < 0 ? ys : oY1;
-                        inM = dL2 < 0 ? ml : (dL2 > 0 ? mr : oM2);
-                    } else {
-                        const int xmy = L0 + 2 * k;
-                        const int Rm1 = bm1 == kEmpty ? Lm1 - 2 : Lm1 + 2 * band_wm1_narrow(bm1);
-                        const int Rm2 = bm2 == kEmpty ? Lm2 - 2 : Lm2 + 2 * band_wm1_narrow(bm2);
-                        inX = gather(oX1, xmy + 1, Lm1, Rm1, act, rowBase, k);
-                        inY = gather(oY1, xmy - 1, Lm1, Rm1, act, rowBase, k);
-                        inM = gather(oM2, xmy, Lm2, Rm2, act, rowBase, k);
-                    }
-                    // B = T in
-                    float q0 = T00 * inM + T01 * inX + T02 * inY;
-                    float q1 = T10 * inM + T11 * inX + T12 * inY;
-                    float q2 = T20 * inM + T21 * inX + T22 * inY;
-                    if (init && dj == d) {
-                        q0 = iv0;
-                        q1 = iv1;
-                        q2 = iv2;
-                    }
-                    if (!act) q0 = q1 = q2 = 0.f;
-                    if (j == K - 1 && dj == d && !init) {
-                        const float m = rmax<SEG>(fmaxf(q0, fmaxf(q1, q2)));
-                        const int e2 = m > 0.f ? __builtin_amdgcn_frexp_expf(m) : 0;
-                        q0 = ldexpf(q0, -e2);
-                        q1 = ldexpf(q1, -e2);
-                        q2 = ldexpf(q2, -e2);
-                        oM1 = ldexpf(oM1, -e2);
-                        S += e2;
-                    }
-                    if (!post) {
-                        if (dj == d && k == 0) myExp[cpos * 4] = S;
-                        ringCell[(size_t)j * 64] = F3{q0, q1, q2};
-                    } else if (dj <= from) {
-                        const F3 f = rC[j];
-                        const float sc = ldexpf(totalInv, eC + S - Stot);
-                        const float pM = f.x * q0 * sc, pX = f.y * q1 * sc, pY = f.z * q2 * sc;
-                        const bool emit = act && (pM >= thr || (withIndels && (pX >= thr || pY >= thr)));
-                        const uint64_t m = __ballot(emit);
-                        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        const int first = rbcast<SEG>(below, rowBase, 0);
-                        const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
-                        const int idx = cnt + (below - first);  // ascending cell order
-                        if (emit && idx < cap)
-                            kp.out[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX),
-                                                              __float_as_int(pY), (int)rec_code(dj, k));
-                        cnt += last - first;
-                    }
-                    if (dj == stop) reached = true;
-                    // send u = e (.) B
-                    oM2 = oM1;
-                    oX1 = e.eX * q1;
-                    oY1 = e.eY * q2;
-                    oM1 = e.eM * q0;
-                    Lm2 = Lm1;
-                    bm2 = bm1;
-                    Lm1 = L0;
-                    bm1 = b;
-                }
-            }
+            if (full && anyPost) static_for<0, K>([&](auto J) { bwdStep(J, T_{}, T_{}, sc); });
+            else if (full) static_for<0, K>([&](auto J) { bwdStep(J, T_{}, F_{}, sc); });
+            else static_for<0, K>([&](auto J) { bwdStep(J, F_{}, T_{}, sc); });
+            if (full) reached = stop == gs;
         }
         init = false;
 #pragma unroll
         for (int j = 0; j < K; j++) {
             bC[j] = bN[j];
             bN[j] = bN2[j];
-            sxC[j] = sxN[j];
-            syC[j] = syN[j];
-            rC[j] = rN[j];
         }
         eC = eN;
 
@@ -503,15 +580,16 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
                 publishCrossing();
             } else if (!bwd) {
                 // forward window end at tb
-                if (k == 0) kp.winEnd[2 * (winOff + win)] = cnt;
+                if (k == 0) gWinEnd[2 * (winOff + win)] = cnt;
                 if (win + 1 == nWin) {
-                    if (k == 0) kp.counts[2 * job] = cnt;
+                    if (k == 0) gCounts[2 * job] = cnt;
                     phase = S_JWAIT;
                 } else {
                     win += 1;
                     const int next = tb + 1;
                     loadWin();
                     post = false;
+                    postFrom = -1;
                     d = next;
                     gs = d & ~(K - 1);
                     reload = true;
@@ -523,7 +601,6 @@ __device__ void split_class(const KParams &kp, const float *tab, SplitShared *sh
         } else {
             gs += bwd ? -K : K;
             d = bwd ? gs + K - 1 : gs;
-            rpos = rposN;
             cpos = cposN;
         }
     }

@@ -24,7 +24,7 @@ struct WideCtx {
     __device__ float *slot(int d, int st) const { return xb + ((size_t)(((d % 3) + 3) % 3) * 3 + st) * W; }
     __device__ void band(int d, int &L, int &R) const {
         const int b = kp.band[bandOff + d];
-        L = band_L(b);
+        L = 2 * band_xL(b) - d;
         R = L + 2 * band_wm1_wide(b);
     }
 };
@@ -122,9 +122,9 @@ __global__ __launch_bounds__(64) void pairhmm_wide_kernel(KParams kp) {
                         }
                     }
                     mx = fmaxf(mx, fmaxf(p0, fmaxf(p1, p2)));
-                    cx.slot(d, 0)[i] = T[0] * p0 + T[3] * p1 + T[6] * p2;
-                    cx.slot(d, 1)[i] = T[1] * p0 + T[4] * p1 + T[7] * p2;
-                    cx.slot(d, 2)[i] = T[2] * p0 + T[5] * p1 + T[8] * p2;
+                    cx.slot(d, 0)[i] = fmaf(T[6], p2, fmaf(T[3], p1, T[0] * p0));
+                    cx.slot(d, 1)[i] = fmaf(T[7], p2, fmaf(T[4], p1, T[1] * p0));
+                    cx.slot(d, 2)[i] = fmaf(T[8], p2, fmaf(T[5], p1, T[2] * p0));
                     if (!fwdOnly) ring[(size_t)(d % R) * W + i] = make_float4(p0, p1, p2, __int_as_float(S + e2));
                     if (d == wn.tb) {
                         const float e0 = rgR && atEnd ? kp.endV[1][0] : kp.endV[0][0];
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(64) void pairhmm_wide_kernel(KParams kp) {
             const int SFtb = S;
             const float totalInv = t > 0.f ? 1.f / t : 0.f;
             const double lt = log((double)t) + (double)SFtb * 0.69314718055994530942;
-            if (atEnd && lane == 0) kp.logProb[job] = lt;
+            if (atEnd && lane == 0) kp.logTotal[job] = make_int2(__float_as_int(t), SFtb);
             likeAcc += (double)(wn.from - wn.lo) * lt;
             if (fwdOnly) break;
             // ---- backward from tb down to lo + stopOff ----
@@ -184,9 +184,9 @@ __global__ __launch_bounds__(64) void pairhmm_wide_kernel(KParams kp) {
                             inX = (sX >= L1 && sX <= R1) ? cx.slot(d + 1, 1)[(sX - L1) >> 1] : 0.f;
                             inY = (sY >= L1 && sY <= R1) ? cx.slot(d + 1, 2)[(sY - L1) >> 1] : 0.f;
                             inM = (xmy >= L2 && xmy <= R2) ? cx.slot(d + 2, 0)[(xmy - L2) >> 1] * ldexpf(1.f, -pendMB) : 0.f;
-                            q0 = (T[0] * inM + T[1] * inX + T[2] * inY) * scl;
-                            q1 = (T[3] * inM + T[4] * inX + T[5] * inY) * scl;
-                            q2 = (T[6] * inM + T[7] * inX + T[8] * inY) * scl;
+                            q0 = fmaf(T[2], inY, fmaf(T[1], inX, T[0] * inM)) * scl;
+                            q1 = fmaf(T[5], inY, fmaf(T[4], inX, T[3] * inM)) * scl;
+                            q2 = fmaf(T[8], inY, fmaf(T[7], inX, T[6] * inM)) * scl;
                         }
                     }
                     mx = fmaxf(mx, fmaxf(q0, fmaxf(q1, q2)));
