@@ -17,10 +17,11 @@ namespace mpk {
 // Emission table (floats, copied to LDS).  Both strands share the run-length
 // table: the strand only changes which of its four base rows a reference
 // base selects (polisher.c:1756-1765: row = strand ? base : 3 - base, N -> A).
-//   GX[strand][5], GY[strand][5], E[strand][5][5], RL[4][51][51]
+//   [0] = 0, GX[strand][5] at 1, GY[strand][5] at 11, E[strand][5][5] at 21,
+//   [71] = 0, RL[4][51][51] at 72
 // ---------------------------------------------------------------------------
-constexpr int kTabGX = 0, kTabGY = 10, kTabE = 20, kTabRL = 72;
-constexpr int kRL = 51;                        // run lengths 0..50
+constexpr int kTabGX = 1, kTabGY = 11, kTabE = 21, kTabZeroE = 71, kTabRL = 72;
+constexpr int kRL = 51;                             // run lengths 0..50
 constexpr int kTabFloats = kTabRL + 4 * kRL * kRL;  // 10476
 // Every byte offset an encoded symbol pair can form stays below this bound,
 // so the LDS allocation is padded to it (no read can leave the allocation).
@@ -31,24 +32,27 @@ constexpr uint32_t kRLLastByte = (4 * kRL * kRL - 1) * 4;
 // and strand).  Reference side (x):
 //   bits  0-15  byte offset of the run-length row: (row * 51 + rl_x) * 51 * 4
 //   bits 16-23  byte offset of the match row:       (strand * 25 + base_x * 5) * 4
-//   bits 24-31  byte offset of the gap-X entry:     (strand * 5 + base_x) * 4
+//   bits 24-31  byte offset of the gap-X entry in the table
 // Read side (y):
 //   bits  0-15  rl_y * 4
 //   bits 16-23  base_y * 4
-//   bits 24-31  byte offset of the gap-Y entry:     (strand * 5 + base_y) * 4
-// so x + y carries the run-length and match-table offsets in one add.
+//   bits 24-31  byte offset of the gap-Y entry in the table
+// so x + y carries the run-length and match-table offsets in one add.  The
+// zero symbol, read for both x and y by cells outside the band, selects the
+// table's two zero entries: all three emissions are 0, so such cells send 0.
 __host__ __device__ inline uint32_t enc_x(uint32_t sym, int strand) {
     const uint32_t base = (sym & 255u) < 4u ? (sym & 255u) : 4u;
     const uint32_t rl = (sym >> 8) < 50u ? (sym >> 8) : 50u;
     const uint32_t row = (base & 3u) ^ (strand ? 0u : 3u);
     return ((row * kRL + rl) * kRL * 4u) | (((uint32_t)strand * 25u + base * 5u) * 4u << 16) |
-           (((uint32_t)strand * 5u + base) * 4u << 24);
+           (((uint32_t)kTabGX + (uint32_t)strand * 5u + base) * 4u << 24);
 }
 __host__ __device__ inline uint32_t enc_y(uint32_t sym, int strand) {
     const uint32_t base = (sym & 255u) < 4u ? (sym & 255u) : 4u;
     const uint32_t rl = (sym >> 8) < 50u ? (sym >> 8) : 50u;
-    return (rl * 4u) | ((base * 4u) << 16) | ((((uint32_t)strand * 5u + base) * 4u) << 24);
+    return (rl * 4u) | ((base * 4u) << 16) | ((((uint32_t)kTabGY + (uint32_t)strand * 5u + base) * 4u) << 24);
 }
+constexpr uint32_t kZeroSym = (uint32_t)((kTabZeroE - kTabE) * 4 / 2) << 16;
 
 struct Emis {
     float eX, eY, eM;
@@ -61,8 +65,8 @@ __device__ __forceinline__ Emis emissions(const float *tab, uint32_t sx, uint32_
     // inside the table; in-band cells never reach it.
     const float rl = *reinterpret_cast<const float *>(t + kTabRL * 4 + min(s & 0xffffu, kRLLastByte));
     const float e = *reinterpret_cast<const float *>(t + kTabE * 4 + ((s >> 16) & 0xffu));
-    const float ex = *reinterpret_cast<const float *>(t + kTabGX * 4 + (sx >> 24));
-    const float ey = *reinterpret_cast<const float *>(t + kTabGY * 4 + (sy >> 24));
+    const float ex = *reinterpret_cast<const float *>(t + (sx >> 24));
+    const float ey = *reinterpret_cast<const float *>(t + (sy >> 24));
     return Emis{ex, ey, e * rl};
 }
 
@@ -124,6 +128,7 @@ struct KParams {
     int32_t ringWideLen, wideC, wideBegin, wideEnd;
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;     // anti-diagonals per ring (multiple of kK)
+    int32_t outLen;      // records in `out` (< kMaxRecords)
     int32_t withIndels;
     int32_t mode;
     float threshold;
@@ -131,6 +136,27 @@ struct KParams {
     float startV[2][3];  // [ragged][state]
     float endV[2][3];
 };
+
+// ---------------------------------------------------------------------------
+// Buffer resources.  A buffer store whose offset (raw) or index (structured)
+// lies beyond the resource's extent is dropped by the address unit: a masked
+// store without an EXEC branch, so the steps around it stay one scheduling
+// region.  Word 3 of a gfx9 resource: 32-bit data format, no swizzle.
+// ---------------------------------------------------------------------------
+constexpr int kBufWord3 = 0x00020000;
+constexpr uint32_t kBufOff = 0xfffffff0u;  // a raw offset beyond every extent used here
+constexpr int64_t kMaxRecords = (int64_t)1 << 28;  // record buffer extent < 2^32 bytes
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void buf_store3(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c) {
+    const u32x3 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
+    __builtin_amdgcn_raw_buffer_store_b96(v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, uint32_t off, int4 v) {
+    const u32x4 w = {(uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w};
+    __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, 0);
+}
 
 // ---------------------------------------------------------------------------
 // Wavefront helpers
@@ -152,6 +178,7 @@ __device__ __forceinline__ int dppi(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
 }
 constexpr int kRowShr1 = 0x111, kRowShl1 = 0x101, kWaveShr1 = 0x138, kWaveShl1 = 0x130;
+constexpr int kRowRor1 = 0x121, kRowRor15 = 0x12F;  // lane i <- lane i-1 / i+1 (mod 16)
 constexpr int kQuadSwap1 = 0xB1, kQuadSwap2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowBcast0 = 0x150, kRowBcast15 = 0x15F;
 

@@ -322,6 +322,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // Symbol pool with padding at both ends (kernels read up to one row width
     // beyond a sequence without clamping).
     std::vector<uint32_t> sym(kSymPad, enc_x(4, 0));
+    sym[0] = kZeroSym;  // read by cells outside the band
     std::vector<HostJob> jobs;
     std::vector<std::vector<int64_t>> jobAnchors;
     std::vector<int64_t> sp, anc;
@@ -460,13 +461,29 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     // xL = x of the band's first cell: (d + xmyL) / 2
                     XL.resize(D + 1);
                     for (int64_t d = 0; d <= D; d++) XL[d] = (int32_t)((d + L[d]) / 2);
+                    if (hj.cls == 0) {
+                        // rows of 16 x-aligned lanes: the cells of three consecutive
+                        // anti-diagonals (and their neighbours) must span at most 16 x
+                        auto xr = [&](int64_t d) { return XL[d] + (R[d] - L[d]) / 2; };
+                        bool fits = true;
+                        for (int64_t d = 0; d <= D && fits; d++) {
+                            int64_t lo = XL[d] - 1, hi = xr(d);
+                            for (int64_t e = d - 2; e < d; e++)
+                                if (e >= 0) { lo = std::min<int64_t>(lo, XL[e]); hi = std::max<int64_t>(hi, xr(e)); }
+                            int64_t lo2 = XL[d], hi2 = xr(d) + 1;
+                            for (int64_t e = d + 1; e <= d + 2; e++)
+                                if (e <= D) { lo2 = std::min<int64_t>(lo2, XL[e]); hi2 = std::max<int64_t>(hi2, xr(e)); }
+                            fits = hi - lo <= 15 && hi2 - lo2 <= 15;
+                        }
+                        if (!fits) hj.cls = 1;
+                    }
                     for (int64_t d = 0; d <= D; d++) {
                         const int32_t wd = (R[d] - L[d]) / 2 + 1;
                         bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
                     }
                     const int64_t padEnd = bandOff[j + 1] - bandOff[j];
                     for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
-                    if (narrow) {
+                    if (narrow && hj.cls > 0) {
                         // cells whose neighbour exchange is not a one-lane shift
                         auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
                         for (int64_t d = 0; d <= D; d++) {
@@ -582,6 +599,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     for (size_t q = 0; q < ord.size(); q++) b->jobs[q] = jobs[ord[q]];
     b->ringLen = ringLen;
     b->outTotal = outOff;
+    // the split kernel addresses records with 32-bit buffer offsets
+    if (outOff >= kMaxRecords) {
+        delete b;
+        return MP_EUNSUPPORTED;
+    }
 
     // ---- 4. upload ----
     hipStream_t s = e->stream;
@@ -634,6 +656,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.band = b->dBand.p;
     kp.win = b->dWin.p;
     kp.out = b->dOut.p;
+    kp.outLen = (int32_t)std::max<int64_t>(outOff, 1);
     kp.counts = b->dCounts.p;
     kp.winEnd = b->dWinEnd.p;
     kp.logTotal = b->dLogTotal.p;
@@ -742,8 +765,10 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
             off += (int64_t)dj[j].cap[0] + dj[j].cap[1];
         }
         HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
+        if (off >= kMaxRecords) return MP_EUNSUPPORTED;
         if ((st = b->dOut.alloc(std::max<int64_t>(off, 1)))) return st;
         b->kp.out = b->dOut.p;
+        b->kp.outLen = (int32_t)std::max<int64_t>(off, 1);
         b->outTotal = off;
         return batch_run_impl(b, plain);
     }
@@ -864,12 +889,41 @@ static mp_status materialize(mp_batch *b) {
                     c[l]++;
                 }
             };
+            // one anti-diagonal's records, cells descending
+            auto emitBlock = [&](const int4 *r, int n) {
+                int ord[64];
+                const int m = std::min(n, 64);
+                for (int t = 0; t < m; t++) {
+                    int u = t;
+                    const uint32_t key = (uint32_t)r[t].w & 2047u;
+                    while (u > 0 && ((uint32_t)r[ord[u - 1]].w & 2047u) < key) {
+                        ord[u] = ord[u - 1];
+                        u--;
+                    }
+                    ord[u] = t;
+                }
+                for (int t = 0; t < m; t++) emitRec(r[ord[t]]);
+                for (int t = m; t < n; t++) emitRec(r[t]);
+            };
+            auto diagOf = [](const int4 &r) { return (uint32_t)r.w >> 11; };
             for (int32_t w = hj.nWin - 1; w >= 0; w--) {
                 const int32_t g = hj.winOff + w;
                 const int32_t fEnd = wend[2 * g], bEnd = wend[2 * g + 1];
                 const int32_t fBeg = w == 0 ? 0 : wend[2 * (g - 1)], bBeg = w == 0 ? 0 : wend[2 * (g - 1) + 1];
-                for (int32_t t = bEnd - 1; t >= bBeg; t--) emitRec(br[t]);
-                for (int32_t t = fBeg; t < fEnd; t++) emitRec(fr[t]);
+                // backward side: anti-diagonals descending, so walk its blocks from the end
+                for (int32_t t = bEnd; t > bBeg;) {
+                    int32_t u = t - 1;
+                    while (u > bBeg && diagOf(br[u - 1]) == diagOf(br[t - 1])) u--;
+                    emitBlock(br + u, t - u);
+                    t = u;
+                }
+                // forward side: anti-diagonals ascending
+                for (int32_t t = fBeg; t < fEnd;) {
+                    int32_t u = t + 1;
+                    while (u < fEnd && diagOf(fr[u]) == diagOf(fr[t])) u++;
+                    emitBlock(fr + t, u - t);
+                    t = u;
+                }
             }
         }
         if (pass == 0)

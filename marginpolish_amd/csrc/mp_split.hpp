@@ -27,10 +27,16 @@
 // band two).  Macro steps that cover all 8 anti-diagonals of every running
 // row with one-lane neighbour shifts (the common case) run a branch-free body.
 //
+// Rows of 16 lanes (the common case) hold cell x of an anti-diagonal on lane
+// x mod 16 ("x-aligned"); when every three consecutive anti-diagonals of the
+// band fit in 16 consecutive x (checked on the host) each neighbour input is
+// a fixed one-lane rotation inside the row.  Rows of 32 and 64 lanes hold
+// cell k on lane k and shift by band geometry.  Cells outside the band read
+// a zero-emission symbol, so they send zero.
+//
 // Output: one record per cell with any posterior >= threshold:
-// (pM, pX, pY, (d << 11) | cell).  The forward side writes each anti-diagonal's
-// cells in descending order, the backward side in ascending order, so that
-// the host can lay the records out in the reference's list order.
+// (pM, pX, pY, (d << 11) | cell); the host orders the records of each
+// anti-diagonal.
 #pragma once
 
 #include "mp_device.hpp"
@@ -110,6 +116,7 @@ __device__ __forceinline__ void bwd_in_slow(int xL, int b1, int b2, float oX1, f
 template <int SEG>
 __device__ __forceinline__ void split_class(const KParams &kp, const float *tab, SplitShared *sh, int cls) {
     constexpr int K = kK;
+    constexpr bool XAL = SEG == 16;  // x-aligned lanes
     // Every parameter is read once here: the loop's acquire loads would otherwise
     // force the compiler to re-read the parameter block after each of them.
     const DevJob *const gJobs = kp.jobs;
@@ -139,6 +146,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int *myExp = kp.ringExp + slot * (size_t)RC * 4;
     const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4;
     const uint32_t laneRing = (uint32_t)lane * kRingLaneBytes;
+    // masked stores go through buffer resources (out-of-range offsets are dropped)
+    const __amdgpu_buffer_rsrc_t ringRs = __builtin_amdgcn_make_buffer_rsrc(myRing, 0, RC * kRingCellBytes, kBufWord3);
+    const __amdgpu_buffer_rsrc_t outRs = __builtin_amdgcn_make_buffer_rsrc(gOut, 0, (int)((uint32_t)kp.outLen * 16u), kBufWord3);
     const char *symB = reinterpret_cast<const char *>(kp.sym);
     const char *bandB = reinterpret_cast<const char *>(kp.band);
     const float T00 = kp.T[0], T01 = kp.T[1], T02 = kp.T[2], T10 = kp.T[3], T11 = kp.T[4], T12 = kp.T[5],
@@ -157,7 +167,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int phase = bwd ? S_JWAIT : S_FETCH;
     int jn = 0, xn = 0;
     int job = -1, D = 0, lastCell = 0, nWin = 0, winOff = 0, jflags = 0, cap = 0, win = 0;
-    uint32_t xrow = 0, yrow = 0, bandRow = 0;  // byte offsets: x/y symbols of cell k, the job's band
+    uint32_t xrow = 0, yrow = 0, bandRow = 0;  // byte offsets: x/y symbol bases, the job's band
     int64_t outBase = 0;
     int tb = 0, lo = 0, from = 0, cx = 0, postFrom = -1;
     int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, cpos = 0;
@@ -177,19 +187,30 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         rC[j] = F3{0.f, 0.f, 0.f};
     }
 
+    // cell index of this lane on an anti-diagonal starting at xL
+    auto cellOf = [&](int xL) { return XAL ? ((k - xL) & 15) : k; };
     auto bandCell = [&](int c, int *dst) {
         const int4 *p = reinterpret_cast<const int4 *>(bandB + bandRow + (uint32_t)c * 32u);
         const int4 a = p[0], b = p[1];
         dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
         dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
     };
+    // symbol byte offsets of this lane's cell on anti-diagonal dj (band entry b);
+    // cells outside the band read the zero symbol at offset 0
+    auto symAddr = [&](int dj, int b, uint32_t &xa, uint32_t &ya) {
+        const int xL = band_xL(b), kx = cellOf(xL);
+        const bool in = kx <= band_wm1_narrow(b);
+        const uint32_t x = (uint32_t)(xL + kx);
+        xa = in ? xrow + (x << 2) : 0u;
+        ya = in ? yrow + (((uint32_t)dj - x) << 2) : 0u;
+    };
     auto symCell = [&](int c, const int *bb, uint32_t *sx, uint32_t *sy) {
-        const uint32_t yg = yrow + (uint32_t)c * (K * 4);
 #pragma unroll
         for (int j = 0; j < K; j++) {
-            const int xL = band_xL(bb[j]);
-            sx[j] = *reinterpret_cast<const uint32_t *>(symB + (xrow + ((uint32_t)xL << 2)));
-            sy[j] = *reinterpret_cast<const uint32_t *>(symB + (yg - ((uint32_t)xL << 2)) + j * 4);
+            uint32_t xa, ya;
+            symAddr(c * K + j, bb[j], xa, ya);
+            sx[j] = *reinterpret_cast<const uint32_t *>(symB + xa);
+            sy[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
         }
     };
     auto peerCell = [&](int cp, F3 *r) {
@@ -225,8 +246,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     };
     auto setupJob = [&]() {
         const DevJob jb = gJobs[job];
-        xrow = (uint32_t)(jb.symX - 1 + k) * 4u;
-        yrow = (uint32_t)(jb.symY - 1 - k) * 4u;
+        xrow = (uint32_t)(jb.symX - 1) * 4u;
+        yrow = (uint32_t)(jb.symY - 1) * 4u;
         bandRow = (uint32_t)jb.bandOff * 4u;
         D = jb.lX + jb.lY;
         lastCell = D / K + 1;  // bands are padded by one extra cell
@@ -261,15 +282,25 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     auto publishCrossing = [&]() {
         float v0, v1, v2, v3;
         if (!bwd) {
-            const int *bandJ = reinterpret_cast<const int *>(bandB + bandRow);
-            const int b1 = bandJ[cx + 1], b2 = bandJ[cx + 2];
-            const int xL1 = band_xL(b1), xL2 = band_xL(b2);
-            const bool a1 = k <= band_wm1_narrow(b1), a2 = k <= band_wm1_narrow(b2) && cx + 2 <= D;
-            const int xc = band_xL(bm1), xcm = band_xL(bm2), wc = band_wm1_narrow(bm1), wcm = band_wm1_narrow(bm2);
-            v0 = gather(oX1, k + xL1 - xc - 1, wc, a1, rowBase, k);
-            v1 = gather(oY1, k + xL1 - xc, wc, a1, rowBase, k);
-            v2 = gather(oM2, k + xL1 - xcm - 1, wcm, a1, rowBase, k);
-            v3 = gather(oM1, k + xL2 - xc - 1, wc, a2, rowBase, k);
+            if constexpr (XAL) {
+                // No masking: lanes outside the bands of cx+1 / cx+2 meet zero
+                // backward sends.  (A predicated DPP would also read 0 from the
+                // disabled source lanes.)
+                v0 = dpp<kRowRor1, false>(oX1);
+                v1 = oY1;
+                v2 = dpp<kRowRor1, false>(oM2);
+                v3 = dpp<kRowRor1, false>(oM1);
+            } else {
+                const int *bandJ = reinterpret_cast<const int *>(bandB + bandRow);
+                const int b1 = bandJ[cx + 1], b2 = bandJ[cx + 2];
+                const int xL1 = band_xL(b1), xL2 = band_xL(b2);
+                const bool a1 = k <= band_wm1_narrow(b1), a2 = k <= band_wm1_narrow(b2) && cx + 2 <= D;
+                const int xc = band_xL(bm1), xcm = band_xL(bm2), wc = band_wm1_narrow(bm1), wcm = band_wm1_narrow(bm2);
+                v0 = gather(oX1, k + xL1 - xc - 1, wc, a1, rowBase, k);
+                v1 = gather(oY1, k + xL1 - xc, wc, a1, rowBase, k);
+                v2 = gather(oM2, k + xL1 - xcm - 1, wcm, a1, rowBase, k);
+                v3 = gather(oM1, k + xL2 - xc - 1, wc, a2, rowBase, k);
+            }
         } else {
             v0 = oX1;
             v1 = oY1;
@@ -390,22 +421,23 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         const int eN = peerExp[cposN * 4 + row];
         // rolling prefetch: after step j consumes its symbols and partner values,
         // the same registers are reloaded for step j of the next cell
-        const uint32_t ygN = yrow + (uint32_t)cN * (K * 4);
         const uint32_t peerN = (uint32_t)cposN * kRingCellBytes + laneRing;
         // Loads are never predicated (a predicated load would be copied into its
         // register with a wait for every load in flight): cells outside the next
         // band read the symbol pool's first entry, and rows not emitting
         // posteriors read one fixed ring slot, both cache hits.
-        auto prefetchStep = [&](auto J) {
+        // Partner values are only fetched in bodies where some row emits: a row
+        // starts emitting after its crossing, which reloads the whole cell.
+        auto prefetchStep = [&](auto J, auto POSTc) {
             constexpr int j = decltype(J)::value;
-            const int xLn = band_xL(bN[j]);
-            const bool in = k <= band_wm1_narrow(bN[j]);
-            const uint32_t xa = in ? xrow + ((uint32_t)xLn << 2) : 0u;
-            const uint32_t ya = in ? ygN - ((uint32_t)xLn << 2) + j * 4 : 0u;
-            const uint32_t pa = (in & post) ? peerN + j * 12 : 0u;
+            uint32_t xa, ya;
+            symAddr(cN * K + j, bN[j], xa, ya);
             sxC[j] = *reinterpret_cast<const uint32_t *>(symB + xa);
             syC[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
-            rC[j] = *reinterpret_cast<const F3 *>(peerRing + pa);
+            if constexpr (decltype(POSTc)::value) {
+                const uint32_t pa = ((xa != 0u) & post) ? peerN + j * 12 : 0u;
+                rC[j] = *reinterpret_cast<const F3 *>(peerRing + pa);
+            }
         };
 
         const bool slowRow = (bC[0] & (bwd ? kSlowB : kSlowF)) != 0;
@@ -425,25 +457,31 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             constexpr bool FULL = decltype(FULLc)::value, POST = decltype(POSTc)::value;
             const int dj = gs + j;
             if (!FULL && !(dj >= d && dj <= stop)) {
-                prefetchStep(J);
+                prefetchStep(J, POSTc);
                 return;
             }
             const int b = bC[j];
             const int xL = band_xL(b);
-            const bool act = k <= band_wm1_narrow(b);
+            const int kx = cellOf(xL);
+            const bool act = kx <= band_wm1_narrow(b);
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
-            if (FULL || !slowWave) fwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
-            else fwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
-            float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
-            if (!FULL && j == 0 && init) {
-                p0 = iv0;
-                p1 = iv1;
-                p2 = iv2;
+            if constexpr (XAL) {
+                inX = dpp<kRowRor1, false>(oX1);
+                inY = oY1;
+                inM = dpp<kRowRor1, false>(oM2);
+            } else if (FULL || !slowWave) {
+                fwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+            } else {
+                fwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
             }
-            p0 = act ? p0 : 0.f;
-            p1 = act ? p1 : 0.f;
-            p2 = act ? p2 : 0.f;
+            float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
+            // (cells outside the band have zero emissions, hence p = 0)
+            if (!FULL && j == 0 && init) {
+                p0 = act ? iv0 : 0.f;
+                p1 = act ? iv1 : 0.f;
+                p2 = act ? iv2 : 0.f;
+            }
             if (j == 0 && (FULL || dj == d)) {
                 // exact power-of-two rescale at the start of every grid cell
                 const float m = rmaxpos<SEG>(fmaxf(p0, fmaxf(p1, p2)));
@@ -457,7 +495,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // kept for the backward row (before the crossing) and for the next
             // window's lower posterior diagonals (above `from`)
-            if (act & (!post | (dj > from))) *reinterpret_cast<F3 *>(myRing + (cbase + j * 12)) = F3{p0, p1, p2};
+            buf_store3(ringRs, (act & (!post | (dj > from))) ? cbase + j * 12 : kBufOff, p0, p1, p2);
             if constexpr (POST) {
                 if (!FULL || j == 0) sc = ldexpf(totalInv, S + eC - Stot);
                 const F3 pb = rC[j];
@@ -469,14 +507,15 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 const int first = rbcast<SEG>(below, rowBase, 0);
                 const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
-                const int idx = cnt + (last - below - 1);  // descending cell order
-                if (emit && idx < cap)
-                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
-                                                      (int)rec_code(dj, k));
+                const int idx = cnt + (below - first);
+                if (emit & (idx < cap))
+                    buf_store4(outRs, (uint32_t)(outBase + idx) * 16u,
+                               make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                         (int)rec_code(dj, kx)));
                 cnt += last - first;
             }
             if (!FULL && dj == stop) reached = true;
-            prefetchStep(J);
+            prefetchStep(J, POSTc);
             const float q0 = fmaf(T20, p2, fmaf(T10, p1, T00 * p0));
             const float q1 = fmaf(T21, p2, fmaf(T11, p1, T01 * p0));
             const float q2 = fmaf(T22, p2, fmaf(T12, p1, T02 * p0));
@@ -492,16 +531,24 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             constexpr bool FULL = decltype(FULLc)::value, POST = decltype(POSTc)::value;
             const int dj = gs + j;
             if (!FULL && !(dj <= d && dj >= stop)) {
-                prefetchStep(std::integral_constant<int, j>{});
+                prefetchStep(std::integral_constant<int, j>{}, POSTc);
                 return;
             }
             const int b = bC[j];
             const int xL = band_xL(b);
-            const bool act = k <= band_wm1_narrow(b);
+            const int kx = cellOf(xL);
+            const bool act = kx <= band_wm1_narrow(b);
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
-            if (FULL || !slowWave) bwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
-            else bwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
+            if constexpr (XAL) {
+                inX = dpp<kRowRor15, false>(oX1);
+                inY = oY1;
+                inM = dpp<kRowRor15, false>(oM2);
+            } else if (FULL || !slowWave) {
+                bwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
+            } else {
+                bwd_in_slow(xL, bm1, bm2, oX1, oY1, oM2, act, rowBase, k, inX, inY, inM);
+            }
             // B = T in
             float q0 = fmaf(T02, inY, fmaf(T01, inX, T00 * inM));
             float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
@@ -511,9 +558,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 q1 = iv1;
                 q2 = iv2;
             }
-            q0 = act ? q0 : 0.f;
-            q1 = act ? q1 : 0.f;
-            q2 = act ? q2 : 0.f;
+            // (cells outside the band keep q but send e . q = 0; their q is never stored or emitted)
             if (j == K - 1 && (FULL || (dj == d && !init))) {
                 const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
                 const int e2 = __builtin_amdgcn_frexp_expf(m);
@@ -525,7 +570,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // the cell's exponent is final after its first processed diagonal
             if (FULL ? j == K - 1 : dj == d) *expSlot = S;
-            if (act & !post) *reinterpret_cast<F3 *>(myRing + (cbase + j * 12)) = F3{q0, q1, q2};
+            buf_store3(ringRs, (act & !post) ? cbase + j * 12 : kBufOff, q0, q1, q2);
             if constexpr (POST) {
                 if (!FULL || j == K - 1) sc = ldexpf(totalInv, eC + S - Stot);
                 const F3 pf = rC[j];
@@ -537,14 +582,15 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 const int first = rbcast<SEG>(below, rowBase, 0);
                 const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
-                const int idx = cnt + (below - first);  // ascending cell order
-                if (emit && idx < cap)
-                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
-                                                      (int)rec_code(dj, k));
+                const int idx = cnt + (below - first);
+                if (emit & (idx < cap))
+                    buf_store4(outRs, (uint32_t)(outBase + idx) * 16u,
+                               make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                         (int)rec_code(dj, kx)));
                 cnt += last - first;
             }
             if (!FULL && dj == stop) reached = true;
-            prefetchStep(std::integral_constant<int, j>{});
+            prefetchStep(std::integral_constant<int, j>{}, POSTc);
             // send u = e (.) B
             oM2 = oM1;
             oX1 = e.eX * q1;
