@@ -54,9 +54,10 @@ struct SplitShared {
 constexpr int kSplitLdsBytes =
     ((kTabFloats * 4 + (int)sizeof(SplitShared)) > kTabReach ? (kTabFloats * 4 + (int)sizeof(SplitShared))
                                                               : kTabReach);
-// Ring geometry: [wave slot][cell][64 lanes][8 anti-diagonals] of 3 floats.
-constexpr int kRingLaneBytes = kK * 12;              // 96
-constexpr int kRingCellBytes = 64 * kRingLaneBytes;  // 6144
+// Ring geometry: [wave slot][cell][8 anti-diagonals][64 lanes] of 3 floats, so
+// that one step's 64 lanes store and load 768 contiguous bytes.
+constexpr int kRingStepBytes = 64 * 12;              // 768: one anti-diagonal, lane-major
+constexpr int kRingCellBytes = kK * kRingStepBytes;  // 6144
 
 enum { S_FETCH = 0, S_RUN = 1, S_XWAIT = 2, S_JWAIT = 3, S_EXIT = 4 };
 
@@ -145,7 +146,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     const char *peerRing = reinterpret_cast<const char *>(kp.ring) + (slot ^ 1) * (size_t)RC * kRingCellBytes;
     int *myExp = kp.ringExp + slot * (size_t)RC * 4;
     const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4;
-    const uint32_t laneRing = (uint32_t)lane * kRingLaneBytes;
+    const uint32_t laneRing = (uint32_t)lane * 12u;
     // masked stores go through buffer resources (out-of-range offsets are dropped)
     const __amdgpu_buffer_rsrc_t ringRs = __builtin_amdgcn_make_buffer_rsrc(myRing, 0, RC * kRingCellBytes, kBufWord3);
     const __amdgpu_buffer_rsrc_t outRs = __builtin_amdgcn_make_buffer_rsrc(gOut, 0, (int)((uint32_t)kp.outLen * 16u), kBufWord3);
@@ -214,10 +215,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         }
     };
     auto peerCell = [&](int cp, F3 *r) {
-        const float4 *p = reinterpret_cast<const float4 *>(peerRing + ((uint32_t)cp * kRingCellBytes + laneRing));
-        const float4 a = p[0], b = p[1], c = p[2], e = p[3], f = p[4], g = p[5];
-        r[0] = F3{a.x, a.y, a.z}; r[1] = F3{a.w, b.x, b.y}; r[2] = F3{b.z, b.w, c.x}; r[3] = F3{c.y, c.z, c.w};
-        r[4] = F3{e.x, e.y, e.z}; r[5] = F3{e.w, f.x, f.y}; r[6] = F3{f.z, f.w, g.x}; r[7] = F3{g.y, g.z, g.w};
+        const char *p = peerRing + ((uint32_t)cp * kRingCellBytes + laneRing);
+#pragma unroll
+        for (int j = 0; j < K; j++) r[j] = *reinterpret_cast<const F3 *>(p + j * kRingStepBytes);
     };
     auto resetExchange = [&]() {
         oX1 = oY1 = oM1 = oM2 = 0.f;
@@ -435,7 +435,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             sxC[j] = *reinterpret_cast<const uint32_t *>(symB + xa);
             syC[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
             if constexpr (decltype(POSTc)::value) {
-                const uint32_t pa = ((xa != 0u) & post) ? peerN + j * 12 : 0u;
+                const uint32_t pa = ((xa != 0u) & post) ? peerN + j * kRingStepBytes : 0u;
                 rC[j] = *reinterpret_cast<const F3 *>(peerRing + pa);
             }
         };
@@ -495,7 +495,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // kept for the backward row (before the crossing) and for the next
             // window's lower posterior diagonals (above `from`)
-            buf_store3(ringRs, (act & (!post | (dj > from))) ? cbase + j * 12 : kBufOff, p0, p1, p2);
+            // (whole rows: a full 192-byte row segment makes whole-line writes;
+            // lanes outside the band store values that are never read)
+            buf_store3(ringRs, (!post | (dj > from)) ? cbase + j * kRingStepBytes : kBufOff, p0, p1, p2);
             if constexpr (POST) {
                 if (!FULL || j == 0) sc = ldexpf(totalInv, S + eC - Stot);
                 const F3 pb = rC[j];
@@ -558,7 +560,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 q1 = iv1;
                 q2 = iv2;
             }
-            // (cells outside the band keep q but send e . q = 0; their q is never stored or emitted)
+            // (cells outside the band keep q but send e . q = 0; their stored q is never read)
             if (j == K - 1 && (FULL || (dj == d && !init))) {
                 const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
                 const int e2 = __builtin_amdgcn_frexp_expf(m);
@@ -570,7 +572,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // the cell's exponent is final after its first processed diagonal
             if (FULL ? j == K - 1 : dj == d) *expSlot = S;
-            buf_store3(ringRs, (act & !post) ? cbase + j * 12 : kBufOff, q0, q1, q2);
+            buf_store3(ringRs, !post ? cbase + j * kRingStepBytes : kBufOff, q0, q1, q2);
             if constexpr (POST) {
                 if (!FULL || j == K - 1) sc = ldexpf(totalInv, eC + S - Stot);
                 const F3 pf = rC[j];
