@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=1200)
     ap.add_argument("--err-pairs", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (chunks shard across GPUs "
@@ -45,6 +48,16 @@ def main():
 
     import numpy as np
     from marginpolish_amd import crop, pairhmm, sim
+
+    if args.copy_calib_mib > 0:
+        import torch
+        n = args.copy_calib_mib << 18  # float32 elements
+        a = torch.ones(n, device="cuda")
+        c = torch.empty_like(a)
+        c.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calib_mib, args.copy_calib_mib))
+        del a, c
 
     t0 = time.time()
     w = sim.config1()
@@ -102,7 +115,7 @@ def main():
     }
     achieved = FLOPS_PER_CELL * cells / (kernel_ms * 1e-3) / 1e12
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_kernel_config1.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_split_kernel_config1.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS,
