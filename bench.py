@@ -19,6 +19,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "pair-HMM GCUPS + reads/s: batched FP32 vs plain, posterior err <=1e-4"
+WORKLOADS = {
+    "config1": "config1: %d chunks x 10 kb (+-500 bp), 400 kb ref, 60x reads mean 8 kb, 9%% error, RLE, anchor "
+               "expansion 10, allParams.np.json, one POA realign iteration",
+    "config2": "config2: %d chunks x 100 kb (+-5 kb), 5 Mb ref, 50x reads mean 20 kb (max 100 kb), 8%% error, RLE, "
+               "anchor expansion 10, allParams.np.json, one POA realign iteration",
+}
 FLOPS_PER_CELL = 42          # forward 18 + backward 18 + posterior 6 (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
 PARAMS = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.json")
@@ -37,6 +43,9 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=1200)
     ap.add_argument("--err-pairs", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
+                    help="config1 (the headline batch) or config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks)")
+    ap.add_argument("--max-chunks", type=int, default=None, help="limit the number of chunks (config2)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -60,7 +69,7 @@ def main():
         del a, c
 
     t0 = time.time()
-    w = sim.config1()
+    w = sim.config1() if args.workload == "config1" else sim.config2(max_chunks=args.max_chunks)
     pairs = [(p.ref_sym, p.read_sym, p.anchors, p.strand) for p in w.pairs]
     log("workload: %d pairs built in %.1f s" % (len(pairs), time.time() - t0))
     pp = pairhmm.PolishParams(PARAMS)
@@ -105,8 +114,7 @@ def main():
         "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "config1: 40 chunks x 10 kb (+-500 bp), 400 kb ref, 60x reads mean 8 kb, 9% error, "
-                               "RLE, anchor expansion 10, allParams.np.json, one POA realign iteration",
+        "config": {"workload": WORKLOADS[args.workload] % w.chunks,
                    "pairs": len(pairs), "cells": int(cells), "band_half_width": 10},
         "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
         "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),

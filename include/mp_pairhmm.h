@@ -110,6 +110,14 @@ mp_status mp_model_tables(const mp_model *m, int32_t strand, double *trans9, dou
 /* repeatSubMatrix_getLogProb (polisher.c:1756-1775): log10 P(observed | underlying, base, strand). */
 double mp_model_repeat_log10(const mp_model *m, int32_t base, int32_t strand, int32_t observed, int32_t underlying);
 int32_t mp_model_max_repeat_count(const mp_model *m);  /* maximumRepeatLength (51) */
+/* stateMachine3_loadAsymmetric's transition part (stateMachine.c:632-668), as
+ * hmm_getStateMachine (676-689) applies an Hmm's trained transitions: replace
+ * both strands' natural-log transitions (order of mp_model_tables' trans9).
+ * pinEnd != 0 keeps the end-state vector (stateMachine.c:537-557) of the
+ * transitions in force at the first pinned call, so that the derivative of a
+ * pair's log-likelihood in each log transition is exactly its expected count
+ * (the parameter-fitting harness); pinEnd = 0 unpins. */
+mp_status mp_model_set_transitions(mp_model *m, const double *logTrans9, int32_t pinEnd);
 void mp_model_free(mp_model *m);
 
 /* ---- host geometry (no device needed) ---- */
@@ -154,6 +162,11 @@ mp_status mp_pairhmm_posteriors(mp_engine *e, const mp_model *m, const mp_pair *
  * traceback windows, band-area job ordering, output sizing. */
 mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
                           const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out);
+/* Re-read the model's transitions and emission tables into an existing batch
+ * (same pairs, bands and outputs): the per-iteration model update of
+ * parameter fitting.  No counterpart in the reference, whose state machine is
+ * a pointer read at every cell. */
+mp_status mp_batch_set_model(mp_batch *b, const mp_model *m);
 /* One persistent launch over every pair (outputs stay on the device). */
 mp_status mp_batch_run(mp_batch *b);
 /* The same work as one kernel launch per pair (the plain per-read path). */

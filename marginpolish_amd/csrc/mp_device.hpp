@@ -128,7 +128,6 @@ struct KParams {
     int32_t ringWideLen, wideC, wideBegin, wideEnd;
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;     // anti-diagonals per ring (multiple of kK)
-    int32_t outLen;      // records in `out` (< kMaxRecords)
     int32_t withIndels;
     int32_t mode;
     float threshold;
@@ -146,18 +145,12 @@ struct KParams {
 constexpr int kBufWord3 = 0x00020000;
 constexpr int kCachePolicyNT = 2;           // gfx94x/gfx950 cache-policy bit "nt"
 constexpr uint32_t kBufOff = 0xfffffff0u;  // a raw offset beyond every extent used here
-constexpr int64_t kMaxRecords = (int64_t)1 << 28;  // record buffer extent < 2^32 bytes
 typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // (non-temporal: ring values are read back by the partner sweep much later)
 __device__ __forceinline__ void buf_store3(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c) {
     const u32x3 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
     __builtin_amdgcn_raw_buffer_store_b96(v, r, (int)off, 0, kCachePolicyNT);
-}
-__device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, uint32_t off, int4 v) {
-    const u32x4 w = {(uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w};
-    __builtin_amdgcn_raw_buffer_store_b128(w, r, (int)off, 0, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -411,6 +411,22 @@ extern "C" mp_status mp_model_default_nucleotide(mp_model **out) {
     return MP_OK;
 }
 
+extern "C" mp_status mp_model_set_transitions(mp_model *m, const double *logTrans9, int32_t pinEnd) {
+    if (!m || !logTrans9) {
+        mp::set_error("mp_model_set_transitions: null argument");
+        return MP_EINVAL;
+    }
+    for (int i = 0; i < 9; i++)
+        if (std::isnan(logTrans9[i]) || logTrans9[i] > 0.0) {
+            mp::set_error("mp_model_set_transitions: transitions must be log probabilities (<= 0, not NaN)");
+            return MP_EINVAL;
+        }
+    if (pinEnd && !m->endPinned) std::memcpy(m->endTrans, m->mach[1].trans, sizeof(m->endTrans));
+    m->endPinned = pinEnd != 0;
+    for (int s = 0; s < 2; s++) std::memcpy(m->mach[s].trans, logTrans9, sizeof(m->mach[s].trans));
+    return MP_OK;
+}
+
 extern "C" mp_status mp_model_band_params(const mp_model *m, mp_band_params *out) {
     if (!m || !out) {
         mp::set_error("mp_model_band_params: null argument");

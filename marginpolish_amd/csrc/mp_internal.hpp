@@ -32,6 +32,8 @@ struct mp_model {
     mp::Machine mach[2];          // [0] reverse strand, [1] forward strand
     std::vector<double> repeatLog10;  // [5][51][51] log10, [base][underlying][observed]
     mp_band_params p;
+    bool endPinned = false;       // end-state vector from endTrans instead of the transitions
+    double endTrans[9] = {};
 };
 
 namespace mp {

@@ -207,12 +207,14 @@ void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
     kp.T[6] = P(mp::kMatchFromGapY); kp.T[7] = P(mp::kGapSwitchToX); kp.T[8] = P(mp::kGapExtendY);
     kp.startV[0][0] = 1.f; kp.startV[0][1] = 0.f; kp.startV[0][2] = 0.f;
     kp.startV[1][0] = 0.f; kp.startV[1][1] = 1.f; kp.startV[1][2] = 1.f;
-    kp.endV[0][0] = P(mp::kMatchContinue);
-    kp.endV[0][1] = P(mp::kMatchFromGapX);
-    kp.endV[0][2] = P(mp::kMatchFromGapY);
-    kp.endV[1][0] = (float)std::exp((f.trans[mp::kGapOpenX] + f.trans[mp::kGapOpenY]) / 2.0);
-    kp.endV[1][1] = P(mp::kGapExtendX);
-    kp.endV[1][2] = P(mp::kGapExtendY);
+    const double *et = m->endPinned ? m->endTrans : f.trans;
+    auto E = [&](int i) { return (float)std::exp(et[i]); };
+    kp.endV[0][0] = E(mp::kMatchContinue);
+    kp.endV[0][1] = E(mp::kMatchFromGapX);
+    kp.endV[0][2] = E(mp::kMatchFromGapY);
+    kp.endV[1][0] = (float)std::exp((et[mp::kGapOpenX] + et[mp::kGapOpenY]) / 2.0);
+    kp.endV[1][1] = E(mp::kGapExtendX);
+    kp.endV[1][2] = E(mp::kGapExtendY);
 }
 
 // Crossing diagonal of window k for the concurrent sweeps: the forward row
@@ -599,11 +601,6 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     for (size_t q = 0; q < ord.size(); q++) b->jobs[q] = jobs[ord[q]];
     b->ringLen = ringLen;
     b->outTotal = outOff;
-    // the split kernel addresses records with 32-bit buffer offsets
-    if (outOff >= kMaxRecords) {
-        delete b;
-        return MP_EUNSUPPORTED;
-    }
 
     // ---- 4. upload ----
     hipStream_t s = e->stream;
@@ -656,7 +653,6 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.band = b->dBand.p;
     kp.win = b->dWin.p;
     kp.out = b->dOut.p;
-    kp.outLen = (int32_t)std::max<int64_t>(outOff, 1);
     kp.counts = b->dCounts.p;
     kp.winEnd = b->dWinEnd.p;
     kp.logTotal = b->dLogTotal.p;
@@ -765,14 +761,31 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
             off += (int64_t)dj[j].cap[0] + dj[j].cap[1];
         }
         HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
-        if (off >= kMaxRecords) return MP_EUNSUPPORTED;
         if ((st = b->dOut.alloc(std::max<int64_t>(off, 1)))) return st;
         b->kp.out = b->dOut.p;
-        b->kp.outLen = (int32_t)std::max<int64_t>(off, 1);
         b->outTotal = off;
         return batch_run_impl(b, plain);
     }
     b->ran = true;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
+    if (!b || !m) {
+        mp::set_error("mp_batch_set_model: null argument");
+        return MP_EINVAL;
+    }
+    std::vector<float> tab;
+    KParams kp = b->kp;
+    build_tables(m, tab, kp);
+    mp_status st;
+    if ((st = b->dTables.upload(tab, b->eng->stream))) return st;
+    std::memcpy(b->kp.T, kp.T, sizeof(kp.T));
+    std::memcpy(b->kp.startV, kp.startV, sizeof(kp.startV));
+    std::memcpy(b->kp.endV, kp.endV, sizeof(kp.endV));
+    b->kp.tables = b->dTables.p;
+    b->ran = false;
+    b->listsReady = false;
     return MP_OK;
 }
 

@@ -147,9 +147,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int *myExp = kp.ringExp + slot * (size_t)RC * 4;
     const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4;
     const uint32_t laneRing = (uint32_t)lane * 12u;
-    // masked stores go through buffer resources (out-of-range offsets are dropped)
+    // masked ring stores go through a buffer resource (out-of-range offsets are dropped)
     const __amdgpu_buffer_rsrc_t ringRs = __builtin_amdgcn_make_buffer_rsrc(myRing, 0, RC * kRingCellBytes, kBufWord3);
-    const __amdgpu_buffer_rsrc_t outRs = __builtin_amdgcn_make_buffer_rsrc(gOut, 0, (int)((uint32_t)kp.outLen * 16u), kBufWord3);
     const char *symB = reinterpret_cast<const char *>(kp.sym);
     const char *bandB = reinterpret_cast<const char *>(kp.band);
     const float T00 = kp.T[0], T01 = kp.T[1], T02 = kp.T[2], T10 = kp.T[3], T11 = kp.T[4], T12 = kp.T[5],
@@ -511,9 +510,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
                 const int idx = cnt + (below - first);
                 if (emit & (idx < cap))
-                    buf_store4(outRs, (uint32_t)(outBase + idx) * 16u,
-                               make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
-                                         (int)rec_code(dj, kx)));
+                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                                    (int)rec_code(dj, kx));
                 cnt += last - first;
             }
             if (!FULL && dj == stop) reached = true;
@@ -586,9 +584,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const int last = rbcast<SEG>(below + (emit ? 1 : 0), rowBase, SEG - 1);
                 const int idx = cnt + (below - first);
                 if (emit & (idx < cap))
-                    buf_store4(outRs, (uint32_t)(outBase + idx) * 16u,
-                               make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
-                                         (int)rec_code(dj, kx)));
+                    gOut[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX), __float_as_int(pY),
+                                                    (int)rec_code(dj, kx));
                 cnt += last - first;
             }
             if (!FULL && dj == stop) reached = true;

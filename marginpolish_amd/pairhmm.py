@@ -84,6 +84,12 @@ class StateMachines:
         check(lib().mp_model_tables(self._h, int(strand), t, m, gx, gy, C.byref(rle)))
         return dict(trans=list(t), match=list(m), gapX=list(gx), gapY=list(gy), rle=bool(rle.value))
 
+    def set_transitions(self, log_trans9, pin_end=False):
+        """Replace both strands' natural-log transitions (order of
+        tables()['trans']); pin_end keeps the end-state vector fixed."""
+        t = (C.c_double * 9)(*[float(v) for v in log_trans9])
+        check(lib().mp_model_set_transitions(self._h, t, int(bool(pin_end))))
+
     def repeat_log10(self, base, strand, observed, underlying):
         return lib().mp_model_repeat_log10(self._h, base, strand, observed, underlying)
 
@@ -217,6 +223,10 @@ class Batch:
                 check(lib().mp_batch_fetch_list(self._h, li, buf.ctypes.data_as(C.POINTER(_lib.AlignedPair))))
             lists.append(np.split(buf[:tot], np.cumsum(counts[:, li])[:-1]) if self.npairs else [])
         return [(lists[0][i], lists[1][i], lists[2][i], float(lp[i])) for i in range(self.npairs)]
+
+    def set_model(self, machines):
+        """Re-read a model's transitions and emission tables (same pairs)."""
+        check(lib().mp_batch_set_model(self._h, machines.handle))
 
     def expectations(self):
         """Expectation mode: (npairs, 9) expected transition counts T[from*3+to]

@@ -202,3 +202,37 @@ def test_engine_without_device_is_an_error():
     with pytest.raises(_lib.MPError) as ei:
         ph.Engine(0)
     assert ei.value.status == 7
+
+
+def test_fit_sampler_and_gradient_plumbing():
+    """The pair-HMM sampler of the fitting harness follows its transition
+    matrix (path frequencies), its anchors are valid, and the autograd
+    function returns the evaluator's counts as the gradient."""
+    import torch
+    from marginpolish_amd import fit
+    T = np.array([[0.90, 0.05, 0.05], [0.70, 0.25, 0.05], [0.70, 0.05, 0.25]])
+    pairs = fit.simulate_pairs(300, 400, T, np.full(16, 1 / 16), np.full(4, 0.25), np.full(4, 0.25), 10, seed=1)
+    n = np.zeros((3, 3))
+    for sx, sy, an, strand in pairs:
+        assert strand == 1 and an.shape[1] == 3 and np.all(an[:, 2] == 10)
+        assert np.all(np.diff(an[:, 0]) > 0) and np.all(np.diff(an[:, 1]) > 0)
+        assert an[:, 0].max() < sx.size and an[:, 1].max() < sy.size and sx.size == 400
+        # recover the state path from the anchors: between consecutive matches,
+        # dx - 1 gap-X and dy - 1 gap-Y states (in some order) are hidden, so
+        # count only match-to-match steps, which identify T[M, M]
+        d = np.diff(an[:, :2], axis=0)
+        n[0, 0] += np.sum((d[:, 0] == 1) & (d[:, 1] == 1))
+        n[0, 1] += np.sum((d[:, 0] > 1) | (d[:, 1] > 1))
+    assert abs(n[0, 0] / (n[0, 0] + n[0, 1]) - T[0, 0]) < 0.01
+    assert fit.matrix_from_named(fit.named_from_matrix(np.log(T))).round(12).tolist() == np.log(T).round(12).tolist()
+
+    class Quadratic:
+        pairs = [0]
+
+        def evaluate(self, logT):
+            c = np.arange(9.0).reshape(3, 3)
+            return float((c * logT).sum()), c
+
+    logT = torch.zeros(3, 3, dtype=torch.float64, requires_grad=True)
+    fit.log_likelihood(logT, Quadratic()).backward()
+    assert np.array_equal(logT.grad.numpy(), np.arange(9.0).reshape(3, 3))
