@@ -124,7 +124,7 @@ def main():
     achieved = FLOPS_PER_CELL * cells / (kernel_ms * 1e-3) / 1e12
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_split_kernel_config1.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and args.workload == "config1":  # measured on the config1 launch
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS,
                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic}
