@@ -16,12 +16,16 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mp_internal.hpp"
+#include "mp_lists.hpp"
 #include "mp_seq.hpp"
 #include "mp_split.hpp"
 #include "mp_wide.hpp"
@@ -33,16 +37,6 @@ namespace {
 __global__ void fill_i32(int32_t *p, int32_t v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// Dense copy of the used part of every record region (one segment per block).
-__global__ void gather_records(const int4 *src, const int64_t *segSrc, const int64_t *segDst, const int32_t *segN,
-                               int nseg, int4 *dst) {
-    for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const int64_t a = segSrc[s], b = segDst[s];
-        const int n = segN[s];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) dst[b + i] = src[a + i];
-    }
 }
 
 #define HIP_TRY(call)                                                                         \
@@ -147,7 +141,7 @@ struct mp_batch {
     std::vector<int32_t> hCounts;  // 2 per job, fetched after each run
     // materialized lists
     bool listsReady = false;
-    std::vector<mp_aligned_pair> lists[3];
+    DevBuf<mpk::ListTuple> dList[3];      // the lists on the device (copied out by fetch)
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
 };
@@ -321,69 +315,124 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     const bool split = use_split(mode);
 
     // ---- 1. encoded symbols, cropping, split points -> jobs (pair order) ----
-    // Symbol pool with padding at both ends (kernels read up to one row width
-    // beyond a sequence without clamping).
-    std::vector<uint32_t> sym(kSymPad, enc_x(4, 0));
-    sym[0] = kZeroSym;  // read by cells outside the band
-    std::vector<HostJob> jobs;
-    std::vector<std::vector<int64_t>> jobAnchors;
-    std::vector<int64_t> sp, anc;
-    for (int64_t i = 0; i < npairs; i++) {
+    // Two parallel passes over the pairs: (A) checks, reference crop, split
+    // points and sizes; (B) after prefix sums, symbol encoding into the pool and
+    // the jobs with their anchors.  The first failing pair (in order) is reported.
+    const int nthreads = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    auto parallel_for = [&](int64_t n, const std::function<void(int64_t)> &fn) {
+        std::atomic<int64_t> next{0};
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; t++)
+            th.emplace_back([&]() {
+                for (;;) {
+                    const int64_t i0 = next.fetch_add(16);
+                    if (i0 >= n) break;
+                    for (int64_t i = i0; i < std::min(n, i0 + 16); i++) fn(i);
+                }
+            });
+        for (auto &t : th) t.join();
+    };
+    struct PairPrep {
+        int64_t first = 0, end = 0;
+        std::vector<int64_t> sp;  // split points (x1, y1, x2, y2)
+        int err = 0;              // 1 bad arguments, 2 bad anchors, 3 sub-problem too long
+    };
+    std::vector<PairPrep> prep(npairs);
+    parallel_for(npairs, [&](int64_t i) {
         const mp_pair &pr = pairs[i];
+        PairPrep &pp = prep[i];
         if (pr.lX < 0 || pr.lY < 0 || (pr.lX > 0 && !pr.sX) || (pr.lY > 0 && !pr.sY) ||
             (pr.nAnchors > 0 && !pr.anchors) || pr.nAnchors < 0 || pr.lX > INT32_MAX / 2 || pr.lY > INT32_MAX / 2) {
-            delete b;
+            pp.err = 1;
+            return;
+        }
+        if (!mp::anchors_valid(pr.anchors, pr.nAnchors, pr.lX, pr.lY)) {
+            pp.err = 2;
+            return;
+        }
+        pp.first = 0;
+        pp.end = pr.lX;
+        if ((flags & MP_BATCH_CROP_REFERENCE) && pr.nAnchors > 0) {
+            const int64_t *a0 = pr.anchors, *al = pr.anchors + 3 * (pr.nAnchors - 1);
+            pp.first = std::max<int64_t>(a0[0] - a0[1], 0);
+            pp.end = std::min<int64_t>(1 + al[0] + (pr.lY - al[1]), pr.lX);
+        }
+        const int64_t lX = pp.end - pp.first, lY = pr.lY;
+        if (mode == MP_MODE_FORWARD_ONLY) {  // computeForwardProbability does not split
+            pp.sp.assign({0, 0, lX, lY});
+        } else {
+            std::vector<int64_t> anc(pr.anchors, pr.anchors + 3 * pr.nAnchors);
+            for (int64_t a = 0; a < pr.nAnchors; a++) anc[3 * a] -= pp.first;
+            mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
+                             pr.raggedRight != 0, pp.sp);
+        }
+        for (size_t s = 0; s + 3 < pp.sp.size(); s += 4)
+            if ((pp.sp[s + 2] - pp.sp[s]) + (pp.sp[s + 3] - pp.sp[s + 1]) >= kMaxDiagonals) pp.err = 3;
+    });
+    for (int64_t i = 0; i < npairs; i++) {
+        if (!prep[i].err) continue;
+        delete b;
+        if (prep[i].err == 1) {
             mp::set_error("pair " + std::to_string(i) + ": bad sequence or anchor arguments");
             return MP_EINVAL;
         }
-        if (!mp::anchors_valid(pr.anchors, pr.nAnchors, pr.lX, pr.lY)) {
-            delete b;
+        if (prep[i].err == 2) {
             mp::set_error("pair " + std::to_string(i) + ": anchors must be strictly increasing, inside the "
                           "sequences, with even non-negative expansion");
             return MP_EANCHORS;
         }
-        int64_t first = 0, end = pr.lX;
-        if ((flags & MP_BATCH_CROP_REFERENCE) && pr.nAnchors > 0) {
-            const int64_t *a0 = pr.anchors, *al = pr.anchors + 3 * (pr.nAnchors - 1);
-            first = std::max<int64_t>(a0[0] - a0[1], 0);
-            end = std::min<int64_t>(1 + al[0] + (pr.lY - al[1]), pr.lX);
-        }
-        b->pairCrop[i] = first;
-        const int64_t lX = end - first, lY = pr.lY;
-        anc.assign(pr.anchors ? pr.anchors : nullptr, pr.anchors ? pr.anchors + 3 * pr.nAnchors : nullptr);
-        for (int64_t a = 0; a < pr.nAnchors; a++) anc[3 * a] -= first;
-        // Each sequence is preceded by an N entry of its own kind: the cells x = 0
-        // (y = 0) of the band read the symbol before the sequence (stateMachine
-        // emissions there are never used, pairwiseAligner.c:528-549, but must be
-        // finite).
+        mp::set_error("pair " + std::to_string(i) + ": a sub-problem has more than " +
+                      std::to_string(kMaxDiagonals - 1) + " anti-diagonals");
+        return MP_EUNSUPPORTED;
+    }
+    // Symbol pool with padding at both ends (kernels read up to one row width
+    // beyond a sequence without clamping).  Each sequence is preceded by an N
+    // entry of its own kind: the cells x = 0 (y = 0) of the band read the symbol
+    // before the sequence (stateMachine emissions there are never used,
+    // pairwiseAligner.c:528-549, but must be finite).
+    std::vector<int64_t> symOff(npairs + 1, kSymPad);
+    int64_t njobs = 0;
+    for (int64_t i = 0; i < npairs; i++) {
+        symOff[i + 1] = symOff[i] + 2 + (prep[i].end - prep[i].first) + pairs[i].lY;
+        b->pairFirstJob[i] = njobs;
+        b->pairNJobs[i] = (int32_t)(prep[i].sp.size() / 4);
+        b->pairCrop[i] = prep[i].first;
+        njobs += (int64_t)prep[i].sp.size() / 4;
+    }
+    if (symOff[npairs] + kSymPad >= ((int64_t)1 << 30)) {
+        delete b;
+        mp::set_error("batch too large: more than 2^30 symbols (split it into several batches)");
+        return MP_EUNSUPPORTED;
+    }
+    std::vector<uint32_t> sym((size_t)(symOff[npairs] + kSymPad), enc_x(4, 0));
+    sym[0] = kZeroSym;  // read by cells outside the band
+    std::vector<HostJob> jobs((size_t)njobs);
+    std::vector<std::vector<int64_t>> jobAnchors((size_t)njobs);
+    parallel_for(npairs, [&](int64_t i) {
+        const mp_pair &pr = pairs[i];
+        const PairPrep &pp = prep[i];
         const int strand = pr.strand ? 1 : 0;
-        sym.push_back(enc_x(4, strand));
-        const int64_t baseX = (int64_t)sym.size();
-        for (int64_t x = first; x < end; x++) sym.push_back(enc_x(pr.sX[x], strand));
-        sym.push_back(enc_y(4, strand));
-        const int64_t baseY = (int64_t)sym.size();
-        for (int64_t y = 0; y < lY; y++) sym.push_back(enc_y(pr.sY[y], strand));
-        if (mode == MP_MODE_FORWARD_ONLY) {  // computeForwardProbability does not split
-            sp.assign({0, 0, lX, lY});
-        } else {
-            mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
-                             pr.raggedRight != 0, sp);
-        }
-        const int64_t ns = (int64_t)sp.size() / 4;
-        b->pairFirstJob[i] = (int64_t)jobs.size();
+        uint32_t *o = sym.data() + symOff[i];
+        *o++ = enc_x(4, strand);
+        const int64_t baseX = symOff[i] + 1;
+        for (int64_t x = pp.first; x < pp.end; x++) *o++ = enc_x(pr.sX[x], strand);
+        *o++ = enc_y(4, strand);
+        const int64_t baseY = baseX + (pp.end - pp.first) + 1;
+        for (int64_t y = 0; y < pr.lY; y++) *o++ = enc_y(pr.sY[y], strand);
+        const int64_t ns = (int64_t)pp.sp.size() / 4;
         int64_t j = 0;
         for (int64_t s = 0; s < ns; s++) {
-            const int64_t x1 = sp[4 * s], y1 = sp[4 * s + 1], x2 = sp[4 * s + 2], y2 = sp[4 * s + 3];
-            std::vector<int64_t> sub;
-            while (j < pr.nAnchors) {
-                const int64_t ax = anc[3 * j], ay = anc[3 * j + 1];
-                if (ax + ay >= x2 + y2) break;
-                sub.push_back(ax - x1);
-                sub.push_back(ay - y1);
-                sub.push_back(anc[3 * j + 2]);
-                j++;
+            const int64_t x1 = pp.sp[4 * s], y1 = pp.sp[4 * s + 1], x2 = pp.sp[4 * s + 2], y2 = pp.sp[4 * s + 3];
+            std::vector<int64_t> &sub = jobAnchors[b->pairFirstJob[i] + s];
+            const int64_t j0 = j;
+            while (j < pr.nAnchors && (pr.anchors[3 * j] - pp.first) + pr.anchors[3 * j + 1] < x2 + y2) j++;
+            sub.resize(3 * (size_t)(j - j0));
+            for (int64_t a = j0; a < j; a++) {
+                sub[3 * (a - j0)] = pr.anchors[3 * a] - pp.first - x1;
+                sub[3 * (a - j0) + 1] = pr.anchors[3 * a + 1] - y1;
+                sub[3 * (a - j0) + 2] = pr.anchors[3 * a + 2];
             }
-            HostJob hj{};
+            HostJob &hj = jobs[b->pairFirstJob[i] + s];
             hj.pair = i;
             hj.xOff = x1;
             hj.yOff = y1;
@@ -393,23 +442,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
             hj.lY = (int32_t)(y2 - y1);
             const bool rl = pr.raggedLeft || s > 0, rr = pr.raggedRight || s < ns - 1;
             hj.flags = strand | (rl ? 2 : 0) | (rr ? 4 : 0);
-            if ((int64_t)hj.lX + hj.lY >= kMaxDiagonals) {
-                delete b;
-                mp::set_error("pair " + std::to_string(i) + ": a sub-problem has more than " +
-                              std::to_string(kMaxDiagonals - 1) + " anti-diagonals");
-                return MP_EUNSUPPORTED;
-            }
-            jobs.push_back(hj);
-            jobAnchors.push_back(std::move(sub));
         }
-        b->pairNJobs[i] = (int32_t)ns;
-    }
-    sym.resize(sym.size() + kSymPad, enc_x(4, 0));
-    if (sym.size() >= ((size_t)1 << 30)) {
-        delete b;
-        mp::set_error("batch too large: more than 2^30 symbols (split it into several batches)");
-        return MP_EUNSUPPORTED;
-    }
+    });
 
     // ---- 2. bands, windows, classes (parallel over jobs) ----
     const int64_t nj = (int64_t)jobs.size();
@@ -815,20 +849,32 @@ extern "C" int64_t mp_batch_npairs(const mp_batch *b) { return b ? b->npairs : 0
 // reversed (pairwiseAligner.c:1072-1074); so a job contributes, for windows
 // last to first, its backward-side records reversed followed by its
 // forward-side records as written.
+// Exclusive scan of one list flag over the emission-ordered records.
+static mp_status scan_list_flag(const int4 *wf, int32_t *pos, int64_t n, int l, hipStream_t s) {
+    hipcub::TransformInputIterator<int32_t, mpk::ListBit, const int4 *> it(wf, mpk::ListBit{l});
+    size_t tmpBytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, it, pos, (int)n, s));
+    DevBuf<char> tmp;
+    mp_status st;
+    if ((st = tmp.alloc(std::max<size_t>(tmpBytes, 1)))) return st;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmpBytes, it, pos, (int)n, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return MP_OK;
+}
+
+// Per-pair log-likelihoods and, in posterior modes, the three lists in the
+// reference order (built on the device, mp_lists.hpp).
 static mp_status materialize(mp_batch *b) {
     if (b->listsReady) return MP_OK;
     const int32_t nj = (int32_t)b->jobs.size();
     const int64_t np = b->npairs;
     for (int l = 0; l < 3; l++) {
-        b->lists[l].clear();
+        b->dList[l].alloc(0);
         b->listStart[l].assign(np + 1, 0);
     }
     b->pairLogProb.assign(np, 0.0);
     std::vector<double> lp(std::max(nj, 1), 0.0);
     std::vector<int2> lt(std::max(nj, 1));
-    std::vector<int32_t> wend(2 * std::max<size_t>(b->winHost.size(), 1), 0);
-    std::vector<int4> rec;
-    std::vector<int64_t> dense(2 * (size_t)std::max(nj, 1) + 1, 0);
     if (nj > 0) {
         HIP_TRY(hipSetDevice(b->eng->device));
         HIP_TRY(hipMemcpy(lt.data(), b->dLogTotal.p, sizeof(int2) * nj, hipMemcpyDeviceToHost));
@@ -837,142 +883,94 @@ static mp_status materialize(mp_batch *b) {
             std::memcpy(&t, &lt[j].x, sizeof(float));
             lp[j] = std::log((double)t) + (double)lt[j].y * 0.69314718055994530942;
         }
-        const bool post = b->mode <= MP_MODE_MATCH_ONLY;
-        if (post) {
-            if (!b->winHost.empty())
-                HIP_TRY(hipMemcpy(wend.data(), b->dWinEnd.p, sizeof(int32_t) * 2 * b->winHost.size(),
-                                  hipMemcpyDeviceToHost));
-            std::vector<int64_t> segSrc(2 * nj), segDst(2 * nj);
-            std::vector<int32_t> segN(2 * nj);
-            for (int32_t j = 0; j < nj; j++)
-                for (int sd = 0; sd < 2; sd++) {
-                    const int32_t n = b->hCounts[2 * j + sd];
-                    segSrc[2 * j + sd] = b->jobs[j].outOff + (sd ? b->jobs[j].cap[0] : 0);
-                    segN[2 * j + sd] = n;
-                    segDst[2 * j + sd] = dense[2 * j + sd];
-                    dense[2 * j + sd + 1] = dense[2 * j + sd] + n;
-                }
-            const int64_t total = dense[2 * nj];
-            if (total > 0) {
-                DevBuf<int64_t> dSrc, dDst;
-                DevBuf<int32_t> dN;
-                DevBuf<int4> dDense;
-                mp_status st;
-                hipStream_t s = b->eng->stream;
-                if ((st = dSrc.upload(segSrc, s)) || (st = dDst.upload(segDst, s)) || (st = dN.upload(segN, s)) ||
-                    (st = dDense.alloc(total)))
-                    return st;
-                hipLaunchKernelGGL(gather_records, dim3(std::min(2 * nj, 65535)), dim3(256), 0, s, b->dOut.p, dSrc.p,
-                                   dDst.p, dN.p, 2 * nj, dDense.p);
-                HIP_TRY(hipGetLastError());
-                rec.resize(total);
-                HIP_TRY(hipMemcpyAsync(rec.data(), dDense.p, sizeof(int4) * total, hipMemcpyDeviceToHost, s));
-                HIP_TRY(hipStreamSynchronize(s));
-            }
-        }
-    }
-    const float thr = b->kp.threshold;
-    const bool withIndels = b->kp.withIndels != 0;
-    // per pair: visit records in list order; pass 0 counts, pass 1 writes
-    auto visit = [&](int64_t i, int pass, int64_t *pos) {
-        int64_t c[3] = {0, 0, 0};
-        const int64_t crop = b->pairCrop[i];
-        for (int32_t sj = 0; sj < b->pairNJobs[i]; sj++) {
-            const int32_t q = b->order[b->pairFirstJob[i] + sj];
-            if (q < 0) continue;
-            const HostJob &hj = b->jobs[q];
-            const int32_t *bj = b->bandHost.data() + hj.bandOff;
-            const int4 *fr = rec.data() + dense[2 * q], *br = rec.data() + dense[2 * q + 1];
-            auto emitRec = [&](const int4 &r) {
-                const uint32_t code = (uint32_t)r.w;
-                const int d = (int)(code >> 11), cellIdx = (int)(code & 2047u);
-                const int x = band_xL(bj[d]) + cellIdx, y = d - x;
-                float pv[3];
-                std::memcpy(pv, &r, sizeof(pv));
-                const bool ok[3] = {pv[0] >= thr && x > 0 && y > 0, withIndels && pv[1] >= thr && x > 0,
-                                    withIndels && pv[2] >= thr && y > 0};
-                for (int l = 0; l < 3; l++) {
-                    if (!ok[l]) continue;
-                    if (pass == 1) {
-                        mp_aligned_pair &o = b->lists[l][pos[l] + c[l]];
-                        o.weight = (int64_t)std::floor(std::fmin(pv[l], 1.f) * 1e7f);
-                        o.x = (int32_t)(x - 1 + hj.xOff + crop);
-                        o.y = (int32_t)(y - 1 + hj.yOff);
-                    }
-                    c[l]++;
-                }
-            };
-            // one anti-diagonal's records, cells descending
-            auto emitBlock = [&](const int4 *r, int n) {
-                int ord[64];
-                const int m = std::min(n, 64);
-                for (int t = 0; t < m; t++) {
-                    int u = t;
-                    const uint32_t key = (uint32_t)r[t].w & 2047u;
-                    while (u > 0 && ((uint32_t)r[ord[u - 1]].w & 2047u) < key) {
-                        ord[u] = ord[u - 1];
-                        u--;
-                    }
-                    ord[u] = t;
-                }
-                for (int t = 0; t < m; t++) emitRec(r[ord[t]]);
-                for (int t = m; t < n; t++) emitRec(r[t]);
-            };
-            auto diagOf = [](const int4 &r) { return (uint32_t)r.w >> 11; };
-            for (int32_t w = hj.nWin - 1; w >= 0; w--) {
-                const int32_t g = hj.winOff + w;
-                const int32_t fEnd = wend[2 * g], bEnd = wend[2 * g + 1];
-                const int32_t fBeg = w == 0 ? 0 : wend[2 * (g - 1)], bBeg = w == 0 ? 0 : wend[2 * (g - 1) + 1];
-                // backward side: anti-diagonals descending, so walk its blocks from the end
-                for (int32_t t = bEnd; t > bBeg;) {
-                    int32_t u = t - 1;
-                    while (u > bBeg && diagOf(br[u - 1]) == diagOf(br[t - 1])) u--;
-                    emitBlock(br + u, t - u);
-                    t = u;
-                }
-                // forward side: anti-diagonals ascending
-                for (int32_t t = fBeg; t < fEnd;) {
-                    int32_t u = t + 1;
-                    while (u < fEnd && diagOf(fr[u]) == diagOf(fr[t])) u++;
-                    emitBlock(fr + t, u - t);
-                    t = u;
-                }
-            }
-        }
-        if (pass == 0)
-            for (int l = 0; l < 3; l++) b->listStart[l][i + 1] = c[l];
-    };
-    auto parallel = [&](int pass) {
-        const int nth = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-        std::atomic<int64_t> next{0};
-        std::vector<std::thread> th;
-        for (int t = 0; t < nth; t++)
-            th.emplace_back([&]() {
-                for (;;) {
-                    const int64_t i0 = next.fetch_add(64);
-                    if (i0 >= np) break;
-                    for (int64_t i = i0; i < std::min(np, i0 + 64); i++) {
-                        int64_t pos[3] = {b->listStart[0][i], b->listStart[1][i], b->listStart[2][i]};
-                        visit(i, pass, pos);
-                    }
-                }
-            });
-        for (auto &t : th) t.join();
-    };
-    if (b->mode <= MP_MODE_MATCH_ONLY && !rec.empty()) {
-        parallel(0);
-        for (int l = 0; l < 3; l++) {
-            for (int64_t i = 0; i < np; i++) b->listStart[l][i + 1] += b->listStart[l][i];
-            b->lists[l].resize(b->listStart[l][np]);
-        }
-        parallel(1);
     }
     for (int64_t i = 0; i < np; i++)
         for (int32_t sj = 0; sj < b->pairNJobs[i]; sj++) {
             const int32_t q = b->order[b->pairFirstJob[i] + sj];
             if (q >= 0) b->pairLogProb[i] += lp[q];
         }
+    if (b->mode <= MP_MODE_MATCH_ONLY && nj > 0) {
+        // emission index base of every job: pairs in order, a pair's jobs in split order
+        std::vector<mpk::JobLists> jl(nj);
+        std::vector<int64_t> pairStart(np + 1, 0);
+        int64_t total = 0;
+        for (int64_t i = 0; i < np; i++) {
+            pairStart[i] = total;
+            for (int32_t sj = 0; sj < b->pairNJobs[i]; sj++) {
+                const int32_t q = b->order[b->pairFirstJob[i] + sj];
+                if (q < 0) continue;
+                const HostJob &hj = b->jobs[q];
+                mpk::JobLists &J = jl[q];
+                J.outF = hj.outOff;
+                J.outB = hj.outOff + hj.cap[0];
+                J.base = total;
+                J.bandOff = hj.bandOff;
+                J.nF = b->hCounts[2 * q];
+                J.nB = b->hCounts[2 * q + 1];
+                J.winOff = hj.winOff;
+                J.nWin = hj.nWin;
+                J.xAdd = (int32_t)(hj.xOff + b->pairCrop[i] - 1);
+                J.yAdd = (int32_t)(hj.yOff - 1);
+                total += (int64_t)J.nF + J.nB;
+            }
+        }
+        pairStart[np] = total;
+        if (total >= INT32_MAX) {
+            mp::set_error("batch too large: more than 2^31 posterior records (split it into several batches)");
+            return MP_EUNSUPPORTED;
+        }
+        if (total > 0) {
+            hipStream_t s = b->eng->stream;
+            DevBuf<mpk::JobLists> dJL;
+            DevBuf<int64_t> dPairStart, dStarts;
+            DevBuf<int2> dXY;
+            DevBuf<int4> dWF;
+            DevBuf<int32_t> dPos[3];
+            mp_status st;
+            if ((st = dJL.upload(jl, s)) || (st = dPairStart.upload(pairStart, s)) || (st = dXY.alloc(total)) ||
+                (st = dWF.alloc(total)) || (st = dPos[0].alloc(total)) || (st = dPos[1].alloc(total)) ||
+                (st = dPos[2].alloc(total)) || (st = dStarts.alloc(3 * (np + 1))))
+                return st;
+            hipLaunchKernelGGL(mpk::order_records, dim3((unsigned)std::min<int64_t>(2 * (int64_t)nj, 65535)),
+                               dim3(256), 0, s, b->dOut.p, dJL.p, b->dWinEnd.p, b->dBand.p, b->kp.threshold,
+                               b->kp.withIndels, 2 * nj, dXY.p, dWF.p);
+            HIP_TRY(hipGetLastError());
+            for (int l = 0; l < 3; l++)
+                if ((st = scan_list_flag(dWF.p, dPos[l].p, total, l, s))) return st;
+            const unsigned gs = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
+            hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, dPairStart.p, np, total, dWF.p,
+                               dPos[0].p, dPos[1].p, dPos[2].p, dStarts.p);
+            HIP_TRY(hipGetLastError());
+            std::vector<int64_t> starts(3 * (np + 1));
+            HIP_TRY(hipMemcpyAsync(starts.data(), dStarts.p, sizeof(int64_t) * starts.size(), hipMemcpyDeviceToHost,
+                                   s));
+            HIP_TRY(hipStreamSynchronize(s));
+            DevBuf<mpk::ListTuple> *dList = b->dList;
+            for (int l = 0; l < 3; l++) {
+                for (int64_t i = 0; i <= np; i++) b->listStart[l][i] = starts[l * (np + 1) + i];
+                if ((st = dList[l].alloc(std::max<int64_t>(b->listStart[l][np], 1)))) return st;
+            }
+            const unsigned ge = (unsigned)std::min<int64_t>((total + 255) / 256, 65535 * 16);
+            hipLaunchKernelGGL(mpk::emit_lists, dim3(ge), dim3(256), 0, s, dXY.p, dWF.p, dPos[0].p, dPos[1].p,
+                               dPos[2].p, total, dList[0].p, dList[1].p, dList[2].p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+    }
     b->listsReady = true;
+    return MP_OK;
+}
+
+// First n tuples of list l (device) to host memory.
+static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t n) {
+    static_assert(sizeof(mpk::ListTuple) == sizeof(mp_aligned_pair), "tuple layout");
+    if (n <= 0) return MP_OK;
+    if (!out) {
+        mp::set_error("mp_batch_fetch_list: null output");
+        return MP_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(b->eng->device));
+    HIP_TRY(hipMemcpy(out, b->dList[l].p, sizeof(mp_aligned_pair) * n, hipMemcpyDeviceToHost));
     return MP_OK;
 }
 
@@ -1005,14 +1003,7 @@ extern "C" mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_p
     }
     mp_status st = materialize(b);
     if (st) return st;
-    if (!b->lists[list].empty()) {
-        if (!out) {
-            mp::set_error("mp_batch_fetch_list: null output");
-            return MP_EINVAL;
-        }
-        std::memcpy(out, b->lists[list].data(), sizeof(mp_aligned_pair) * b->lists[list].size());
-    }
-    return MP_OK;
+    return copy_list(b, list, out, b->listStart[list][b->npairs]);
 }
 
 extern "C" mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, double *likelihood) {
@@ -1106,7 +1097,10 @@ extern "C" mp_status mp_pairhmm_posteriors(mp_engine *e, const mp_model *m, cons
     for (int l = 0; l < 3; l++) {
         if (cnt[l]) *cnt[l] = counts[l];
         if (!dst[l] || counts[l] == 0) continue;
-        std::memcpy(dst[l], b->lists[l].data(), sizeof(mp_aligned_pair) * std::min(counts[l], caps[l]));
+        if ((st = copy_list(b, l, dst[l], std::min(counts[l], caps[l])))) {
+            mp_batch_free(b);
+            return st;
+        }
     }
     if (logProb) *logProb = lp;
     mp_batch_free(b);
