@@ -118,6 +118,25 @@ int32_t mp_model_max_repeat_count(const mp_model *m);  /* maximumRepeatLength (5
  * pair's log-likelihood in each log transition is exactly its expected count
  * (the parameter-fitting harness); pinEnd = 0 unpins. */
 mp_status mp_model_set_transitions(mp_model *m, const double *logTrans9, int32_t pinEnd);
+/* Five-state machine: no counterpart in the reference, whose StateMachineType
+ * has the 3-state types 2 and 3 only (stateMachine.h:96-99).  States M, short
+ * gap X, short gap Y, long gap X, long gap Y; a gap state of either length
+ * emits like the 3-state gap of its side, and a cell's gap-X (gap-Y)
+ * posterior is the sum over its two gap-X (gap-Y) states.  Boundary vectors:
+ * start in M; ragged start in any gap state; end through each state's
+ * transition to M; ragged end: M at exp of the mean of the two short-gap open
+ * logs, each gap state at its own extension.  The JSON loader reads such a
+ * machine from an hmm of "type": 5 with 25 "transitions" [from * 5 + to].
+ * Five-state models run every job on a one-wavefront kernel with a per-cell
+ * exponent (band widths up to 1280 cells; wider is MP_EUNSUPPORTED) in the
+ * posterior, match-only and forward-only modes; expected counts are
+ * MP_EUNSUPPORTED.
+ * mp_model_set_transitions5 makes the model five-state with these natural-log
+ * transitions [from * 5 + to] on both strands; mp_model_transitions5 reads
+ * them back; mp_model_num_states returns 3 or 5 (0 for a null model). */
+mp_status mp_model_set_transitions5(mp_model *m, const double *logTrans25);
+mp_status mp_model_transitions5(const mp_model *m, double *logTrans25);
+int32_t mp_model_num_states(const mp_model *m);
 void mp_model_free(mp_model *m);
 
 /* ---- host geometry (no device needed) ---- */

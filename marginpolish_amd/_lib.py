@@ -51,7 +51,8 @@ EXPORTS = [
     "mp_batch_run_plain", "mp_batch_kernel_ms", "mp_batch_cells", "mp_batch_npairs", "mp_batch_fetch_counts",
     "mp_batch_fetch_list", "mp_batch_free", "mp_band_construct", "mp_split_points", "mp_traceback_windows",
     "mp_batch_fetch_expectations", "mp_pairhmm_forward_log_prob", "mp_pairhmm_expectations",
-    "mp_model_set_transitions", "mp_batch_set_model",
+    "mp_model_set_transitions", "mp_batch_set_model", "mp_model_set_transitions5", "mp_model_transitions5",
+    "mp_model_num_states",
 ]
 
 _lib = None
@@ -98,6 +99,9 @@ def lib():
         "mp_pairhmm_forward_log_prob": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl)]),
         "mp_pairhmm_expectations": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl), P(dbl)]),
         "mp_model_set_transitions": (C.c_int, [vp, P(dbl), i32]),
+        "mp_model_set_transitions5": (C.c_int, [vp, P(dbl)]),
+        "mp_model_transitions5": (C.c_int, [vp, P(dbl)]),
+        "mp_model_num_states": (i32, [vp]),
         "mp_batch_set_model": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():

@@ -99,20 +99,36 @@ static void floats(const Value &v, double *out, size_t n, const char *what) {
     for (size_t i = 0; i < n; i++) out[i] = num(v.arr[i], what);
 }
 
+// The 3-state slots of a five-state machine hold its short-gap sub-machine
+// (M, short X, short Y), for inspection through mp_model_tables.
+static void short_gap_slots(Machine &m) {
+    const double *l5 = m.trans5;
+    m.trans[kMatchContinue] = l5[0];
+    m.trans[kGapOpenX] = l5[1];
+    m.trans[kGapOpenY] = l5[2];
+    m.trans[kMatchFromGapX] = l5[5];
+    m.trans[kGapExtendX] = l5[6];
+    m.trans[kGapSwitchToY] = l5[7];
+    m.trans[kMatchFromGapY] = l5[10];
+    m.trans[kGapSwitchToX] = l5[11];
+    m.trans[kGapExtendY] = l5[12];
+}
+
 // hmm_jsonParse + hmm_getStateMachine (stateMachine.c:206-266, 632-689).
 static Machine parse_hmm(const Value &h) {
     if (h.kind != Value::Object || h.obj.size() < 2 || h.obj[0].first != "type" || h.obj[1].first != "emissionsType")
         bad("hmm json must start with 'type' then 'emissionsType'");
     int type = (int)num(h.obj[0].second, "type");
     int etype = (int)num(h.obj[1].second, "emissionsType");
-    if (type != 2 && type != 3) bad("unrecognised state machine type " + std::to_string(type));
+    if (type != 2 && type != 3 && type != 5) bad("unrecognised state machine type " + std::to_string(type));
     if (etype != 0) bad("unrecognised emission type " + std::to_string(etype));
-    double t[9], e[24];
+    const int nt = type == 5 ? 25 : 9;
+    double t[25], e[24];
     bool gotT = false, gotE = false;
     for (size_t i = 2; i < h.obj.size(); i++) {
         const std::string &k = h.obj[i].first;
         if (k == "transitions") {
-            floats(h.obj[i].second, t, 9, "transitions");
+            floats(h.obj[i].second, t, (size_t)nt, "transitions");
             gotT = true;
         } else if (k == "emissions") {
             floats(h.obj[i].second, e, 24, "emissions");
@@ -125,16 +141,27 @@ static Machine parse_hmm(const Value &h) {
     }
     if (!gotE) bad("did not find emissions in hmm json");
     if (!gotT) bad("did not find transitions in hmm json");
+    Machine m;
+    if (type == 5) {
+        // Five-state extension (no counterpart in the reference, which knows
+        // types 2 and 3 only): 25 probabilities [from * 5 + to] over
+        // (M, short X, short Y, long X, long Y).
+        for (int i = 0; i < 25; i++) {
+            if (!(t[i] >= 0.0)) bad("five-state transitions must be non-negative probabilities");
+            m.trans5[i] = lg(t[i]);
+        }
+        m.nStates = 5;
+        short_gap_slots(m);
+    }
     auto T = [&](int a, int b) { return t[a * 3 + b]; };
     const int M = 0, X = 1, Y = 2;
-    Machine m;
     if (type == 2) {  // symmetric: average the X and Y variants in probability space
         m.trans[kMatchContinue] = lg(T(M, M));
         m.trans[kMatchFromGapX] = m.trans[kMatchFromGapY] = lg((T(X, M) + T(Y, M)) / 2.0);
         m.trans[kGapOpenX] = m.trans[kGapOpenY] = lg((T(M, X) + T(M, Y)) / 2.0);
         m.trans[kGapExtendX] = m.trans[kGapExtendY] = lg((T(X, X) + T(Y, Y)) / 2.0);
         m.trans[kGapSwitchToX] = m.trans[kGapSwitchToY] = lg((T(Y, X) + T(X, Y)) / 2.0);
-    } else {
+    } else if (type == 3) {
         m.trans[kMatchContinue] = lg(T(M, M));
         m.trans[kMatchFromGapX] = lg(T(X, M));
         m.trans[kMatchFromGapY] = lg(T(Y, M));
@@ -421,9 +448,48 @@ extern "C" mp_status mp_model_set_transitions(mp_model *m, const double *logTran
             mp::set_error("mp_model_set_transitions: transitions must be log probabilities (<= 0, not NaN)");
             return MP_EINVAL;
         }
+    if (m->mach[1].nStates != 3) {
+        mp::set_error("mp_model_set_transitions: the model is a five-state machine (use mp_model_set_transitions5)");
+        return MP_EINVAL;
+    }
     if (pinEnd && !m->endPinned) std::memcpy(m->endTrans, m->mach[1].trans, sizeof(m->endTrans));
     m->endPinned = pinEnd != 0;
     for (int s = 0; s < 2; s++) std::memcpy(m->mach[s].trans, logTrans9, sizeof(m->mach[s].trans));
+    return MP_OK;
+}
+
+extern "C" int32_t mp_model_num_states(const mp_model *m) { return m ? m->mach[1].nStates : 0; }
+
+extern "C" mp_status mp_model_set_transitions5(mp_model *m, const double *logTrans25) {
+    if (!m || !logTrans25) {
+        mp::set_error("mp_model_set_transitions5: null argument");
+        return MP_EINVAL;
+    }
+    for (int i = 0; i < 25; i++)
+        if (std::isnan(logTrans25[i]) || logTrans25[i] > 0.0) {
+            mp::set_error("mp_model_set_transitions5: transitions must be log probabilities (<= 0, not NaN)");
+            return MP_EINVAL;
+        }
+    for (int s = 0; s < 2; s++) {
+        mp::Machine &mc = m->mach[s];
+        std::memcpy(mc.trans5, logTrans25, sizeof(mc.trans5));
+        mc.nStates = 5;
+        mp::short_gap_slots(mc);
+    }
+    m->endPinned = false;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_model_transitions5(const mp_model *m, double *logTrans25) {
+    if (!m || !logTrans25) {
+        mp::set_error("mp_model_transitions5: null argument");
+        return MP_EINVAL;
+    }
+    if (m->mach[1].nStates != 5) {
+        mp::set_error("mp_model_transitions5: the model is a three-state machine");
+        return MP_EINVAL;
+    }
+    std::memcpy(logTrans25, m->mach[1].trans5, sizeof(m->mach[1].trans5));
     return MP_OK;
 }
 

@@ -17,7 +17,14 @@ constexpr int kRepeatEntries = 5 * kMaxRepeat * kMaxRepeat;
 enum Trans { kMatchContinue, kMatchFromGapX, kMatchFromGapY, kGapOpenX, kGapOpenY,
              kGapExtendX, kGapExtendY, kGapSwitchToX, kGapSwitchToY };
 
+// Five-state machine (no counterpart in the reference): M, short gap X, short
+// gap Y, long gap X, long gap Y.  A gap state of either length emits like the
+// three-state gap of its side.
+enum State5 { kS5Match, kS5ShortX, kS5ShortY, kS5LongX, kS5LongY };
+
 struct Machine {
+    int nStates = 3;
+    double trans5[25] = {};  // natural log, [from * 5 + to], when nStates == 5
     double trans[9];
     double match[16], gapX[4], gapY[4];
     int rle = 0;
@@ -29,7 +36,7 @@ void set_error(const std::string &msg);
 }  // namespace mp
 
 struct mp_model {
-    mp::Machine mach[2];          // [0] reverse strand, [1] forward strand
+    mp::Machine mach[2];          // [0] reverse strand, [1] forward strand (same nStates and transitions)
     std::vector<double> repeatLog10;  // [5][51][51] log10, [base][underlying][observed]
     mp_band_params p;
     bool endPinned = false;       // end-state vector from endTrans instead of the transitions

@@ -29,6 +29,7 @@
 #include "mp_seq.hpp"
 #include "mp_split.hpp"
 #include "mp_wide.hpp"
+#include "mp_wide5.hpp"
 
 using namespace mpk;
 
@@ -161,7 +162,10 @@ mp_status launch_narrow(mp_batch *b, int grid, int block, const KParams &kp) {
 }
 
 mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
-    hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), kWideLdsBytes, b->eng->stream, kp);
+    if (kp.nStates == 5)
+        hipLaunchKernelGGL(pairhmm_wide5_kernel, dim3(grid), dim3(64), kWide5LdsBytes, b->eng->stream, kp);
+    else
+        hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), kWideLdsBytes, b->eng->stream, kp);
     HIP_TRY(hipGetLastError());
     return MP_OK;
 }
@@ -209,6 +213,23 @@ void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
     kp.endV[1][0] = (float)std::exp((et[mp::kGapOpenX] + et[mp::kGapOpenY]) / 2.0);
     kp.endV[1][1] = E(mp::kGapExtendX);
     kp.endV[1][2] = E(mp::kGapExtendY);
+    kp.nStates = f.nStates;
+    if (f.nStates == 5) {
+        // Boundary vectors of the five-state machine, the three-state ones
+        // (stateMachine.c:518-557) extended to the long gaps: start in M;
+        // ragged start in any gap state; end through each state's transition
+        // to M; ragged end: M at the mean of the short-gap opens (log space),
+        // each gap state at its own extension.
+        const double *l5 = f.trans5;
+        for (int i = 0; i < 25; i++) kp.T5[i] = (float)std::exp(l5[i]);
+        for (int st = 0; st < 5; st++) {
+            kp.startV5[0][st] = st == mp::kS5Match ? 1.f : 0.f;
+            kp.startV5[1][st] = st == mp::kS5Match ? 0.f : 1.f;
+            kp.endV5[0][st] = (float)std::exp(l5[st * 5 + mp::kS5Match]);
+            kp.endV5[1][st] = st == mp::kS5Match ? (float)std::exp((l5[mp::kS5ShortX] + l5[mp::kS5ShortY]) / 2.0)
+                                                 : (float)std::exp(l5[st * 5 + st]);
+        }
+    }
 }
 
 // Crossing diagonal of window k for the concurrent sweeps: the forward row
@@ -258,6 +279,8 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
                                 kSeqLdsBytes));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 kWideLdsBytes));
+    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kWide5LdsBytes));
     int bpc = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void *)pairhmm_split_kernel, 512,
                                                          kSplitLdsBytes));
@@ -301,6 +324,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("mp_batch_create: band parameters violate the reference prerequisites "
                       "(pairwiseAligner.c:692-696)");
         return MP_EINVAL;
+    }
+    const bool five = m->mach[1].nStates == 5;
+    if (five && mode == MP_MODE_EXPECTATIONS) {
+        mp::set_error("mp_batch_create: expected transition counts are computed for three-state machines only");
+        return MP_EUNSUPPORTED;
     }
     *out = nullptr;
     HIP_TRY(hipSetDevice(e->device));
@@ -490,7 +518,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     }
                     hj.wmax = wmax;
                     hj.cells = D > 0 ? cells : 0;
-                    hj.cls = wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3));
+                    hj.cls = five ? 3 : (wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
                     hj.bandOff = bandOff[j];
                     int32_t *bj = band.data() + bandOff[j];
                     const bool narrow = hj.cls < 3;
@@ -556,11 +584,13 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("invalid band (anchors inconsistent with the sequences)");
         return MP_EANCHORS;
     }
+    const int32_t wideLimit = 64 * (five ? kWide5MaxC : kWideMaxC);
     for (int64_t j = 0; j < nj; j++) {
-        if (jobs[j].wmax > 64 * kWideMaxC) {
+        if (jobs[j].wmax > wideLimit) {
             delete b;
             mp::set_error("pair " + std::to_string(jobs[j].pair) + ": band width " + std::to_string(jobs[j].wmax) +
-                          " exceeds the " + std::to_string(64 * kWideMaxC) + "-cell kernel limit");
+                          " exceeds the " + std::to_string(wideLimit) + "-cell kernel limit" +
+                          (five ? " of the five-state machine" : ""));
             return MP_EUNSUPPORTED;
         }
     }
@@ -673,8 +703,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
     if (b->wideEnd > b->wideBegin) {
-        // wide jobs: one wavefront each, ring of float4 per cell; bounded to ~48 GB
-        const size_t perWave = (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * sizeof(float4);
+        // wide jobs: one wavefront each, ring of one float4 per cell (two with five states: values and
+        // the cell's exponent); bounded to ~48 GB
+        const size_t perWave =
+            (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * sizeof(float4) * (five ? 2 : 1);
         const size_t cap = std::max<size_t>(1, ((size_t)48 << 30) / perWave);
         b->wideWaves = (int32_t)std::min<size_t>({(size_t)(b->wideEnd - b->wideBegin), (size_t)e->cus * 4, cap});
         if ((st = b->dRingWide.alloc((size_t)b->wideWaves * perWave / sizeof(float)))) {
@@ -809,6 +841,10 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
         mp::set_error("mp_batch_set_model: null argument");
         return MP_EINVAL;
     }
+    if (m->mach[1].nStates != b->kp.nStates) {
+        mp::set_error("mp_batch_set_model: the model's state count differs from the batch's");
+        return MP_EINVAL;
+    }
     std::vector<float> tab;
     KParams kp = b->kp;
     build_tables(m, tab, kp);
@@ -817,6 +853,9 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
     std::memcpy(b->kp.T, kp.T, sizeof(kp.T));
     std::memcpy(b->kp.startV, kp.startV, sizeof(kp.startV));
     std::memcpy(b->kp.endV, kp.endV, sizeof(kp.endV));
+    std::memcpy(b->kp.T5, kp.T5, sizeof(kp.T5));
+    std::memcpy(b->kp.startV5, kp.startV5, sizeof(kp.startV5));
+    std::memcpy(b->kp.endV5, kp.endV5, sizeof(kp.endV5));
     b->kp.tables = b->dTables.p;
     b->ran = false;
     b->listsReady = false;

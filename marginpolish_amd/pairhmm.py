@@ -20,6 +20,7 @@ libmp_pairhmm.so; errors raise MPError (the reference aborts instead).
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -45,8 +46,36 @@ class PairwiseAlignmentParameters:
                                int(bool(self.dynamicAnchorExpansion)))
 
 
+# Long-gap defaults of the five-state machine: open 0.0018 from M to each
+# side, extension 0.9965 (mean long-gap length 1 / 0.0035 ~ 286 positions).
+LONG_GAP_OPEN = 0.0018
+LONG_GAP_EXTEND = 0.9965
+
+
+def with_long_gaps(trans9, long_open=LONG_GAP_OPEN, long_extend=LONG_GAP_EXTEND):
+    """5x5 natural-log transitions [from, to] over (M, short X, short Y,
+    long X, long Y) from 3-state natural-log transitions (order of
+    StateMachines.tables()['trans']): the short-gap rows and columns are the
+    3-state ones, M opens each long gap with probability long_open (taken from
+    M -> M), and a long gap extends with long_extend or returns to M."""
+    t = [float(v) for v in trans9]
+    mm = math.exp(t[0]) - 2.0 * long_open
+    if not (mm > 0.0 and 0.0 <= long_open and 0.0 < long_extend < 1.0):
+        raise ValueError("long-gap parameters leave no match-continue probability or are out of range")
+    lg = lambda v: math.log(v) if v > 0.0 else -math.inf
+    T = np.full((5, 5), -math.inf)
+    T[0, 0], T[0, 1], T[0, 2], T[0, 3], T[0, 4] = math.log(mm), t[3], t[4], lg(long_open), lg(long_open)
+    T[1, 0], T[1, 1], T[1, 2] = t[1], t[5], t[8]
+    T[2, 0], T[2, 1], T[2, 2] = t[2], t[7], t[6]
+    T[3, 0], T[3, 3] = math.log(1.0 - long_extend), math.log(long_extend)
+    T[4, 0], T[4, 4] = math.log(1.0 - long_extend), math.log(long_extend)
+    return T
+
+
 class StateMachines:
-    """Both strands' 3-state machines and tables (an mp_model)."""
+    """Both strands' state machines and tables (an mp_model): the reference's
+    3-state machine, or the five-state extension (M, short X, short Y,
+    long X, long Y) made by set_transitions5 or a "type": 5 hmm."""
 
     def __init__(self, handle):
         self._h = handle
@@ -89,6 +118,26 @@ class StateMachines:
         tables()['trans']); pin_end keeps the end-state vector fixed."""
         t = (C.c_double * 9)(*[float(v) for v in log_trans9])
         check(lib().mp_model_set_transitions(self._h, t, int(bool(pin_end))))
+
+    @property
+    def num_states(self):
+        return int(lib().mp_model_num_states(self._h))
+
+    def set_transitions5(self, log_trans25):
+        """Make both strands five-state with these natural-log transitions,
+        a 5x5 matrix (or 25 values) [from, to] over (M, Xs, Ys, Xl, Yl)."""
+        v = [float(x) for x in np.asarray(log_trans25, dtype=np.float64).reshape(25)]
+        check(lib().mp_model_set_transitions5(self._h, (C.c_double * 25)(*v)))
+
+    def transitions5(self):
+        t = (C.c_double * 25)()
+        check(lib().mp_model_transitions5(self._h, t))
+        return np.array(list(t)).reshape(5, 5)
+
+    def add_long_gaps(self, long_open=LONG_GAP_OPEN, long_extend=LONG_GAP_EXTEND):
+        """Make the machine five-state: its 3-state transitions become the
+        short-gap part and long gaps are added (with_long_gaps)."""
+        self.set_transitions5(with_long_gaps(self.tables(1)["trans"], long_open, long_extend))
 
     def repeat_log10(self, base, strand, observed, underlying):
         return lib().mp_model_repeat_log10(self._h, base, strand, observed, underlying)

@@ -12,7 +12,7 @@ extern "C" {
 #endif
 
 enum { OR_LOGADD_POLY = 0, OR_LOGADD_EXACT = 1 };
-enum { OR_OK = 0, OR_EBAND = 1 };
+enum { OR_OK = 0, OR_EBAND = 1, OR_EUNSUPPORTED = 2 };
 
 /* A resolved 3-state machine in natural-log space (StateMachine3 fields,
  * stateMachine.c:490-500) plus the emission tables it reads. */
@@ -24,6 +24,12 @@ typedef struct {
     int strand;                 /* 1 forward, 0 reverse (selects the repeat-table row) */
     int maxRepeat;              /* repeat-table side, 51 */
     const double *repeatLog10;  /* [5][maxRepeat][maxRepeat] log10, [base][underlying][observed] */
+    /* Five-state extension (no reference counterpart): nStates == 5 selects
+     * states M, short X, short Y, long X, long Y with natural-log transitions
+     * trans5[from * 5 + to]; the 3-state fields are then unused.  0 or 3
+     * select the 3-state machine above. */
+    int nStates;
+    double trans5[25];
 } or_model;
 
 /* PairwiseAlignmentParameters (pairwiseAligner.h:28-49), the fields on the path. */

@@ -38,6 +38,7 @@ class OrModel(C.Structure):
         ("match", C.c_double * 16), ("gapX", C.c_double * 4), ("gapY", C.c_double * 4),
         ("rle", C.c_int), ("strand", C.c_int), ("maxRepeat", C.c_int),
         ("repeatLog10", C.POINTER(C.c_double)),
+        ("nStates", C.c_int), ("trans5", C.c_double * 25),
     ]
 
 
@@ -117,9 +118,13 @@ def _log(v):
 
 
 class Model:
-    """A resolved 3-state machine (log space) plus the RLE repeat table."""
+    """A resolved 3-state machine (log space) plus the RLE repeat table.
+    With trans5 (a 5x5 natural-log matrix [from, to] over M, short X, short
+    Y, long X, long Y) it is the five-state machine instead, and `trans` is
+    not used by the recursions."""
 
-    def __init__(self, trans, match, gapX, gapY, rle=False, strand=1, repeat=None):
+    def __init__(self, trans, match, gapX, gapY, rle=False, strand=1, repeat=None, trans5=None):
+        self.trans5 = None if trans5 is None else np.asarray(trans5, dtype=np.float64).reshape(5, 5).copy()
         self.trans = dict(trans)
         self.match = list(match)
         self.gapX = list(gapX)
@@ -142,6 +147,11 @@ class Model:
             m.strand = self.strand
             m.maxRepeat = MAX_REPEAT
             m.repeatLog10 = self.repeat.ctypes.data_as(C.POINTER(C.c_double))
+            if self.trans5 is not None:
+                m.nStates = 5
+                m.trans5 = (C.c_double * 25)(*[float(v) for v in self.trans5.reshape(25)])
+            else:
+                m.nStates = 3
             self._c = m
         return self._c
 
@@ -149,7 +159,8 @@ class Model:
         """nucleotideEmissions_reverseComplement (stateMachine.c:454-470):
         M'[i][j] = M[3-i][3-j], G'[i] = G[3-i]."""
         m = [self.match[(3 - i) * 4 + (3 - j)] for i in range(4) for j in range(4)]
-        return Model(self.trans, m, self.gapX[::-1], self.gapY[::-1], self.rle, self.strand, self.repeat)
+        return Model(self.trans, m, self.gapX[::-1], self.gapY[::-1], self.rle, self.strand, self.repeat,
+                     self.trans5)
 
 
 def default_nucleotide_model():
@@ -366,6 +377,8 @@ def expectations(model, sX, sY, anchors, params, ragged_left=False, ragged_right
     r = L.or_expectations(C.byref(model.c()), _ptr(sx, C.c_uint16), len(sx), _ptr(sy, C.c_uint16), len(sy),
                           _ptr(a, C.c_int64), n, C.byref(pc), int(ragged_left), int(ragged_right), mode)
     try:
+        if L.or_result_status(r) != 0:
+            raise ValueError("oracle status %d" % L.or_result_status(r))
         e = np.zeros(9)
         L.or_result_expectations(r, _ptr(e, C.c_double))
         return e, L.or_result_likelihood(r)

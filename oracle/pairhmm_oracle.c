@@ -25,6 +25,12 @@
  * points vs test_getSplitPoints, AGCG/AGTTCG posterior set vs
  * test_diagonalDPCalculations, forward==backward totals, brute-force path
  * enumeration on tiny inputs, removeOverlap crop (6,2).
+ *
+ * Five-state machine (M, short X, short Y, long X, long Y): no reference
+ * counterpart; the same driver with a generic 5x5 cell update, pinned by
+ * brute-force path enumeration, by forward == backward totals, by the
+ * per-x / per-y posterior sums of a single-window global alignment, and by
+ * reducing to the 3-state machine when the long-gap states are unreachable.
  */
 #include <math.h>
 #include <stdint.h>
@@ -41,6 +47,12 @@
 #define PROB_ONE_INT 10000000   /* PAIR_ALIGNMENT_PROB_1, pairwiseAligner.h:22 */
 
 enum { ST_M = 0, ST_X = 1, ST_Y = 2 };
+enum { S5_M = 0, S5_XS = 1, S5_YS = 2, S5_XL = 3, S5_YL = 4 };
+#define MAX_NS 5
+
+static inline int nstates(const or_model *m) { return m->nStates == 5 ? 5 : 3; }
+/* state kind: 0 match (from (x-1, y-1)), 1 gap X (from (x-1, y)), 2 gap Y (from (x, y-1)) */
+static inline int kind5(int s) { return s == 0 ? 0 : ((s & 1) ? 1 : 2); }
 
 /* ------------------------------------------------------------------------ */
 /* logAdd (pairwiseAligner.c:279-299)                                        */
@@ -230,6 +242,7 @@ int64_t or_split_points(const int64_t *anchors, int64_t n, int64_t lX, int64_t l
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
+    int ns;                 /* states per cell, 3 or 5 */
     int64_t D;              /* last anti-diagonal index (lX + lY) */
     const int64_t *L, *R;   /* band */
     int64_t *off;           /* cell offset of each diagonal */
@@ -240,7 +253,7 @@ typedef struct {
 static double *cellp(const Dp *dp, double *base, int64_t xay, int64_t xmy) {
     if (xay < 0 || xay > dp->D) return NULL;
     if (xmy < dp->L[xay] || xmy > dp->R[xay]) return NULL;
-    return base + 3 * (dp->off[xay] + (xmy - dp->L[xay]) / 2);
+    return base + dp->ns * (dp->off[xay] + (xmy - dp->L[xay]) / 2);
 }
 
 static inline uint16_t xsym(const uint16_t *sX, int64_t xay, int64_t xmy) {
@@ -255,9 +268,39 @@ static inline uint16_t ysym(const uint16_t *sY, int64_t xay, int64_t xmy) {
 
 /* stateMachine3_cellCalculate (stateMachine.c:559-581) with doTransitionForward
  * (pairwiseAligner.c:311-314). */
+/* Five-state cell update: each target state t reads the cell its kind steps
+ * from, with that kind's emission, through T5[s][t] from every state s. */
+static void cell_forward5(const Ctx *c, double *cur, const double *lo, const double *mid, const double *up,
+                          uint16_t cx, uint16_t cy) {
+    const or_model *m = c->m;
+    for (int t = 0; t < 5; t++) {
+        const int k = kind5(t);
+        const double *src = k == 0 ? mid : (k == 1 ? lo : up);
+        if (!src) continue;
+        const double e = k == 0 ? emit_match(m, cx, cy) : (k == 1 ? emit_gap_x(m, cx) : emit_gap_y(m, cy));
+        for (int s = 0; s < 5; s++) cur[t] = LADD(c, cur[t], src[s] + (e + m->trans5[s * 5 + t]));
+    }
+}
+
+static void cell_backward5(const Ctx *c, const double *cur, double *lo, double *mid, double *up,
+                           uint16_t cx, uint16_t cy) {
+    const or_model *m = c->m;
+    for (int t = 0; t < 5; t++) {
+        const int k = kind5(t);
+        double *src = k == 0 ? mid : (k == 1 ? lo : up);
+        if (!src) continue;
+        const double e = k == 0 ? emit_match(m, cx, cy) : (k == 1 ? emit_gap_x(m, cx) : emit_gap_y(m, cy));
+        for (int s = 0; s < 5; s++) src[s] = LADD(c, src[s], cur[t] + (e + m->trans5[s * 5 + t]));
+    }
+}
+
 static void cell_forward(const Ctx *c, double *cur, const double *lo, const double *mid, const double *up,
                          uint16_t cx, uint16_t cy) {
     const or_model *m = c->m;
+    if (m->nStates == 5) {
+        cell_forward5(c, cur, lo, mid, up, cx, cy);
+        return;
+    }
     if (lo) {
         double e = emit_gap_x(m, cx);
         cur[ST_X] = LADD(c, cur[ST_X], lo[ST_M] + (e + m->gapOpenX));
@@ -282,6 +325,10 @@ static void cell_forward(const Ctx *c, double *cur, const double *lo, const doub
 static void cell_backward(const Ctx *c, const double *cur, double *lo, double *mid, double *up,
                           uint16_t cx, uint16_t cy) {
     const or_model *m = c->m;
+    if (m->nStates == 5) {
+        cell_backward5(c, cur, lo, mid, up, cx, cy);
+        return;
+    }
     if (lo) {
         double e = emit_gap_x(m, cx);
         lo[ST_M] = LADD(c, lo[ST_M], cur[ST_X] + (e + m->gapOpenX));
@@ -302,14 +349,23 @@ static void cell_backward(const Ctx *c, const double *cur, double *lo, double *m
     }
 }
 
-/* Boundary vectors (stateMachine.c:518-557). */
+/* Boundary vectors (stateMachine.c:518-557).  Five states: start in M;
+ * ragged start in any gap state; end through each state's transition to M;
+ * ragged end: M at the mean of the short-gap open logs, each gap state at its
+ * own extension. */
 static void start_vec(const or_model *m, int ragged, double *v) {
-    (void)m;
-    if (ragged) { v[0] = LOG_ZERO; v[1] = 0.0; v[2] = 0.0; }
-    else { v[0] = 0.0; v[1] = LOG_ZERO; v[2] = LOG_ZERO; }
+    const int ns = nstates(m);
+    for (int s = 0; s < ns; s++) v[s] = ragged ? (s == 0 ? LOG_ZERO : 0.0) : (s == 0 ? 0.0 : LOG_ZERO);
 }
 
 static void end_vec(const or_model *m, int ragged, double *v) {
+    if (m->nStates == 5) {
+        for (int s = 0; s < 5; s++) {
+            if (ragged) v[s] = s == 0 ? (m->trans5[S5_XS] + m->trans5[S5_YS]) / 2.0 : m->trans5[s * 5 + s];
+            else v[s] = m->trans5[s * 5 + S5_M];
+        }
+        return;
+    }
     if (ragged) {
         v[0] = (m->gapOpenX + m->gapOpenY) / 2.0;
         v[1] = m->gapExtendX;
@@ -323,14 +379,13 @@ static void end_vec(const or_model *m, int ragged, double *v) {
 
 static void diag_fill(const Dp *dp, double *base, int64_t xay, const double *v) {
     int64_t w = (dp->R[xay] - dp->L[xay]) / 2 + 1;
-    double *p = base + 3 * dp->off[xay];
-    for (int64_t i = 0; i < w; i++) {
-        p[3 * i] = v[0]; p[3 * i + 1] = v[1]; p[3 * i + 2] = v[2];
-    }
+    double *p = base + dp->ns * dp->off[xay];
+    for (int64_t i = 0; i < w; i++)
+        for (int s = 0; s < dp->ns; s++) p[dp->ns * i + s] = v[s];
 }
 
 static void diag_zero(const Dp *dp, double *base, int64_t xay) {
-    double z[3] = {LOG_ZERO, LOG_ZERO, LOG_ZERO};
+    double z[MAX_NS] = {LOG_ZERO, LOG_ZERO, LOG_ZERO, LOG_ZERO, LOG_ZERO};
     diag_fill(dp, base, xay, z);
 }
 
@@ -360,14 +415,18 @@ static double cell_dot(const Ctx *c, const double *a, const double *b) {
     double t = a[0] + b[0];
     t = LADD(c, t, a[1] + b[1]);
     t = LADD(c, t, a[2] + b[2]);
+    if (c->m->nStates == 5) {
+        t = LADD(c, t, a[3] + b[3]);
+        t = LADD(c, t, a[4] + b[4]);
+    }
     return t;
 }
 
 static double diag_dot(const Ctx *c, const Dp *dp, const double *A, int64_t xa, const double *Bv, int64_t xb) {
     double t = LOG_ZERO;
     for (int64_t xmy = dp->L[xa]; xmy <= dp->R[xa]; xmy += 2) {
-        const double *a = A + 3 * (dp->off[xa] + (xmy - dp->L[xa]) / 2);
-        const double *b = Bv + 3 * (dp->off[xb] + (xmy - dp->L[xb]) / 2);
+        const double *a = A + dp->ns * (dp->off[xa] + (xmy - dp->L[xa]) / 2);
+        const double *b = Bv + dp->ns * (dp->off[xb] + (xmy - dp->L[xb]) / 2);
         t = LADD(c, t, cell_dot(c, a, b));
     }
     return t;
@@ -379,16 +438,16 @@ static double diag_total(const Ctx *c, Dp *dp, const uint16_t *sX, const uint16_
     if (xay + 1 <= dp->D && dp->hasB[xay + 1] && xay >= 1 && dp->hasF[xay - 1]) {
         int64_t d1 = xay + 1;
         int64_t w = (dp->R[d1] - dp->L[d1]) / 2 + 1;
-        double *mt = (double *)malloc(sizeof(double) * 3 * (size_t)w);
-        for (int64_t i = 0; i < 3 * w; i++) mt[i] = LOG_ZERO;
+        double *mt = (double *)malloc(sizeof(double) * dp->ns * (size_t)w);
+        for (int64_t i = 0; i < dp->ns * w; i++) mt[i] = LOG_ZERO;
         double acc = LOG_ZERO;
         for (int64_t xmy = dp->L[d1], i = 0; xmy <= dp->R[d1]; xmy += 2, i++) {
             const double *mid = cellp(dp, dp->F, xay - 1, xmy);
-            cell_forward(c, mt + 3 * i, NULL, mid, NULL, xsym(sX, d1, xmy), ysym(sY, d1, xmy));
+            cell_forward(c, mt + dp->ns * i, NULL, mid, NULL, xsym(sX, d1, xmy), ysym(sY, d1, xmy));
         }
         for (int64_t xmy = dp->L[d1], i = 0; xmy <= dp->R[d1]; xmy += 2, i++) {
             const double *b = cellp(dp, dp->B, d1, xmy);
-            acc = LADD(c, acc, cell_dot(c, mt + 3 * i, b));
+            acc = LADD(c, acc, cell_dot(c, mt + dp->ns * i, b));
         }
         free(mt);
         t = LADD(c, t, acc);
@@ -476,7 +535,13 @@ static void diag_posterior(const Ctx *c, Dp *dp, const uint16_t *sX, const uint1
         const double *f = cellp(dp, dp->F, xay, xmy);
         const double *b = cellp(dp, dp->B, xay, xmy);
         if (x > 0 && y > 0) add_post(&sub[0], x, y, exp((f[ST_M] + b[ST_M]) - total), thr);
-        if (kind == PF_ALL) {
+        if (kind == PF_ALL && dp->ns == 5) {
+            /* gap posteriors of the five-state machine: short + long */
+            if (x > 0)
+                add_post(&sub[1], x, y, exp((f[S5_XS] + b[S5_XS]) - total) + exp((f[S5_XL] + b[S5_XL]) - total), thr);
+            if (y > 0)
+                add_post(&sub[2], x, y, exp((f[S5_YS] + b[S5_YS]) - total) + exp((f[S5_YL] + b[S5_YL]) - total), thr);
+        } else if (kind == PF_ALL) {
             if (x > 0) add_post(&sub[1], x, y, exp((f[ST_X] + b[ST_X]) - total), thr);
             if (y > 0) add_post(&sub[2], x, y, exp((f[ST_Y] + b[ST_Y]) - total), thr);
         }
@@ -502,12 +567,13 @@ static int banded_fb(const Ctx *c, const int64_t *anchors, int64_t n, const uint
     for (int64_t d = 0; d <= D; d++) { dp.off[d] = tot; tot += (R[d] - L[d]) / 2 + 1; }
     dp.off[D + 1] = tot;
     res->cells += tot;
-    dp.F = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
-    dp.B = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
+    dp.ns = nstates(c->m);
+    dp.F = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
+    dp.B = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
     dp.hasF = (unsigned char *)calloc((size_t)D + 1, 1);
     dp.hasB = (unsigned char *)calloc((size_t)D + 1, 1);
 
-    double v[3];
+    double v[MAX_NS];
     start_vec(c->m, raggedL, v);
     diag_fill(&dp, dp.F, 0, v);
     dp.hasF[0] = 1;
@@ -558,6 +624,10 @@ static or_result *run_split(const or_model *m, const uint16_t *sX, int64_t lX, c
                             int kind, int mode) {
     Ctx c = {m, mode};
     or_result *res = (or_result *)calloc(1, sizeof(or_result));
+    if (kind == PF_EXPECT && nstates(m) == 5) {
+        res->status = OR_EUNSUPPORTED;  /* expected counts: 3-state machine only */
+        return res;
+    }
     int64_t cap = 16, ns;
     int64_t *sp = NULL;
     for (;;) {
@@ -625,11 +695,12 @@ double or_forward_prob(const or_model *m, const uint16_t *sX, int64_t lX, const 
     int64_t tot = 0;
     for (int64_t d = 0; d <= D; d++) { dp.off[d] = tot; tot += (R[d] - L[d]) / 2 + 1; }
     dp.off[D + 1] = tot;
-    dp.F = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
-    dp.B = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
+    dp.ns = nstates(m);
+    dp.F = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
+    dp.B = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
     dp.hasF = (unsigned char *)calloc((size_t)D + 1, 1);
     dp.hasB = (unsigned char *)calloc((size_t)D + 1, 1);
-    double v[3];
+    double v[MAX_NS];
     start_vec(m, raggedL, v);
     diag_fill(&dp, dp.F, 0, v);
     dp.hasF[0] = 1;
@@ -662,22 +733,23 @@ int or_full_dp(const or_model *m, const uint16_t *sX, int64_t lX, const uint16_t
     dp.off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(D + 2));
     int64_t tot = 0;
     for (int64_t d = 0; d <= D; d++) { dp.off[d] = tot; tot += (R[d] - L[d]) / 2 + 1; }
-    dp.F = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
-    dp.B = (double *)malloc(sizeof(double) * 3 * (size_t)tot);
+    dp.ns = nstates(m);
+    dp.F = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
+    dp.B = (double *)malloc(sizeof(double) * dp.ns * (size_t)tot);
     dp.hasF = (unsigned char *)calloc((size_t)D + 1, 1);
     dp.hasB = (unsigned char *)calloc((size_t)D + 1, 1);
     for (int64_t d = 0; d <= D; d++) {
         diag_zero(&dp, dp.F, d); diag_zero(&dp, dp.B, d);
         dp.hasF[d] = dp.hasB[d] = 1;
     }
-    double v[3];
+    double v[MAX_NS];
     start_vec(m, 0, v);
     diag_fill(&dp, dp.F, 0, v);
     end_vec(m, 0, v);
     diag_fill(&dp, dp.B, D, v);
     for (int64_t d = 1; d <= D; d++) diag_forward(&c, &dp, sX, sY, d);
     for (int64_t d = D; d > 0; d--) diag_backward(&c, &dp, sX, sY, d);
-    double e[3], s[3];
+    double e[MAX_NS], s[MAX_NS];
     end_vec(m, 0, e);
     start_vec(m, 0, s);
     double *fc = cellp(&dp, dp.F, D, lX - lY);
