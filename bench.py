@@ -43,8 +43,11 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=1200)
     ap.add_argument("--err-pairs", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("config1", "config2"), default="config1",
-                    help="config1 (the headline batch) or config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks)")
+    ap.add_argument("--workload", choices=("config1", "config2", "config4"), default="config1",
+                    help="config1 (the headline batch), config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks) "
+                         "or config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at one --band)")
+    ap.add_argument("--band", type=int, default=128, help="config4: anchor expansion W (128, 256, 512, 1024)")
+    ap.add_argument("--pairs", type=int, default=256, help="config4: number of 100 kb pairs")
     ap.add_argument("--max-chunks", type=int, default=None, help="limit the number of chunks (config2)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
@@ -57,6 +60,9 @@ def main():
 
     import numpy as np
     from marginpolish_amd import crop, pairhmm, sim
+
+    if args.workload == "config4":
+        return run_config4(args)
 
     if args.copy_calib_mib > 0:
         import torch
@@ -171,6 +177,141 @@ def main():
                     worst = 1.0  # an entry far above threshold on one side only
         out["max_abs_posterior_err"] = worst
         out["posterior_err_sample_pairs"] = int(min(args.err_pairs, len(pick)))
+    print(json.dumps(out), flush=True)
+
+
+FLOPS_PER_CELL_5 = 120  # five states: forward 55 (5 emission muls, 25 FMA sends), backward 55, posterior 10
+
+
+def _keyed(res, li):
+    """All pairs' tuples of list li as (sorted keys pair<<42 | (x+1)<<21 | (y+1), weights); x or y is -1 for
+    gap tuples before the first base of the other sequence."""
+    import numpy as np
+    keys, ws = [], []
+    for i, r in enumerate(res):
+        a = r[li]
+        keys.append((np.int64(i) << 42) | ((a["x"].astype(np.int64) + 1) << 21) | (a["y"].astype(np.int64) + 1))
+        ws.append(a["weight"].astype(np.int64))
+    k = np.concatenate(keys) if keys else np.zeros(0, np.int64)
+    w = np.concatenate(ws) if ws else np.zeros(0, np.int64)
+    o = np.argsort(k, kind="stable")
+    return k[o], w[o]
+
+
+def _list_err(ra, rb, thr, tol):
+    """Max |p_a - p_b| over tuples both hold, the largest weight held by one
+    side only (it must stay within tol of the threshold), and the number of
+    repeated (pair, x, y) keys (0 in a correct list)."""
+    import numpy as np
+    worst, edge, dups = 0.0, 0.0, 0
+    for li in range(3):
+        ka, wa = _keyed(ra, li)
+        kb, wb = _keyed(rb, li)
+        dups += int(np.sum(ka[1:] == ka[:-1]) + np.sum(kb[1:] == kb[:-1]))
+        pa = np.minimum(np.searchsorted(kb, ka), max(kb.size - 1, 0))
+        inA = (kb.size > 0) & (kb[pa] == ka) if kb.size else np.zeros(ka.size, bool)
+        pb = np.minimum(np.searchsorted(ka, kb), max(ka.size - 1, 0))
+        inB = (ka.size > 0) & (ka[pb] == kb) if ka.size else np.zeros(kb.size, bool)
+        if inA.any():
+            worst = max(worst, float(np.abs(wa[inA] - wb[pa[inA]]).max()) / 1e7)
+        for w, only in ((wa, ~inA), (wb, ~inB)):
+            if only.any():
+                edge = max(edge, float(w[only].max()) / 1e7)
+    return worst, edge, dups
+
+
+def run_config4(args):
+    """Config 4: the five-state machine on 100 kb pairs at one band W, FP32
+    (per-cell exponents) vs the FP64 GPU path, and both vs the CPU FP64
+    oracle on a bounded sample (the first ~6k RLE positions of one pair)."""
+    import numpy as np
+    from marginpolish_amd import pairhmm, sim
+    t0 = time.time()
+    w = sim.config4(n_pairs=args.pairs, W=args.band)
+    pairs = [(p.ref_sym, p.read_sym, p.anchors, p.strand) for p in w.pairs]
+    log("config4: %d pairs, W=%d, built in %.1f s" % (len(pairs), args.band, time.time() - t0))
+    pp = pairhmm.PolishParams(PARAMS)
+    mach = pp.machines
+    mach.add_long_gaps()
+    bp = pairhmm.PairwiseAlignmentParameters(threshold=0.01, minDiagsBetweenTraceBack=10000, traceBackDiagonals=40,
+                                             diagonalExpansion=args.band, splitMatrixBiggerThanThis=250_000_000,
+                                             dynamicAnchorExpansion=True)
+    eng = pairhmm.default_engine()
+    legs = {}
+    for tag, fp64 in (("fp32", False), ("fp64", True)):
+        t1 = time.time()
+        b = pairhmm.Batch(mach, pairs, bp, engine=eng, fp64=fp64)
+        b.run()
+        res = b.fetch()
+        t_first = time.time() - t1
+        kms = []
+        for _ in range(args.steps):
+            b.run()
+            kms.append(b.kernel_ms)
+        kernel_ms = float(np.mean(kms))
+        legs[tag] = {"kernel_ms": kernel_ms, "gcups": b.cells / (kernel_ms * 1e-3) / 1e9, "cells": int(b.cells),
+                     "first_iteration_s": t_first, "res": res}
+        log("%s: %.1f ms kernel, %.2f GCUPS, first iteration %.1f s" % (tag, kernel_ms, legs[tag]["gcups"], t_first))
+        b.close()
+        del b
+    r32, r64 = legs["fp32"]["res"], legs["fp64"]["res"]
+    err, edge, dups = _list_err(r32, r64, 0.01, 1e-4)
+    lp32 = np.array([r[3] for r in r32])
+    lp64 = np.array([r[3] for r in r64])
+    drift = np.abs(lp32 - lp64)
+    cells = legs["fp32"]["cells"]
+    kernel_ms = legs["fp32"]["kernel_ms"]
+    achieved = FLOPS_PER_CELL_5 * cells / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": "five-state pair-HMM GCUPS (long-band stress), FP32 per-cell exponents vs FP64 GPU vs CPU FP64",
+        "value": round(legs["fp32"]["gcups"], 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(kernel_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config4: %d pairs of a 100 kb read vs its reference (RLE ~65 kb), 12%% error, "
+                               "structural indels 50-500 bp over 0.5%%, five-state machine (long gaps open 0.0018, "
+                               "extend 0.9965), anchor expansion W=%d, windows every 10000 anti-diagonals"
+                               % (len(pairs), args.band), "pairs": len(pairs), "cells": cells,
+                   "band_half_width": args.band},
+        "fp64_gcups": round(legs["fp64"]["gcups"], 3), "fp64_kernel_ms": round(legs["fp64"]["kernel_ms"], 3),
+        "fp32_vs_fp64_gpu": {"max_abs_posterior_err": err, "max_one_sided_posterior": edge, "repeated_keys": dups,
+                             "max_abs_ll_drift": float(drift.max()),
+                             "max_rel_ll_drift": float((drift / np.abs(lp64)).max())},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "flops_per_cell": FLOPS_PER_CELL_5},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        op = O.PolishParams(PARAMS)
+        T5 = mach.transitions5()
+        sx, sy, an, strand = pairs[0]
+        n = 6000
+        an = an[(an[:, 0] < n) & (an[:, 1] < n)]
+        cut = int(an[-1, 0]) + 1, int(an[-1, 1]) + 1
+        sx, sy, an = sx[: cut[0]], sy[: cut[1]], an[:-1]
+        base = op.forward if strand else op.reverse
+        om5 = O.Model(base.trans, base.match, base.gapX, base.gapY, base.rle, base.strand, base.repeat, trans5=T5)
+        opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=10000, traceBackDiagonals=40,
+                        diagonalExpansion=args.band, splitMatrixBiggerThanThis=250_000_000,
+                        dynamicAnchorExpansion=True)
+        t3 = time.perf_counter()
+        o = O.posteriors(om5, sx, sy, an, opar, mode=O.LOGADD_EXACT)
+        tc = time.perf_counter() - t3
+        cpu = {"value": round(o[4] / tc / 1e9, 6), "unit": "GCUPS", "cores": 1, "kind": "port",
+               "sample": "first %d x %d RLE positions of pair 0 (%d cells), five-state O_exact (FP64 log space, "
+                         "exact log-sum-exp), one thread" % (cut[0], cut[1], o[4])}
+        for tag, fp64 in (("fp32", False), ("fp64", True)):
+            b = pairhmm.Batch(mach, [(sx, sy, an, strand)], bp, engine=eng, fp64=fp64)
+            b.run()
+            g = b.fetch()
+            orows = [{"weight": l[:, 0], "x": l[:, 1], "y": l[:, 2]} for l in o[:3]]
+            e, ed, du = _list_err(g, [tuple(orows) + (o[3],)], 0.01, 1e-4)
+            cpu[tag + "_max_abs_posterior_err"] = e
+            cpu[tag + "_max_one_sided_posterior"] = ed
+            cpu[tag + "_repeated_keys"] = du
+            cpu[tag + "_ll_drift"] = abs(g[0][3] - o[3])
+        out["cpu_baseline"] = cpu
+        log("cpu sample: %.1f s, %s" % (tc, json.dumps(cpu)))
     print(json.dumps(out), flush=True)
 
 

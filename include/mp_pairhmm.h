@@ -52,7 +52,9 @@ enum {
 
 /* Batch flags. */
 enum {
-    MP_BATCH_CROP_REFERENCE = 1  /* crop sX per pair as poa_realign's caller does (polisher.c:605-654) */
+    MP_BATCH_CROP_REFERENCE = 1, /* crop sX per pair as poa_realign's caller does (polisher.c:605-654) */
+    MP_BATCH_FP64 = 2            /* FP64 arithmetic on the per-cell-exponent kernel (no reference counterpart:
+                                    the precision comparison path); bands up to 1216 cells, no expectations */
 };
 
 typedef struct mp_model mp_model;          /* both strands' state machines + tables */
@@ -128,7 +130,7 @@ mp_status mp_model_set_transitions(mp_model *m, const double *logTrans9, int32_t
  * logs, each gap state at its own extension.  The JSON loader reads such a
  * machine from an hmm of "type": 5 with 25 "transitions" [from * 5 + to].
  * Five-state models run every job on a one-wavefront kernel with a per-cell
- * exponent (band widths up to 1280 cells; wider is MP_EUNSUPPORTED) in the
+ * exponent (band widths up to 2048 cells; wider is MP_EUNSUPPORTED) in the
  * posterior, match-only and forward-only modes; expected counts are
  * MP_EUNSUPPORTED.
  * mp_model_set_transitions5 makes the model five-state with these natural-log

@@ -18,6 +18,7 @@ STATUS = {0: "MP_OK", 1: "MP_EINVAL", 2: "MP_EPARAMS", 3: "MP_EANCHORS", 4: "MP_
 LIST_MATCH, LIST_GAPX, LIST_GAPY = 0, 1, 2
 MODE_POSTERIORS, MODE_MATCH_ONLY, MODE_FORWARD_ONLY, MODE_EXPECTATIONS = 0, 1, 2, 3
 BATCH_CROP_REFERENCE = 1
+BATCH_FP64 = 2
 
 
 class MPError(RuntimeError):

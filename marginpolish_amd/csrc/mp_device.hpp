@@ -134,11 +134,12 @@ struct KParams {
     float T[9];          // probability transitions, [from * 3 + to], M = 0, X = 1, Y = 2
     float startV[2][3];  // [ragged][state]
     float endV[2][3];
-    // five-state machine (M, short X, short Y, long X, long Y): pairhmm_wide5_kernel
-    int32_t nStates;     // 3 or 5
-    float T5[25];        // [from * 5 + to]
-    float startV5[2][5];
-    float endV5[2][5];
+    // pairhmm_wide5_kernel: the five-state machine (M, short X, short Y, long X,
+    // long Y), or a 3-state machine with the long gaps unreachable (FP64 batches)
+    int32_t nStates;     // 5 when the five-state kernel runs the batch, else 3
+    double T5[25];       // [from * 5 + to]
+    double startV5[2][5];
+    double endV5[2][5];
 };
 
 // ---------------------------------------------------------------------------

@@ -135,6 +135,7 @@ struct mp_batch {
     DevBuf<int32_t> dError;
     DevBuf<float> dRingWide;
     int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
+    int32_t modelStates = 3;
     int32_t narrowBlocks = 0;
     int64_t outTotal = 0;
     double kernelMs = 0.0;
@@ -162,8 +163,12 @@ mp_status launch_narrow(mp_batch *b, int grid, int block, const KParams &kp) {
 }
 
 mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
-    if (kp.nStates == 5)
-        hipLaunchKernelGGL(pairhmm_wide5_kernel, dim3(grid), dim3(64), kWide5LdsBytes, b->eng->stream, kp);
+    if (kp.nStates == 5 && (b->flags & MP_BATCH_FP64))
+        hipLaunchKernelGGL(pairhmm_wide5_kernel<double>, dim3(grid), dim3(64), wide5_lds_bytes<double>(kp.wideC),
+                           b->eng->stream, kp);
+    else if (kp.nStates == 5)
+        hipLaunchKernelGGL(pairhmm_wide5_kernel<float>, dim3(grid), dim3(64), wide5_lds_bytes<float>(kp.wideC),
+                           b->eng->stream, kp);
     else
         hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), kWideLdsBytes, b->eng->stream, kp);
     HIP_TRY(hipGetLastError());
@@ -176,7 +181,9 @@ mp_status reset_i32(mp_batch *b, int32_t *q, int n) {
     return MP_OK;
 }
 
-void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
+// fp64: every job runs on the five-state kernel in FP64; a 3-state machine is
+// embedded there with its long gaps unreachable.
+void build_tables(const mp_model *m, bool fp64, std::vector<float> &tab, KParams &kp) {
     tab.assign(kTabFloats, 0.f);
     for (int s = 0; s < 2; s++) {
         const mp::Machine &mc = m->mach[s];
@@ -213,7 +220,7 @@ void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
     kp.endV[1][0] = (float)std::exp((et[mp::kGapOpenX] + et[mp::kGapOpenY]) / 2.0);
     kp.endV[1][1] = E(mp::kGapExtendX);
     kp.endV[1][2] = E(mp::kGapExtendY);
-    kp.nStates = f.nStates;
+    kp.nStates = f.nStates == 5 || fp64 ? 5 : 3;
     if (f.nStates == 5) {
         // Boundary vectors of the five-state machine, the three-state ones
         // (stateMachine.c:518-557) extended to the long gaps: start in M;
@@ -221,14 +228,37 @@ void build_tables(const mp_model *m, std::vector<float> &tab, KParams &kp) {
         // to M; ragged end: M at the mean of the short-gap opens (log space),
         // each gap state at its own extension.
         const double *l5 = f.trans5;
-        for (int i = 0; i < 25; i++) kp.T5[i] = (float)std::exp(l5[i]);
+        for (int i = 0; i < 25; i++) kp.T5[i] = std::exp(l5[i]);
         for (int st = 0; st < 5; st++) {
-            kp.startV5[0][st] = st == mp::kS5Match ? 1.f : 0.f;
-            kp.startV5[1][st] = st == mp::kS5Match ? 0.f : 1.f;
-            kp.endV5[0][st] = (float)std::exp(l5[st * 5 + mp::kS5Match]);
-            kp.endV5[1][st] = st == mp::kS5Match ? (float)std::exp((l5[mp::kS5ShortX] + l5[mp::kS5ShortY]) / 2.0)
-                                                 : (float)std::exp(l5[st * 5 + st]);
+            kp.startV5[0][st] = st == mp::kS5Match ? 1.0 : 0.0;
+            kp.startV5[1][st] = st == mp::kS5Match ? 0.0 : 1.0;
+            kp.endV5[0][st] = std::exp(l5[st * 5 + mp::kS5Match]);
+            kp.endV5[1][st] = st == mp::kS5Match ? std::exp((l5[mp::kS5ShortX] + l5[mp::kS5ShortY]) / 2.0)
+                                                 : std::exp(l5[st * 5 + st]);
         }
+    } else if (fp64) {
+        // the 3-state machine on the five-state kernel: long gaps unreachable
+        const double *t = f.trans;
+        const int M = mp::kS5Match, X = mp::kS5ShortX, Y = mp::kS5ShortY;
+        for (int i = 0; i < 25; i++) kp.T5[i] = 0.0;
+        kp.T5[M * 5 + M] = std::exp(t[mp::kMatchContinue]);
+        kp.T5[M * 5 + X] = std::exp(t[mp::kGapOpenX]);
+        kp.T5[M * 5 + Y] = std::exp(t[mp::kGapOpenY]);
+        kp.T5[X * 5 + M] = std::exp(t[mp::kMatchFromGapX]);
+        kp.T5[X * 5 + X] = std::exp(t[mp::kGapExtendX]);
+        kp.T5[X * 5 + Y] = std::exp(t[mp::kGapSwitchToY]);
+        kp.T5[Y * 5 + M] = std::exp(t[mp::kMatchFromGapY]);
+        kp.T5[Y * 5 + X] = std::exp(t[mp::kGapSwitchToX]);
+        kp.T5[Y * 5 + Y] = std::exp(t[mp::kGapExtendY]);
+        for (int st = 0; st < 5; st++) kp.startV5[0][st] = kp.startV5[1][st] = kp.endV5[0][st] = kp.endV5[1][st] = 0.0;
+        kp.startV5[0][M] = 1.0;
+        kp.startV5[1][X] = kp.startV5[1][Y] = 1.0;
+        kp.endV5[0][M] = std::exp(et[mp::kMatchContinue]);
+        kp.endV5[0][X] = std::exp(et[mp::kMatchFromGapX]);
+        kp.endV5[0][Y] = std::exp(et[mp::kMatchFromGapY]);
+        kp.endV5[1][M] = std::exp((et[mp::kGapOpenX] + et[mp::kGapOpenY]) / 2.0);
+        kp.endV5[1][X] = std::exp(et[mp::kGapExtendX]);
+        kp.endV5[1][Y] = std::exp(et[mp::kGapExtendY]);
     }
 }
 
@@ -279,8 +309,10 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
                                 kSeqLdsBytes));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 kWideLdsBytes));
-    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kWide5LdsBytes));
+    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, wide5_lds_bytes<float>(kWide5MaxC32)));
+    HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel<double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, wide5_lds_bytes<double>(kWide5MaxC64)));
     int bpc = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void *)pairhmm_split_kernel, 512,
                                                          kSplitLdsBytes));
@@ -325,9 +357,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                       "(pairwiseAligner.c:692-696)");
         return MP_EINVAL;
     }
-    const bool five = m->mach[1].nStates == 5;
+    const bool fp64 = (flags & MP_BATCH_FP64) != 0;
+    const bool five = m->mach[1].nStates == 5 || fp64;  // every job on the five-state kernel
     if (five && mode == MP_MODE_EXPECTATIONS) {
-        mp::set_error("mp_batch_create: expected transition counts are computed for three-state machines only");
+        mp::set_error("mp_batch_create: expected transition counts are computed for three-state machines in FP32 "
+                      "only");
         return MP_EUNSUPPORTED;
     }
     *out = nullptr;
@@ -568,11 +602,13 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     wins[j].clear();
                     for (auto &x : w) wins[j].push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
                     if (split && narrow) choose_crossings(wins[j]);
-                    // live ring entries: forward and backward values of (lo_k, tb_{k+1}]
+                    // live ring entries.  Narrow kernels (concurrent sweeps): forward and
+                    // backward values of (lo_k, tb_{k+1}].  Wide kernels (the backward of
+                    // window k runs after its forward): forward values of [lo_k, tb_k].
                     int64_t need = 1;
                     for (size_t k = 0; k < wins[j].size(); k++) {
-                        const int64_t hi = wins[j][std::min(k + 1, wins[j].size() - 1)].tb;
-                        need = std::max<int64_t>(need, hi - wins[j][k].lo + 3 * kK);
+                        const int64_t hi = narrow ? wins[j][std::min(k + 1, wins[j].size() - 1)].tb : wins[j][k].tb;
+                        need = std::max<int64_t>(need, hi - wins[j][k].lo + (narrow ? 3 * kK : 2));
                     }
                     hj.ringNeed = (int32_t)((need + kK - 1) / kK * kK);
                 }
@@ -584,13 +620,13 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("invalid band (anchors inconsistent with the sequences)");
         return MP_EANCHORS;
     }
-    const int32_t wideLimit = 64 * (five ? kWide5MaxC : kWideMaxC);
+    const int32_t wideLimit = 64 * (fp64 ? kWide5MaxC64 : (five ? kWide5MaxC32 : kWideMaxC));
     for (int64_t j = 0; j < nj; j++) {
         if (jobs[j].wmax > wideLimit) {
             delete b;
             mp::set_error("pair " + std::to_string(jobs[j].pair) + ": band width " + std::to_string(jobs[j].wmax) +
                           " exceeds the " + std::to_string(wideLimit) + "-cell kernel limit" +
-                          (five ? " of the five-state machine" : ""));
+                          (fp64 ? " of FP64 batches" : (five ? " of the five-state machine" : "")));
             return MP_EUNSUPPORTED;
         }
     }
@@ -671,7 +707,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     std::vector<float> tab;
     KParams &kp = b->kp;
     std::memset(&kp, 0, sizeof(kp));
-    build_tables(m, tab, kp);
+    build_tables(m, fp64, tab, kp);
+    b->modelStates = m->mach[1].nStates;
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
     if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
@@ -703,11 +740,14 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
     if (b->wideEnd > b->wideBegin) {
-        // wide jobs: one wavefront each, ring of one float4 per cell (two with five states: values and
-        // the cell's exponent); bounded to ~48 GB
-        const size_t perWave =
-            (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * sizeof(float4) * (five ? 2 : 1);
-        const size_t cap = std::max<size_t>(1, ((size_t)48 << 30) / perWave);
+        // wide jobs: one wavefront each; ring of one float4 per cell (3-state: values and the exponent)
+        // or six values (five-state kernel: values and the cell's exponent, FP32 or FP64); the resident
+        // waves' rings take at most ~70% of the free device memory
+        const size_t perCell = five ? 6 * (fp64 ? sizeof(double) : sizeof(float)) : sizeof(float4);
+        const size_t perWave = (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * perCell;
+        size_t freeB = 0, totalB = 0;
+        HIP_TRY(hipMemGetInfo(&freeB, &totalB));
+        const size_t cap = std::max<size_t>(1, freeB / 10 * 7 / perWave);
         b->wideWaves = (int32_t)std::min<size_t>({(size_t)(b->wideEnd - b->wideBegin), (size_t)e->cus * 4, cap});
         if ((st = b->dRingWide.alloc((size_t)b->wideWaves * perWave / sizeof(float)))) {
             delete b;
@@ -841,13 +881,13 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
         mp::set_error("mp_batch_set_model: null argument");
         return MP_EINVAL;
     }
-    if (m->mach[1].nStates != b->kp.nStates) {
+    if (m->mach[1].nStates != b->modelStates) {
         mp::set_error("mp_batch_set_model: the model's state count differs from the batch's");
         return MP_EINVAL;
     }
     std::vector<float> tab;
     KParams kp = b->kp;
-    build_tables(m, tab, kp);
+    build_tables(m, (b->flags & MP_BATCH_FP64) != 0, tab, kp);
     mp_status st;
     if ((st = b->dTables.upload(tab, b->eng->stream))) return st;
     std::memcpy(b->kp.T, kp.T, sizeof(kp.T));

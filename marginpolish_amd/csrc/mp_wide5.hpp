@@ -1,20 +1,27 @@
 // mp_wide5.hpp -- the five-state machine (M, short gap X, short gap Y, long
-// gap X, long gap Y), every job, gfx950.  No counterpart in the reference.
+// gap X, long gap Y), every job, in FP32 or FP64, gfx950.  No counterpart in
+// the reference.
 //
 // Same sweep structure as the 3-state wide kernel: one job per wavefront,
-// cell i = lane + 64c (c < C), one anti-diagonal per step, the per-target
-// "send" values of the last three anti-diagonals in LDS, read by band
-// coordinate; forward to tb, backward down with posterior records
-// (anti-diagonals descending, cells ascending); same windows and outputs.
+// cell i = lane + 64c (c < C), one anti-diagonal per step, neighbours read
+// from LDS by band coordinate; forward to tb, backward down with posterior
+// records (anti-diagonals descending, cells ascending); same windows and
+// outputs.
 //
 // Scaling is per cell: each cell keeps its five values normalised to a
-// maximum in [0.5, 1) with an int exponent of its own.  A cell aligns its
+// maximum in [0.5, 1) with an int exponent of its own, and aligns its
 // neighbours' inputs to the largest neighbour exponent before the update.  A
 // long gap extends with probability ~0.997 at the gap emission, so along one
 // anti-diagonal the long-gap values at the band edges and the match values
 // near the alignment path differ by more than the FP32 range once the band is
 // a few hundred cells wide; one exponent per anti-diagonal (the 3-state
 // kernels' scheme) would flush the path cells to zero.
+//
+// LDS: the emission table (only in-band cells form emission offsets, and the
+// table's clamps keep them inside it), then per cell the "send" values of the
+// last anti-diagonals -- the match target of three ([3][W]: the match input
+// comes from two back), the four gap targets of two ([2][4][W]) -- and the
+// cell exponents of three ([3][W]).
 //
 // A gap state of either length emits like the 3-state gap of its side; the
 // gap-X (gap-Y) posterior of a cell is the sum over its two gap-X (gap-Y)
@@ -26,85 +33,112 @@
 namespace mpk {
 
 constexpr int kNS5 = 5;
-constexpr int kWide5MaxC = 20;  // up to 64 * 20 = 1280 cells per anti-diagonal
-// LDS: emission table, then send values [3 slots][5 targets][W] and the
-// cells' exponents [3 slots][W].
-constexpr int wide5_lds_bytes(int c) { return kTabReach + 3 * (kNS5 + 1) * 64 * c * 4; }
-constexpr int kWide5LdsBytes = wide5_lds_bytes(kWide5MaxC);
+constexpr int kWide5TabBytes = kTabFloats * 4;  // 41904, a multiple of 16
+template <class Real>
+constexpr int wide5_lds_bytes(int c) {
+    return kWide5TabBytes + 64 * c * (11 * (int)sizeof(Real) + 3 * 4);
+}
+template <class Real>
+constexpr int wide5_max_c() {  // columns of 64 cells that fit 160 KB of LDS
+    return (160 * 1024 - kWide5TabBytes) / (64 * (11 * (int)sizeof(Real) + 3 * 4));
+}
+constexpr int kWide5MaxC32 = wide5_max_c<float>() < 32 ? wide5_max_c<float>() : 32;  // 2048 cells
+constexpr int kWide5MaxC64 = wide5_max_c<double>();                                  // 19: 1216 cells
 
 // state kind: 0 match (input from (x-1, y-1)), 1 gap X (from (x-1, y)), 2 gap Y (from (x, y-1))
 __host__ __device__ constexpr int kind5(int s) { return s == 0 ? 0 : ((s & 1) ? 1 : 2); }
 
-struct Wide5Ctx {
-    float *val;  // [3][5][W]
-    int *ex;     // [3][W]
-    int W;
-    __device__ static int slotOf(int d) { return ((d % 3) + 3) % 3; }
-    __device__ float *v(int d, int s) const { return val + (size_t)(slotOf(d) * kNS5 + s) * W; }
-    __device__ int *e(int d) const { return ex + (size_t)slotOf(d) * W; }
-};
+// Exponent of a cell without mass (all five values 0).  It loses every
+// maximum against a cell with mass, differences with it stay far from int
+// overflow, and ldexp of them is 0.
+constexpr int kNoMass = -(1 << 29);
+
+__device__ __forceinline__ int frexp_exp(float v) { return __builtin_amdgcn_frexp_expf(v); }
+__device__ __forceinline__ int frexp_exp(double v) { return __builtin_amdgcn_frexp_exp(v); }
+__device__ __forceinline__ float ldexp_r(float v, int k) { return ldexpf(v, k); }
+__device__ __forceinline__ double ldexp_r(double v, int k) { return ldexp(v, k); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return fma(a, b, c); }
+// an int exponent kept in a ring row of the value type (its bit pattern, not its value)
+__device__ __forceinline__ float exp_slot(int e, float) { return __int_as_float(e); }
+__device__ __forceinline__ double exp_slot(int e, double) { return __longlong_as_double((long long)e); }
+__device__ __forceinline__ int slot_exp(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int slot_exp(double v) { return (int)__double_as_longlong(v); }
 
 __device__ __forceinline__ int wave_max_i(int v) {
     for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
-__device__ __forceinline__ float wave_sum5(float v) {
+template <class Real>
+__device__ __forceinline__ Real wave_sum_r(Real v) {
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
 // (acc, accE) += (v, ve): a running sum of values with exponents of their own
-__device__ __forceinline__ void acc_add(float &acc, int &accE, float v, int ve) {
-    if (v == 0.f) return;
-    if (acc == 0.f) {
+template <class Real>
+__device__ __forceinline__ void acc_add(Real &acc, int &accE, Real v, int ve) {
+    if (v == Real(0)) return;
+    if (acc == Real(0)) {
         acc = v;
         accE = ve;
     } else if (ve > accE) {
-        acc = ldexpf(acc, accE - ve) + v;
+        acc = ldexp_r(acc, accE - ve) + v;
         accE = ve;
     } else {
-        acc += ldexpf(v, ve - accE);
+        acc += ldexp_r(v, ve - accE);
     }
 }
 
-// Exponent of a cell without mass (all five values 0).  It loses every
-// maximum against a cell with mass, and differences with it stay far from
-// int overflow while ldexpf of them is 0.
-constexpr int kNoMass = -(1 << 29);
-
 // Scale the five values (of exponent E) to a maximum in [0.5, 1); returns the
 // new exponent, kNoMass when all values are 0.
-__device__ __forceinline__ int renorm5(float *v, int E) {
-    float m = v[0];
+template <class Real>
+__device__ __forceinline__ int renorm5(Real *v, int E) {
+    Real m = v[0];
 #pragma unroll
-    for (int s = 1; s < kNS5; s++) m = fmaxf(m, v[s]);
-    if (m == 0.f) return kNoMass;
-    const int k = __builtin_amdgcn_frexp_expf(m);
-    const float sc = ldexpf(1.f, -k);
+    for (int s = 1; s < kNS5; s++) m = v[s] > m ? v[s] : m;
+    if (m == Real(0)) return kNoMass;
+    const int k = frexp_exp(m);
+    const Real sc = ldexp_r(Real(1), -k);
 #pragma unroll
     for (int s = 0; s < kNS5; s++) v[s] *= sc;
     return E + k;
 }
 
+template <class Real>
+struct Wide5Lds {
+    Real *sendM;  // [3][W]    match target
+    Real *sendG;  // [2][4][W] targets short X, short Y, long X, long Y
+    int *ex;      // [3][W]
+    int W;
+    __device__ static int m3(int d) { return ((d % 3) + 3) % 3; }
+    __device__ Real *m(int d) const { return sendM + (size_t)m3(d) * W; }
+    __device__ Real *g(int d, int s) const { return sendG + (size_t)((d & 1) * 4 + s - 1) * W; }  // s = 1..4
+    __device__ int *e(int d) const { return ex + (size_t)m3(d) * W; }
+};
+
+template <class Real>
 __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
     extern __shared__ float ldsw5[];
     float *tabL = ldsw5;
     const int lane = threadIdx.x;
     for (int i = lane; i < kTabFloats; i += 64) tabL[i] = kp.tables[i];
-    __syncthreads();
     const int C = kp.wideC, W = 64 * C;
-    Wide5Ctx cx{ldsw5 + kTabReach / 4, reinterpret_cast<int *>(ldsw5 + kTabReach / 4 + 3 * kNS5 * W), W};
-    float4 *ring = reinterpret_cast<float4 *>(kp.ringWide) + (size_t)blockIdx.x * kp.ringWideLen * W * 2;
+    Real *arr = reinterpret_cast<Real *>(reinterpret_cast<char *>(ldsw5) + kWide5TabBytes);
+    const Wide5Lds<Real> lx{arr, arr + 3 * W, reinterpret_cast<int *>(arr + 11 * W), W};
+    // ring: per anti-diagonal six rows of W entries (five values, the exponent)
+    Real *ring = reinterpret_cast<Real *>(kp.ringWide) + (size_t)blockIdx.x * kp.ringWideLen * W * 6;
     const int R = kp.ringWideLen;
-    auto ringAt = [&](int d, int k, int i) -> float4 & { return ring[((size_t)(d % R) * 2 + k) * W + i]; };
+    auto ringRow = [&](int d, int s) { return ring + ((size_t)(d % R) * 6 + s) * W; };
     const uint64_t below = (1ull << lane) - 1ull;
     const bool fwdOnly = kp.mode == MP_MODE_FORWARD_ONLY;
     const bool postMode = kp.mode <= MP_MODE_MATCH_ONLY;
     const bool withIndels = kp.withIndels != 0;
     const float thr = kp.threshold;
-    float T[kNS5 * kNS5];
+    Real T[kNS5 * kNS5];
 #pragma unroll
-    for (int q = 0; q < kNS5 * kNS5; q++) T[q] = kp.T5[q];
+    for (int q = 0; q < kNS5 * kNS5; q++) T[q] = (Real)kp.T5[q];
+    __syncthreads();
 
     for (;;) {
         int jn = 0;
@@ -114,7 +148,7 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
         if (job >= kp.wideEnd) break;
         const DevJob jb = kp.jobs[job];
         const int D = jb.lX + jb.lY;
-        const bool rgL = (jb.flags >> 1) & 1, rgR = (jb.flags >> 2) & 1;
+        const int rStart = (jb.flags >> 1) & 1, rgR = (jb.flags >> 2) & 1;
         auto band = [&](int d, int &L, int &Rr) {
             const int b = kp.band[jb.bandOff + d];
             L = 2 * band_xL(b) - d;
@@ -124,13 +158,14 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
         const int cap = jb.cap[1];
         int cnt = 0;
         int dStart = 0;
-        for (int i = lane; i < 3 * (kNS5 + 1) * W; i += 64) cx.val[i] = 0.f;
+        for (int i = lane; i < 11 * W; i += 64) lx.sendM[i] = Real(0);
+        for (int i = lane; i < 3 * W; i += 64) lx.ex[i] = kNoMass;
         __syncthreads();
         for (int win = 0; win < jb.nWin; win++) {
             const DevWin wn = kp.win[jb.winOff + win];
             const bool atEnd = wn.tb == D;
             const int rEnd = rgR && atEnd ? 1 : 0;
-            float acc = 0.f;  // this lane's part of the total at tb
+            Real acc = Real(0);  // this lane's part of the total at tb
             int accE = 0;
             // ---- forward from dStart to tb ----
             for (int d = dStart; d <= wn.tb; d++) {
@@ -142,59 +177,65 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                     const int i = lane + 64 * c, xmy = L0 + 2 * i;
                     const bool act = xmy <= R0;
                     const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
-                    float p[kNS5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+                    Real p[kNS5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
                     int E = kNoMass;
                     if (act) {
-                        const Emis em = emissions(tabL, kp.sym[jb.symX + x - 1], kp.sym[jb.symY + y - 1]);
                         if (d == 0) {
 #pragma unroll
-                            for (int s = 0; s < kNS5; s++) p[s] = s == 0 ? (rgL ? 0.f : 1.f) : (rgL ? 1.f : 0.f);
+                            for (int s = 0; s < kNS5; s++) p[s] = (Real)kp.startV5[rStart][s];
                             E = renorm5(p, 0);
                         } else {
+                            const Emis em = emissions(tabL, kp.sym[jb.symX + x - 1], kp.sym[jb.symY + y - 1]);
                             const int sX = xmy - 1, sY = xmy + 1;
                             const bool okX = sX >= L1 && sX <= R1, okY = sY >= L1 && sY <= R1;
                             const bool okM = xmy >= L2 && xmy <= R2;
                             const int jX = (sX - L1) >> 1, jY = (sY - L1) >> 1, jM = (xmy - L2) >> 1;
-                            const int eXn = okX ? cx.e(d - 1)[jX] : kNoMass, eYn = okY ? cx.e(d - 1)[jY] : kNoMass;
-                            const int eMn = okM ? cx.e(d - 2)[jM] : kNoMass;
+                            const int eXn = okX ? lx.e(d - 1)[jX] : kNoMass, eYn = okY ? lx.e(d - 1)[jY] : kNoMass;
+                            const int eMn = okM ? lx.e(d - 2)[jM] : kNoMass;
                             const int Ein = max(eXn, max(eYn, eMn));
-                            const float fX = okX ? em.eX * ldexpf(1.f, eXn - Ein) : 0.f;
-                            const float fY = okY ? em.eY * ldexpf(1.f, eYn - Ein) : 0.f;
-                            const float fM = okM ? em.eM * ldexpf(1.f, eMn - Ein) : 0.f;
-                            p[0] = okM ? fM * cx.v(d - 2, 0)[jM] : 0.f;
-                            p[1] = okX ? fX * cx.v(d - 1, 1)[jX] : 0.f;
-                            p[3] = okX ? fX * cx.v(d - 1, 3)[jX] : 0.f;
-                            p[2] = okY ? fY * cx.v(d - 1, 2)[jY] : 0.f;
-                            p[4] = okY ? fY * cx.v(d - 1, 4)[jY] : 0.f;
+                            const Real fX = okX ? (Real)em.eX * ldexp_r(Real(1), eXn - Ein) : Real(0);
+                            const Real fY = okY ? (Real)em.eY * ldexp_r(Real(1), eYn - Ein) : Real(0);
+                            const Real fM = okM ? (Real)em.eM * ldexp_r(Real(1), eMn - Ein) : Real(0);
+                            p[0] = okM ? fM * lx.m(d - 2)[jM] : Real(0);
+                            p[1] = okX ? fX * lx.g(d - 1, 1)[jX] : Real(0);
+                            p[3] = okX ? fX * lx.g(d - 1, 3)[jX] : Real(0);
+                            p[2] = okY ? fY * lx.g(d - 1, 2)[jY] : Real(0);
+                            p[4] = okY ? fY * lx.g(d - 1, 4)[jY] : Real(0);
                             E = renorm5(p, Ein);
                         }
                     }
 #pragma unroll
                     for (int t = 0; t < kNS5; t++) {
-                        float a = T[t] * p[0];
+                        Real a = T[t] * p[0];
 #pragma unroll
-                        for (int s = 1; s < kNS5; s++) a = fmaf(T[s * kNS5 + t], p[s], a);
-                        cx.v(d, t)[i] = a;
+                        for (int s = 1; s < kNS5; s++) a = fma_r(T[s * kNS5 + t], p[s], a);
+                        if (t == 0) lx.m(d)[i] = a;
+                        else lx.g(d, t)[i] = a;
                     }
-                    cx.e(d)[i] = E;
+                    lx.e(d)[i] = E;
                     if (!fwdOnly) {
-                        ringAt(d, 0, i) = make_float4(p[0], p[1], p[2], p[3]);
-                        ringAt(d, 1, i) = make_float4(p[4], __int_as_float(E), 0.f, 0.f);
+#pragma unroll
+                        for (int s = 0; s < kNS5; s++) ringRow(d, s)[i] = p[s];
+                        ringRow(d, 5)[i] = exp_slot(E, Real(0));
                     }
                     if (d == wn.tb && act) {
-                        float tv = p[0] * kp.endV5[rEnd][0];
+                        Real tv = p[0] * (Real)kp.endV5[rEnd][0];
 #pragma unroll
-                        for (int s = 1; s < kNS5; s++) tv = fmaf(p[s], kp.endV5[rEnd][s], tv);
+                        for (int s = 1; s < kNS5; s++) tv = fma_r(p[s], (Real)kp.endV5[rEnd][s], tv);
                         acc_add(acc, accE, tv, E);
                     }
                 }
                 __syncthreads();
             }
             // total at tb: t * 2^SFtb
-            const int SFtb = wave_max_i(acc > 0.f ? accE : kNoMass);
-            const float t = wave_sum5(acc > 0.f ? ldexpf(acc, accE - SFtb) : 0.f);
-            const float totalInv = t > 0.f ? 1.f / t : 0.f;
-            if (atEnd && lane == 0) kp.logTotal[job] = make_int2(__float_as_int(t), t > 0.f ? SFtb : 0);
+            const int SFtb = wave_max_i(acc > Real(0) ? accE : kNoMass);
+            const Real t = wave_sum_r(acc > Real(0) ? ldexp_r(acc, accE - SFtb) : Real(0));
+            const Real totalInv = t > Real(0) ? Real(1) / t : Real(0);
+            if (atEnd && lane == 0) {
+                // (t, S) with t in [0.5, 1) as FP32: the host's log t + S ln 2 loses < 1e-7 to the rounding of t
+                const int k = t > Real(0) ? frexp_exp(t) : 0;
+                kp.logTotal[job] = make_int2(__float_as_int((float)ldexp_r(t, -k)), t > Real(0) ? SFtb + k : 0);
+            }
             if (fwdOnly) break;
             // ---- backward from tb down to lo + 1 ----
             for (int d = wn.tb; d >= wn.lo + 1; d--) {
@@ -206,40 +247,40 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                     const int i = lane + 64 * c, xmy = L0 + 2 * i;
                     const bool act = xmy <= R0;
                     const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
-                    float q[kNS5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-                    float eX = 0.f, eY = 0.f, eM = 0.f;
+                    Real q[kNS5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
+                    Real eX = Real(0), eY = Real(0), eM = Real(0);
                     int E = kNoMass;
                     if (act) {
                         const Emis em = emissions(tabL, kp.sym[jb.symX + x - 1], kp.sym[jb.symY + y - 1]);
-                        eX = em.eX;
-                        eY = em.eY;
-                        eM = em.eM;
+                        eX = (Real)em.eX;
+                        eY = (Real)em.eY;
+                        eM = (Real)em.eM;
                         if (d == wn.tb) {
 #pragma unroll
-                            for (int s = 0; s < kNS5; s++) q[s] = kp.endV5[rEnd][s];
+                            for (int s = 0; s < kNS5; s++) q[s] = (Real)kp.endV5[rEnd][s];
                             E = renorm5(q, 0);
                         } else {
                             const int sX = xmy + 1, sY = xmy - 1;
                             const bool okX = sX >= L1 && sX <= R1, okY = sY >= L1 && sY <= R1;
                             const bool okM = xmy >= L2 && xmy <= R2;
                             const int jX = (sX - L1) >> 1, jY = (sY - L1) >> 1, jM = (xmy - L2) >> 1;
-                            const int eXn = okX ? cx.e(d + 1)[jX] : kNoMass, eYn = okY ? cx.e(d + 1)[jY] : kNoMass;
-                            const int eMn = okM ? cx.e(d + 2)[jM] : kNoMass;
+                            const int eXn = okX ? lx.e(d + 1)[jX] : kNoMass, eYn = okY ? lx.e(d + 1)[jY] : kNoMass;
+                            const int eMn = okM ? lx.e(d + 2)[jM] : kNoMass;
                             const int Ein = max(eXn, max(eYn, eMn));
-                            const float fX = okX ? ldexpf(1.f, eXn - Ein) : 0.f;
-                            const float fY = okY ? ldexpf(1.f, eYn - Ein) : 0.f;
-                            const float fM = okM ? ldexpf(1.f, eMn - Ein) : 0.f;
-                            float in[kNS5];
-                            in[0] = okM ? fM * cx.v(d + 2, 0)[jM] : 0.f;
-                            in[1] = okX ? fX * cx.v(d + 1, 1)[jX] : 0.f;
-                            in[3] = okX ? fX * cx.v(d + 1, 3)[jX] : 0.f;
-                            in[2] = okY ? fY * cx.v(d + 1, 2)[jY] : 0.f;
-                            in[4] = okY ? fY * cx.v(d + 1, 4)[jY] : 0.f;
+                            const Real fX = okX ? ldexp_r(Real(1), eXn - Ein) : Real(0);
+                            const Real fY = okY ? ldexp_r(Real(1), eYn - Ein) : Real(0);
+                            const Real fM = okM ? ldexp_r(Real(1), eMn - Ein) : Real(0);
+                            Real in[kNS5];
+                            in[0] = okM ? fM * lx.m(d + 2)[jM] : Real(0);
+                            in[1] = okX ? fX * lx.g(d + 1, 1)[jX] : Real(0);
+                            in[3] = okX ? fX * lx.g(d + 1, 3)[jX] : Real(0);
+                            in[2] = okY ? fY * lx.g(d + 1, 2)[jY] : Real(0);
+                            in[4] = okY ? fY * lx.g(d + 1, 4)[jY] : Real(0);
 #pragma unroll
                             for (int s = 0; s < kNS5; s++) {
-                                float a = T[s * kNS5] * in[0];
+                                Real a = T[s * kNS5] * in[0];
 #pragma unroll
-                                for (int tt = 1; tt < kNS5; tt++) a = fmaf(T[s * kNS5 + tt], in[tt], a);
+                                for (int tt = 1; tt < kNS5; tt++) a = fma_r(T[s * kNS5 + tt], in[tt], a);
                                 q[s] = a;
                             }
                             E = renorm5(q, Ein);
@@ -248,11 +289,13 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                     if (postMode && d <= wn.from) {
                         float pM = 0.f, pX = 0.f, pY = 0.f;
                         if (act) {
-                            const float4 f0 = ringAt(d, 0, i), f1 = ringAt(d, 1, i);
-                            const float sc = ldexpf(totalInv, __float_as_int(f1.y) + E - SFtb);
-                            pM = f0.x * q[0] * sc;
-                            pX = fmaf(f0.w, q[3], f0.y * q[1]) * sc;
-                            pY = fmaf(f1.x, q[4], f0.z * q[2]) * sc;
+                            Real f[kNS5];
+#pragma unroll
+                            for (int s = 0; s < kNS5; s++) f[s] = ringRow(d, s)[i];
+                            const Real sc = ldexp_r(totalInv, slot_exp(ringRow(d, 5)[i]) + E - SFtb);
+                            pM = (float)(f[0] * q[0] * sc);
+                            pX = (float)(fma_r(f[3], q[3], f[1] * q[1]) * sc);
+                            pY = (float)(fma_r(f[4], q[4], f[2] * q[2]) * sc);
                         }
                         const bool emit = act && (pM >= thr || (withIndels && (pX >= thr || pY >= thr)));
                         const uint64_t bm = __ballot(emit);
@@ -264,12 +307,12 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                         }
                         cnt += __popcll(bm);
                     }
-#pragma unroll
-                    for (int s = 0; s < kNS5; s++) {
-                        const int k = kind5(s);
-                        cx.v(d, s)[i] = (k == 0 ? eM : (k == 1 ? eX : eY)) * q[s];
-                    }
-                    cx.e(d)[i] = E;
+                    lx.m(d)[i] = eM * q[0];
+                    lx.g(d, 1)[i] = eX * q[1];
+                    lx.g(d, 3)[i] = eX * q[3];
+                    lx.g(d, 2)[i] = eY * q[2];
+                    lx.g(d, 4)[i] = eY * q[4];
+                    lx.e(d)[i] = E;
                 }
                 __syncthreads();
             }
@@ -287,23 +330,26 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 for (int c = 0; c < C; c++) {
                     const int i = lane + 64 * c;
                     const bool a0 = L0 + 2 * i <= R0, a1 = L1 + 2 * i <= R1;
-                    const float4 f0 = ringAt(tb, 0, i), f1 = ringAt(tb, 1, i);
-                    const float4 g0 = ringAt(tb - 1, 0, i), g1 = ringAt(tb - 1, 1, i);
-                    const float f[kNS5] = {f0.x, f0.y, f0.z, f0.w, f1.x};
-                    const float g[kNS5] = {g0.x, g0.y, g0.z, g0.w, g1.x};
+                    Real f[kNS5], g[kNS5];
+#pragma unroll
+                    for (int s = 0; s < kNS5; s++) {
+                        f[s] = ringRow(tb, s)[i];
+                        g[s] = ringRow(tb - 1, s)[i];
+                    }
 #pragma unroll
                     for (int tt = 0; tt < kNS5; tt++) {
-                        float a = T[tt] * f[0];
+                        Real a = T[tt] * f[0];
 #pragma unroll
-                        for (int s = 1; s < kNS5; s++) a = fmaf(T[s * kNS5 + tt], f[s], a);
-                        cx.v(tb, tt)[i] = a0 ? a : 0.f;
+                        for (int s = 1; s < kNS5; s++) a = fma_r(T[s * kNS5 + tt], f[s], a);
+                        if (tt == 0) lx.m(tb)[i] = a0 ? a : Real(0);
+                        else lx.g(tb, tt)[i] = a0 ? a : Real(0);
                     }
-                    cx.e(tb)[i] = a0 ? __float_as_int(f1.y) : kNoMass;
-                    float a = T[0] * g[0];
+                    lx.e(tb)[i] = a0 ? slot_exp(ringRow(tb, 5)[i]) : kNoMass;
+                    Real a = T[0] * g[0];
 #pragma unroll
-                    for (int s = 1; s < kNS5; s++) a = fmaf(T[s * kNS5], g[s], a);
-                    cx.v(tb - 1, 0)[i] = a1 ? a : 0.f;
-                    cx.e(tb - 1)[i] = a1 ? __float_as_int(g1.y) : kNoMass;
+                    for (int s = 1; s < kNS5; s++) a = fma_r(T[s * kNS5], g[s], a);
+                    lx.m(tb - 1)[i] = a1 ? a : Real(0);
+                    lx.e(tb - 1)[i] = a1 ? slot_exp(ringRow(tb - 1, 5)[i]) : kNoMass;
                 }
                 __syncthreads();
                 dStart = tb + 1;

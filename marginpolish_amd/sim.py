@@ -22,6 +22,10 @@ Workloads (sizes follow the benchmark configurations):
   config2  5 Mb reference (GC 50%, runs ~ Geometric(0.5), 1% tandem
            repeats), 100 kb chunks with 5 kb boundaries, 50x reads (mean
            20 kb, sd 15 kb, max 100 kb), 8% error, W=10
+  config4  256 pairs of a 100 kb read against its own 100 kb reference
+           (GC 50%, runs ~ Geometric(0.65): RLE ~65 kb), structural indels of
+           50-500 bp covering 0.5% of the length, then 12% error; band W in
+           {128, 256, 512, 1024}; for the five-state machine
 """
 from __future__ import annotations
 
@@ -210,3 +214,45 @@ def config1(seed=1, max_chunks=None):
 def config2(seed=2, max_chunks=None):
     return chunked("config2", seed, 5_000_000, 0.50, 0.5, 0.01, 100_000, 5_000, 50, 20_000, 15_000, 500, 100_000,
                    0.08, 10, max_chunks)
+
+
+def _structural(rng, ref, frac, lo=50, hi=500):
+    """Insert and delete segments of lo..hi bases (half each, non-overlapping)
+    until they cover frac of the length.  Returns (sequence, source
+    reference index of each base or -1 for inserted bases)."""
+    n = ref.size
+    events, covered, tries = [], 0, 0
+    while covered < frac * n and tries < 1000:
+        tries += 1
+        L = int(rng.integers(lo, hi + 1))
+        at = int(rng.integers(hi, max(hi + 1, n - 2 * hi)))
+        if any(abs(at - a) < 2 * hi for a, _, _ in events):
+            continue
+        events.append((at, L, bool(rng.random() < 0.5)))
+        covered += L
+    events.sort()
+    out, src, prev = [], [], 0
+    for at, L, is_del in events:
+        out.append(ref[prev:at])
+        src.append(np.arange(prev, at))
+        if is_del:
+            prev = at + L
+        else:
+            out.append(rng.integers(0, 4, size=L, dtype=np.uint8))
+            src.append(np.full(L, -1, dtype=np.int64))
+            prev = at
+    out.append(ref[prev:])
+    src.append(np.arange(prev, n))
+    return np.concatenate(out), np.concatenate(src)
+
+
+def config4(n_pairs=256, length=100_000, seed=4, err=0.12, sv_frac=0.005, W=128, strands=(1, 0)):
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for r in range(n_pairs):
+        ref = _ref_runs(rng, length, 0.5, 0.65)
+        sv, src = _structural(rng, ref, sv_frac)
+        read, pos = _mutate(rng, sv, err)
+        keep = (src >= 0) & (pos >= 0)
+        pairs.append(_pair(ref, read, src[keep], pos[keep], W, strands[r % len(strands)], r, r))
+    return Workload("config4", pairs, W, length, n_pairs)

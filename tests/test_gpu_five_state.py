@@ -232,7 +232,7 @@ def test_five_state_forward_only(ph, five):
 
 
 def test_five_state_refusals(ph, five):
-    """Expected counts are 3-state only; bands wider than 1280 cells are
+    """Expected counts are 3-state only; bands wider than 2048 cells are
     refused; a batch keeps its state count across set_model."""
     mach = five[1][0]
     p = ph.PairwiseAlignmentParameters()
@@ -241,10 +241,50 @@ def test_five_state_refusals(ph, five):
     with pytest.raises(ph.MPError) as ei:
         ph.Batch(mach, [(sx, sx, None, 1)], p, mode=3)
     assert ei.value.status == 6
-    big = _rand_syms(rng, 1400, False)
+    big = _rand_syms(rng, 2100, False)
     with pytest.raises(ph.MPError) as ei:
         ph.getAlignedPairs(mach, 1, big, big, p)
     assert ei.value.status == 6
     b = ph.Batch(mach, [(sx, sx, None, 1)], p)
     with pytest.raises(ph.MPError):
         b.set_model(ph.StateMachines.default_nucleotide())
+
+
+def test_fp64_paths(ph, five):
+    """FP64 batches (the per-cell-exponent kernel in double): the five-state
+    machine and the 3-state machine (long gaps unreachable) against O_exact
+    at 1e-6 per posterior (the emission tables stay FP32: ~6e-8 relative per
+    factor) and 1e-5 relative per log-likelihood; bands wider than 1216 cells
+    are refused."""
+    from marginpolish_amd import crop, sim
+    mach5, models5, _ = five[0]
+    pp = ph.PolishParams(PARAMS)
+    op = O.PolishParams(PARAMS)
+    jobs = []
+    for k, W in enumerate((10, 200, 512)):
+        w = sim.config0(n_reads=2, ref_len=1200, seed=90 + k, W=W, strands=(1, 0))
+        for i, pr in enumerate(w.pairs):
+            sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+            jobs.append((sx, sy, an[::5], pr.strand, bool(i & 1), bool(k & 1)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=500, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    for tag, mach, models in (("five-state", mach5, models5), ("three-state", pp.machines, (op.reverse, op.forward))):
+        say("case: FP64 %s, %d pairs" % (tag, len(jobs)))
+        b = ph.Batch(mach, jobs, _bp(ph, opar), fp64=True)
+        b.run()
+        worst = 0.0
+        for (sx, sy, an, strand, rl, rr), (gm, gx, gy, glp) in zip(jobs, b.fetch()):
+            om, ox, oy, olp, _ = O.posteriors(models[strand], sx, sy, an, opar, rl, rr, mode=O.LOGADD_EXACT)
+            for g, o in ((gm, om), (gx, ox), (gy, oy)):
+                err, _, _ = compare(as_rows(g), o, opar.threshold, 1e-6)
+                worst = max(worst, err)
+                assert order_consistent(as_rows(g), o)
+            assert abs(glp - olp) <= 1e-5 * abs(olp), (glp, olp)
+        assert worst <= 1e-6, worst
+        say("FP64 %s max-abs posterior err vs O_exact: %.3g" % (tag, worst))
+    rng = random.Random(6)
+    big = _rand_syms(rng, 1300, False)
+    with pytest.raises(ph.MPError) as ei:
+        ph.Batch(ph.StateMachines.default_nucleotide(), [(big, big, None, 1)], ph.PairwiseAlignmentParameters(),
+                 fp64=True)
+    assert ei.value.status == 6
