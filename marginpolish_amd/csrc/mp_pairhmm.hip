@@ -163,11 +163,12 @@ mp_status launch_narrow(mp_batch *b, int grid, int block, const KParams &kp) {
 }
 
 mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
+    const dim3 block5(64 * wide5_waves(kp.wideC));
     if (kp.nStates == 5 && (b->flags & MP_BATCH_FP64))
-        hipLaunchKernelGGL(pairhmm_wide5_kernel<double>, dim3(grid), dim3(64), wide5_lds_bytes<double>(kp.wideC),
+        hipLaunchKernelGGL(pairhmm_wide5_kernel<double>, dim3(grid), block5, wide5_lds_bytes<double>(kp.wideC),
                            b->eng->stream, kp);
     else if (kp.nStates == 5)
-        hipLaunchKernelGGL(pairhmm_wide5_kernel<float>, dim3(grid), dim3(64), wide5_lds_bytes<float>(kp.wideC),
+        hipLaunchKernelGGL(pairhmm_wide5_kernel<float>, dim3(grid), block5, wide5_lds_bytes<float>(kp.wideC),
                            b->eng->stream, kp);
     else
         hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), kWideLdsBytes, b->eng->stream, kp);

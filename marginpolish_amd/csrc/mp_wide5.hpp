@@ -2,11 +2,14 @@
 // gap X, long gap Y), every job, in FP32 or FP64, gfx950.  No counterpart in
 // the reference.
 //
-// Same sweep structure as the 3-state wide kernel: one job per wavefront,
-// cell i = lane + 64c (c < C), one anti-diagonal per step, neighbours read
-// from LDS by band coordinate; forward to tb, backward down with posterior
-// records (anti-diagonals descending, cells ascending); same windows and
-// outputs.
+// Same sweep structure as the 3-state wide kernel, with one job per
+// workgroup of up to 8 wavefronts: cell i = lane + 64c (c < C), wavefront w
+// takes the columns c = w, w + NW, ...; one anti-diagonal per step (one
+// workgroup barrier), neighbours read from LDS by band coordinate; forward to
+// tb, backward down with posterior records; same windows and outputs.  The
+// records of one anti-diagonal are contiguous in a window's record stream (a
+// shared LDS counter hands out slots), cells in any order inside it: the list
+// assembly ranks them by cell.
 //
 // Scaling is per cell: each cell keeps its five values normalised to a
 // maximum in [0.5, 1) with an int exponent of its own, and aligns its
@@ -34,14 +37,18 @@ namespace mpk {
 
 constexpr int kNS5 = 5;
 constexpr int kWide5TabBytes = kTabFloats * 4;  // 41904, a multiple of 16
+constexpr int kWide5MaxWaves = 8;
+constexpr int kWide5CtlBytes = kWide5MaxWaves * 16 + 16;  // per-wave reduction slots, job and record counters
 template <class Real>
 constexpr int wide5_lds_bytes(int c) {
-    return kWide5TabBytes + 64 * c * (11 * (int)sizeof(Real) + 3 * 4);
+    return kWide5TabBytes + 64 * c * (11 * (int)sizeof(Real) + 3 * 4) + kWide5CtlBytes;
 }
 template <class Real>
 constexpr int wide5_max_c() {  // columns of 64 cells that fit 160 KB of LDS
-    return (160 * 1024 - kWide5TabBytes) / (64 * (11 * (int)sizeof(Real) + 3 * 4));
+    return (160 * 1024 - kWide5TabBytes - kWide5CtlBytes) / (64 * (11 * (int)sizeof(Real) + 3 * 4));
 }
+// wavefronts per workgroup for a batch whose widest band needs c columns
+__host__ __device__ constexpr int wide5_waves(int c) { return c < kWide5MaxWaves ? c : kWide5MaxWaves; }
 constexpr int kWide5MaxC32 = wide5_max_c<float>() < 32 ? wide5_max_c<float>() : 32;  // 2048 cells
 constexpr int kWide5MaxC64 = wide5_max_c<double>();                                  // 19: 1216 cells
 
@@ -118,14 +125,18 @@ struct Wide5Lds {
 };
 
 template <class Real>
-__global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
+__global__ __launch_bounds__(64 * kWide5MaxWaves) void pairhmm_wide5_kernel(KParams kp) {
     extern __shared__ float ldsw5[];
     float *tabL = ldsw5;
-    const int lane = threadIdx.x;
-    for (int i = lane; i < kTabFloats; i += 64) tabL[i] = kp.tables[i];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, NW = blockDim.x >> 6;
+    for (int i = tid; i < kTabFloats; i += blockDim.x) tabL[i] = kp.tables[i];
     const int C = kp.wideC, W = 64 * C;
     Real *arr = reinterpret_cast<Real *>(reinterpret_cast<char *>(ldsw5) + kWide5TabBytes);
     const Wide5Lds<Real> lx{arr, arr + 3 * W, reinterpret_cast<int *>(arr + 11 * W), W};
+    // control: per-wave totals (value, exponent), the job index, the record counter
+    Real *redT = reinterpret_cast<Real *>(lx.ex + 3 * W);
+    int *redE = reinterpret_cast<int *>(redT + kWide5MaxWaves);
+    int *ctl = redE + kWide5MaxWaves;
     // ring: per anti-diagonal six rows of W entries (five values, the exponent)
     Real *ring = reinterpret_cast<Real *>(kp.ringWide) + (size_t)blockIdx.x * kp.ringWideLen * W * 6;
     const int R = kp.ringWideLen;
@@ -141,10 +152,12 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
     __syncthreads();
 
     for (;;) {
-        int jn = 0;
-        if (lane == 0) jn = atomicAdd(&kp.queue[3], 1);
-        jn = __shfl(jn, 0, 64);
-        const int job = kp.wideBegin + jn;
+        if (tid == 0) {
+            ctl[0] = atomicAdd(&kp.queue[3], 1);
+            ctl[1] = 0;
+        }
+        __syncthreads();
+        const int job = kp.wideBegin + ctl[0];
         if (job >= kp.wideEnd) break;
         const DevJob jb = kp.jobs[job];
         const int D = jb.lX + jb.lY;
@@ -156,10 +169,9 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
         };
         const int64_t outBase = jb.outOff + jb.cap[0];
         const int cap = jb.cap[1];
-        int cnt = 0;
         int dStart = 0;
-        for (int i = lane; i < 11 * W; i += 64) lx.sendM[i] = Real(0);
-        for (int i = lane; i < 3 * W; i += 64) lx.ex[i] = kNoMass;
+        for (int i = tid; i < 11 * W; i += blockDim.x) lx.sendM[i] = Real(0);
+        for (int i = tid; i < 3 * W; i += blockDim.x) lx.ex[i] = kNoMass;
         __syncthreads();
         for (int win = 0; win < jb.nWin; win++) {
             const DevWin wn = kp.win[jb.winOff + win];
@@ -173,7 +185,7 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 band(d, L0, R0);
                 if (d >= 1) band(d - 1, L1, R1);
                 if (d >= 2) band(d - 2, L2, R2);
-                for (int c = 0; c < C; c++) {
+                for (int c = wave; c < C; c += NW) {
                     const int i = lane + 64 * c, xmy = L0 + 2 * i;
                     const bool act = xmy <= R0;
                     const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
@@ -227,11 +239,24 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 }
                 __syncthreads();
             }
-            // total at tb: t * 2^SFtb
-            const int SFtb = wave_max_i(acc > Real(0) ? accE : kNoMass);
-            const Real t = wave_sum_r(acc > Real(0) ? ldexp_r(acc, accE - SFtb) : Real(0));
+            // total at tb: t * 2^SFtb, over the wave, then over the waves in wave order
+            {
+                const int e = wave_max_i(acc > Real(0) ? accE : kNoMass);
+                const Real tw = wave_sum_r(acc > Real(0) ? ldexp_r(acc, accE - e) : Real(0));
+                if (lane == 0) {
+                    redT[wave] = tw;
+                    redE[wave] = e;
+                }
+            }
+            __syncthreads();
+            int SFtb = kNoMass;
+            for (int w = 0; w < NW; w++)
+                if (redT[w] > Real(0)) SFtb = max(SFtb, redE[w]);
+            Real t = Real(0);
+            for (int w = 0; w < NW; w++)
+                if (redT[w] > Real(0)) t += ldexp_r(redT[w], redE[w] - SFtb);
             const Real totalInv = t > Real(0) ? Real(1) / t : Real(0);
-            if (atEnd && lane == 0) {
+            if (atEnd && tid == 0) {
                 // (t, S) with t in [0.5, 1) as FP32: the host's log t + S ln 2 loses < 1e-7 to the rounding of t
                 const int k = t > Real(0) ? frexp_exp(t) : 0;
                 kp.logTotal[job] = make_int2(__float_as_int((float)ldexp_r(t, -k)), t > Real(0) ? SFtb + k : 0);
@@ -243,7 +268,7 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 band(d, L0, R0);
                 if (d + 1 <= wn.tb) band(d + 1, L1, R1);
                 if (d + 2 <= wn.tb) band(d + 2, L2, R2);
-                for (int c = 0; c < C; c++) {
+                for (int c = wave; c < C; c += NW) {
                     const int i = lane + 64 * c, xmy = L0 + 2 * i;
                     const bool act = xmy <= R0;
                     const int x = (d + xmy) >> 1, y = (d - xmy) >> 1;
@@ -299,13 +324,15 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                         }
                         const bool emit = act && (pM >= thr || (withIndels && (pX >= thr || pY >= thr)));
                         const uint64_t bm = __ballot(emit);
+                        int base = 0;
+                        if (lane == 0 && bm) base = atomicAdd(&ctl[1], __popcll(bm));
+                        base = __shfl(base, 0, 64);
                         if (emit) {
-                            const int idx = cnt + __popcll(bm & below);
+                            const int idx = base + __popcll(bm & below);
                             if (idx < cap)
                                 kp.out[outBase + idx] = make_int4(__float_as_int(pM), __float_as_int(pX),
                                                                   __float_as_int(pY), (int)rec_code(d, i));
                         }
-                        cnt += __popcll(bm);
                     }
                     lx.m(d)[i] = eM * q[0];
                     lx.g(d, 1)[i] = eX * q[1];
@@ -316,9 +343,9 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 }
                 __syncthreads();
             }
-            if (lane == 0) {
+            if (tid == 0) {
                 kp.winEnd[2 * (jb.winOff + win)] = 0;
-                kp.winEnd[2 * (jb.winOff + win) + 1] = cnt;
+                kp.winEnd[2 * (jb.winOff + win) + 1] = ctl[1];
             }
             // ---- resume the forward sweep after tb: its send values at tb and
             // the match send values at tb - 1, rebuilt from the ring ----
@@ -327,7 +354,7 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 int L0, R0, L1, R1;
                 band(tb, L0, R0);
                 band(tb - 1, L1, R1);
-                for (int c = 0; c < C; c++) {
+                for (int c = wave; c < C; c += NW) {
                     const int i = lane + 64 * c;
                     const bool a0 = L0 + 2 * i <= R0, a1 = L1 + 2 * i <= R1;
                     Real f[kNS5], g[kNS5];
@@ -355,9 +382,9 @@ __global__ __launch_bounds__(64) void pairhmm_wide5_kernel(KParams kp) {
                 dStart = tb + 1;
             }
         }
-        if (lane == 0) {
+        if (tid == 0) {
             kp.counts[2 * job] = 0;
-            kp.counts[2 * job + 1] = cnt;
+            kp.counts[2 * job + 1] = ctl[1];
         }
         __syncthreads();
     }
