@@ -136,6 +136,19 @@ mp_status mp_model_set_transitions(mp_model *m, const double *logTrans9, int32_t
  * mp_model_set_transitions5 makes the model five-state with these natural-log
  * transitions [from * 5 + to] on both strands; mp_model_transitions5 reads
  * them back; mp_model_num_states returns 3 or 5 (0 for a null model). */
+/* Emission part of stateMachine3_loadAsymmetric / hmm_getStateMachine
+ * (stateMachine.c:632-689), for the parameter-fitting harness: replace the
+ * forward strand's natural-log emissions (match16 = joint P(x base, y base)
+ * [x * 4 + y], gapX4, gapY4); the reverse strand takes their reverse
+ * complement (nucleotideEmissions_reverseComplement, stateMachine.c:454-470). */
+mp_status mp_model_set_emissions(mp_model *m, const double *logMatch16, const double *logGapX4,
+                                 const double *logGapY4);
+/* repeatSubMatrix_setLogProb (polisher.c:1777-1794) for every entry: rows 0-3
+ * of the repeat table, log10 P(observed | underlying, base) [base][underlying]
+ * [observed] (51 x 51 per base; NULL keeps the table), and enable = the
+ * run-length term in the match emission (rleNucleotideEmissions_construct,
+ * parser.c:391-401) on both strands. */
+mp_status mp_model_set_repeat_log10(mp_model *m, const double *log10Repeat, int32_t enable);
 mp_status mp_model_set_transitions5(mp_model *m, const double *logTrans25);
 mp_status mp_model_transitions5(const mp_model *m, double *logTrans25);
 int32_t mp_model_num_states(const mp_model *m);
@@ -207,6 +220,18 @@ mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_pair *out);
  * (M=0, X=1, Y=2) summed over the posterior diagonals of every window and
  * split, and the reference's per-diagonal likelihood sum (pairwiseAligner.c:674). */
 mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, double *likelihood);
+/* Expected emission counts of an expectation-mode batch, summed over its
+ * pairs, on the forward-strand parameters: match16 [x base * 4 + y base],
+ * gapX4, gapY4 and the repeat table's rows 0-3 [base][underlying][observed]
+ * (4 x 51 x 51; the derivative of the log-likelihood in a log10 entry is
+ * 2.3025 times its count).  Each is the sum over cells on posterior
+ * anti-diagonals of the posterior of the state that reads the entry there;
+ * with one traceback window per pair it is the derivative of the summed
+ * log-likelihood in the natural-log parameter.  No counterpart in the
+ * reference, whose emission expectations are disabled (pairwiseAligner.c:
+ * 359-362).  Any output may be NULL. */
+mp_status mp_batch_fetch_emission_counts(mp_batch *b, double *match16, double *gapX4, double *gapY4,
+                                         double *repeat);
 void mp_batch_free(mp_batch *b);
 
 /* computeForwardProbability (pairwiseAligner.c:813-865) for one pair. */

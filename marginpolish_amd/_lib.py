@@ -53,7 +53,7 @@ EXPORTS = [
     "mp_batch_fetch_list", "mp_batch_free", "mp_band_construct", "mp_split_points", "mp_traceback_windows",
     "mp_batch_fetch_expectations", "mp_pairhmm_forward_log_prob", "mp_pairhmm_expectations",
     "mp_model_set_transitions", "mp_batch_set_model", "mp_model_set_transitions5", "mp_model_transitions5",
-    "mp_model_num_states",
+    "mp_model_num_states", "mp_model_set_emissions", "mp_model_set_repeat_log10", "mp_batch_fetch_emission_counts",
 ]
 
 _lib = None
@@ -103,6 +103,9 @@ def lib():
         "mp_model_set_transitions5": (C.c_int, [vp, P(dbl)]),
         "mp_model_transitions5": (C.c_int, [vp, P(dbl)]),
         "mp_model_num_states": (i32, [vp]),
+        "mp_model_set_emissions": (C.c_int, [vp, P(dbl), P(dbl), P(dbl)]),
+        "mp_model_set_repeat_log10": (C.c_int, [vp, P(dbl), i32]),
+        "mp_batch_fetch_emission_counts": (C.c_int, [vp, P(dbl), P(dbl), P(dbl), P(dbl)]),
         "mp_batch_set_model": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():

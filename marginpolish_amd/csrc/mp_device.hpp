@@ -123,6 +123,7 @@ struct KParams {
     const float *tables;
     int32_t *queue;      // kNumClasses + 1 counters (the last for wide jobs)
     double *expect;      // per job: 9 expected transition counts + likelihood
+    double *emitCount;   // per batch: expected emission counts by emission-table entry (kTabFloats)
     int32_t *error;      // set when a wavefront gives up waiting for its partner (never in a correct run)
     float *ringWide;
     int32_t ringWideLen, wideC, wideBegin, wideEnd;

@@ -458,6 +458,48 @@ extern "C" mp_status mp_model_set_transitions(mp_model *m, const double *logTran
     return MP_OK;
 }
 
+extern "C" mp_status mp_model_set_emissions(mp_model *m, const double *logMatch16, const double *logGapX4,
+                                            const double *logGapY4) {
+    if (!m || !logMatch16 || !logGapX4 || !logGapY4) {
+        mp::set_error("mp_model_set_emissions: null argument");
+        return MP_EINVAL;
+    }
+    for (int i = 0; i < 24; i++) {
+        const double v = i < 16 ? logMatch16[i] : (i < 20 ? logGapX4[i - 16] : logGapY4[i - 20]);
+        if (std::isnan(v) || v > 0.0) {
+            mp::set_error("mp_model_set_emissions: emissions must be log probabilities (<= 0, not NaN)");
+            return MP_EINVAL;
+        }
+    }
+    mp::Machine &f = m->mach[1];
+    std::memcpy(f.match, logMatch16, sizeof(f.match));
+    std::memcpy(f.gapX, logGapX4, sizeof(f.gapX));
+    std::memcpy(f.gapY, logGapY4, sizeof(f.gapY));
+    const mp::Machine r = mp::reverse_complement(f);
+    std::memcpy(m->mach[0].match, r.match, sizeof(r.match));
+    std::memcpy(m->mach[0].gapX, r.gapX, sizeof(r.gapX));
+    std::memcpy(m->mach[0].gapY, r.gapY, sizeof(r.gapY));
+    return MP_OK;
+}
+
+extern "C" mp_status mp_model_set_repeat_log10(mp_model *m, const double *log10Repeat, int32_t enable) {
+    if (!m || (enable && !log10Repeat)) {
+        mp::set_error("mp_model_set_repeat_log10: null argument");
+        return MP_EINVAL;
+    }
+    if (log10Repeat) {
+        for (int i = 0; i < 4 * mp::kMaxRepeat * mp::kMaxRepeat; i++)
+            if (std::isnan(log10Repeat[i]) || log10Repeat[i] > 0.0) {
+                mp::set_error("mp_model_set_repeat_log10: entries must be log10 probabilities (<= 0, not NaN)");
+                return MP_EINVAL;
+            }
+        m->repeatLog10.resize(mp::kRepeatEntries, 0.0);
+        std::memcpy(m->repeatLog10.data(), log10Repeat, sizeof(double) * 4 * mp::kMaxRepeat * mp::kMaxRepeat);
+    }
+    m->mach[0].rle = m->mach[1].rle = enable ? 1 : 0;
+    return MP_OK;
+}
+
 extern "C" int32_t mp_model_num_states(const mp_model *m) { return m ? m->mach[1].nStates : 0; }
 
 extern "C" mp_status mp_model_set_transitions5(mp_model *m, const double *logTrans25) {

@@ -132,6 +132,7 @@ struct mp_batch {
     DevBuf<float> dTables;
     DevBuf<int32_t> dQueue;
     DevBuf<double> dExpect;
+    DevBuf<double> dEmit;  // expected emission counts by emission-table entry (expectation mode)
     DevBuf<int32_t> dError;
     DevBuf<float> dRingWide;
     int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
@@ -156,7 +157,8 @@ mp_status launch_narrow(mp_batch *b, int grid, int block, const KParams &kp) {
     if (use_split(b->mode)) {
         hipLaunchKernelGGL(pairhmm_split_kernel, dim3(grid), dim3(block), kSplitLdsBytes, b->eng->stream, kp);
     } else {
-        hipLaunchKernelGGL(pairhmm_seq_kernel, dim3(grid), dim3(block), kSeqLdsBytes, b->eng->stream, kp);
+        hipLaunchKernelGGL(pairhmm_seq_kernel, dim3(grid), dim3(block),
+                           b->mode == MP_MODE_EXPECTATIONS ? kSeqLdsBytesExp : kSeqLdsBytes, b->eng->stream, kp);
     }
     HIP_TRY(hipGetLastError());
     return MP_OK;
@@ -307,7 +309,7 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_split_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 kSplitLdsBytes));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_seq_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kSeqLdsBytes));
+                                kSeqLdsBytesExp));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 kWideLdsBytes));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel<float>,
@@ -717,7 +719,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
         (st = b->dQueue.alloc(kNumClasses + 1)) || (st = b->dError.alloc(1)) ||
-        (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1))) {
+        (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1)) ||
+        (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
         delete b;
         return st;
     }
@@ -776,6 +779,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.withIndels = mode == MP_MODE_POSTERIORS;
     kp.mode = mode;
     kp.expect = b->dExpect.p;
+    kp.emitCount = b->dEmit.p;
     kp.error = b->dError.p;
     kp.threshold = (float)p->threshold;
     for (int c = 0; c < kNumClasses; c++) {
@@ -802,6 +806,9 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     mp_status st;
     float ms = 0.f;
     if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
+        return st;
+    if (b->mode == MP_MODE_EXPECTATIONS &&
+        (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dEmit.p), 2 * kTabFloats)))
         return st;
     if ((st = reset_i32(b, b->dError.p, 1))) return st;
     const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
@@ -1111,6 +1118,45 @@ extern "C" mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, dou
         for (int k = 0; k < 9; k++) trans[9 * i + k] = t[k];
         if (likelihood) likelihood[i] = t[9];
     }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_fetch_emission_counts(mp_batch *b, double *match16, double *gapX4, double *gapY4,
+                                                    double *repeat) {
+    if (!b) {
+        mp::set_error("mp_batch_fetch_emission_counts: null batch");
+        return MP_EINVAL;
+    }
+    if (!b->ran || b->mode != MP_MODE_EXPECTATIONS) {
+        mp::set_error("mp_batch_fetch_emission_counts: batch has not run in expectation mode");
+        return MP_EINVAL;
+    }
+    std::vector<double> h(kTabFloats, 0.0);
+    if (!b->jobs.empty()) {
+        HIP_TRY(hipSetDevice(b->eng->device));
+        HIP_TRY(hipMemcpy(h.data(), b->dEmit.p, sizeof(double) * kTabFloats, hipMemcpyDeviceToHost));
+    }
+    // table entries -> forward-strand parameters: the reverse strand reads the
+    // reverse complement (M'[i][j] = M[3-i][3-j], G'[i] = G[3-i]); N entries are
+    // constants, not parameters; run-length rows are already strand-resolved
+    if (match16)
+        for (int i = 0; i < 16; i++) match16[i] = 0.0;
+    if (gapX4)
+        for (int i = 0; i < 4; i++) gapX4[i] = 0.0;
+    if (gapY4)
+        for (int i = 0; i < 4; i++) gapY4[i] = 0.0;
+    for (int st = 0; st < 2; st++)
+        for (int i = 0; i < 4; i++) {
+            const int fi = st ? i : 3 - i;
+            if (gapX4) gapX4[fi] += h[kTabGX + st * 5 + i];
+            if (gapY4) gapY4[fi] += h[kTabGY + st * 5 + i];
+            for (int j = 0; j < 4; j++) {
+                const int fj = st ? j : 3 - j;
+                if (match16) match16[fi * 4 + fj] += h[kTabE + st * 25 + i * 5 + j];
+            }
+        }
+    if (repeat)
+        for (int i = 0; i < 4 * kRL * kRL; i++) repeat[i] = h[kTabRL + i];
     return MP_OK;
 }
 

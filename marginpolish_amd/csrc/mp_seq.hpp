@@ -21,7 +21,7 @@ enum { Q_FETCH = 0, Q_FWD = 1, Q_BWD = 2, Q_DONE = 3 };
 // traceback window are kept in an HBM ring (float4: 3 states + the cell's
 // exponent) for the backward sweep.
 template <int SEG>
-__device__ __forceinline__ void seq_class(const KParams &kp, const float *tab, int cls) {
+__device__ __forceinline__ void seq_class(const KParams &kp, const float *tab, float *hist, int cls) {
     constexpr int K = kK;
     constexpr int kEmpty = kEmptyBand;
     const int lane = threadIdx.x & 63;
@@ -236,6 +236,17 @@ __device__ __forceinline__ void seq_class(const KParams &kp, const float *tab, i
                         e00 += a0 * (T00 * tM); e01 += a0 * (T01 * tX); e02 += a0 * (T02 * tY);
                         e10 += a1 * (T10 * tM); e11 += a1 * (T11 * tX); e12 += a1 * (T12 * tY);
                         e20 += a2 * (T20 * tM); e21 += a2 * (T21 * tX); e22 += a2 * (T22 * tY);
+                        // expected emission counts: this cell's state posteriors, binned by
+                        // the emission-table entry each state reads here
+                        if (dj > lo && dj <= from) {
+                            const float scq = ldexpf(totalInv, __float_as_int(f.w) + S - SFtb);
+                            const uint32_t sx = sxC[j], sy = syC[j], sxy = sx + sy;
+                            const float pM = f.x * q0 * scq;
+                            atomicAdd(&hist[kTabE + (((sxy >> 16) & 0xffu) >> 2)], pM);
+                            atomicAdd(&hist[kTabRL + (min(sxy & 0xffffu, kRLLastByte) >> 2)], pM);
+                            atomicAdd(&hist[(sx >> 24) >> 2], f.y * q1 * scq);
+                            atomicAdd(&hist[(sy >> 24) >> 2], f.z * q2 * scq);
+                        }
                     }
                     if (dj == lo) winDone = true;
                     oM2 = oM1;
@@ -319,13 +330,28 @@ __device__ __forceinline__ void seq_class(const KParams &kp, const float *tab, i
     }
 }
 
+// Expectation mode adds an FP32 histogram of expected emission counts over
+// the emission-table entries (one per workgroup, in LDS after the table),
+// added to kp.emitCount in FP64 when the workgroup ends.
+constexpr int kSeqLdsBytesExp = kTabReach + kTabFloats * 4;
+
 __global__ __launch_bounds__(512) void pairhmm_seq_kernel(KParams kp) {
     extern __shared__ float lds[];
-    for (int i = threadIdx.x; i < kTabFloats; i += blockDim.x) lds[i] = kp.tables[i];
+    float *hist = lds + kTabReach / 4;
+    const bool expMode = kp.mode == MP_MODE_EXPECTATIONS;
+    for (int i = threadIdx.x; i < kTabFloats; i += blockDim.x) {
+        lds[i] = kp.tables[i];
+        if (expMode) hist[i] = 0.f;
+    }
     __syncthreads();
-    seq_class<64>(kp, lds, 2);
-    seq_class<32>(kp, lds, 1);
-    seq_class<16>(kp, lds, 0);
+    seq_class<64>(kp, lds, hist, 2);
+    seq_class<32>(kp, lds, hist, 1);
+    seq_class<16>(kp, lds, hist, 0);
+    if (expMode) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < kTabFloats; i += blockDim.x)
+            if (hist[i] != 0.f) atomicAdd(&kp.emitCount[i], (double)hist[i]);
+    }
 }
 
 }  // namespace mpk

@@ -199,6 +199,17 @@ __global__ __launch_bounds__(64) void pairhmm_wide_kernel(KParams kp) {
                         ep[0] += a0 * T[0] * tM; ep[1] += a0 * T[1] * tX; ep[2] += a0 * T[2] * tY;
                         ep[3] += a1 * T[3] * tM; ep[4] += a1 * T[4] * tX; ep[5] += a1 * T[5] * tY;
                         ep[6] += a2 * T[6] * tM; ep[7] += a2 * T[7] * tX; ep[8] += a2 * T[8] * tY;
+                        // expected emission counts (state posteriors by emission-table entry)
+                        if (d > wn.lo && d <= wn.from) {
+                            const float scq = ldexpf(totalInv, __float_as_int(f.w) + SB + e2 - SFtb);
+                            const uint32_t sx = kp.sym[jb.symX + x - 1], sy = kp.sym[jb.symY + y - 1];
+                            const uint32_t sxy = sx + sy;
+                            const double pM = (double)(f.x * q0 * scq);
+                            atomicAdd(&kp.emitCount[kTabE + (((sxy >> 16) & 0xffu) >> 2)], pM);
+                            atomicAdd(&kp.emitCount[kTabRL + (min(sxy & 0xffffu, kRLLastByte) >> 2)], pM);
+                            atomicAdd(&kp.emitCount[(sx >> 24) >> 2], (double)(f.y * q1 * scq));
+                            atomicAdd(&kp.emitCount[(sy >> 24) >> 2], (double)(f.z * q2 * scq));
+                        }
                     }
                     if (postMode && d <= wn.from) {
                         const float sc = ldexpf(totalInv, __float_as_int(f.w) + SB + e2 - SFtb);

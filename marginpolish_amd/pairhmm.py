@@ -119,6 +119,22 @@ class StateMachines:
         t = (C.c_double * 9)(*[float(v) for v in log_trans9])
         check(lib().mp_model_set_transitions(self._h, t, int(bool(pin_end))))
 
+    def set_emissions(self, log_match16, log_gap_x4, log_gap_y4):
+        """Replace the forward strand's natural-log emissions (joint match
+        [x * 4 + y], gap X, gap Y); the reverse strand takes their reverse
+        complement."""
+        f = lambda v, n: (C.c_double * n)(*[float(x) for x in np.asarray(v, dtype=np.float64).reshape(n)])
+        check(lib().mp_model_set_emissions(self._h, f(log_match16, 16), f(log_gap_x4, 4), f(log_gap_y4, 4)))
+
+    def set_repeat_log10(self, log10_repeat, enable=True):
+        """Rows 0-3 of the repeat table (log10, [base][underlying][observed],
+        4 x 51 x 51) and whether the match emission uses it."""
+        if log10_repeat is None:
+            check(lib().mp_model_set_repeat_log10(self._h, None, int(bool(enable))))
+            return
+        a = np.ascontiguousarray(np.asarray(log10_repeat, dtype=np.float64).reshape(4 * 51 * 51))
+        check(lib().mp_model_set_repeat_log10(self._h, a.ctypes.data_as(C.POINTER(C.c_double)), int(bool(enable))))
+
     @property
     def num_states(self):
         return int(lib().mp_model_num_states(self._h))
@@ -289,6 +305,16 @@ class Batch:
         check(lib().mp_batch_fetch_expectations(self._h, t.ctypes.data_as(C.POINTER(C.c_double)),
                                                 lk.ctypes.data_as(C.POINTER(C.c_double))))
         return t[: self.npairs], lk[: self.npairs]
+
+    def emission_counts(self):
+        """Expectation mode: expected emission counts summed over the pairs,
+        on the forward-strand parameters: (match 4x4 [x, y], gap X 4, gap Y 4,
+        repeat table 4x51x51 [base, underlying, observed])."""
+        m, gx, gy = np.zeros(16), np.zeros(4), np.zeros(4)
+        rl = np.zeros(4 * 51 * 51)
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        check(lib().mp_batch_fetch_emission_counts(self._h, P(m), P(gx), P(gy), P(rl)))
+        return m.reshape(4, 4), gx, gy, rl.reshape(4, 51, 51)
 
     def close(self):
         if getattr(self, "_h", None):

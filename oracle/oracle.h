@@ -67,6 +67,9 @@ double or_result_total(const or_result *r);
 double or_result_likelihood(const or_result *r);
 int64_t or_result_cells(const or_result *r);
 void or_result_expectations(const or_result *r, double *out9);
+/* expected emission counts (expectations runs), forward-strand parameters:
+ * match[16] [x*4+y], gapX[4], gapY[4], repeat[4][51][51] [base][underlying][observed] */
+void or_result_emissions(const or_result *r, double *out);
 void or_result_free(or_result *r);
 
 int64_t or_batch_cells(const or_model *models, int64_t npairs,

@@ -99,6 +99,8 @@ def lib():
         L.or_result_cells.argtypes = [C.c_void_p]
         L.or_result_expectations.restype = None
         L.or_result_expectations.argtypes = [C.c_void_p, dp]
+        L.or_result_emissions.restype = None
+        L.or_result_emissions.argtypes = [C.c_void_p, dp]
         L.or_result_free.restype = None
         L.or_result_free.argtypes = [C.c_void_p]
         L.or_batch_cells.restype = C.c_int64
@@ -382,6 +384,27 @@ def expectations(model, sX, sY, anchors, params, ragged_left=False, ragged_right
         e = np.zeros(9)
         L.or_result_expectations(r, _ptr(e, C.c_double))
         return e, L.or_result_likelihood(r)
+    finally:
+        L.or_result_free(r)
+
+
+def emission_counts(model, sX, sY, anchors, params, ragged_left=False, ragged_right=False, mode=LOGADD_EXACT):
+    """Expected emission counts of getExpectationsUsingAnchors' diagonals (the
+    reference disables them, pairwiseAligner.c:359-362): (match 4x4 [x, y],
+    gapX 4, gapY 4, repeat 4x51x51 [base, underlying, observed]) on the
+    forward-strand parameters."""
+    L = lib()
+    sx, sy = _u16(sX), _u16(sY)
+    a, n = _anchors(anchors)
+    pc = params.c()
+    r = L.or_expectations(C.byref(model.c()), _ptr(sx, C.c_uint16), len(sx), _ptr(sy, C.c_uint16), len(sy),
+                          _ptr(a, C.c_int64), n, C.byref(pc), int(ragged_left), int(ragged_right), mode)
+    try:
+        if L.or_result_status(r) != 0:
+            raise ValueError("oracle status %d" % L.or_result_status(r))
+        e = np.zeros(16 + 4 + 4 + 4 * 51 * 51)
+        L.or_result_emissions(r, _ptr(e, C.c_double))
+        return e[:16].reshape(4, 4), e[16:20], e[20:24], e[24:].reshape(4, 51, 51)
     finally:
         L.or_result_free(r)
 

@@ -473,9 +473,13 @@ static void list_push(List *l, int64_t w, int64_t x, int64_t y) {
     l->n++;
 }
 
+#define OR_EMIT_N (16 + 4 + 4 + 4 * 51 * 51)
+
 struct or_result {
     List lists[3];        /* match, gapX, gapY */
     double expect[9];     /* transition expectations T[from*3+to] */
+    double emit[OR_EMIT_N]; /* expected emission counts, forward-strand parameters: match[x*4+y],
+                               gapX[4], gapY[4], repeat[base][underlying][observed] */
     double likelihood;    /* per-diagonal accumulated total (expectation hack) */
     double totalLogProb;  /* sum over sub-problems of the last window's total (forward log-likelihood) */
     int64_t cells;        /* band cells of all sub-problems */
@@ -527,6 +531,27 @@ static void diag_posterior(const Ctx *c, Dp *dp, const uint16_t *sX, const uint1
                 EXPT(ST_X, ST_Y, up[ST_X], cur[ST_Y], e, m->gapSwitchToY);
             }
 #undef EXPT
+            /* expected emission counts: the posterior of each state at this cell,
+             * on the forward-strand parameter that state's emission reads (no
+             * reference counterpart: emission expectations are disabled at
+             * pairwiseAligner.c:359-362) */
+            const double *f = cellp(dp, dp->F, xay, xmy);
+            int64_t x = xcoord(xay, xmy), y = ycoord(xay, xmy);
+            int st = m->strand;
+            int bx = m->rle ? (cx & 255) : cx, by = m->rle ? (cy & 255) : cy;
+            double pM = exp((f[ST_M] + cur[ST_M]) - total);
+            double pX = exp((f[ST_X] + cur[ST_X]) - total);
+            double pY = exp((f[ST_Y] + cur[ST_Y]) - total);
+            if (x > 0 && y > 0) {
+                if (bx < 4 && by < 4) res->emit[st ? bx * 4 + by : (3 - bx) * 4 + (3 - by)] += pM;
+                if (m->rle) {
+                    int b = bx >= 4 ? 0 : bx;
+                    int row = st ? b : 3 - b;
+                    res->emit[24 + (row * 51 + (cx >> 8)) * 51 + (cy >> 8)] += pM;
+                }
+            }
+            if (x > 0 && bx < 4) res->emit[16 + (st ? bx : 3 - bx)] += pX;
+            if (y > 0 && by < 4) res->emit[20 + (st ? by : 3 - by)] += pY;
         }
         return;
     }
@@ -783,6 +808,7 @@ double or_result_total(const or_result *r) { return r->totalLogProb; }
 double or_result_likelihood(const or_result *r) { return r->likelihood; }
 int64_t or_result_cells(const or_result *r) { return r->cells; }
 void or_result_expectations(const or_result *r, double *out9) { memcpy(out9, r->expect, sizeof(double) * 9); }
+void or_result_emissions(const or_result *r, double *out) { memcpy(out, r->emit, sizeof(double) * OR_EMIT_N); }
 void or_result_free(or_result *r) {
     if (!r) return;
     for (int i = 0; i < 3; i++) free(r->lists[i].v);

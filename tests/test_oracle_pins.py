@@ -323,3 +323,53 @@ def test_faithful_vs_exact_gap_is_bounded():
         common = set(da) & set(db)
         assert len(common) > 0.9 * max(len(da), len(db))
         assert max(abs(da[k] - db[k]) for k in common) < 2e-2 * 1e7
+
+
+def _brute_emission_counts(m, sx, sy, post):
+    """Expected emission counts from exact state posteriors, on the
+    forward-strand parameters (reverse strand: reverse complement)."""
+    match, gx, gy = np.zeros((4, 4)), np.zeros(4), np.zeros(4)
+    rl = np.zeros((4, 51, 51))
+    st = m.strand
+    for x in range(len(sx) + 1):
+        for y in range(len(sy) + 1):
+            cx = int(sx[x - 1]) if x > 0 else 4
+            cy = int(sy[y - 1]) if y > 0 else 4
+            bx = (cx & 255) if m.rle else cx
+            by = (cy & 255) if m.rle else cy
+            if x > 0 and y > 0:
+                if bx < 4 and by < 4:
+                    match[(bx, by) if st else (3 - bx, 3 - by)] += post[0, x, y]
+                if m.rle:
+                    b = 0 if bx >= 4 else bx
+                    rl[b if st else 3 - b, cx >> 8, cy >> 8] += post[0, x, y]
+            if x > 0 and bx < 4:
+                gx[bx if st else 3 - bx] += post[1, x, y]
+            if y > 0 and by < 4:
+                gy[by if st else 3 - by] += post[2, x, y]
+    return match, gx, gy, rl
+
+
+@pytest.mark.parametrize("ragged", [(False, False), (True, True)])
+def test_emission_counts_vs_brute_force(ragged):
+    """Expected emission counts (one window) equal the exact state
+    posteriors of path enumeration summed per parameter, for the default
+    machine and both RLE strands (the reverse strand maps through the reverse
+    complement), and every x (y) is emitted once: match + gap-X counts sum to
+    the number of non-N reference bases."""
+    rng = random.Random(23)
+    p = O.Params(threshold=0.0, diagonalExpansion=20, minDiagsBetweenTraceBack=1000)
+    for name, m in _models():
+        for _ in range(4):
+            lX, lY = rng.randint(1, 5), rng.randint(1, 5)
+            sx = _rand_rle(rng, lX) if m.rle else O.symbols("".join(rng.choice("ACGT") for _ in range(lX)))
+            sy = _rand_rle(rng, lY) if m.rle else O.symbols("".join(rng.choice("ACGT") for _ in range(lY)))
+            _, post = brute.enumerate_paths(m, sx, sy, *ragged)
+            want = _brute_emission_counts(m, sx, sy, post)
+            got = O.emission_counts(m, sx, sy, [], p, *ragged)
+            for g, w in zip(got, want):
+                assert np.allclose(g, w, atol=1e-12, rtol=1e-10), (name, g, w)
+            assert abs(got[0].sum() + got[1].sum() - lX) < 1e-9
+            assert abs(got[0].sum() + got[2].sum() - lY) < 1e-9
+            if m.rle:
+                assert abs(got[3].sum() - got[0].sum()) < 1e-9
