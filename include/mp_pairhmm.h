@@ -53,8 +53,10 @@ enum {
 /* Batch flags. */
 enum {
     MP_BATCH_CROP_REFERENCE = 1, /* crop sX per pair as poa_realign's caller does (polisher.c:605-654) */
-    MP_BATCH_FP64 = 2            /* FP64 arithmetic on the per-cell-exponent kernel (no reference counterpart:
+    MP_BATCH_FP64 = 2,           /* FP64 arithmetic on the per-cell-exponent kernel (no reference counterpart:
                                     the precision comparison path); bands up to 1216 cells, no expectations */
+    MP_BATCH_NO_GRAPH = 4        /* enqueue each run's launches directly instead of replaying the batch's
+                                    hipGraph (captured at the first run, re-captured after mp_batch_set_model) */
 };
 
 typedef struct mp_model mp_model;          /* both strands' state machines + tables */

@@ -19,6 +19,7 @@ LIST_MATCH, LIST_GAPX, LIST_GAPY = 0, 1, 2
 MODE_POSTERIORS, MODE_MATCH_ONLY, MODE_FORWARD_ONLY, MODE_EXPECTATIONS = 0, 1, 2, 3
 BATCH_CROP_REFERENCE = 1
 BATCH_FP64 = 2
+BATCH_NO_GRAPH = 4
 
 
 class MPError(RuntimeError):

@@ -137,6 +137,7 @@ struct mp_batch {
     DevBuf<float> dRingWide;
     int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
     int32_t modelStates = 3;
+    hipGraphExec_t graph = nullptr;  // the run's resets and launches, captured once per parameter block
     int32_t narrowBlocks = 0;
     int64_t outTotal = 0;
     double kernelMs = 0.0;
@@ -791,6 +792,57 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     return MP_OK;
 }
 
+static void drop_graph(mp_batch *b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+// The device work of one batched run: clear the counters, then the narrow
+// and wide launches.  Enqueued on the batch's stream, or captured into a graph.
+static mp_status enqueue_run(mp_batch *b) {
+    const int32_t nj = (int32_t)b->jobs.size();
+    mp_status st;
+    if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
+        return st;
+    if (b->mode == MP_MODE_EXPECTATIONS &&
+        (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dEmit.p), 2 * kTabFloats)))
+        return st;
+    if ((st = reset_i32(b, b->dError.p, 1))) return st;
+    if ((st = reset_i32(b, b->dQueue.p, kNumClasses + 1))) return st;
+    const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
+                           b->classEnd[2] > b->classBegin[2];
+    if (anyNarrow && (st = launch_narrow(b, b->narrowBlocks, 512, b->kp))) return st;
+    if (b->wideEnd > b->wideBegin && (st = launch_wide(b, b->wideWaves, b->kp))) return st;
+    return MP_OK;
+}
+
+// hipGraph of enqueue_run (the per-iteration graph: capture once, replay every
+// run until the parameter block changes).  The graph holds kernel nodes only;
+// counters are cleared by fill kernels, not memset nodes.
+static mp_status capture_run(mp_batch *b) {
+    hipStream_t s = b->eng->stream;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const mp_status st = enqueue_run(b);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ec != hipSuccess) {
+        mp::set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        return MP_EHIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        b->graph = nullptr;
+        mp::set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        return MP_EHIP;
+    }
+    return MP_OK;
+}
+
 static mp_status batch_run_impl(mp_batch *b, bool plain) {
     mp_engine *e = b->eng;
     HIP_TRY(hipSetDevice(e->device));
@@ -805,24 +857,26 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     }
     mp_status st;
     float ms = 0.f;
-    if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
-        return st;
-    if (b->mode == MP_MODE_EXPECTATIONS &&
-        (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dEmit.p), 2 * kTabFloats)))
-        return st;
-    if ((st = reset_i32(b, b->dError.p, 1))) return st;
-    const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
-                           b->classEnd[2] > b->classBegin[2];
-    const int narrowBlock = use_split(b->mode) ? 512 : 512;
     if (!plain) {
-        if ((st = reset_i32(b, b->dQueue.p, kNumClasses + 1))) return st;
+        const bool useGraph = (b->flags & MP_BATCH_NO_GRAPH) == 0;
+        if (useGraph && !b->graph && (st = capture_run(b))) return st;
         HIP_TRY(hipEventRecord(e->ev0, s));
-        if (anyNarrow && (st = launch_narrow(b, b->narrowBlocks, narrowBlock, b->kp))) return st;
-        if (b->wideEnd > b->wideBegin && (st = launch_wide(b, b->wideWaves, b->kp))) return st;
+        if (useGraph) {
+            HIP_TRY(hipGraphLaunch(b->graph, s));
+        } else if ((st = enqueue_run(b))) {
+            return st;
+        }
         HIP_TRY(hipEventRecord(e->ev1, s));
         HIP_TRY(hipEventSynchronize(e->ev1));
         HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     } else {
+        if (b->mode == MP_MODE_EXPECTATIONS &&
+            (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
+            return st;
+        if (b->mode == MP_MODE_EXPECTATIONS &&
+            (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dEmit.p), 2 * kTabFloats)))
+            return st;
+        if ((st = reset_i32(b, b->dError.p, 1))) return st;
         // one launch per job (one row, or one forward/backward pair of rows), in device order
         DevBuf<int32_t> q;
         const int nq = kNumClasses + 1;
@@ -877,6 +931,7 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
         if ((st = b->dOut.alloc(std::max<int64_t>(off, 1)))) return st;
         b->kp.out = b->dOut.p;
+        drop_graph(b);
         b->outTotal = off;
         return batch_run_impl(b, plain);
     }
@@ -905,6 +960,7 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
     std::memcpy(b->kp.startV5, kp.startV5, sizeof(kp.startV5));
     std::memcpy(b->kp.endV5, kp.endV5, sizeof(kp.endV5));
     b->kp.tables = b->dTables.p;
+    drop_graph(b);
     b->ran = false;
     b->listsReady = false;
     return MP_OK;
@@ -1192,6 +1248,7 @@ extern "C" mp_status mp_pairhmm_expectations(mp_engine *e, const mp_model *m, co
 extern "C" void mp_batch_free(mp_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->eng->device);
+    drop_graph(b);
     delete b;
 }
 

@@ -232,10 +232,12 @@ class Batch:
     """A device-resident batch of read/reference pairs (mp_batch)."""
 
     def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None,
-                 fp64=False):
+                 fp64=False, graph=True):
         """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight]).
         fp64: FP64 arithmetic on the per-cell-exponent kernel (the precision
-        comparison path; 3-state machines run with the long gaps unreachable)."""
+        comparison path; 3-state machines run with the long gaps unreachable).
+        graph: replay the batch's captured hipGraph on every run (False:
+        enqueue the launches directly)."""
         self.engine = engine or default_engine()
         self._keep = []
         n = len(pairs)
@@ -253,7 +255,8 @@ class Batch:
         pc = p.c()
         check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
                                     (_lib.BATCH_CROP_REFERENCE if crop_reference else 0) |
-                                    (_lib.BATCH_FP64 if fp64 else 0), C.byref(h)))
+                                    (_lib.BATCH_FP64 if fp64 else 0) | (0 if graph else _lib.BATCH_NO_GRAPH),
+                                    C.byref(h)))
         self._h = h
         self.npairs = n
         self.mode = mode

@@ -402,3 +402,31 @@ def test_expectations_parity(ph, params):
     for (sx, sy, _, _), tg in zip(cases, t):
         te, _ = O.expectations(odm, sx, sy, [], O.Params(threshold=0.0, diagonalExpansion=20))
         assert np.allclose(tg, te, rtol=1e-4, atol=1e-5), (tg, te)
+
+
+def test_graph_replays_equal_eager_runs(ph, params, config1):
+    """The batch's hipGraph (counter-clearing fill kernels + launches) replayed
+    several times, with eager batches run and synchronised between the
+    replays, gives the eager result every time; re-capture after set_model."""
+    from marginpolish_amd import crop
+    pp, op = params
+    jobs = []
+    for pr in config1.pairs[:200]:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    say("case: graph replays vs eager, 200 pairs")
+    eager = ph.Batch(pp.machines, jobs, pp.p, graph=False).run().fetch()
+    g = ph.Batch(pp.machines, jobs, pp.p)
+    for rep in range(4):
+        g.run()
+        got = g.fetch()
+        other = ph.Batch(pp.machines, jobs[rep::4], pp.p, graph=False).run()  # eager work between replays
+        other.fetch()
+        for r1, r2 in zip(got, eager):
+            for l in range(3):
+                assert np.array_equal(r1[l], r2[l]), (rep, l)
+            assert r1[3] == r2[3]
+    g.set_model(pp.machines)
+    g.run()
+    for r1, r2 in zip(g.fetch(), eager):
+        assert np.array_equal(r1[0], r2[0])
