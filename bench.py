@@ -81,16 +81,22 @@ def main():
     pp = pairhmm.PolishParams(PARAMS)
     eng = pairhmm.default_engine()
 
-    # reads/s: one POA iteration end to end (host prep + upload + kernel + result handoff)
-    t1 = time.time()
-    b = pairhmm.Batch(pp.machines, pairs, pp.p, crop_reference=True, engine=eng)
-    t_prep = time.time() - t1
-    b.run()
-    res = b.fetch()
-    t_iter = time.time() - t1
+    # reads/s: one POA iteration end to end (host prep + upload + kernel + result handoff), timed on the
+    # second iteration of the engine (the first also pays one-time allocations: pinned staging, code load)
+    iters = []
+    for it in range(2):
+        t1 = time.time()
+        b = pairhmm.Batch(pp.machines, pairs, pp.p, crop_reference=True, engine=eng)
+        t_prep = time.time() - t1
+        b.run()
+        res = b.fetch()
+        iters.append((t_prep, time.time() - t1))
+        if it == 0:
+            b.close()
+    t_prep, t_iter = iters[1]
     reads_per_s = len(pairs) / t_iter
     cells = b.cells
-    log("iteration: prep %.3f s, total %.3f s, %d cells" % (t_prep, t_iter, cells))
+    log("iteration: first %.3f s; second: prep %.3f s, total %.3f s, %d cells" % (iters[0][1], t_prep, t_iter, cells))
 
     for _ in range(args.warmup):
         b.run()
@@ -124,6 +130,7 @@ def main():
                    "pairs": len(pairs), "cells": int(cells), "band_half_width": 10},
         "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
         "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),
+        "first_iteration_s": round(iters[0][1], 4),
         "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
         "plain_sample_pairs": len(samp),
     }

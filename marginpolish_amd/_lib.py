@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmp_pairhmm.so")
 
@@ -38,6 +40,14 @@ class Pair(C.Structure):
     _fields_ = [("sX", C.POINTER(C.c_uint16)), ("lX", C.c_int64), ("sY", C.POINTER(C.c_uint16)),
                 ("lY", C.c_int64), ("anchors", C.POINTER(C.c_int64)), ("nAnchors", C.c_int64),
                 ("strand", C.c_int32), ("raggedLeft", C.c_int32), ("raggedRight", C.c_int32)]
+
+
+# mp_pair as a numpy record (pointers as uint64), for filling pair arrays column by column
+PAIR_DTYPE = np.dtype([("sX", "<u8"), ("lX", "<i8"), ("sY", "<u8"), ("lY", "<i8"), ("anchors", "<u8"),
+                       ("nAnchors", "<i8"), ("strand", "<i4"), ("raggedLeft", "<i4"), ("raggedRight", "<i4")],
+                      align=True)
+assert PAIR_DTYPE.itemsize == C.sizeof(Pair) and all(
+    PAIR_DTYPE.fields[f][1] == getattr(Pair, f).offset for f, _ in Pair._fields_)
 
 
 class AlignedPair(C.Structure):

@@ -20,6 +20,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <memory>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -85,12 +86,13 @@ struct DevBuf {
         }
         return MP_OK;
     }
-    mp_status upload(const std::vector<T> &v, hipStream_t s) {
-        mp_status st = alloc(v.size());
-        if (st != MP_OK || v.empty()) return st;
-        HIP_TRY(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    mp_status upload(const T *v, size_t count, hipStream_t s) {
+        mp_status st = alloc(count);
+        if (st != MP_OK || count == 0) return st;
+        HIP_TRY(hipMemcpyAsync(p, v, count * sizeof(T), hipMemcpyHostToDevice, s));
         return MP_OK;
     }
+    mp_status upload(const std::vector<T> &v, hipStream_t s) { return upload(v.data(), v.size(), s); }
 };
 
 }  // namespace
@@ -101,6 +103,10 @@ struct mp_engine {
     int splitBlocksPerCU = 1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pinned double buffer for large device-to-host result copies (allocated at first use)
+    static constexpr size_t kStageBytes = (size_t)64 << 20;
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stageEv[2] = {nullptr, nullptr};
 };
 
 struct mp_batch {
@@ -113,7 +119,6 @@ struct mp_batch {
     std::vector<int64_t> pairFirstJob;  // index into `order` of a pair's first job
     std::vector<int32_t> order;         // pair-major list of device job indices (split order; -1 = empty)
     std::vector<int32_t> pairNJobs;
-    std::vector<int32_t> bandHost;      // packed band entries (record decoding)
     std::vector<DevWin> winHost;
     int64_t cells = 0;
     int32_t ringLen = kK;
@@ -337,6 +342,10 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
 extern "C" void mp_engine_free(mp_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    for (int k = 0; k < 2; k++) {
+        if (e->stage[k]) (void)hipHostFree(e->stage[k]);
+        if (e->stageEv[k]) (void)hipEventDestroy(e->stageEv[k]);
+    }
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -470,7 +479,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("batch too large: more than 2^30 symbols (split it into several batches)");
         return MP_EUNSUPPORTED;
     }
-    std::vector<uint32_t> sym((size_t)(symOff[npairs] + kSymPad), enc_x(4, 0));
+    // every entry is written below (pairs in parallel, padding here): no value-initialisation pass
+    const size_t nsym = (size_t)(symOff[npairs] + kSymPad);
+    std::unique_ptr<uint32_t[]> symBuf(new uint32_t[nsym]);
+    uint32_t *sym = symBuf.get();
+    for (int64_t i = 0; i < kSymPad; i++) sym[i] = enc_x(4, 0);
+    for (size_t i = (size_t)symOff[npairs]; i < nsym; i++) sym[i] = enc_x(4, 0);
     sym[0] = kZeroSym;  // read by cells outside the band
     std::vector<HostJob> jobs((size_t)njobs);
     std::vector<std::vector<int64_t>> jobAnchors((size_t)njobs);
@@ -478,7 +492,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         const mp_pair &pr = pairs[i];
         const PairPrep &pp = prep[i];
         const int strand = pr.strand ? 1 : 0;
-        uint32_t *o = sym.data() + symOff[i];
+        uint32_t *o = sym + symOff[i];
         *o++ = enc_x(4, strand);
         const int64_t baseX = symOff[i] + 1;
         for (int64_t x = pp.first; x < pp.end; x++) *o++ = enc_x(pr.sX[x], strand);
@@ -521,8 +535,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         mp::set_error("batch too large: more than 2^30 anti-diagonals (split it into several batches)");
         return MP_EUNSUPPORTED;
     }
-    std::vector<int32_t> &band = b->bandHost;
-    band.assign((size_t)bandOff[nj], 0);
+    // every entry of a job's region is written by its band pass below (anti-diagonals 0..D, then padding)
+    std::unique_ptr<int32_t[]> bandBuf(new int32_t[(size_t)bandOff[nj]]);
+    int32_t *band = bandBuf.get();
     std::vector<std::vector<DevWin>> wins(nj);
     std::atomic<int> err{0};
     {
@@ -558,7 +573,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     hj.cells = D > 0 ? cells : 0;
                     hj.cls = five ? 3 : (wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
                     hj.bandOff = bandOff[j];
-                    int32_t *bj = band.data() + bandOff[j];
+                    int32_t *bj = band + bandOff[j];
                     const bool narrow = hj.cls < 3;
                     // xL = x of the band's first cell: (d + xmyL) / 2
                     XL.resize(D + 1);
@@ -715,7 +730,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->modelStates = m->mach[1].nStates;
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
-    if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, s)) || (st = b->dBand.upload(band, s)) ||
+    if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, nsym, s)) ||
+        (st = b->dBand.upload(band, (size_t)bandOff[nj], s)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
@@ -1113,7 +1129,47 @@ static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t
         return MP_EINVAL;
     }
     HIP_TRY(hipSetDevice(b->eng->device));
-    HIP_TRY(hipMemcpy(out, b->dList[l].p, sizeof(mp_aligned_pair) * n, hipMemcpyDeviceToHost));
+    mp_engine *e = b->eng;
+    const size_t bytes = sizeof(mp_aligned_pair) * (size_t)n;
+    if (bytes < ((size_t)8 << 20)) {
+        HIP_TRY(hipMemcpy(out, b->dList[l].p, bytes, hipMemcpyDeviceToHost));
+        return MP_OK;
+    }
+    // Large lists: DMA into a pinned double buffer in 64 MB chunks; while the
+    // next chunk is in flight, host threads copy the previous one out.
+    if (!e->stage[0]) {
+        for (int k = 0; k < 2; k++) {
+            HIP_TRY(hipHostMalloc(&e->stage[k], mp_engine::kStageBytes, hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&e->stageEv[k], hipEventDisableTiming));
+        }
+    }
+    const char *src = reinterpret_cast<const char *>(b->dList[l].p);
+    char *dst = reinterpret_cast<char *>(out);
+    const size_t chunk = mp_engine::kStageBytes;
+    const size_t nchunk = (bytes + chunk - 1) / chunk;
+    const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    auto drain = [&](size_t k) {  // copy chunk k out of its staging buffer (after its DMA)
+        const size_t off = k * chunk, len = std::min(chunk, bytes - off);
+        const char *sb = reinterpret_cast<const char *>(e->stage[k & 1]);
+        const size_t part = (len + nth - 1) / nth;
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; t++)
+            if ((size_t)t * part < len)
+                th.emplace_back([=]() { std::memcpy(dst + off + t * part, sb + t * part, std::min(part, len - t * part)); });
+        std::memcpy(dst + off, sb, std::min(part, len));
+        for (auto &x : th) x.join();
+    };
+    for (size_t k = 0; k < nchunk; k++) {
+        const size_t off = k * chunk, len = std::min(chunk, bytes - off);
+        HIP_TRY(hipMemcpyAsync(e->stage[k & 1], src + off, len, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipEventRecord(e->stageEv[k & 1], e->stream));
+        if (k > 0) {
+            HIP_TRY(hipEventSynchronize(e->stageEv[(k - 1) & 1]));
+            drain(k - 1);
+        }
+    }
+    HIP_TRY(hipEventSynchronize(e->stageEv[(nchunk - 1) & 1]));
+    drain(nchunk - 1);
     return MP_OK;
 }
 

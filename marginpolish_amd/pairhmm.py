@@ -239,18 +239,25 @@ class Batch:
         graph: replay the batch's captured hipGraph on every run (False:
         enqueue the launches directly)."""
         self.engine = engine or default_engine()
-        self._keep = []
         n = len(pairs)
-        arr = (_lib.Pair * max(n, 1))()
-        for i, pr in enumerate(pairs):
-            sx, sy, an, strand = pr[0], pr[1], pr[2], pr[3]
-            rl = pr[4] if len(pr) > 4 else False
-            rr = pr[5] if len(pr) > 5 else False
-            sx, sy, an = _as_u16(sx), _as_u16(sy), _as_anchors(an)
-            self._keep += [sx, sy, an]
-            arr[i] = _lib.Pair(sx.ctypes.data_as(C.POINTER(C.c_uint16)), len(sx),
-                               sy.ctypes.data_as(C.POINTER(C.c_uint16)), len(sy),
-                               an.ctypes.data_as(C.POINTER(C.c_int64)), an.shape[0], int(strand), int(rl), int(rr))
+        # the mp_pair array as a numpy record array of the same layout, filled column by column
+        arr = np.zeros(max(n, 1), dtype=_lib.PAIR_DTYPE)
+        sxs = [_as_u16(pr[0]) for pr in pairs]
+        sys_ = [_as_u16(pr[1]) for pr in pairs]
+        ans = [_as_anchors(pr[2]) for pr in pairs]
+        self._keep = (sxs, sys_, ans)
+        if n:
+            arr["sX"][:n] = [a.ctypes.data for a in sxs]
+            arr["lX"][:n] = [a.size for a in sxs]
+            arr["sY"][:n] = [a.ctypes.data for a in sys_]
+            arr["lY"][:n] = [a.size for a in sys_]
+            arr["anchors"][:n] = [a.ctypes.data for a in ans]
+            arr["nAnchors"][:n] = [a.shape[0] for a in ans]
+            arr["strand"][:n] = [int(pr[3]) for pr in pairs]
+            arr["raggedLeft"][:n] = [int(bool(pr[4])) if len(pr) > 4 else 0 for pr in pairs]
+            arr["raggedRight"][:n] = [int(bool(pr[5])) if len(pr) > 5 else 0 for pr in pairs]
+        self._arr = arr
+        arr = arr.ctypes.data_as(C.POINTER(_lib.Pair))
         h = C.c_void_p()
         pc = p.c()
         check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
@@ -290,10 +297,11 @@ class Batch:
         lists = []
         for li in range(3):
             tot = int(counts[:, li].sum())
-            buf = np.zeros(max(tot, 1), dtype=ALIGNED_PAIR_DTYPE)
+            buf = np.empty(max(tot, 1), dtype=ALIGNED_PAIR_DTYPE)
             if tot:
                 check(lib().mp_batch_fetch_list(self._h, li, buf.ctypes.data_as(C.POINTER(_lib.AlignedPair))))
-            lists.append(np.split(buf[:tot], np.cumsum(counts[:, li])[:-1]) if self.npairs else [])
+            ends = np.cumsum(counts[:, li]).tolist()
+            lists.append([buf[a:e] for a, e in zip([0] + ends[:-1], ends)] if self.npairs else [])
         return [(lists[0][i], lists[1][i], lists[2][i], float(lp[i])) for i in range(self.npairs)]
 
     def set_model(self, machines):
