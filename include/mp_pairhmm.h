@@ -205,6 +205,15 @@ mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs,
 mp_status mp_batch_set_model(mp_batch *b, const mp_model *m);
 /* One persistent launch over every pair (outputs stay on the device). */
 mp_status mp_batch_run(mp_batch *b);
+/* Autotuner (hardware tiling only; no reference counterpart): times this batch
+ * under each launch geometry -- narrow-kernel workgroups of 512 or 256 threads
+ * (the same resident wavefronts), five-state workgroups of 8/4/2/1 waves --
+ * `repeats` runs each (best of), keeps the fastest for this batch and caches it
+ * in the engine for later batches of the same shape (mode, kernel family,
+ * dominant band class, widest band, log2 job count).  Results are identical
+ * under every geometry; the batch is left run.  Outputs may be NULL. */
+mp_status mp_batch_autotune(mp_batch *b, int32_t repeats, int32_t *narrowThreads, int32_t *wideWaves,
+                            float *kernelMs);
 /* The same work as one kernel launch per pair (the plain per-read path). */
 mp_status mp_batch_run_plain(mp_batch *b);
 /* Device time of the last run's pair-HMM kernel launch(es), milliseconds (HIP events on the engine's stream). */

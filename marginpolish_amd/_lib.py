@@ -65,6 +65,7 @@ EXPORTS = [
     "mp_batch_fetch_expectations", "mp_pairhmm_forward_log_prob", "mp_pairhmm_expectations",
     "mp_model_set_transitions", "mp_batch_set_model", "mp_model_set_transitions5", "mp_model_transitions5",
     "mp_model_num_states", "mp_model_set_emissions", "mp_model_set_repeat_log10", "mp_batch_fetch_emission_counts",
+    "mp_batch_autotune",
 ]
 
 _lib = None
@@ -117,6 +118,7 @@ def lib():
         "mp_model_set_emissions": (C.c_int, [vp, P(dbl), P(dbl), P(dbl)]),
         "mp_model_set_repeat_log10": (C.c_int, [vp, P(dbl), i32]),
         "mp_batch_fetch_emission_counts": (C.c_int, [vp, P(dbl), P(dbl), P(dbl), P(dbl)]),
+        "mp_batch_autotune": (C.c_int, [vp, i32, P(i32), P(i32), P(C.c_float)]),
         "mp_batch_set_model": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():

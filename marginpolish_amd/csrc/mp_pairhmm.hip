@@ -20,6 +20,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -107,6 +108,8 @@ struct mp_engine {
     static constexpr size_t kStageBytes = (size_t)64 << 20;
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stageEv[2] = {nullptr, nullptr};
+    // autotuner results: batch shape key -> (narrow workgroup threads, five-state waves per workgroup)
+    std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;
 };
 
 struct mp_batch {
@@ -144,6 +147,10 @@ struct mp_batch {
     int32_t modelStates = 3;
     hipGraphExec_t graph = nullptr;  // the run's resets and launches, captured once per parameter block
     int32_t narrowBlocks = 0;
+    int32_t narrowThreads = 512;  // workgroup size of the narrow kernels (the autotuner may pick 256)
+    int32_t narrowWaves = 0;      // resident narrow wavefronts the rings are sized for
+    int32_t wide5Waves = 0;       // waves per five-state workgroup (0: min(columns, 8))
+    uint64_t tuneKey = 0;
     int64_t outTotal = 0;
     double kernelMs = 0.0;
     bool ran = false;
@@ -171,7 +178,8 @@ mp_status launch_narrow(mp_batch *b, int grid, int block, const KParams &kp) {
 }
 
 mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
-    const dim3 block5(64 * wide5_waves(kp.wideC));
+    const int w5 = b->wide5Waves > 0 ? std::min(b->wide5Waves, wide5_waves(kp.wideC)) : wide5_waves(kp.wideC);
+    const dim3 block5(64 * w5);
     if (kp.nStates == 5 && (b->flags & MP_BATCH_FP64))
         hipLaunchKernelGGL(pairhmm_wide5_kernel<double>, dim3(grid), block5, wide5_lds_bytes<double>(kp.wideC),
                            b->eng->stream, kp);
@@ -744,9 +752,30 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // Rings: one slice per resident wavefront.
     const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
                            b->classEnd[2] > b->classBegin[2];
-    b->narrowBlocks = e->cus * (split ? e->splitBlocksPerCU : 1);
+    b->narrowWaves = e->cus * (split ? e->splitBlocksPerCU : 1) * 8;
+    b->narrowBlocks = b->narrowWaves * 64 / b->narrowThreads;
+    {
+        // batch shape key for the autotuner's cache: mode, kernel family, dominant narrow class, widest
+        // band in columns, log2 of the job count
+        int domCls = 0, domN = -1;
+        for (int c = 0; c < kNumClasses; c++)
+            if (b->classEnd[c] - b->classBegin[c] > domN) {
+                domN = b->classEnd[c] - b->classBegin[c];
+                domCls = c;
+            }
+        int lg = 0;
+        while ((1ll << lg) < (int64_t)ord.size() && lg < 40) lg++;
+        b->tuneKey = (uint64_t)mode | ((uint64_t)(five ? 1 : 0) << 2) | ((uint64_t)(fp64 ? 1 : 0) << 3) |
+                     ((uint64_t)domCls << 4) | ((uint64_t)std::min(b->wideC, 63) << 6) | ((uint64_t)lg << 12);
+        auto it = e->tuned.find(b->tuneKey);
+        if (it != e->tuned.end()) {
+            b->narrowThreads = it->second.first;
+            b->narrowBlocks = b->narrowWaves * 64 / b->narrowThreads;
+            b->wide5Waves = it->second.second;
+        }
+    }
     if (anyNarrow) {
-        const size_t waves = (size_t)b->narrowBlocks * 8;
+        const size_t waves = (size_t)b->narrowWaves;
         if (split) {
             if ((st = b->dRing.alloc(waves * (size_t)(ringLen / kK) * (kRingCellBytes / 4))) ||
                 (st = b->dRingExp.alloc(waves * (size_t)(ringLen / kK) * 4))) {
@@ -827,7 +856,7 @@ static mp_status enqueue_run(mp_batch *b) {
     if ((st = reset_i32(b, b->dQueue.p, kNumClasses + 1))) return st;
     const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
                            b->classEnd[2] > b->classBegin[2];
-    if (anyNarrow && (st = launch_narrow(b, b->narrowBlocks, 512, b->kp))) return st;
+    if (anyNarrow && (st = launch_narrow(b, b->narrowBlocks, b->narrowThreads, b->kp))) return st;
     if (b->wideEnd > b->wideBegin && (st = launch_wide(b, b->wideWaves, b->kp))) return st;
     return MP_OK;
 }
@@ -979,6 +1008,54 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
     drop_graph(b);
     b->ran = false;
     b->listsReady = false;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_autotune(mp_batch *b, int32_t repeats, int32_t *narrowThreads, int32_t *wideWaves,
+                                       float *kernelMs) {
+    if (!b) {
+        mp::set_error("mp_batch_autotune: null batch");
+        return MP_EINVAL;
+    }
+    repeats = std::max(1, std::min(repeats, 16));
+    // hardware tiling only: the results do not depend on it (jobs come from a queue, each job's
+    // records land in its own region)
+    std::vector<int32_t> nt = {512, 256};
+    std::vector<int32_t> ww = {0};
+    if (b->kp.nStates == 5) {
+        ww.clear();
+        for (int w : {wide5_waves(b->wideC), 4, 2, 1})
+            if (w <= wide5_waves(b->wideC) && std::find(ww.begin(), ww.end(), w) == ww.end()) ww.push_back(w);
+    }
+    float best = 1e30f;
+    int32_t bestT = b->narrowThreads, bestW = b->wide5Waves;
+    mp_status st;
+    for (int32_t t : nt)
+        for (int32_t w : ww) {
+            b->narrowThreads = t;
+            b->narrowBlocks = b->narrowWaves * 64 / t;
+            b->wide5Waves = w;
+            drop_graph(b);
+            float ms = 1e30f;
+            for (int r = 0; r < repeats; r++) {
+                if ((st = batch_run_impl(b, false))) return st;
+                ms = std::min(ms, (float)b->kernelMs);
+            }
+            if (ms < best) {
+                best = ms;
+                bestT = t;
+                bestW = w;
+            }
+        }
+    b->narrowThreads = bestT;
+    b->narrowBlocks = b->narrowWaves * 64 / bestT;
+    b->wide5Waves = bestW;
+    drop_graph(b);
+    b->eng->tuned[b->tuneKey] = {bestT, bestW};
+    if ((st = batch_run_impl(b, false))) return st;
+    if (narrowThreads) *narrowThreads = bestT;
+    if (wideWaves) *wideWaves = bestW;
+    if (kernelMs) *kernelMs = best;
     return MP_OK;
 }
 

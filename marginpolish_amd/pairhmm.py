@@ -272,6 +272,14 @@ class Batch:
         check(lib().mp_batch_run(self._h))
         return self
 
+    def autotune(self, repeats=3):
+        """Pick the fastest launch geometry for this batch (cached in the
+        engine for batches of the same shape); returns (narrow workgroup
+        threads, five-state waves per workgroup or 0, best kernel ms)."""
+        t, w, ms = C.c_int32(), C.c_int32(), C.c_float()
+        check(lib().mp_batch_autotune(self._h, int(repeats), C.byref(t), C.byref(w), C.byref(ms)))
+        return t.value, w.value, ms.value
+
     def run_plain(self):
         check(lib().mp_batch_run_plain(self._h))
         return self

@@ -430,3 +430,26 @@ def test_graph_replays_equal_eager_runs(ph, params, config1):
     g.run()
     for r1, r2 in zip(g.fetch(), eager):
         assert np.array_equal(r1[0], r2[0])
+
+
+def test_autotune_keeps_results(ph, params, config1):
+    """The autotuner changes only the launch geometry: after it the batch's
+    lists equal the untuned ones, and a new batch of the same shape starts
+    from the cached choice."""
+    from marginpolish_amd import crop
+    pp, op = params
+    jobs = []
+    for pr in config1.pairs[:300]:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    say("case: autotune, 300 pairs")
+    ref = ph.Batch(pp.machines, jobs, pp.p).run().fetch()
+    b = ph.Batch(pp.machines, jobs, pp.p)
+    threads, waves, ms = b.autotune(repeats=2)
+    assert threads in (256, 512) and ms > 0
+    for r1, r2 in zip(b.fetch(), ref):
+        for l in range(3):
+            assert np.array_equal(r1[l], r2[l])
+    b2 = ph.Batch(pp.machines, jobs, pp.p).run()
+    for r1, r2 in zip(b2.fetch(), ref):
+        assert np.array_equal(r1[0], r2[0])
