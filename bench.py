@@ -166,12 +166,26 @@ def main():
         t1c = time.perf_counter() - t4
         log("cpu: %d pairs on %d threads %.2f s; %d pairs on 1 thread %.2f s" % (len(cpu_jobs), ncores, tc,
                                                                              len(one), t1c))
+        # the exact-arithmetic oracle (O_exact) on the single-thread sample, for reference
+        t5 = time.perf_counter()
+        c2, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_EXACT, 1)
+        t2c = time.perf_counter() - t5
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+        except OSError:
+            pass
         out["cpu_baseline"] = {"value": round(ccells / tc / 1e9, 5), "unit": "GCUPS", "cores": ncores,
                                "kind": "port",
                                "sample": "%d of %d config1 pairs (%d cells), reference algorithm restated in C "
                                          "(FP64 log space, polynomial logAdd), OpenMP over pairs"
                                          % (len(cpu_jobs), len(pairs), ccells),
-                               "single_core_gcups": round(c1 / t1c / 1e9, 5)}
+                               "reads_per_s": round(len(cpu_jobs) / tc, 2),
+                               "single_core_gcups": round(c1 / t1c / 1e9, 5),
+                               "single_core_reads_per_s": round(len(one) / t1c, 3),
+                               "o_exact_single_core_gcups": round(c2 / t2c / 1e9, 5),
+                               "cpu_model": cpu_model}
         # max-abs posterior error vs the exact FP64 oracle on a sample
         worst = 0.0
         for i in pick[: args.err_pairs]:
@@ -187,6 +201,19 @@ def main():
                     worst = 1.0  # an entry far above threshold on one side only
         out["max_abs_posterior_err"] = worst
         out["posterior_err_sample_pairs"] = int(min(args.err_pairs, len(pick)))
+        # the reference arithmetic (polynomial logAdd) against exact log-sum-exp on the same sample
+        fe = 0.0
+        for i in pick[: args.err_pairs]:
+            sx, sy, an, _ = crop.crop_pair(*pairs[i][:3])
+            m = op.forward if pairs[i][3] else op.reverse
+            ex = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_EXACT)
+            fa = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_POLY)
+            for a, b_ in zip(fa[:3], ex[:3]):
+                try:
+                    fe = max(fe, compare(a, b_, op.p.threshold, 1.0)[0])
+                except AssertionError:
+                    fe = 1.0
+        out["o_faithful_vs_o_exact_max_abs"] = fe
     print(json.dumps(out), flush=True)
 
 
