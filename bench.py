@@ -98,6 +98,45 @@ def main():
     cells = b.cells
     log("iteration: first %.3f s; second: prep %.3f s, total %.3f s, %d cells" % (iters[0][1], t_prep, t_iter, cells))
 
+    # the same iteration through the POA augmentation: every chunk's graph built from the batch's lists inside
+    # the library (no list handoff to Python); timed on the second run, plus the augment alone on one thread
+    from marginpolish_amd import poa as mpoa
+    chunk_ids = sorted(set(p.chunk for p in w.pairs))
+    first_pair = {}
+    for p in w.pairs:
+        first_pair.setdefault(p.chunk, p)
+    refs = [mpoa.rle_from_symbols(first_pair[c].ref_sym, first_pair[c].ref_runs) for c in chunk_ids]
+    cidx = {c: k for k, c in enumerate(chunk_ids)}
+    pair_poa = np.array([cidx[p.chunk] for p in w.pairs], dtype=np.int64)
+    read_no = np.zeros(len(pairs), dtype=np.int64)
+    seen = {}
+    for i, p in enumerate(w.pairs):
+        read_no[i] = seen.get(p.chunk, 0)
+        seen[p.chunk] = read_no[i] + 1
+    rle_reads = [mpoa.rle_from_symbols(p.read_sym, p.read_runs) for p in w.pairs]
+    poa_iters = []
+    for it in range(2):
+        t1 = time.time()
+        bb = pairhmm.Batch(pp.machines, pairs, pp.p, crop_reference=True, engine=eng)
+        bb.run()
+        ta = time.time()
+        graphs = [mpoa.Poa(r) for r in refs]
+        mpoa.augment_batch(bb, graphs, pair_poa, rle_reads, read_no)
+        poa_iters.append((time.time() - ta, time.time() - t1))
+        if it == 1:
+            tb = time.time()
+            g1 = [mpoa.Poa(r) for r in refs]
+            mpoa.augment_batch(bb, g1, pair_poa, rle_reads, read_no, threads=1)
+            aug1 = time.time() - tb
+            gsz = [g.sizes() for g in graphs]
+            del g1
+        bb.close()
+        del graphs
+    t_aug, t_poa_iter = poa_iters[1]
+    log("POA iteration: %.3f s (augment %.3f s on all cores, %.3f s on one thread; %d graphs, %d inserts, "
+        "%d deletes)" % (t_poa_iter, t_aug, aug1, len(refs), sum(s["inserts"] for s in gsz),
+                         sum(s["deletes"] for s in gsz)))
+
     # hardware tiling picked by the autotuner on this batch (launch geometry only; results unchanged)
     tuned = b.autotune(3)
     log("autotune: %d-thread narrow workgroups, best %.3f ms" % (tuned[0], tuned[2]))
@@ -134,6 +173,8 @@ def main():
         "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
         "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),
         "first_iteration_s": round(iters[0][1], 4),
+        "poa_iteration_s": round(t_poa_iter, 4), "poa_reads_per_s": round(len(pairs) / t_poa_iter, 1),
+        "augment_s": round(t_aug, 4), "augment_1thread_s": round(aug1, 4),
         "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
         "plain_sample_pairs": len(samp), "autotuned_narrow_threads": tuned[0],
     }

@@ -56,6 +56,22 @@ class AlignedPair(C.Structure):
 
 ALIGNED_PAIR_DTYPE = [("weight", "<i8"), ("x", "<i4"), ("y", "<i4")]
 
+
+class RleString(C.Structure):
+    _fields_ = [("bases", C.c_void_p), ("repeatCounts", C.c_void_p), ("length", C.c_int64)]
+
+
+RLE_STRING_DTYPE = np.dtype([("bases", "<u8"), ("repeatCounts", "<u8"), ("length", "<i8")], align=True)
+assert RLE_STRING_DTYPE.itemsize == C.sizeof(RleString)
+
+
+class PoaSizes(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("nodes", "maxRepeatCount", "inserts", "insertSymbols", "deletes",
+                                         "nodeObservations", "insertObservations", "deleteObservations")]
+
+
+OBSERVATION_DTYPE = np.dtype([("readNo", "<i8"), ("offset", "<i8"), ("weight", "<f8")])
+
 EXPORTS = [
     "mp_last_error", "mp_version", "mp_model_load_json", "mp_model_default_nucleotide", "mp_model_band_params",
     "mp_model_tables", "mp_model_repeat_log10", "mp_model_max_repeat_count", "mp_model_free", "mp_engine_create",
@@ -65,7 +81,8 @@ EXPORTS = [
     "mp_batch_fetch_expectations", "mp_pairhmm_forward_log_prob", "mp_pairhmm_expectations",
     "mp_model_set_transitions", "mp_batch_set_model", "mp_model_set_transitions5", "mp_model_transitions5",
     "mp_model_num_states", "mp_model_set_emissions", "mp_model_set_repeat_log10", "mp_batch_fetch_emission_counts",
-    "mp_batch_autotune",
+    "mp_batch_autotune", "mp_poa_reference_graph", "mp_poa_free", "mp_poa_augment", "mp_batch_poa_augment",
+    "mp_poa_get_sizes", "mp_poa_get_nodes", "mp_poa_get_inserts", "mp_poa_get_deletes", "mp_poa_get_observations",
 ]
 
 _lib = None
@@ -120,6 +137,15 @@ def lib():
         "mp_batch_fetch_emission_counts": (C.c_int, [vp, P(dbl), P(dbl), P(dbl), P(dbl)]),
         "mp_batch_autotune": (C.c_int, [vp, i32, P(i32), P(i32), P(C.c_float)]),
         "mp_batch_set_model": (C.c_int, [vp, vp]),
+        "mp_poa_reference_graph": (C.c_int, [P(RleString), i64, P(vp)]),
+        "mp_poa_free": (None, [vp]),
+        "mp_poa_augment": (C.c_int, [vp, P(RleString), i32, i64, vp, i64, vp, i64, vp, i64, i32]),
+        "mp_batch_poa_augment": (C.c_int, [vp, P(vp), i64, vp, vp, vp, i32, i32]),
+        "mp_poa_get_sizes": (C.c_int, [vp, P(PoaSizes)]),
+        "mp_poa_get_nodes": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mp_poa_get_inserts": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "mp_poa_get_deletes": (C.c_int, [vp, vp, vp, vp, vp]),
+        "mp_poa_get_observations": (C.c_int, [vp, i32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

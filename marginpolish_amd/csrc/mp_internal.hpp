@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mp_pairhmm.h"
+#include "mp_poa.h"
 
 namespace mp {
 
@@ -66,5 +67,12 @@ void split_points(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY, int
                   bool raggedR, std::vector<int64_t> &out);
 // Anchor validity: strictly increasing, inside the sequences, even expansion.
 bool anchors_valid(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY);
+
+// The augment loop of poa_realign over the pairs of a batch (mp_poa.cpp):
+// pair i's lists are lists[l][starts[l][i] .. starts[l][i + 1]).
+mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *pairPoa, const int64_t *pairReadNo,
+                            const mp_rle_string *reads, const uint8_t *strand, int64_t npairs,
+                            const mp_aligned_pair *const lists[3], const int64_t *const starts[3], bool cmp,
+                            int threads);
 
 }  // namespace mp

@@ -119,6 +119,7 @@ struct mp_batch {
     int64_t npairs = 0;
     std::vector<HostJob> jobs;          // in device order
     std::vector<int64_t> pairCrop;      // reference crop offset per pair
+    std::vector<uint8_t> pairStrand;    // 1 = forward-strand read
     std::vector<int64_t> pairFirstJob;  // index into `order` of a pair's first job
     std::vector<int32_t> order;         // pair-major list of device job indices (split order; -1 = empty)
     std::vector<int32_t> pairNJobs;
@@ -160,6 +161,9 @@ struct mp_batch {
     DevBuf<mpk::ListTuple> dList[3];      // the lists on the device (copied out by fetch)
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
+    // host copies of the lists for the POA augmentation (kept between runs)
+    std::unique_ptr<mp_aligned_pair[]> hostList[3];
+    int64_t hostListCap[3] = {0, 0, 0};
 };
 
 namespace {
@@ -393,6 +397,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->flags = flags;
     b->npairs = npairs;
     b->pairCrop.assign(npairs, 0);
+    b->pairStrand.resize(npairs);
+    for (int64_t i = 0; i < npairs; i++) b->pairStrand[i] = pairs[i].strand ? 1 : 0;
     b->pairNJobs.assign(npairs, 0);
     b->pairFirstJob.assign(npairs, 0);
     const bool split = use_split(mode);
@@ -1280,6 +1286,39 @@ extern "C" mp_status mp_batch_fetch_list(mp_batch *b, int32_t list, mp_aligned_p
     mp_status st = materialize(b);
     if (st) return st;
     return copy_list(b, list, out, b->listStart[list][b->npairs]);
+}
+
+extern "C" mp_status mp_batch_poa_augment(mp_batch *b, mp_poa *const *poas, int64_t nPoas, const int64_t *pairPoa,
+                                          const int64_t *pairReadNo, const mp_rle_string *reads,
+                                          int32_t compareRepeatCounts, int32_t threads) {
+    if (!b || !poas || nPoas < 0 || (b->npairs && (!pairPoa || !reads))) {
+        mp::set_error("mp_batch_poa_augment: null argument");
+        return MP_EINVAL;
+    }
+    if (!b->ran || b->mode != MP_MODE_POSTERIORS) {
+        mp::set_error("mp_batch_poa_augment: needs a posterior-mode batch that has run");
+        return MP_EINVAL;
+    }
+    mp_status st = materialize(b);
+    if (st) return st;
+    const mp_aligned_pair *lists[3];
+    const int64_t *starts[3];
+    for (int l = 0; l < 3; l++) {
+        const int64_t n = b->listStart[l][b->npairs];
+        if (n > b->hostListCap[l]) {
+            b->hostList[l].reset(new (std::nothrow) mp_aligned_pair[n]);
+            b->hostListCap[l] = b->hostList[l] ? n : 0;
+            if (!b->hostList[l]) {
+                mp::set_error("mp_batch_poa_augment: out of host memory");
+                return MP_ENOMEM;
+            }
+        }
+        if ((st = copy_list(b, l, b->hostList[l].get(), n))) return st;
+        lists[l] = b->hostList[l].get();
+        starts[l] = b->listStart[l].data();
+    }
+    return mp::poa_augment_pairs(poas, nPoas, pairPoa, pairReadNo, reads, b->pairStrand.data(), b->npairs, lists,
+                                 starts, compareRepeatCounts != 0, threads);
 }
 
 extern "C" mp_status mp_batch_fetch_expectations(mp_batch *b, double *trans, double *likelihood) {

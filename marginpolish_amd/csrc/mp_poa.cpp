@@ -1,0 +1,631 @@
+// mp_poa.cpp -- POA graph augmentation from the posterior lists (host side).
+//
+// The reference builds the graph read by read (poa_realign,
+// impl/polisher.c:656-703): match weights into the nodes, then the read's
+// complete inserts and complete deletes, each moved to its leftmost
+// equivalent position and merged into the node's list.  Here the per-read
+// work that does not touch the graph (ordering the gap lists, finding the
+// complete runs, left-shifting them) is done for all reads in parallel, and
+// each graph then applies its reads' events in read order, so the lists come
+// out in the reference's order and the weight sums in the reference's
+// summation order.
+#include <algorithm>
+#include <atomic>
+#include <cctype>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mp_internal.hpp"
+#include "mp_poa.h"
+
+namespace {
+
+constexpr int kAlphabet = 5;  // A C G T N (alphabet_constructNucleotide, stateMachine.c:63-71)
+constexpr double kUintMax = 4294967295.0;  // the reference's initial run minimum (UINT_MAX)
+
+inline int base_symbol(char c) {  // convertNucleotideCharToSymbol (stateMachine.c:25-42)
+    switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+    }
+}
+
+// A run-length string view: runs [0, n) of (b, c).
+struct Str {
+    const char *b;
+    const uint64_t *c;
+    int64_t n;
+};
+
+struct InsEntry {
+    uint64_t hash;
+    int64_t sym, len;  // placed string in the graph's symbol pool
+    double wF = 0.0, wR = 0.0;
+    int32_t next = -1;  // next entry of the same node
+    std::vector<mp_poa_observation> obs;
+};
+
+struct DelEntry {
+    int64_t len;
+    double wF = 0.0, wR = 0.0;
+    int32_t next = -1;
+    std::vector<mp_poa_observation> obs;
+};
+
+// A complete insert of one read, placed: node `pos`, the read's runs
+// [start, start + len) rotated right by `rot`, run minimum weight `w`.
+struct InsEvent {
+    int64_t pos, start, len, rot;
+    double w;
+    uint64_t hash;
+};
+
+struct DelEvent {
+    int64_t pos, len, off;
+    double w;
+};
+
+inline uint64_t mix(uint64_t h, char b, uint64_t c) {
+    h ^= (uint64_t)(unsigned char)b;
+    h *= 0x100000001b3ULL;
+    h ^= c;
+    h *= 0x100000001b3ULL;
+    return h;
+}
+
+}  // namespace
+
+struct mp_poa {
+    int64_t maxRC = 0;
+    std::vector<char> refB;
+    std::vector<uint64_t> refC;
+    std::vector<char> nodeBase;
+    std::vector<uint64_t> nodeRC;
+    std::vector<double> baseW, rcW;  // [node][5], [node][maxRC]
+    std::vector<int32_t> obsNode;    // node observations in append order
+    std::vector<mp_poa_observation> obs;
+    std::vector<int32_t> insHead, insTail, delHead, delTail;
+    std::vector<InsEntry> ins;
+    std::vector<char> insB;
+    std::vector<uint64_t> insC;
+    std::vector<DelEntry> del;
+
+    Str ref() const { return Str{refB.data(), refC.data(), (int64_t)refB.size()}; }
+    int64_t nodes() const { return (int64_t)nodeBase.size(); }
+};
+
+namespace {
+
+// matchesReferenceSubstring (polisher.c:237-249): s[0, len) == ref[at, at + len).
+inline bool ref_matches(const Str &ref, int64_t at, const char *sb, const uint64_t *sc, int64_t len, bool cmp) {
+    for (int64_t l = 0; l < len; l++)
+        if (ref.b[at + l] != sb[l] || (cmp && ref.c[at + l] != sc[l])) return false;
+    return true;
+}
+
+// Smallest k such that s is whole copies of its first k runs (hasInternalRepeat, polisher.c:251-267).
+int64_t min_period(const char *sb, const uint64_t *sc, int64_t n, bool cmp) {
+    for (int64_t k = 1; k < n; k++) {
+        if (n % k) continue;
+        bool ok = true;
+        for (int64_t i = k; i < n && ok; i++) ok = sb[i] == sb[i - k] && (!cmp || sc[i] == sc[i - k]);
+        if (ok) return k;
+    }
+    return std::max<int64_t>(n, 1);
+}
+
+// getShift (polisher.c:269-297): the leftmost start reached by stepping back
+// one period at a time while the reference repeats the period; then, for a
+// one-run string with counts compared, one more run if the base agrees.
+int64_t left_shift(const Str &ref, int64_t at, const char *sb, const uint64_t *sc, int64_t n, bool cmp) {
+    const int64_t m = min_period(sb, sc, n, cmp);
+    for (int64_t k = at - m; k >= 0 && ref_matches(ref, k, sb, sc, m, cmp); k -= m) at = k;
+    if (n == 1 && cmp && at > 0 && ref.b[at - 1] == sb[0]) at--;
+    return at;
+}
+
+// getMaxCommonSuffixLength (polisher.c:299-314) of ref[0, end) and s.
+int64_t common_suffix(const Str &ref, int64_t end, const char *sb, const uint64_t *sc, int64_t n, bool cmp) {
+    int64_t i = 0;
+    while (end - i - 1 >= 0 && n - i - 1 >= 0) {
+        if (ref.b[end - 1 - i] != sb[n - 1 - i] || (cmp && ref.c[end - 1 - i] != sc[n - 1 - i])) break;
+        i++;
+    }
+    return i;
+}
+
+// Per-thread scratch for one read's events.
+struct Scratch {
+    std::vector<int32_t> cnt;        // counting-sort buckets
+    std::vector<int32_t> mY;         // match y values bucketed by x
+    std::vector<int32_t> mStart;     // bucket starts (x range + 1)
+    int64_t mX0 = 0, mX1 = -1;
+    std::vector<mp_aligned_pair> sorted;
+
+    void index_matches(const mp_aligned_pair *m, int64_t n) {
+        mX0 = INT64_MAX;
+        mX1 = INT64_MIN;
+        for (int64_t i = 0; i < n; i++) {
+            mX0 = std::min<int64_t>(mX0, m[i].x);
+            mX1 = std::max<int64_t>(mX1, m[i].x);
+        }
+        if (n == 0) {
+            mX0 = 0;
+            mX1 = -1;
+        }
+        const int64_t span = mX1 - mX0 + 1;
+        mStart.assign(span + 1, 0);
+        for (int64_t i = 0; i < n; i++) mStart[m[i].x - mX0 + 1]++;
+        for (int64_t k = 0; k < span; k++) mStart[k + 1] += mStart[k];
+        mY.resize(n);
+        cnt.assign(mStart.begin(), mStart.end());
+        for (int64_t i = 0; i < n; i++) mY[cnt[m[i].x - mX0]++] = m[i].y;
+    }
+    bool is_match(int64_t x, int64_t y) const {
+        if (x < mX0 || x > mX1) return false;
+        for (int32_t k = mStart[x - mX0]; k < mStart[x - mX0 + 1]; k++)
+            if (mY[k] == y) return true;
+        return false;
+    }
+    // Stable sort of gap tuples by (major, minor), major = x (inserts) or y (deletes).
+    void sort_gaps(const mp_aligned_pair *g, int64_t n, bool byX) {
+        sorted.resize(n);
+        if (n == 0) return;
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t k = byX ? g[i].x : g[i].y;
+            lo = std::min(lo, k);
+            hi = std::max(hi, k);
+        }
+        cnt.assign(hi - lo + 2, 0);
+        for (int64_t i = 0; i < n; i++) cnt[(byX ? g[i].x : g[i].y) - lo + 1]++;
+        for (int64_t k = 0; k + 1 < (int64_t)cnt.size(); k++) cnt[k + 1] += cnt[k];
+        for (int64_t i = 0; i < n; i++) sorted[cnt[(byX ? g[i].x : g[i].y) - lo]++] = g[i];
+        // bucket k now ends at cnt[k]; order each bucket by the minor coordinate (buckets are small)
+        int64_t s = 0;
+        for (int64_t k = 0; k + 1 < (int64_t)cnt.size(); k++) {
+            const int64_t e = cnt[k];
+            for (int64_t i = s + 1; i < e; i++) {
+                const mp_aligned_pair t = sorted[i];
+                const int32_t key = byX ? t.y : t.x;
+                int64_t j = i;
+                while (j > s && (byX ? sorted[j - 1].y : sorted[j - 1].x) > key) {
+                    sorted[j] = sorted[j - 1];
+                    j--;
+                }
+                sorted[j] = t;
+            }
+            s = e;
+        }
+    }
+};
+
+bool check_read(const Str &ref, const Str &read, const mp_aligned_pair *m, int64_t nm, const mp_aligned_pair *ins,
+                int64_t ni, const mp_aligned_pair *del, int64_t nd) {
+    for (int64_t i = 0; i < nm; i++)
+        if (m[i].x < 0 || m[i].x >= ref.n || m[i].y < 0 || m[i].y >= read.n) return false;
+    for (int64_t i = 0; i < ni; i++)
+        if (ins[i].x < -1 || ins[i].x >= ref.n || ins[i].y < 0 || ins[i].y >= read.n) return false;
+    for (int64_t i = 0; i < nd; i++)
+        if (del[i].x < 0 || del[i].x >= ref.n || del[i].y < -1 || del[i].y >= read.n) return false;
+    return true;
+}
+
+// The complete inserts and deletes of one read (polisher.c:341-530), placed
+// and in the reference's enumeration order.
+void read_events(const Str &ref, const Str &read, const mp_aligned_pair *m, int64_t nm, const mp_aligned_pair *ins,
+                 int64_t ni, const mp_aligned_pair *del, int64_t nd, bool cmp, Scratch &S,
+                 std::vector<InsEvent> &iev, std::vector<DelEvent> &dev) {
+    iev.clear();
+    dev.clear();
+    S.index_matches(m, nm);
+    // inserts: runs of equal x and consecutive y; sub-runs [k, l] flanked by
+    // a match (x, y_k - 1) or the read start and a match (x + 1, y_l + 1) or the read end
+    S.sort_gaps(ins, ni, true);
+    for (int64_t i = 0; i < ni;) {
+        const int64_t x0 = S.sorted[i].x, y0 = S.sorted[i].y;
+        int64_t j = i + 1;
+        while (j < ni && S.sorted[j].x == x0 && S.sorted[j].y == y0 + (j - i)) j++;
+        for (int64_t k = i; k < j; k++) {
+            const int64_t yp = y0 + (k - i) - 1;
+            if (yp > -1 && !S.is_match(x0, yp)) continue;
+            double w = kUintMax;
+            for (int64_t l = k; l < j; l++) {
+                w = std::min(w, (double)S.sorted[l].weight);
+                const int64_t yn = y0 + (l - i) + 1;
+                if (yn < read.n && !S.is_match(x0 + 1, yn)) continue;
+                const int64_t start = y0 + (k - i), len = l - k + 1;
+                const char *sb = read.b + start;
+                const uint64_t *sc = read.c + start;
+                int64_t pos = left_shift(ref, x0 + 1, sb, sc, len, cmp);
+                const int64_t cs = common_suffix(ref, pos, sb, sc, len, cmp);
+                const int64_t rot = cs % len;  // rotating by the whole length is the identity
+                pos -= cs;
+                uint64_t h = 0xcbf29ce484222325ULL ^ (uint64_t)len;
+                for (int64_t q = 0; q < len; q++) {
+                    const int64_t src = (q - rot + len) % len;
+                    h = mix(h, sb[src], sc[src]);
+                }
+                iev.push_back(InsEvent{pos, start, len, rot, w, h});
+            }
+        }
+        i = j;
+    }
+    // deletes: runs of equal y and consecutive x, flanked by a match
+    // (x_k - 1, y) or the reference start and (x_l + 1, y + 1) or its end
+    S.sort_gaps(del, nd, false);
+    for (int64_t i = 0; i < nd;) {
+        const int64_t x0 = S.sorted[i].x, y0 = S.sorted[i].y;
+        int64_t j = i + 1;
+        while (j < nd && S.sorted[j].y == y0 && S.sorted[j].x == x0 + (j - i)) j++;
+        for (int64_t k = i; k < j; k++) {
+            const int64_t xp = x0 + (k - i) - 1;
+            if (xp > -1 && !S.is_match(xp, y0)) continue;
+            double w = kUintMax;
+            for (int64_t l = k; l < j; l++) {
+                w = std::min(w, (double)S.sorted[l].weight);
+                const int64_t xn = x0 + (l - i) + 1;
+                if (xn < ref.n && !S.is_match(xn, y0 + 1)) continue;
+                const int64_t found = x0 + (k - i), len = l - k + 1;
+                int64_t pos = left_shift(ref, found, ref.b + found, ref.c + found, len, cmp);
+                pos -= common_suffix(ref, pos, ref.b + found, ref.c + found, len, cmp);
+                dev.push_back(DelEvent{pos, len, y0, w});
+            }
+        }
+        i = j;
+    }
+}
+
+// Graph updates of one read, in the reference's order: matches, inserts, deletes.
+void apply_read(mp_poa *g, const Str &read, bool strand, int64_t readNo, const mp_aligned_pair *m, int64_t nm,
+                const std::vector<InsEvent> &iev, const std::vector<DelEvent> &dev) {
+    const int64_t mrc = g->maxRC;
+    for (int64_t i = 0; i < nm; i++) {
+        const int64_t node = (int64_t)m[i].x + 1, y = m[i].y;
+        const double w = (double)m[i].weight;
+        g->baseW[node * kAlphabet + base_symbol(read.b[y])] += w;
+        const uint64_t rc = read.c[y] >= (uint64_t)mrc ? (uint64_t)(mrc - 1) : read.c[y];
+        g->rcW[node * mrc + (int64_t)rc] += w;
+        g->obsNode.push_back((int32_t)node);
+        g->obs.push_back(mp_poa_observation{readNo, y, w});
+    }
+    for (const InsEvent &e : iev) {
+        const char *sb = read.b + e.start;
+        const uint64_t *sc = read.c + e.start;
+        int32_t hit = -1;
+        for (int32_t q = g->insHead[e.pos]; q >= 0; q = g->ins[q].next) {
+            const InsEntry &E = g->ins[q];
+            if (E.hash != e.hash || E.len != e.len) continue;
+            bool same = true;
+            for (int64_t t = 0; t < e.len && same; t++) {
+                const int64_t src = (t - e.rot + e.len) % e.len;
+                same = g->insB[E.sym + t] == sb[src] && g->insC[E.sym + t] == sc[src];
+            }
+            if (same) {
+                hit = q;
+                break;
+            }
+        }
+        if (hit < 0) {
+            hit = (int32_t)g->ins.size();
+            InsEntry E;
+            E.hash = e.hash;
+            E.sym = (int64_t)g->insB.size();
+            E.len = e.len;
+            for (int64_t t = 0; t < e.len; t++) {
+                const int64_t src = (t - e.rot + e.len) % e.len;
+                g->insB.push_back(sb[src]);
+                g->insC.push_back(sc[src]);
+            }
+            g->ins.push_back(std::move(E));
+            if (g->insTail[e.pos] >= 0)
+                g->ins[g->insTail[e.pos]].next = hit;
+            else
+                g->insHead[e.pos] = hit;
+            g->insTail[e.pos] = hit;
+        }
+        InsEntry &E = g->ins[hit];
+        (strand ? E.wF : E.wR) += e.w;
+        E.obs.push_back(mp_poa_observation{readNo, e.start, e.w});
+    }
+    for (const DelEvent &e : dev) {
+        int32_t hit = -1;
+        for (int32_t q = g->delHead[e.pos]; q >= 0; q = g->del[q].next)
+            if (g->del[q].len == e.len) {
+                hit = q;
+                break;
+            }
+        if (hit < 0) {
+            hit = (int32_t)g->del.size();
+            DelEntry E;
+            E.len = e.len;
+            g->del.push_back(std::move(E));
+            if (g->delTail[e.pos] >= 0)
+                g->del[g->delTail[e.pos]].next = hit;
+            else
+                g->delHead[e.pos] = hit;
+            g->delTail[e.pos] = hit;
+        }
+        DelEntry &E = g->del[hit];
+        (strand ? E.wF : E.wR) += e.w;
+        E.obs.push_back(mp_poa_observation{readNo, e.off, e.w});
+    }
+}
+
+bool valid_string(const mp_rle_string *s) {
+    return s && s->length >= 0 && (s->length == 0 || (s->bases && s->repeatCounts));
+}
+
+// Entries of one kind in read-out order: nodes ascending, each node's list in order.
+template <class E>
+void list_order(const std::vector<int32_t> &head, const std::vector<E> &ent, std::vector<std::pair<int64_t, int32_t>> &out) {
+    out.clear();
+    for (int64_t n = 0; n < (int64_t)head.size(); n++)
+        for (int32_t q = head[n]; q >= 0; q = ent[q].next) out.emplace_back(n, q);
+}
+
+}  // namespace
+
+namespace mp {
+
+mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *pairPoa, const int64_t *pairReadNo,
+                            const mp_rle_string *reads, const uint8_t *strand, int64_t npairs,
+                            const mp_aligned_pair *const lists[3], const int64_t *const starts[3], bool cmp,
+                            int threads) {
+    for (int64_t i = 0; i < npairs; i++) {
+        if (pairPoa[i] < 0 || pairPoa[i] >= nPoas || !poas[pairPoa[i]] || !valid_string(&reads[i])) {
+            set_error("mp_batch_poa_augment: pair " + std::to_string(i) + " has no graph or a bad read string");
+            return MP_EINVAL;
+        }
+    }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, std::max<int64_t>(npairs, 1)));
+    // each graph's pairs, in batch order
+    std::vector<std::vector<int64_t>> byPoa(nPoas);
+    for (int64_t i = 0; i < npairs; i++) byPoa[pairPoa[i]].push_back(i);
+    std::vector<int64_t> poaOrder;
+    for (int64_t p = 0; p < nPoas; p++)
+        if (!byPoa[p].empty()) poaOrder.push_back(p);
+    // larger graphs (more list tuples) first
+    std::vector<int64_t> poaWork(nPoas, 0);
+    for (int64_t i = 0; i < npairs; i++)
+        for (int l = 0; l < 3; l++) poaWork[pairPoa[i]] += starts[l][i + 1] - starts[l][i];
+    std::stable_sort(poaOrder.begin(), poaOrder.end(), [&](int64_t a, int64_t b) { return poaWork[a] > poaWork[b]; });
+
+    std::vector<std::vector<InsEvent>> iev(npairs);
+    std::vector<std::vector<DelEvent>> dev(npairs);
+    std::atomic<int64_t> nextPair{0}, nextPoa{0};
+    std::atomic<int64_t> badPair{-1};
+    // phase 1: events of every read (graphs read-only)
+    auto phase1 = [&]() {
+        Scratch S;
+        for (int64_t i; (i = nextPair.fetch_add(1)) < npairs;) {
+            const mp_poa *g = poas[pairPoa[i]];
+            const Str ref = g->ref();
+            const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
+            const mp_aligned_pair *m = lists[0] + starts[0][i], *gx = lists[1] + starts[1][i],
+                                  *gy = lists[2] + starts[2][i];
+            const int64_t nm = starts[0][i + 1] - starts[0][i], nx = starts[1][i + 1] - starts[1][i],
+                          ny = starts[2][i + 1] - starts[2][i];
+            if (!check_read(ref, rd, m, nm, gy, ny, gx, nx)) {
+                int64_t none = -1;
+                badPair.compare_exchange_strong(none, i);
+                continue;
+            }
+            read_events(ref, rd, m, nm, gy, ny, gx, nx, cmp, S, iev[i], dev[i]);
+        }
+    };
+    // phase 2: each graph applies its reads in order
+    auto phase2 = [&]() {
+        for (int64_t k; (k = nextPoa.fetch_add(1)) < (int64_t)poaOrder.size();) {
+            mp_poa *g = poas[poaOrder[k]];
+            for (int64_t i : byPoa[poaOrder[k]]) {
+                const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
+                apply_read(g, rd, strand[i] != 0, pairReadNo ? pairReadNo[i] : i, lists[0] + starts[0][i],
+                           starts[0][i + 1] - starts[0][i], iev[i], dev[i]);
+                std::vector<InsEvent>().swap(iev[i]);
+                std::vector<DelEvent>().swap(dev[i]);
+            }
+        }
+    };
+    try {
+        std::vector<std::thread> th;
+        for (int t = 1; t < threads; t++) th.emplace_back(phase1);
+        phase1();
+        for (auto &t : th) t.join();
+        if (badPair.load() >= 0) {
+            set_error("mp_batch_poa_augment: pair " + std::to_string(badPair.load()) +
+                      " has list coordinates outside its graph's reference or its read");
+            return MP_EINVAL;
+        }
+        th.clear();
+        const int t2 = (int)std::min<int64_t>(threads, (int64_t)poaOrder.size());
+        for (int t = 1; t < t2; t++) th.emplace_back(phase2);
+        phase2();
+        for (auto &t : th) t.join();
+    } catch (const std::bad_alloc &) {
+        set_error("mp_batch_poa_augment: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+
+}  // namespace mp
+
+extern "C" mp_status mp_poa_reference_graph(const mp_rle_string *reference, int64_t maxRepeatCount, mp_poa **out) {
+    if (!out || !valid_string(reference) || maxRepeatCount < 1) {
+        mp::set_error("mp_poa_reference_graph: bad argument");
+        return MP_EINVAL;
+    }
+    *out = nullptr;
+    try {
+        std::unique_ptr<mp_poa> g(new mp_poa);
+        const int64_t n = reference->length;
+        g->maxRC = maxRepeatCount;
+        g->refB.assign(reference->bases, reference->bases + n);
+        g->refC.assign(reference->repeatCounts, reference->repeatCounts + n);
+        g->nodeBase.resize(n + 1);
+        g->nodeRC.resize(n + 1);
+        g->nodeBase[0] = 'N';  // prefix node for events before the first reference run
+        g->nodeRC[0] = 1;
+        for (int64_t i = 0; i < n; i++) {  // poaNode_construct (polisher.c:90-107)
+            const char u = (char)std::toupper((unsigned char)reference->bases[i]);
+            g->nodeBase[i + 1] = base_symbol(u) < 4 ? u : 'N';
+            g->nodeRC[i + 1] = reference->repeatCounts[i];
+        }
+        g->baseW.assign((n + 1) * kAlphabet, 0.0);
+        g->rcW.assign((n + 1) * maxRepeatCount, 0.0);
+        g->insHead.assign(n + 1, -1);
+        g->insTail.assign(n + 1, -1);
+        g->delHead.assign(n + 1, -1);
+        g->delTail.assign(n + 1, -1);
+        *out = g.release();
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poa_reference_graph: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+
+extern "C" void mp_poa_free(mp_poa *poa) { delete poa; }
+
+extern "C" mp_status mp_poa_augment(mp_poa *poa, const mp_rle_string *read, int32_t readStrand, int64_t readNo,
+                                    const mp_aligned_pair *matches, int64_t nMatches, const mp_aligned_pair *inserts,
+                                    int64_t nInserts, const mp_aligned_pair *deletes, int64_t nDeletes,
+                                    int32_t compareRepeatCounts) {
+    if (!poa || !valid_string(read) || nMatches < 0 || nInserts < 0 || nDeletes < 0 || (nMatches && !matches) ||
+        (nInserts && !inserts) || (nDeletes && !deletes)) {
+        mp::set_error("mp_poa_augment: bad argument");
+        return MP_EINVAL;
+    }
+    const Str rd{read->bases, read->repeatCounts, read->length};
+    if (!check_read(poa->ref(), rd, matches, nMatches, inserts, nInserts, deletes, nDeletes)) {
+        mp::set_error("mp_poa_augment: list coordinates outside the reference or the read");
+        return MP_EINVAL;
+    }
+    try {
+        Scratch S;
+        std::vector<InsEvent> iev;
+        std::vector<DelEvent> dev;
+        read_events(poa->ref(), rd, matches, nMatches, inserts, nInserts, deletes, nDeletes, compareRepeatCounts != 0,
+                    S, iev, dev);
+        apply_read(poa, rd, readStrand != 0, readNo, matches, nMatches, iev, dev);
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poa_augment: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *s) {
+    if (!poa || !s) {
+        mp::set_error("mp_poa_get_sizes: null argument");
+        return MP_EINVAL;
+    }
+    s->nodes = poa->nodes();
+    s->maxRepeatCount = poa->maxRC;
+    s->inserts = (int64_t)poa->ins.size();
+    s->insertSymbols = (int64_t)poa->insB.size();
+    s->deletes = (int64_t)poa->del.size();
+    s->nodeObservations = (int64_t)poa->obs.size();
+    s->insertObservations = 0;
+    for (const InsEntry &e : poa->ins) s->insertObservations += (int64_t)e.obs.size();
+    s->deleteObservations = 0;
+    for (const DelEntry &e : poa->del) s->deleteObservations += (int64_t)e.obs.size();
+    return MP_OK;
+}
+
+extern "C" mp_status mp_poa_get_nodes(const mp_poa *poa, char *bases, uint64_t *repeatCounts, double *baseWeights,
+                                      double *repeatCountWeights) {
+    if (!poa) {
+        mp::set_error("mp_poa_get_nodes: null graph");
+        return MP_EINVAL;
+    }
+    const size_t n = poa->nodeBase.size();
+    if (bases) std::memcpy(bases, poa->nodeBase.data(), n);
+    if (repeatCounts) std::memcpy(repeatCounts, poa->nodeRC.data(), n * sizeof(uint64_t));
+    if (baseWeights) std::memcpy(baseWeights, poa->baseW.data(), poa->baseW.size() * sizeof(double));
+    if (repeatCountWeights) std::memcpy(repeatCountWeights, poa->rcW.data(), poa->rcW.size() * sizeof(double));
+    return MP_OK;
+}
+
+extern "C" mp_status mp_poa_get_inserts(const mp_poa *poa, int64_t *node, double *wF, double *wR,
+                                        int64_t *symbolStart, int64_t *symbolLength, char *bases,
+                                        uint64_t *repeatCounts) {
+    if (!poa) {
+        mp::set_error("mp_poa_get_inserts: null graph");
+        return MP_EINVAL;
+    }
+    std::vector<std::pair<int64_t, int32_t>> ord;
+    list_order(poa->insHead, poa->ins, ord);
+    int64_t sym = 0;
+    for (size_t k = 0; k < ord.size(); k++) {
+        const InsEntry &e = poa->ins[ord[k].second];
+        if (node) node[k] = ord[k].first;
+        if (wF) wF[k] = e.wF;
+        if (wR) wR[k] = e.wR;
+        if (symbolStart) symbolStart[k] = sym;
+        if (symbolLength) symbolLength[k] = e.len;
+        if (bases) std::memcpy(bases + sym, poa->insB.data() + e.sym, (size_t)e.len);
+        if (repeatCounts) std::memcpy(repeatCounts + sym, poa->insC.data() + e.sym, (size_t)e.len * sizeof(uint64_t));
+        sym += e.len;
+    }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_poa_get_deletes(const mp_poa *poa, int64_t *node, int64_t *length, double *wF, double *wR) {
+    if (!poa) {
+        mp::set_error("mp_poa_get_deletes: null graph");
+        return MP_EINVAL;
+    }
+    std::vector<std::pair<int64_t, int32_t>> ord;
+    list_order(poa->delHead, poa->del, ord);
+    for (size_t k = 0; k < ord.size(); k++) {
+        const DelEntry &e = poa->del[ord[k].second];
+        if (node) node[k] = ord[k].first;
+        if (length) length[k] = e.len;
+        if (wF) wF[k] = e.wF;
+        if (wR) wR[k] = e.wR;
+    }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_poa_get_observations(const mp_poa *poa, int32_t kind, int64_t *start,
+                                             mp_poa_observation *obs) {
+    if (!poa || !start || kind < 0 || kind > 2) {
+        mp::set_error("mp_poa_get_observations: bad argument");
+        return MP_EINVAL;
+    }
+    if (kind == 0) {  // stable counting sort of the append-order observations by node
+        const int64_t n = poa->nodes();
+        std::vector<int64_t> cnt(n + 1, 0);
+        for (int32_t v : poa->obsNode) cnt[v + 1]++;
+        for (int64_t k = 0; k < n; k++) cnt[k + 1] += cnt[k];
+        std::memcpy(start, cnt.data(), (n + 1) * sizeof(int64_t));
+        if (obs)
+            for (size_t i = 0; i < poa->obs.size(); i++) obs[cnt[poa->obsNode[i]]++] = poa->obs[i];
+        return MP_OK;
+    }
+    std::vector<std::pair<int64_t, int32_t>> ord;
+    if (kind == 1)
+        list_order(poa->insHead, poa->ins, ord);
+    else
+        list_order(poa->delHead, poa->del, ord);
+    int64_t at = 0;
+    for (size_t k = 0; k < ord.size(); k++) {
+        const std::vector<mp_poa_observation> &v =
+            kind == 1 ? poa->ins[ord[k].second].obs : poa->del[ord[k].second].obs;
+        start[k] = at;
+        if (obs && !v.empty()) std::memcpy(obs + at, v.data(), v.size() * sizeof(mp_poa_observation));
+        at += (int64_t)v.size();
+    }
+    start[ord.size()] = at;
+    return MP_OK;
+}

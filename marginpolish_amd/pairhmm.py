@@ -268,6 +268,10 @@ class Batch:
         self.npairs = n
         self.mode = mode
 
+    @property
+    def handle(self):
+        return self._h
+
     def run(self):
         check(lib().mp_batch_run(self._h))
         return self

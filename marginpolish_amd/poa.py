@@ -1,0 +1,166 @@
+"""POA augmentation from the pair-HMM's posterior lists: the host mirror of
+the reference's poa_getReferenceGraph / poa_augment / the augment loop of
+poa_realign (impl/polisher.c:118-131, 316-534, 656-703), over the native
+implementation in libmp_pairhmm.so (include/mp_poa.h).
+
+Run-length strings are (bases, repeatCounts): bases a str / bytes / uint8
+array of characters, repeatCounts integers >= 1 (RleString, inc/margin.h).
+Aligned-pair lists are numpy arrays of ALIGNED_PAIR_DTYPE or sequences of
+(weight, x, y) tuples.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+MAXIMUM_REPEAT_LENGTH = 51  # margin.h:123 (exclusive)
+_BASES = np.frombuffer(b"ACGTN", dtype=np.uint8)
+
+
+def rle_from_symbols(sym, runs=None):
+    """An RLE string from uint16 pair-HMM symbols (run length << 8 | base):
+    bases as characters; repeat counts from `runs` (uncapped) when given,
+    else the symbols' own (capped) run lengths."""
+    sym = np.asarray(sym, dtype=np.uint16)
+    bases = _BASES[np.minimum(sym & 0xFF, 4)]
+    counts = np.asarray(runs if runs is not None else (sym >> 8), dtype=np.uint64)
+    return bases, counts
+
+
+def _as_rle(r):
+    bases, counts = r
+    if isinstance(bases, str):
+        bases = bases.encode()
+    b = np.ascontiguousarray(np.frombuffer(bases, dtype=np.uint8) if isinstance(bases, (bytes, bytearray))
+                             else np.asarray(bases, dtype=np.uint8))
+    c = np.ascontiguousarray(np.asarray(counts, dtype=np.uint64))
+    if b.shape != c.shape or b.ndim != 1:
+        raise ValueError("bases and repeatCounts must be 1-d of equal length")
+    return b, c
+
+
+def _rle_struct(b, c):
+    return _lib.RleString(b.ctypes.data, c.ctypes.data, b.size)
+
+
+def _as_pairs(t):
+    if isinstance(t, np.ndarray) and t.dtype == np.dtype(_lib.ALIGNED_PAIR_DTYPE):
+        return np.ascontiguousarray(t)
+    a = np.zeros(len(t), dtype=_lib.ALIGNED_PAIR_DTYPE)
+    if len(t):
+        arr = np.asarray(t, dtype=np.int64).reshape(-1, 3)
+        a["weight"], a["x"], a["y"] = arr[:, 0], arr[:, 1], arr[:, 2]
+    return a
+
+
+class Poa:
+    """A POA graph (struct _Poa, inc/margin.h:590-597) held by the library."""
+
+    def __init__(self, reference, max_repeat_count=MAXIMUM_REPEAT_LENGTH):
+        b, c = _as_rle(reference)
+        h = C.c_void_p()
+        check(lib().mp_poa_reference_graph(C.byref(_rle_struct(b, c)), int(max_repeat_count), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def augment(self, read, read_strand, read_no, matches, inserts, deletes, compare_repeat_counts=True):
+        """poa_augment (polisher.c:316-534)."""
+        b, c = _as_rle(read)
+        m, i, d = _as_pairs(matches), _as_pairs(inserts), _as_pairs(deletes)
+        check(lib().mp_poa_augment(self._h, C.byref(_rle_struct(b, c)), int(bool(read_strand)), int(read_no),
+                                   m.ctypes.data, m.size, i.ctypes.data, i.size, d.ctypes.data, d.size,
+                                   int(bool(compare_repeat_counts))))
+
+    def sizes(self):
+        s = _lib.PoaSizes()
+        check(lib().mp_poa_get_sizes(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _lib.PoaSizes._fields_}
+
+    def nodes(self):
+        """Per node: bases (str), repeatCounts, baseWeights (n, 5), repeatCountWeights (n, maxRepeatCount)."""
+        s = self.sizes()
+        n, mrc = s["nodes"], s["maxRepeatCount"]
+        bases = np.empty(n, dtype=np.uint8)
+        rc = np.empty(n, dtype=np.uint64)
+        bw = np.empty((n, 5), dtype=np.float64)
+        rw = np.empty((n, mrc), dtype=np.float64)
+        check(lib().mp_poa_get_nodes(self._h, bases.ctypes.data, rc.ctypes.data, bw.ctypes.data, rw.ctypes.data))
+        return {"bases": bases.tobytes().decode(), "repeatCounts": rc, "baseWeights": bw, "repeatCountWeights": rw}
+
+    def inserts(self):
+        """Structure-of-arrays over every PoaInsert (nodes ascending, list order)."""
+        s = self.sizes()
+        k, ns = s["inserts"], s["insertSymbols"]
+        node = np.empty(k, np.int64)
+        wf, wr = np.empty(k), np.empty(k)
+        st, ln = np.empty(k, np.int64), np.empty(k, np.int64)
+        bases = np.empty(max(ns, 1), np.uint8)
+        counts = np.empty(max(ns, 1), np.uint64)
+        check(lib().mp_poa_get_inserts(self._h, node.ctypes.data, wf.ctypes.data, wr.ctypes.data, st.ctypes.data,
+                                       ln.ctypes.data, bases.ctypes.data, counts.ctypes.data))
+        return {"node": node, "weightForwardStrand": wf, "weightReverseStrand": wr, "symbolStart": st,
+                "symbolLength": ln, "bases": bases[:ns], "repeatCounts": counts[:ns]}
+
+    def deletes(self):
+        s = self.sizes()
+        k = s["deletes"]
+        node, ln = np.empty(k, np.int64), np.empty(k, np.int64)
+        wf, wr = np.empty(k), np.empty(k)
+        check(lib().mp_poa_get_deletes(self._h, node.ctypes.data, ln.ctypes.data, wf.ctypes.data, wr.ctypes.data))
+        return {"node": node, "length": ln, "weightForwardStrand": wf, "weightReverseStrand": wr}
+
+    def observations(self, kind):
+        """kind 0 = nodes, 1 = inserts, 2 = deletes: (start, records)."""
+        s = self.sizes()
+        ent = {0: s["nodes"], 1: s["inserts"], 2: s["deletes"]}[kind]
+        tot = {0: s["nodeObservations"], 1: s["insertObservations"], 2: s["deleteObservations"]}[kind]
+        start = np.empty(ent + 1, np.int64)
+        obs = np.empty(max(tot, 1), dtype=_lib.OBSERVATION_DTYPE)
+        check(lib().mp_poa_get_observations(self._h, int(kind), start.ctypes.data, obs.ctypes.data))
+        return start, obs[:tot]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mp_poa_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def poa_getReferenceGraph(reference, maxRepeatCount=MAXIMUM_REPEAT_LENGTH):
+    return Poa(reference, maxRepeatCount)
+
+
+def poa_augment(poa, read, readStrand, readNo, matches, inserts, deletes, poaConstructCompareRepeatCounts=True):
+    poa.augment(read, readStrand, readNo, matches, inserts, deletes, poaConstructCompareRepeatCounts)
+
+
+def augment_batch(batch, poas, pair_poa, reads, pair_read_no=None, compare_repeat_counts=True, threads=0):
+    """The augment loop of poa_realign (polisher.c:668-698) for every pair of
+    a posterior batch that has run, straight from the batch's lists: pair i
+    is read pair_read_no[i] (default: its index) of poas[pair_poa[i]], with
+    run-length string reads[i] = (bases, repeatCounts)."""
+    n = batch.npairs
+    if len(reads) != n or len(pair_poa) != n:
+        raise ValueError("one graph index and one read string per pair")
+    keep = [_as_rle(r) for r in reads]
+    rs = np.empty(n, dtype=_lib.RLE_STRING_DTYPE)
+    for i, (b, c) in enumerate(keep):
+        rs[i] = (b.ctypes.data, c.ctypes.data, b.size)
+    pp = np.ascontiguousarray(np.asarray(pair_poa, dtype=np.int64))
+    rn = np.ascontiguousarray(np.asarray(pair_read_no if pair_read_no is not None else np.arange(n), dtype=np.int64))
+    handles = (C.c_void_p * max(len(poas), 1))(*[p.handle for p in poas])
+    check(lib().mp_batch_poa_augment(batch.handle, handles, len(poas), pp.ctypes.data, rn.ctypes.data,
+                                     rs.ctypes.data, int(bool(compare_repeat_counts)), int(threads)))
+    del keep

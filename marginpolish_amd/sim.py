@@ -46,6 +46,8 @@ class Pair:
     strand: int
     chunk: int = 0
     read: int = 0
+    ref_runs: np.ndarray = None   # uncapped run lengths of ref_sym (the RleString's repeat counts)
+    read_runs: np.ndarray = None  # uncapped run lengths of read_sym
 
 
 @dataclasses.dataclass
@@ -141,7 +143,8 @@ def _pair(ref_seg, read_seg, ref_idx, read_idx, W, strand, chunk, read_id):
         keep = np.r_[True, (X[1:] > X[:-1]) & (Y[1:] > Y[:-1])]
         X, Y = X[keep], Y[keep]
     anchors = np.stack([X, Y, np.full(X.size, W, dtype=np.int64)], axis=1).astype(np.int64)
-    return Pair(_symbols(rb, rr), _symbols(qb, qr), anchors, strand, chunk, read_id)
+    return Pair(_symbols(rb, rr), _symbols(qb, qr), anchors, strand, chunk, read_id, rr.astype(np.uint64),
+                qr.astype(np.uint64))
 
 
 def config0(n_reads=20, ref_len=5000, seed=0, err=0.10, W=32, strands=(1,)):

@@ -1,0 +1,261 @@
+"""Oracle for POA graph augmentation from posterior lists -- test infrastructure.
+
+A pure-Python restatement of the reference's reference-graph construction and
+poa_augment (impl/polisher.c), used only by tests/ and bench.py's CPU leg to
+check the library's native implementation.  Nothing in the product imports it.
+
+RLE strings are (bases, counts): a str of base characters and a tuple of run
+lengths (RleString, inc/margin.h).  Aligned pairs are (weight, x, y) tuples
+with x a reference and y a read coordinate, as the posterior lists hold them.
+
+Pins (tests/test_poa_oracle.py): the reference's known-answer tests
+test_poa_augment_example, test_rle_rotateString and the property of
+test_getShift (polisherTest.c:59-240, 869-898), plus brute-force invariants
+(a shifted insert or delete spells the same sequence as the unshifted one,
+complete-insert/delete enumeration against a direct definition).
+"""
+from __future__ import annotations
+
+NUC_SYMBOL = {"A": 0, "a": 0, "C": 1, "c": 1, "G": 2, "g": 2, "T": 3, "t": 3}
+ALPHABET_SIZE = 5  # A C G T N (stateMachine.c:63-71)
+
+
+def char_to_symbol(c):
+    """convertNucleotideCharToSymbol (stateMachine.c:25-42)."""
+    return NUC_SYMBOL.get(c, 4)
+
+
+def rle(s):
+    """rleString_construct (polisher.c:1357-1390)."""
+    bases, counts = [], []
+    for c in s:
+        if bases and bases[-1] == c:
+            counts[-1] += 1
+        else:
+            bases.append(c)
+            counts.append(1)
+    return "".join(bases), tuple(counts)
+
+
+def rle_no_rle(s):
+    """rleString_construct_no_rle: every character its own run of 1."""
+    return s, tuple([1] * len(s))
+
+
+def expand(r):
+    return "".join(b * c for b, c in zip(*r))
+
+
+def rotate(r, k):
+    """rleString_rotateString (polisher.c:1508-1520): character i moves to (i + k) mod length."""
+    b, c = r
+    n = len(b)
+    if n == 0:
+        return r
+    ob, oc = [None] * n, [None] * n
+    for i in range(n):
+        ob[(i + k) % n] = b[i]
+        oc[(i + k) % n] = c[i]
+    return "".join(ob), tuple(oc)
+
+
+def substring(r, start, length):
+    """rleString_copySubstring (polisher.c:1433-1453)."""
+    return r[0][start:start + length], r[1][start:start + length]
+
+
+def _matches_ref(ref, start, s, length, cmp_counts):
+    """matchesReferenceSubstring (polisher.c:237-249)."""
+    for l in range(length):
+        if ref[0][start + l] != s[0][l] or (cmp_counts and ref[1][start + l] != s[1][l]):
+            return False
+    return True
+
+
+def _has_internal_repeat(s, k, cmp_counts):
+    """hasInternalRepeat (polisher.c:251-267)."""
+    n = len(s[0])
+    if n % k:
+        return False
+    for i in range(k, n, k):
+        for j in range(k):
+            if s[0][j] != s[0][j + i] or (cmp_counts and s[1][j] != s[1][j + i]):
+                return False
+    return True
+
+
+def get_shift(ref, ref_start, s, cmp_counts):
+    """getShift (polisher.c:269-297): walk an indel left over whole copies of
+    its minimal internal repeat, then one more symbol for a single-symbol
+    string when repeat counts are compared."""
+    n = len(s[0])
+    m = 1
+    while m < n and not _has_internal_repeat(s, m, cmp_counts):
+        m += 1
+    k = ref_start - m
+    while k >= 0:
+        if not _matches_ref(ref, k, s, m, cmp_counts):
+            break
+        ref_start = k
+        k -= m
+    if n == 1 and cmp_counts and ref_start > 0 and ref[0][ref_start - 1] == s[0][0]:
+        ref_start -= 1
+    return ref_start
+
+
+def common_suffix(ref, length1, s, cmp_counts):
+    """getMaxCommonSuffixLength (polisher.c:299-314): longest suffix of
+    ref[:length1] equal to a suffix of s."""
+    i = 0
+    n = len(s[0])
+    while length1 - i - 1 >= 0 and n - i - 1 >= 0:
+        if ref[0][length1 - 1 - i] != s[0][n - 1 - i] or (cmp_counts and ref[1][length1 - 1 - i] != s[1][n - 1 - i]):
+            break
+        i += 1
+    return i
+
+
+class Node:
+    __slots__ = ("base", "repeat_count", "base_weights", "rc_weights", "inserts", "deletes", "observations")
+
+    def __init__(self, base, rc, max_rc):
+        self.base = base
+        self.repeat_count = rc
+        self.base_weights = [0.0] * ALPHABET_SIZE
+        self.rc_weights = [0.0] * max_rc
+        self.inserts = []   # [rle string, forward weight, reverse weight, observations]
+        self.deletes = []   # [length, forward weight, reverse weight, observations]
+        self.observations = []  # (readNo, offset, weight)
+
+
+class Poa:
+    """poa_getReferenceGraph (polisher.c:118-131): a prefix node 'N'/1 then one
+    node per reference symbol, base upper-cased, non-ACGT -> 'N'
+    (poaNode_construct, polisher.c:90-107)."""
+
+    def __init__(self, ref, max_rc):
+        self.ref = ref
+        self.max_rc = max_rc
+        self.nodes = [Node("N", 1, max_rc)]
+        for b, c in zip(*ref):
+            u = b.upper()
+            self.nodes.append(Node(u if char_to_symbol(u) < 4 else "N", c, max_rc))
+
+
+def _add_insert(node, s, weight, strand, obs):
+    """addToInserts (polisher.c:177-205): first equal RLE string or a new entry at the end."""
+    for e in node.inserts:
+        if e[0] == s:
+            break
+    else:
+        e = [s, 0.0, 0.0, []]
+        node.inserts.append(e)
+    e[1 if strand else 2] += weight
+    e[3].append(obs)
+
+
+def _add_delete(node, length, weight, strand, obs):
+    """addToDeletes (polisher.c:207-235): first entry of the same length or a new one."""
+    for e in node.deletes:
+        if e[0] == length:
+            break
+    else:
+        e = [length, 0.0, 0.0, []]
+        node.deletes.append(e)
+    e[1 if strand else 2] += weight
+    e[3].append(obs)
+
+
+UINT_MAX = 4294967295.0
+
+
+def insert_events(poa, read, mset, inserts, cmp_counts=True):
+    """Complete inserts of one read (polisher.c:351-443): runs of inserts with
+    equal x and consecutive y; every sub-run flanked by a match (or the read's
+    ends) on both sides.  Yields (found_at, placed_at, placed_string,
+    found_string, weight, read_offset) in the reference's order."""
+    read_len = len(read[0])
+    ins = sorted(inserts, key=lambda t: (t[1], t[2]))
+    i = 0
+    while i < len(ins):
+        x0, y0 = ins[i][1], ins[i][2]
+        j = i + 1
+        while j < len(ins) and ins[j][1] == x0 and ins[j][2] == y0 + j - i:
+            j += 1
+        for k in range(i, j):
+            yp = y0 + k - i - 1
+            if (x0, yp) not in mset and yp > -1:
+                continue
+            for l in range(k, j):
+                yn = y0 + l - i + 1
+                if (x0 + 1, yn) not in mset and yn < read_len:
+                    continue
+                s = substring(read, ins[k][2], l + 1 - k)
+                w = UINT_MAX
+                for m in range(k, l + 1):
+                    w = min(w, float(ins[m][0]))
+                pos = get_shift(poa.ref, x0 + 1, s, cmp_counts)
+                cs = common_suffix(poa.ref, pos, s, cmp_counts)
+                placed = rotate(s, cs) if cs > 0 else s
+                yield x0 + 1, pos - cs, placed, s, w, ins[k][2]
+        i = j
+
+
+def delete_events(poa, mset, deletes, cmp_counts=True):
+    """Complete deletes of one read (polisher.c:445-530): runs of deletes with
+    equal y and consecutive x, flanked by matches (or the reference's ends).
+    Yields (found_at, placed_at, length, weight, read_offset)."""
+    ref_len = len(poa.ref[0])
+    dels = sorted(deletes, key=lambda t: (t[2], t[1]))
+    i = 0
+    while i < len(dels):
+        x0, y0 = dels[i][1], dels[i][2]
+        j = i + 1
+        while j < len(dels) and dels[j][2] == y0 and dels[j][1] == x0 + j - i:
+            j += 1
+        for k in range(i, j):
+            xp = x0 + k - i - 1
+            if (xp, y0) not in mset and xp > -1:
+                continue
+            for l in range(k, j):
+                xn = x0 + l - i + 1
+                if (xn, y0 + 1) not in mset and xn < ref_len:
+                    continue
+                length = l - k + 1
+                w = UINT_MAX
+                for m in range(k, l + 1):
+                    w = min(w, float(dels[m][0]))
+                found = x0 + k - i
+                assert found >= 0
+                d = substring(poa.ref, found, length)
+                pos = get_shift(poa.ref, found, d, cmp_counts)
+                pos -= common_suffix(poa.ref, pos, d, cmp_counts)
+                yield found, pos, length, w, y0
+        i = j
+
+
+def augment(poa, read, strand, read_no, matches, inserts, deletes, cmp_counts=True):
+    """poa_augment (polisher.c:316-534).  matches / inserts (gap-Y) / deletes
+    (gap-X) are lists of (weight, x, y)."""
+    nodes = poa.nodes
+    # match weights and base observations, in list order (polisher.c:321-339)
+    for w, x, y in matches:
+        node = nodes[x + 1]
+        node.base_weights[char_to_symbol(read[0][y])] += w
+        node.rc_weights[min(read[1][y], poa.max_rc - 1)] += w
+        node.observations.append((read_no, y, float(w)))
+    mset = set((x, y) for _, x, y in matches)
+    assert len(mset) == len(matches)
+    for _, pos, s, _, w, off in insert_events(poa, read, mset, inserts, cmp_counts):
+        _add_insert(nodes[pos], s, w, strand, (read_no, off, w))
+    for _, pos, length, w, off in delete_events(poa, mset, deletes, cmp_counts):
+        _add_delete(nodes[pos], length, w, strand, (read_no, off, w))
+
+
+def realign_augment(ref, max_rc, reads, lists, cmp_counts=True):
+    """The augment half of poa_realign (polisher.c:656-703) for one chunk:
+    reads = [(rle read, strand)], lists = [(matches, inserts, deletes)] in read order."""
+    poa = Poa(ref, max_rc)
+    for n, ((r, strand), (m, ins, dels)) in enumerate(zip(reads, lists)):
+        augment(poa, r, strand, n, m, ins, dels, cmp_counts)
+    return poa

@@ -1,5 +1,5 @@
 """CPU tests of the shipped library's host side: it loads, exports every
-symbol include/mp_pairhmm.h declares, parses the params JSON like the
+symbol include/mp_pairhmm.h and include/mp_poa.h declare, parses the params JSON like the
 reference, and builds the same band / windows / split points as the oracle.
 No kernel runs here."""
 import ctypes as C
@@ -19,9 +19,12 @@ from tests.conftest import PARAMS, PARAMS_NO_RLE, ROOT
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "mp_pairhmm.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(mp_[a-z0-9_]+)\s*\(", txt)))
+    syms = set()
+    for h in ("mp_pairhmm.h", "mp_poa.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        syms |= set(re.findall(r"\b(mp_[a-z0-9_]+)\s*\(", txt))
+    return sorted(syms)
 
 
 def test_library_exports_every_declared_symbol():

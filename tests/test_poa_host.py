@@ -1,0 +1,161 @@
+"""The library's POA augmentation (host code, include/mp_poa.h) against the
+oracle restatement of poa_augment, bit for bit: node weights, insert and
+delete lists in order, every observation.  Runs on the CPU (no kernel)."""
+import random
+
+import numpy as np
+import pytest
+
+from marginpolish_amd import _lib
+from marginpolish_amd import crop
+from marginpolish_amd import poa as mpoa
+from marginpolish_amd import sim
+from oracle import oracle as O
+from oracle import poa as P
+from tests.conftest import PARAMS
+from tests.poa_compare import assert_same_graph, native_as_oracle, oracle_lists
+
+
+def _both(ref, max_rc=51):
+    return mpoa.Poa(ref, max_rc), P.Poa(ref, max_rc)
+
+
+def test_poa_augment_example_native():
+    """test_poa_augment_example (polisherTest.c:160-241) through the library."""
+    ref, read = P.rle_no_rle("GATTACA"), P.rle_no_rle("GATACGGT")
+    matches = [(100, 0, 0), (100, 1, 1), (50, 2, 2), (50, 3, 2), (100, 4, 3), (100, 5, 4), (50, 6, 5),
+               (25, 6, 6), (25, 6, 7)]
+    inserts = [(50, 5, 5), (25, 5, 6), (50, 6, 6), (75, 6, 7)]
+    deletes = [(50, 2, 1), (50, 3, 2)]
+    g, o = _both(ref)
+    g.augment(read, 1, 0, matches, inserts, deletes, True)
+    P.augment(o, read, True, 0, matches, inserts, deletes, True)
+    nodes = native_as_oracle(g)
+    assert_same_graph(nodes, o)
+    assert [e[0][0] for e in nodes[6].inserts] == ["G", "GG"]
+    assert [e[1] for e in nodes[6].inserts] == [50.0, 25.0]
+    assert [e[0][0] for e in nodes[7].inserts] == ["GT", "T"]
+    assert [(e[0], e[1]) for e in nodes[2].deletes] == [(1, 100.0)]
+
+
+def _rand_rle(rng, n, alphabet="ACGT", max_run=4):
+    b = "".join(rng.choice(alphabet) for _ in range(n))
+    runs = tuple(rng.choice([1, 1, 1, 2, 2, 3, max_run]) for _ in range(n))
+    # adjacent equal bases are allowed in an RLE string built elsewhere; keep them distinct as rleString_construct does
+    out_b, out_c = [], []
+    for c, r in zip(b, runs):
+        if out_b and out_b[-1] == c:
+            continue
+        out_b.append(c)
+        out_c.append(r)
+    return "".join(out_b), tuple(out_c)
+
+
+def _rand_lists(rng, n, m, dens=(0.3, 0.25, 0.25)):
+    matches = [(rng.randint(1, 10 ** 7), x, y) for x in range(n) for y in range(m) if rng.random() < dens[0]]
+    inserts = [(rng.randint(1, 10 ** 7), x, y) for x in range(-1, n) for y in range(m) if rng.random() < dens[1]]
+    deletes = [(rng.randint(1, 10 ** 7), x, y) for x in range(n) for y in range(-1, m) if rng.random() < dens[2]]
+    for lst in (matches, inserts, deletes):
+        rng.shuffle(lst)  # the reference sorts the gap lists itself; match order sets observation order
+    return matches, inserts, deletes
+
+
+@pytest.mark.parametrize("cmp", [True, False])
+def test_random_reads_match_oracle(cmp):
+    """Several reads per graph, random dense lists (long complete runs,
+    repeats, both strands), counts compared or not."""
+    rng = random.Random(17 + cmp)
+    for case in range(40):
+        ref = _rand_rle(rng, rng.randint(1, 25), "ACGT" if case % 5 else "AC")
+        g, o = _both(ref, rng.choice([2, 5, 51]))
+        for r in range(rng.randint(1, 4)):
+            read = _rand_rle(rng, rng.randint(1, 25), "ACGT" if case % 5 else "AC", max_run=60)
+            m, i, d = _rand_lists(rng, len(ref[0]), len(read[0]))
+            strand = rng.random() < 0.5
+            g.augment(read, strand, r, m, i, d, cmp)
+            P.augment(o, read, strand, r, m, i, d, cmp)
+        assert_same_graph(native_as_oracle(g), o)
+
+
+def test_edge_cases():
+    rng = random.Random(4)
+    # empty lists, empty read
+    g, o = _both(P.rle("GATTACA"))
+    g.augment(("", ()), 1, 0, [], [], [], True)
+    P.augment(o, ("", ()), True, 0, [], [], [], True)
+    assert_same_graph(native_as_oracle(g), o)
+    # empty reference: one prefix node; only inserts at x = -1 are legal
+    g, o = _both(("", ()))
+    read = P.rle("ACGT")
+    ins = [(7, -1, 0), (8, -1, 1), (9, -1, 2), (6, -1, 3)]
+    g.augment(read, 0, 5, [], ins, [], True)
+    P.augment(o, read, False, 5, [], ins, [], True)
+    nodes = native_as_oracle(g)
+    assert_same_graph(nodes, o)
+    assert nodes[0].inserts[0][0][0] == "ACGT" and nodes[0].inserts[0][2] == 6.0
+    # lower-case and non-ACGT reference bases, read counts beyond the repeat-count bound
+    ref = ("gaTNxC", (1, 2, 3, 1, 1, 70))
+    read = ("GATNYC", (1, 2, 300, 1, 1, 55))
+    g, o = _both(ref, 51)
+    m = [(10, k, k) for k in range(6)]
+    g.augment(read, 1, 0, m, [], [], True)
+    P.augment(o, read, True, 0, m, [], [], True)
+    nodes = native_as_oracle(g)
+    assert_same_graph(nodes, o)
+    assert "".join(n.base for n in nodes) == "NGATNNC"
+    assert nodes[3].rc_weights[50] == 10.0 and nodes[5].base_weights[4] == 10.0
+    # a homopolymer / tandem repeat reference where indels shift far left
+    ref = P.rle("G" + "AT" * 30 + "C")
+    read = P.rle("G" + "AT" * 31 + "C")
+    n, mlen = len(ref[0]), len(read[0])
+    m = [(9, x, x) for x in range(n - 1)] + [(9, n - 1, mlen - 1)]
+    ins = [(5, n - 2, n - 1), (5, n - 2, n)]
+    g, o = _both(ref)
+    g.augment(read, 1, 0, m, ins, [(3, 10, 9), (3, 11, 9)], True)
+    P.augment(o, read, True, 0, m, ins, [(3, 10, 9), (3, 11, 9)], True)
+    nodes = native_as_oracle(g)
+    assert_same_graph(nodes, o)
+    assert nodes[1].inserts and nodes[1].inserts[0][0][0] in ("AT", "TA")
+    del rng
+
+
+def test_out_of_range_coordinates_are_rejected():
+    g = mpoa.Poa(P.rle("ACGT"))
+    read = P.rle("ACG")
+    for m, i, d in (([(1, 4, 0)], [], []), ([(1, 0, 3)], [], []), ([], [(1, -2, 0)], []), ([], [(1, 0, 3)], []),
+                    ([], [], [(1, -1, 0)]), ([], [], [(1, 0, -2)])):
+        with pytest.raises(_lib.MPError) as e:
+            g.augment(read, 1, 0, m, i, d, True)
+        assert e.value.status == 1
+    assert g.sizes()["nodeObservations"] == 0
+
+
+def test_realistic_lists_match_oracle():
+    """Lists from the exact FP64 pair-HMM oracle on simulated RLE reads
+    (cropped reference, uncropped coordinates), several reads into one
+    graph, forward and reverse strands."""
+    pp = O.PolishParams(PARAMS)
+    w = sim.config0(n_reads=4, ref_len=700, seed=9, strands=(1, 0))
+    ref_sym = w.pairs[0].ref_sym
+    ref = mpoa.rle_from_symbols(ref_sym)
+    ref_o = (ref[0].tobytes().decode(), tuple(int(c) for c in ref[1]))
+    g, o = mpoa.Poa(ref), P.Poa(ref_o, 51)
+    total = 0
+    for r, pr in enumerate(w.pairs):
+        sx, sy, an, first = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        model = pp.forward if pr.strand else pp.reverse
+        m, gx, gy, _, _ = O.posteriors(model, sx, sy, an, pp.p)
+        lists = []
+        for t in (m, gy, gx):
+            t = np.asarray(t, dtype=np.int64).reshape(-1, 3).copy()
+            t[:, 1] += first
+            lists.append([tuple(int(v) for v in row) for row in t])
+        read = mpoa.rle_from_symbols(pr.read_sym)
+        read_o = (read[0].tobytes().decode(), tuple(int(c) for c in read[1]))
+        g.augment(read, pr.strand, r, *lists, True)
+        P.augment(o, read_o, bool(pr.strand), r, *lists, True)
+        total += sum(len(x) for x in lists)
+    nodes = native_as_oracle(g)
+    assert_same_graph(nodes, o)
+    assert total > 3000
+    assert sum(len(n.inserts) for n in nodes) > 10 and sum(len(n.deletes) for n in nodes) > 10
