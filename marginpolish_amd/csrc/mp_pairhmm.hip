@@ -108,6 +108,9 @@ struct mp_engine {
     static constexpr size_t kStageBytes = (size_t)64 << 20;
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stageEv[2] = {nullptr, nullptr};
+    // pinned host copies of a batch's three lists for the POA augmentation (grown on demand, reused)
+    void *listHost[3] = {nullptr, nullptr, nullptr};
+    size_t listHostBytes[3] = {0, 0, 0};
     // autotuner results: batch shape key -> (narrow workgroup threads, five-state waves per workgroup)
     std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;
 };
@@ -161,9 +164,6 @@ struct mp_batch {
     DevBuf<mpk::ListTuple> dList[3];      // the lists on the device (copied out by fetch)
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
-    // host copies of the lists for the POA augmentation (kept between runs)
-    std::unique_ptr<mp_aligned_pair[]> hostList[3];
-    int64_t hostListCap[3] = {0, 0, 0};
 };
 
 namespace {
@@ -358,6 +358,8 @@ extern "C" void mp_engine_free(mp_engine *e) {
         if (e->stage[k]) (void)hipHostFree(e->stage[k]);
         if (e->stageEv[k]) (void)hipEventDestroy(e->stageEv[k]);
     }
+    for (int l = 0; l < 3; l++)
+        if (e->listHost[l]) (void)hipHostFree(e->listHost[l]);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1301,22 +1303,33 @@ extern "C" mp_status mp_batch_poa_augment(mp_batch *b, mp_poa *const *poas, int6
     }
     mp_status st = materialize(b);
     if (st) return st;
+    // the three lists straight into the engine's pinned buffers (one DMA each, no staging copy)
+    mp_engine *e = b->eng;
+    HIP_TRY(hipSetDevice(e->device));
     const mp_aligned_pair *lists[3];
     const int64_t *starts[3];
     for (int l = 0; l < 3; l++) {
-        const int64_t n = b->listStart[l][b->npairs];
-        if (n > b->hostListCap[l]) {
-            b->hostList[l].reset(new (std::nothrow) mp_aligned_pair[n]);
-            b->hostListCap[l] = b->hostList[l] ? n : 0;
-            if (!b->hostList[l]) {
-                mp::set_error("mp_batch_poa_augment: out of host memory");
+        const size_t bytes = sizeof(mp_aligned_pair) * (size_t)std::max<int64_t>(b->listStart[l][b->npairs], 1);
+        if (bytes > e->listHostBytes[l]) {
+            const size_t want = std::max(bytes, e->listHostBytes[l] + e->listHostBytes[l] / 4);
+            if (e->listHost[l]) HIP_TRY(hipHostFree(e->listHost[l]));
+            e->listHost[l] = nullptr;
+            e->listHostBytes[l] = 0;
+            if (hipHostMalloc(&e->listHost[l], want, hipHostMallocDefault) != hipSuccess) {
+                e->listHost[l] = nullptr;
+                mp::set_error("mp_batch_poa_augment: pinned host allocation failed");
                 return MP_ENOMEM;
             }
+            e->listHostBytes[l] = want;
         }
-        if ((st = copy_list(b, l, b->hostList[l].get(), n))) return st;
-        lists[l] = b->hostList[l].get();
+        const int64_t n = b->listStart[l][b->npairs];
+        if (n > 0)
+            HIP_TRY(hipMemcpyAsync(e->listHost[l], b->dList[l].p, sizeof(mp_aligned_pair) * (size_t)n,
+                                   hipMemcpyDeviceToHost, e->stream));
+        lists[l] = static_cast<const mp_aligned_pair *>(e->listHost[l]);
         starts[l] = b->listStart[l].data();
     }
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return mp::poa_augment_pairs(poas, nPoas, pairPoa, pairReadNo, reads, b->pairStrand.data(), b->npairs, lists,
                                  starts, compareRepeatCounts != 0, threads);
 }

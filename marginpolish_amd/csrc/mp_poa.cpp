@@ -46,17 +46,18 @@ struct Str {
 
 struct InsEntry {
     uint64_t hash;
+    int64_t node;
     int64_t sym, len;  // placed string in the graph's symbol pool
-    double wF = 0.0, wR = 0.0;
-    int32_t next = -1;  // next entry of the same node
-    std::vector<mp_poa_observation> obs;
+    double wF, wR;
+    int32_t next;      // next entry of the same node (-1: last)
 };
+
+inline uint64_t slot_key(uint64_t hash, int64_t node) { return hash ^ ((uint64_t)node * 0x9E3779B97F4A7C15ULL); }
 
 struct DelEntry {
     int64_t len;
-    double wF = 0.0, wR = 0.0;
-    int32_t next = -1;
-    std::vector<mp_poa_observation> obs;
+    double wF, wR;
+    int32_t next;
 };
 
 // A complete insert of one read, placed: node `pos`, the read's runs
@@ -89,13 +90,19 @@ struct mp_poa {
     std::vector<char> nodeBase;
     std::vector<uint64_t> nodeRC;
     std::vector<double> baseW, rcW;  // [node][5], [node][maxRC]
-    std::vector<int32_t> obsNode;    // node observations in append order
-    std::vector<mp_poa_observation> obs;
+    // node observations: the match tuples of every augment call in order, and
+    // per call (readNo, index of its first tuple); node = x + 1, offset = y
+    std::vector<mp_aligned_pair> obsPairs;
+    std::vector<std::pair<int64_t, int64_t>> obsSeg;
     std::vector<int32_t> insHead, insTail, delHead, delTail;
     std::vector<InsEntry> ins;
+    std::vector<int32_t> insTab;  // open-addressing index of ins by (node, string); -1 = empty
     std::vector<char> insB;
     std::vector<uint64_t> insC;
     std::vector<DelEntry> del;
+    // insert / delete observations in append order, with the entry each belongs to
+    std::vector<int32_t> insObsEnt, delObsEnt;
+    std::vector<mp_poa_observation> insObs, delObs;
 
     Str ref() const { return Str{refB.data(), refC.data(), (int64_t)refB.size()}; }
     int64_t nodes() const { return (int64_t)nodeBase.size(); }
@@ -283,48 +290,65 @@ void read_events(const Str &ref, const Str &read, const mp_aligned_pair *m, int6
     }
 }
 
-// Graph updates of one read, in the reference's order: matches, inserts, deletes.
-void apply_read(mp_poa *g, const Str &read, bool strand, int64_t readNo, const mp_aligned_pair *m, int64_t nm,
-                const std::vector<InsEvent> &iev, const std::vector<DelEvent> &dev) {
+// Graph updates of one read, in the reference's order: matches, inserts,
+// deletes.  The three touch disjoint parts of the graph, so for a sequence
+// of reads each part may be brought up to date on its own.
+void apply_matches(mp_poa *g, const Str &read, int64_t readNo, const mp_aligned_pair *m, int64_t nm) {
     const int64_t mrc = g->maxRC;
+    if (nm > 0) {
+        g->obsSeg.emplace_back(readNo, (int64_t)g->obsPairs.size());
+        g->obsPairs.insert(g->obsPairs.end(), m, m + nm);
+    }
+    double *bw = g->baseW.data(), *rw = g->rcW.data();
     for (int64_t i = 0; i < nm; i++) {
         const int64_t node = (int64_t)m[i].x + 1, y = m[i].y;
         const double w = (double)m[i].weight;
-        g->baseW[node * kAlphabet + base_symbol(read.b[y])] += w;
+        bw[node * kAlphabet + base_symbol(read.b[y])] += w;
         const uint64_t rc = read.c[y] >= (uint64_t)mrc ? (uint64_t)(mrc - 1) : read.c[y];
-        g->rcW[node * mrc + (int64_t)rc] += w;
-        g->obsNode.push_back((int32_t)node);
-        g->obs.push_back(mp_poa_observation{readNo, y, w});
+        rw[node * mrc + (int64_t)rc] += w;
     }
+}
+
+void apply_inserts(mp_poa *g, const Str &read, bool strand, int64_t readNo, const std::vector<InsEvent> &iev) {
     for (const InsEvent &e : iev) {
         const char *sb = read.b + e.start;
         const uint64_t *sc = read.c + e.start;
+        // the node's entry with the same run-length string, through the (node, string) index
+        if (2 * (g->ins.size() + 1) > g->insTab.size()) {
+            const size_t cap = std::max<size_t>(1024, 2 * g->insTab.size());
+            g->insTab.assign(cap, -1);
+            for (size_t q = 0; q < g->ins.size(); q++) {
+                size_t h = slot_key(g->ins[q].hash, g->ins[q].node) & (cap - 1);
+                while (g->insTab[h] >= 0) h = (h + 1) & (cap - 1);
+                g->insTab[h] = (int32_t)q;
+            }
+        }
+        const size_t mask = g->insTab.size() - 1;
+        size_t slot = slot_key(e.hash, e.pos) & mask;
         int32_t hit = -1;
-        for (int32_t q = g->insHead[e.pos]; q >= 0; q = g->ins[q].next) {
-            const InsEntry &E = g->ins[q];
-            if (E.hash != e.hash || E.len != e.len) continue;
+        for (; g->insTab[slot] >= 0; slot = (slot + 1) & mask) {
+            const InsEntry &E = g->ins[g->insTab[slot]];
+            if (E.hash != e.hash || E.node != e.pos || E.len != e.len) continue;
             bool same = true;
             for (int64_t t = 0; t < e.len && same; t++) {
                 const int64_t src = (t - e.rot + e.len) % e.len;
                 same = g->insB[E.sym + t] == sb[src] && g->insC[E.sym + t] == sc[src];
             }
             if (same) {
-                hit = q;
+                hit = g->insTab[slot];
                 break;
             }
         }
         if (hit < 0) {
             hit = (int32_t)g->ins.size();
-            InsEntry E;
-            E.hash = e.hash;
-            E.sym = (int64_t)g->insB.size();
-            E.len = e.len;
+            g->insTab[slot] = hit;
+            const InsEntry E{e.hash, e.pos, (int64_t)g->insB.size(), e.len, 0.0, 0.0, -1};
             for (int64_t t = 0; t < e.len; t++) {
                 const int64_t src = (t - e.rot + e.len) % e.len;
                 g->insB.push_back(sb[src]);
                 g->insC.push_back(sc[src]);
             }
-            g->ins.push_back(std::move(E));
+            g->ins.push_back(E);
             if (g->insTail[e.pos] >= 0)
                 g->ins[g->insTail[e.pos]].next = hit;
             else
@@ -333,8 +357,12 @@ void apply_read(mp_poa *g, const Str &read, bool strand, int64_t readNo, const m
         }
         InsEntry &E = g->ins[hit];
         (strand ? E.wF : E.wR) += e.w;
-        E.obs.push_back(mp_poa_observation{readNo, e.start, e.w});
+        g->insObsEnt.push_back(hit);
+        g->insObs.push_back(mp_poa_observation{readNo, e.start, e.w});
     }
+}
+
+void apply_deletes(mp_poa *g, bool strand, int64_t readNo, const std::vector<DelEvent> &dev) {
     for (const DelEvent &e : dev) {
         int32_t hit = -1;
         for (int32_t q = g->delHead[e.pos]; q >= 0; q = g->del[q].next)
@@ -344,9 +372,7 @@ void apply_read(mp_poa *g, const Str &read, bool strand, int64_t readNo, const m
             }
         if (hit < 0) {
             hit = (int32_t)g->del.size();
-            DelEntry E;
-            E.len = e.len;
-            g->del.push_back(std::move(E));
+            g->del.push_back(DelEntry{e.len, 0.0, 0.0, -1});
             if (g->delTail[e.pos] >= 0)
                 g->del[g->delTail[e.pos]].next = hit;
             else
@@ -355,7 +381,8 @@ void apply_read(mp_poa *g, const Str &read, bool strand, int64_t readNo, const m
         }
         DelEntry &E = g->del[hit];
         (strand ? E.wF : E.wR) += e.w;
-        E.obs.push_back(mp_poa_observation{readNo, e.off, e.w});
+        g->delObsEnt.push_back(hit);
+        g->delObs.push_back(mp_poa_observation{readNo, e.off, e.w});
     }
 }
 
@@ -422,16 +449,29 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
             read_events(ref, rd, m, nm, gy, ny, gx, nx, cmp, S, iev[i], dev[i]);
         }
     };
-    // phase 2: each graph applies its reads in order
+    // phase 2: per graph, its matches, its inserts and its deletes (three
+    // independent tasks), each applying the graph's reads in batch order
+    std::vector<std::pair<int64_t, int>> tasks;
+    for (int kind : {1, 0, 2})
+        for (int64_t p : poaOrder) tasks.emplace_back(p, kind);
     auto phase2 = [&]() {
-        for (int64_t k; (k = nextPoa.fetch_add(1)) < (int64_t)poaOrder.size();) {
-            mp_poa *g = poas[poaOrder[k]];
-            for (int64_t i : byPoa[poaOrder[k]]) {
+        for (int64_t k; (k = nextPoa.fetch_add(1)) < (int64_t)tasks.size();) {
+            const int64_t p = tasks[k].first;
+            mp_poa *g = poas[p];
+            if (tasks[k].second == 0) {
+                int64_t nmTot = 0;
+                for (int64_t i : byPoa[p]) nmTot += starts[0][i + 1] - starts[0][i];
+                g->obsPairs.reserve(g->obsPairs.size() + nmTot);
+            }
+            for (int64_t i : byPoa[p]) {
                 const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
-                apply_read(g, rd, strand[i] != 0, pairReadNo ? pairReadNo[i] : i, lists[0] + starts[0][i],
-                           starts[0][i + 1] - starts[0][i], iev[i], dev[i]);
-                std::vector<InsEvent>().swap(iev[i]);
-                std::vector<DelEvent>().swap(dev[i]);
+                const int64_t rn = pairReadNo ? pairReadNo[i] : i;
+                if (tasks[k].second == 0)
+                    apply_matches(g, rd, rn, lists[0] + starts[0][i], starts[0][i + 1] - starts[0][i]);
+                else if (tasks[k].second == 1)
+                    apply_inserts(g, rd, strand[i] != 0, rn, iev[i]);
+                else
+                    apply_deletes(g, strand[i] != 0, rn, dev[i]);
             }
         }
     };
@@ -446,7 +486,7 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
             return MP_EINVAL;
         }
         th.clear();
-        const int t2 = (int)std::min<int64_t>(threads, (int64_t)poaOrder.size());
+        const int t2 = (int)std::min<int64_t>(threads, (int64_t)tasks.size());
         for (int t = 1; t < t2; t++) th.emplace_back(phase2);
         phase2();
         for (auto &t : th) t.join();
@@ -516,7 +556,9 @@ extern "C" mp_status mp_poa_augment(mp_poa *poa, const mp_rle_string *read, int3
         std::vector<DelEvent> dev;
         read_events(poa->ref(), rd, matches, nMatches, inserts, nInserts, deletes, nDeletes, compareRepeatCounts != 0,
                     S, iev, dev);
-        apply_read(poa, rd, readStrand != 0, readNo, matches, nMatches, iev, dev);
+        apply_matches(poa, rd, readNo, matches, nMatches);
+        apply_inserts(poa, rd, readStrand != 0, readNo, iev);
+        apply_deletes(poa, readStrand != 0, readNo, dev);
     } catch (const std::bad_alloc &) {
         mp::set_error("mp_poa_augment: out of host memory");
         return MP_ENOMEM;
@@ -534,11 +576,9 @@ extern "C" mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *s) {
     s->inserts = (int64_t)poa->ins.size();
     s->insertSymbols = (int64_t)poa->insB.size();
     s->deletes = (int64_t)poa->del.size();
-    s->nodeObservations = (int64_t)poa->obs.size();
-    s->insertObservations = 0;
-    for (const InsEntry &e : poa->ins) s->insertObservations += (int64_t)e.obs.size();
-    s->deleteObservations = 0;
-    for (const DelEntry &e : poa->del) s->deleteObservations += (int64_t)e.obs.size();
+    s->nodeObservations = (int64_t)poa->obsPairs.size();
+    s->insertObservations = (int64_t)poa->insObs.size();
+    s->deleteObservations = (int64_t)poa->delObs.size();
     return MP_OK;
 }
 
@@ -605,12 +645,25 @@ extern "C" mp_status mp_poa_get_observations(const mp_poa *poa, int32_t kind, in
     }
     if (kind == 0) {  // stable counting sort of the append-order observations by node
         const int64_t n = poa->nodes();
+        const std::vector<mp_aligned_pair> &op = poa->obsPairs;
         std::vector<int64_t> cnt(n + 1, 0);
-        for (int32_t v : poa->obsNode) cnt[v + 1]++;
-        for (int64_t k = 0; k < n; k++) cnt[k + 1] += cnt[k];
+        for (const mp_aligned_pair &t : op) cnt[(int64_t)t.x + 1]++;
+        int64_t run = 0;
+        for (int64_t k = 0; k <= n; k++) {
+            const int64_t c = cnt[k];
+            cnt[k] = run;
+            run += c;
+        }
         std::memcpy(start, cnt.data(), (n + 1) * sizeof(int64_t));
+        start[n] = run;
         if (obs)
-            for (size_t i = 0; i < poa->obs.size(); i++) obs[cnt[poa->obsNode[i]]++] = poa->obs[i];
+            for (size_t sg = 0; sg < poa->obsSeg.size(); sg++) {
+                const int64_t a = poa->obsSeg[sg].second;
+                const int64_t e = sg + 1 < poa->obsSeg.size() ? poa->obsSeg[sg + 1].second : (int64_t)op.size();
+                const int64_t readNo = poa->obsSeg[sg].first;
+                for (int64_t i = a; i < e; i++)
+                    obs[cnt[(int64_t)op[i].x + 1]++] = mp_poa_observation{readNo, op[i].y, (double)op[i].weight};
+            }
         return MP_OK;
     }
     std::vector<std::pair<int64_t, int32_t>> ord;
@@ -618,14 +671,15 @@ extern "C" mp_status mp_poa_get_observations(const mp_poa *poa, int32_t kind, in
         list_order(poa->insHead, poa->ins, ord);
     else
         list_order(poa->delHead, poa->del, ord);
-    int64_t at = 0;
-    for (size_t k = 0; k < ord.size(); k++) {
-        const std::vector<mp_poa_observation> &v =
-            kind == 1 ? poa->ins[ord[k].second].obs : poa->del[ord[k].second].obs;
-        start[k] = at;
-        if (obs && !v.empty()) std::memcpy(obs + at, v.data(), v.size() * sizeof(mp_poa_observation));
-        at += (int64_t)v.size();
-    }
-    start[ord.size()] = at;
+    const std::vector<int32_t> &ent = kind == 1 ? poa->insObsEnt : poa->delObsEnt;
+    const std::vector<mp_poa_observation> &ob = kind == 1 ? poa->insObs : poa->delObs;
+    // entry -> read-out position, then a stable counting sort of the observations by it
+    std::vector<int64_t> rank(ord.size()), cnt(ord.size() + 1, 0);
+    for (size_t k = 0; k < ord.size(); k++) rank[ord[k].second] = (int64_t)k;
+    for (int32_t q : ent) cnt[rank[q] + 1]++;
+    for (size_t k = 0; k < ord.size(); k++) cnt[k + 1] += cnt[k];
+    std::memcpy(start, cnt.data(), (ord.size() + 1) * sizeof(int64_t));
+    if (obs)
+        for (size_t i = 0; i < ob.size(); i++) obs[cnt[rank[ent[i]]]++] = ob[i];
     return MP_OK;
 }
