@@ -26,6 +26,7 @@ WORKLOADS = {
                "anchor expansion 10, allParams.np.json, one POA realign iteration",
 }
 FLOPS_PER_CELL = 42          # forward 18 + backward 18 + posterior 6 (FMA = 2)
+FLOPS_PER_CELL_FWD = 18      # forward only
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
 PARAMS = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.json")
 
@@ -43,9 +44,10 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=1200)
     ap.add_argument("--err-pairs", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("config1", "config2", "config4"), default="config1",
-                    help="config1 (the headline batch), config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks) "
-                         "or config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at one --band)")
+    ap.add_argument("--workload", choices=("config1", "config2", "config4", "bubbles"), default="config1",
+                    help="config1 (the headline batch), config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks), "
+                         "config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at one --band) or "
+                         "bubbles (bubble-graph per-read x per-allele forward likelihoods)")
     ap.add_argument("--band", type=int, default=128, help="config4: anchor expansion W (128, 256, 512, 1024)")
     ap.add_argument("--pairs", type=int, default=256, help="config4: number of 100 kb pairs")
     ap.add_argument("--max-chunks", type=int, default=None, help="limit the number of chunks (config2)")
@@ -63,6 +65,8 @@ def main():
 
     if args.workload == "config4":
         return run_config4(args)
+    if args.workload == "bubbles":
+        return run_bubbles(args)
 
     if args.copy_calib_mib > 0:
         import torch
@@ -390,6 +394,111 @@ def run_config4(args):
             cpu[tag + "_ll_drift"] = abs(g[0][3] - o[3])
         out["cpu_baseline"] = cpu
         log("cpu sample: %.1f s, %s" % (tc, json.dumps(cpu)))
+    print(json.dumps(out), flush=True)
+
+
+def run_bubbles(args):
+    """Bubble-graph likelihoods (bubbleGraph.c:895-928): 4000 bubbles x 60
+    read substrings x 2-4 alleles, every (allele, distinct read) pair's
+    forward log-likelihood in one forward-only launch.  value = allele-read
+    likelihoods per second of the whole call (host string handling + upload
+    + kernel + download); the kernel line is the same pairs as a resident
+    batch."""
+    import numpy as np
+    from marginpolish_amd import pairhmm, sim
+    t0 = time.time()
+    bub = sim.bubbles(n_bubbles=4000, depth=60, seed=5)
+    n_sup = sum(len(a) * len(r) for a, r in bub)
+    log("bubbles: %d bubbles, %d supports, built in %.1f s" % (len(bub), n_sup, time.time() - t0))
+    pp = pairhmm.PolishParams(PARAMS)
+    eng = pairhmm.default_engine()
+    pk = pairhmm.pack_bubbles(bub)  # the strings as a C caller holds them (timed region starts after)
+    calls = []
+    for _ in range(1 + max(1, args.warmup)):
+        t1 = time.perf_counter()
+        flat, npairs = pairhmm.bubble_allele_supports_packed(pp.machines, pp.p, pk, engine=eng)
+        calls.append(time.perf_counter() - t1)
+    t_call = min(calls[1:])
+    got, at = [], 0
+    for alleles, reads in bub:
+        got.append(flat[at:at + len(alleles) * len(reads)].reshape(len(alleles), len(reads)))
+        at += len(alleles) * len(reads)
+    # the same (allele, distinct read) pairs as a resident batch, for the kernel's own rate
+    pairs = []
+    for alleles, reads in bub:
+        seen = set()
+        asym = [pairhmm.rleString_constructSymbolString(a) for a in alleles]
+        for r, strand in reads:
+            key = (bytes(r[0]), tuple(int(x) for x in r[1]))
+            if key in seen:
+                continue
+            seen.add(key)
+            rs = pairhmm.rleString_constructSymbolString(r)
+            pairs.extend((a, rs, None, strand) for a in asym)
+    assert len(pairs) == npairs
+    t4 = time.perf_counter()
+    b = pairhmm.Batch(pp.machines, pairs, pp.p, mode=2, engine=eng)
+    t_create = time.perf_counter() - t4
+    b.run()
+    kms = []
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        kms.append(b.kernel_ms)
+    ms_per_step = (time.perf_counter() - t2) * 1e3 / args.steps
+    kernel_ms = float(np.mean(kms))
+    cells = b.cells
+    log("bubbles: call %.3f s (%d forward pairs), batch create from Python %.3f s, kernel %.3f ms, %.1f GCUPS"
+        % (t_call, npairs, t_create, kernel_ms, cells / kernel_ms / 1e6))
+    achieved = FLOPS_PER_CELL_FWD * cells / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": "bubble-graph allele-read forward log-likelihoods per second", "value": round(n_sup / t_call, 1),
+        "unit": "likelihoods/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_call * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "bubbles: 4000 bubbles of 8-40 bp (+6 bp flanks), 2-4 alleles, 60 read substrings at "
+                               "9% error (20% repeated), allParams.np.json, forward only, no anchors",
+                   "bubbles": len(bub), "supports": n_sup, "forward_pairs": npairs, "cells": int(cells)},
+        "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(cells / (kernel_ms * 1e-3) / 1e9, 3),
+        "batch_ms_per_step": round(ms_per_step, 4), "batch_create_s": round(t_create, 4),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "flops_per_cell": FLOPS_PER_CELL_FWD},
+    }
+    # parity on a sample against the FP64 oracle (computeForwardProbability restated)
+    from oracle import oracle as O
+    op = O.PolishParams(PARAMS)
+    worst = 0.0
+    for (alleles, reads), g in list(zip(bub, got))[:40]:
+        first = {}
+        for k, (r, strand) in enumerate(reads):
+            key = (bytes(r[0]), tuple(int(x) for x in r[1]))
+            if key in first:
+                continue
+            first[key] = k
+            m = op.forward if strand else op.reverse
+            rs = pairhmm.rleString_constructSymbolString(r)
+            for j, a in enumerate(alleles):
+                want = O.forward_prob(m, pairhmm.rleString_constructSymbolString(a), rs, [], op.p)
+                worst = max(worst, abs(float(g[j, k]) - want) / max(1.0, abs(want)))
+    out["max_rel_ll_err_vs_fp64_oracle"] = worst
+    if not args.no_cpu:
+        samp = pairs[:: max(1, len(pairs) // 20000)][:20000]
+        mods = (op.reverse, op.forward)
+        t3 = time.perf_counter()
+        c_cells = 0
+        for a, rs, _, strand in samp:
+            O.forward_prob(mods[strand], a, rs, [], op.p, mode=O.LOGADD_POLY)
+        tc = time.perf_counter() - t3
+        bs = pairhmm.Batch(pp.machines, samp, pp.p, mode=2, engine=eng)
+        c_cells = bs.cells
+        bs.close()
+        out["cpu_baseline"] = {"value": round(len(samp) * (n_sup / npairs) / tc, 1), "unit": "likelihoods/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "%d of %d forward pairs (%d cells), computeForwardProbability restated in C "
+                                         "(FP64 log space, polynomial logAdd), one thread, called per pair"
+                                         % (len(samp), npairs, c_cells),
+                               "gcups": round(c_cells / tc / 1e9, 5)}
     print(json.dumps(out), flush=True)
 
 

@@ -425,8 +425,11 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     };
     struct PairPrep {
         int64_t first = 0, end = 0;
-        std::vector<int64_t> sp;  // split points (x1, y1, x2, y2)
-        int err = 0;              // 1 bad arguments, 2 bad anchors, 3 sub-problem too long
+        int64_t one[4];             // the split tuple when there is exactly one (no allocation)
+        std::vector<int64_t> more;  // split points (x1, y1, x2, y2) when there are several
+        int32_t nsp = 0;            // number of sub-problems
+        int err = 0;                // 1 bad arguments, 2 bad anchors, 3 sub-problem too long
+        const int64_t *sp() const { return more.empty() ? one : more.data(); }
     };
     std::vector<PairPrep> prep(npairs);
     parallel_for(npairs, [&](int64_t i) {
@@ -450,15 +453,24 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
         const int64_t lX = pp.end - pp.first, lY = pr.lY;
         if (mode == MP_MODE_FORWARD_ONLY) {  // computeForwardProbability does not split
-            pp.sp.assign({0, 0, lX, lY});
+            pp.one[0] = pp.one[1] = 0;
+            pp.one[2] = lX;
+            pp.one[3] = lY;
+            pp.nsp = 1;
         } else {
             std::vector<int64_t> anc(pr.anchors, pr.anchors + 3 * pr.nAnchors);
             for (int64_t a = 0; a < pr.nAnchors; a++) anc[3 * a] -= pp.first;
             mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
-                             pr.raggedRight != 0, pp.sp);
+                             pr.raggedRight != 0, pp.more);
+            pp.nsp = (int32_t)(pp.more.size() / 4);
+            if (pp.nsp == 1) {
+                std::copy(pp.more.begin(), pp.more.end(), pp.one);
+                std::vector<int64_t>().swap(pp.more);
+            }
         }
-        for (size_t s = 0; s + 3 < pp.sp.size(); s += 4)
-            if ((pp.sp[s + 2] - pp.sp[s]) + (pp.sp[s + 3] - pp.sp[s + 1]) >= kMaxDiagonals) pp.err = 3;
+        const int64_t *spp = pp.sp();
+        for (int32_t s = 0; s < pp.nsp; s++)
+            if ((spp[4 * s + 2] - spp[4 * s]) + (spp[4 * s + 3] - spp[4 * s + 1]) >= kMaxDiagonals) pp.err = 3;
     });
     for (int64_t i = 0; i < npairs; i++) {
         if (!prep[i].err) continue;
@@ -486,9 +498,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     for (int64_t i = 0; i < npairs; i++) {
         symOff[i + 1] = symOff[i] + 2 + (prep[i].end - prep[i].first) + pairs[i].lY;
         b->pairFirstJob[i] = njobs;
-        b->pairNJobs[i] = (int32_t)(prep[i].sp.size() / 4);
+        b->pairNJobs[i] = prep[i].nsp;
         b->pairCrop[i] = prep[i].first;
-        njobs += (int64_t)prep[i].sp.size() / 4;
+        njobs += prep[i].nsp;
     }
     if (symOff[npairs] + kSymPad >= ((int64_t)1 << 30)) {
         delete b;
@@ -515,14 +527,15 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         *o++ = enc_y(4, strand);
         const int64_t baseY = baseX + (pp.end - pp.first) + 1;
         for (int64_t y = 0; y < pr.lY; y++) *o++ = enc_y(pr.sY[y], strand);
-        const int64_t ns = (int64_t)pp.sp.size() / 4;
+        const int64_t ns = pp.nsp;
+        const int64_t *spp = pp.sp();
         int64_t j = 0;
         for (int64_t s = 0; s < ns; s++) {
-            const int64_t x1 = pp.sp[4 * s], y1 = pp.sp[4 * s + 1], x2 = pp.sp[4 * s + 2], y2 = pp.sp[4 * s + 3];
+            const int64_t x1 = spp[4 * s], y1 = spp[4 * s + 1], x2 = spp[4 * s + 2], y2 = spp[4 * s + 3];
             std::vector<int64_t> &sub = jobAnchors[b->pairFirstJob[i] + s];
             const int64_t j0 = j;
             while (j < pr.nAnchors && (pr.anchors[3 * j] - pp.first) + pr.anchors[3 * j + 1] < x2 + y2) j++;
-            sub.resize(3 * (size_t)(j - j0));
+            if (j > j0) sub.resize(3 * (size_t)(j - j0));
             for (int64_t a = j0; a < j; a++) {
                 sub[3 * (a - j0)] = pr.anchors[3 * a] - pp.first - x1;
                 sub[3 * (a - j0) + 1] = pr.anchors[3 * a + 1] - y1;
@@ -554,18 +567,30 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // every entry of a job's region is written by its band pass below (anti-diagonals 0..D, then padding)
     std::unique_ptr<int32_t[]> bandBuf(new int32_t[(size_t)bandOff[nj]]);
     int32_t *band = bandBuf.get();
-    std::vector<std::vector<DevWin>> wins(nj);
+    // each job's windows live in its thread's pool: (thread, offset, count) per job
+    struct WinRef {
+        int32_t t, n;
+        int64_t off;
+    };
+    std::vector<WinRef> winRef(nj);
+    const int nthW = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<DevWin>> winPool(nthW);
     std::atomic<int> err{0};
     {
-        const int nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        const int nth = nthW;
         std::vector<std::thread> th;
         std::atomic<int64_t> next{0};
         for (int t = 0; t < nth; t++)
-            th.emplace_back([&]() {
+            th.emplace_back([&, t]() {
                 std::vector<int32_t> L, R, XL;
                 std::vector<mp::Window> w;
-                for (;;) {
-                    int64_t j = next.fetch_add(1);
+                std::vector<DevWin> wj;
+                std::vector<DevWin> &pool = winPool[t];
+                for (int64_t j = 0, jEnd = 0;; j++) {
+                    if (j >= jEnd) {  // claim jobs in runs of 32 (one shared counter)
+                        j = next.fetch_add(32);
+                        jEnd = std::min<int64_t>(j + 32, nj);
+                    }
                     if (j >= nj) break;
                     HostJob &hj = jobs[j];
                     const int64_t D = hj.lX + hj.lY;
@@ -634,16 +659,18 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     w.clear();
                     if (D > 0 && mode == MP_MODE_FORWARD_ONLY) w.push_back(mp::Window{(int32_t)D, 0, (int32_t)D});
                     else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, w);
-                    wins[j].clear();
-                    for (auto &x : w) wins[j].push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
-                    if (split && narrow) choose_crossings(wins[j]);
+                    wj.clear();
+                    for (auto &x : w) wj.push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
+                    if (split && narrow) choose_crossings(wj);
+                    winRef[j] = WinRef{t, (int32_t)wj.size(), (int64_t)pool.size()};
+                    pool.insert(pool.end(), wj.begin(), wj.end());
                     // live ring entries.  Narrow kernels (concurrent sweeps): forward and
                     // backward values of (lo_k, tb_{k+1}].  Wide kernels (the backward of
                     // window k runs after its forward): forward values of [lo_k, tb_k].
                     int64_t need = 1;
-                    for (size_t k = 0; k < wins[j].size(); k++) {
-                        const int64_t hi = narrow ? wins[j][std::min(k + 1, wins[j].size() - 1)].tb : wins[j][k].tb;
-                        need = std::max<int64_t>(need, hi - wins[j][k].lo + (narrow ? 3 * kK : 2));
+                    for (size_t k = 0; k < wj.size(); k++) {
+                        const int64_t hi = narrow ? wj[std::min(k + 1, wj.size() - 1)].tb : wj[k].tb;
+                        need = std::max<int64_t>(need, hi - wj[k].lo + (narrow ? 3 * kK : 2));
                     }
                     hj.ringNeed = (int32_t)((need + kK - 1) / kK * kK);
                 }
@@ -667,26 +694,49 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     }
 
     // ---- 3. device order: by class, then band area descending ----
+    // (class ascending, cells descending, job index ascending): a stable LSD
+    // radix sort of the jobs on (class, maxCells - cells)
     std::vector<int32_t> ord;
-    for (int64_t j = 0; j < nj; j++)
-        if (jobs[j].lX + jobs[j].lY > 0) ord.push_back((int32_t)j);
-    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) {
-        if (jobs[a].cls != jobs[c].cls) return jobs[a].cls < jobs[c].cls;
-        return jobs[a].cells > jobs[c].cells;
-    });
+    {
+        int64_t maxCells = 0;
+        for (int64_t j = 0; j < nj; j++)
+            if (jobs[j].lX + jobs[j].lY > 0) {
+                ord.push_back((int32_t)j);
+                maxCells = std::max(maxCells, jobs[j].cells);
+            }
+        int bitsC = 1;
+        while (bitsC < 60 && ((int64_t)1 << bitsC) <= maxCells) bitsC++;
+        std::vector<uint64_t> key((size_t)nj);
+        for (int32_t j : ord) key[j] = ((uint64_t)jobs[j].cls << bitsC) | (uint64_t)(maxCells - jobs[j].cells);
+        std::vector<int32_t> tmp(ord.size());
+        std::vector<size_t> cnt(2049);
+        for (int sh = 0; sh < bitsC + 2; sh += 11) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int32_t j : ord) cnt[((key[j] >> sh) & 2047) + 1]++;
+            for (int k = 0; k < 2048; k++) cnt[k + 1] += cnt[k];
+            for (int32_t j : ord) tmp[cnt[(key[j] >> sh) & 2047]++] = j;
+            ord.swap(tmp);
+        }
+    }
+    const int64_t nq = (int64_t)ord.size();
     std::vector<int32_t> devIndex(nj, -1);
-    std::vector<DevJob> dj(ord.size());
+    std::vector<DevJob> dj((size_t)nq);
     std::vector<DevWin> &dw = b->winHost;
-    dw.clear();
-    int64_t outOff = 0;
-    int32_t ringLen = 4 * kK;
-    for (size_t q = 0; q < ord.size(); q++) {
+    const bool post = mode <= MP_MODE_MATCH_ONLY;
+    // record capacities (two per posterior anti-diagonal of each side; exact re-run on overflow),
+    // gathered per device slot so the offset pass below reads memory in order
+    struct Slot {
+        int64_t cells, out;
+        int32_t nWin, cls, ringNeed, wmax;
+    };
+    std::vector<Slot> slot((size_t)nq);
+    parallel_for(nq, [&](int64_t q) {
         HostJob &hj = jobs[ord[q]];
-        devIndex[ord[q]] = (int32_t)q;
-        const std::vector<DevWin> &wv = wins[ord[q]];
-        // record capacities: two per posterior anti-diagonal of each side (exact re-run on overflow)
+        const WinRef &wr = winRef[ord[q]];
+        const DevWin *wv = winPool[wr.t].data() + wr.off;
         int64_t nF = 0, nB = 0;
-        for (const DevWin &w : wv) {
+        for (int32_t k = 0; k < wr.n; k++) {
+            const DevWin &w = wv[k];
             if (hj.cls < 3) {
                 nF += std::max(0, w.from - w.cx);
                 nB += std::max(0, std::min(w.cx, w.from) - w.lo);
@@ -694,46 +744,55 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                 nB += w.from - w.lo;
             }
         }
-        const bool post = mode <= MP_MODE_MATCH_ONLY;
         hj.cap[0] = post && hj.cls < 3 ? (int32_t)std::min<int64_t>(2 * nF + 64, INT32_MAX / 4) : 0;
         hj.cap[1] = post ? (int32_t)std::min<int64_t>(2 * nB + 64, INT32_MAX / 4) : 0;
-        hj.outOff = outOff;
-        outOff += (int64_t)hj.cap[0] + hj.cap[1];
-        hj.winOff = (int32_t)dw.size();
-        hj.nWin = (int32_t)wv.size();
-        for (const DevWin &w : wv) dw.push_back(w);
-        if (mode != MP_MODE_FORWARD_ONLY && hj.cls < 3) ringLen = std::max(ringLen, hj.ringNeed);
-        if (hj.cls == 3) {
-            b->wideC = std::max(b->wideC, (hj.wmax + 63) / 64);
-            if (mode != MP_MODE_FORWARD_ONLY) b->ringWideLen = std::max(b->ringWideLen, hj.ringNeed);
+        hj.nWin = wr.n;
+        slot[q] = Slot{hj.cells, (int64_t)hj.cap[0] + hj.cap[1], wr.n, hj.cls, hj.ringNeed, hj.wmax};
+    });
+    // offsets in device order, batch-wide maxima, class ranges
+    std::vector<int64_t> slotOut((size_t)nq), slotWin((size_t)nq);
+    int64_t outOff = 0, winTotal = 0;
+    int32_t ringLen = 4 * kK;
+    for (int c = 0; c < kNumClasses; c++) b->classBegin[c] = b->classEnd[c] = (int32_t)nq;
+    b->wideBegin = b->wideEnd = (int32_t)nq;
+    for (int64_t q = 0; q < nq; q++) {
+        const Slot &sl = slot[q];
+        slotOut[q] = outOff;
+        outOff += sl.out;
+        slotWin[q] = winTotal;
+        winTotal += sl.nWin;
+        if (mode != MP_MODE_FORWARD_ONLY && sl.cls < 3) ringLen = std::max(ringLen, sl.ringNeed);
+        if (sl.cls == 3) {
+            b->wideC = std::max(b->wideC, (sl.wmax + 63) / 64);
+            if (mode != MP_MODE_FORWARD_ONLY) b->ringWideLen = std::max(b->ringWideLen, sl.ringNeed);
+            b->wideBegin = std::min<int32_t>(b->wideBegin, (int32_t)q);
+            b->wideEnd = (int32_t)q + 1;
+        } else {
+            b->classBegin[sl.cls] = std::min<int32_t>(b->classBegin[sl.cls], (int32_t)q);
+            b->classEnd[sl.cls] = (int32_t)q + 1;
         }
+        b->cells += sl.cells;
+    }
+    for (int c = 0; c < kNumClasses; c++)
+        if (b->classBegin[c] > b->classEnd[c]) b->classBegin[c] = b->classEnd[c];
+    dw.resize((size_t)winTotal);
+    b->jobs.resize((size_t)nq);
+    parallel_for(nq, [&](int64_t q) {
+        HostJob &hj = jobs[ord[q]];
+        hj.outOff = slotOut[q];
+        hj.winOff = (int32_t)slotWin[q];
+        devIndex[ord[q]] = (int32_t)q;
+        const WinRef &wr = winRef[ord[q]];
+        std::copy(winPool[wr.t].data() + wr.off, winPool[wr.t].data() + wr.off + wr.n, dw.data() + hj.winOff);
         DevJob &d = dj[q];
         d.symX = hj.symX; d.symY = hj.symY; d.bandOff = hj.bandOff; d.outOff = hj.outOff;
         d.lX = hj.lX; d.lY = hj.lY; d.winOff = hj.winOff; d.nWin = hj.nWin;
         d.cap[0] = hj.cap[0]; d.cap[1] = hj.cap[1];
         d.flags = hj.flags;
         d.pad = 0;
-        b->cells += hj.cells;
-    }
-    for (int c = 0; c < kNumClasses; c++) b->classBegin[c] = b->classEnd[c] = (int32_t)ord.size();
-    for (int c = kNumClasses - 1; c >= 0; c--)
-        for (size_t q = 0; q < ord.size(); q++)
-            if (jobs[ord[q]].cls == c) {
-                b->classBegin[c] = std::min<int32_t>(b->classBegin[c], (int32_t)q);
-                b->classEnd[c] = (int32_t)q + 1;
-            }
-    for (int c = 0; c < kNumClasses; c++)
-        if (b->classBegin[c] > b->classEnd[c]) b->classBegin[c] = b->classEnd[c];
-    b->wideBegin = b->wideEnd = (int32_t)ord.size();
-    for (size_t q = 0; q < ord.size(); q++)
-        if (jobs[ord[q]].cls == 3) {
-            b->wideBegin = std::min<int32_t>(b->wideBegin, (int32_t)q);
-            b->wideEnd = (int32_t)q + 1;
-        }
-    b->order.resize(nj);
-    for (int64_t j = 0; j < nj; j++) b->order[j] = devIndex[j];
-    b->jobs.resize(ord.size());
-    for (size_t q = 0; q < ord.size(); q++) b->jobs[q] = jobs[ord[q]];
+        b->jobs[q] = hj;
+    });
+    b->order.assign(devIndex.begin(), devIndex.end());
     b->ringLen = ringLen;
     b->outTotal = outOff;
 

@@ -28,6 +28,8 @@ import numpy as np
 from . import _lib
 from ._lib import ALIGNED_PAIR_DTYPE, MPError, check, lib  # noqa: F401
 
+MAX_REPEAT_EXCLUSIVE = 51  # MAXIMUM_REPEAT_LENGTH (margin.h:123)
+
 
 @dataclass
 class PairwiseAlignmentParameters:
@@ -434,3 +436,105 @@ def realign_all(params, pairs, engine=None):
     out = [(m, gy, gx) for (m, gx, gy, _) in b.fetch()]
     b.close()
     return out
+
+
+_SYMBOL_OF_CHAR = np.full(256, 4, dtype=np.uint16)
+for _c, _v in zip(b"ACGTacgt", (0, 1, 2, 3, 0, 1, 2, 3)):
+    _SYMBOL_OF_CHAR[_c] = _v
+
+
+def rleString_constructSymbolString(rle, includeRepeatCounts=True, maxRepeatCountExclusive=MAX_REPEAT_EXCLUSIVE):
+    """stateMachine.c:124-146: uint16 symbols (run length << 8 | base) of a
+    run-length string (bases, repeatCounts), run lengths capped at
+    maxRepeatCountExclusive - 1."""
+    from .poa import _as_rle
+    b, c = _as_rle(rle)
+    sym = _SYMBOL_OF_CHAR[b]
+    if includeRepeatCounts:
+        sym = sym | (np.minimum(c, maxRepeatCountExclusive - 1).astype(np.uint16) << 8)
+    return sym.astype(np.uint16)
+
+
+def pack_bubbles(bubbles):
+    """Flatten [(alleles, [(read, strand)])] (run-length strings as (bases,
+    repeatCounts)) into the packed arrays bubble_allele_supports_packed takes."""
+    al = [a for alleles, _ in bubbles for a in alleles]
+    rd = [r for _, reads in bubbles for r, _ in reads]
+
+    def cat(strings):
+        bl = [np.frombuffer(b, np.uint8) if isinstance(b, (bytes, bytearray)) else np.asarray(b, np.uint8)
+              for b, _ in strings]
+        lens = np.array([x.size for x in bl], dtype=np.int64)
+        off = np.zeros(lens.size + 1, dtype=np.int64)
+        np.cumsum(lens, out=off[1:])
+        b = np.concatenate(bl) if bl else np.zeros(0, np.uint8)
+        c = np.concatenate([np.asarray(c, np.uint64) for _, c in strings]) if strings else np.zeros(0, np.uint64)
+        return b, c, off
+
+    ab, ac, aoff = cat(al)
+    rb, rc, roff = cat(rd)
+    a_start = np.zeros(len(bubbles) + 1, dtype=np.int64)
+    r_start = np.zeros(len(bubbles) + 1, dtype=np.int64)
+    np.cumsum([len(a) for a, _ in bubbles], out=a_start[1:])
+    np.cumsum([len(r) for _, r in bubbles], out=r_start[1:])
+    strands = np.array([int(bool(st)) for _, reads in bubbles for _, st in reads] or [0], dtype=np.int32)
+    return {"allele_bases": ab, "allele_counts": ac, "allele_off": aoff, "allele_start": a_start,
+            "read_bases": rb, "read_counts": rc, "read_off": roff, "read_start": r_start, "read_strand": strands}
+
+
+def _rle_records(bases, counts, off):
+    n = off.size - 1
+    rec = np.zeros(max(n, 1), dtype=_lib.RLE_STRING_DTYPE)
+    if n:
+        rec["bases"][:n] = bases.ctypes.data + off[:-1]
+        rec["repeatCounts"][:n] = counts.ctypes.data + 8 * off[:-1]
+        rec["length"][:n] = np.diff(off)
+    return rec
+
+
+def bubble_allele_supports_packed(machines, p, pk, include_repeat_counts=True,
+                                  max_repeat_count_exclusive=MAX_REPEAT_EXCLUSIVE, engine=None):
+    """mp_bubble_allele_supports on packed bubbles (pack_bubbles): returns the
+    flat float32 supports (bubble-major, allele-major within a bubble) and the
+    number of forward computations run."""
+    engine = engine or default_engine()
+    for k in ("allele_bases", "read_bases"):
+        if pk[k].size == 0:
+            pk[k] = np.zeros(1, np.uint8)
+    for k in ("allele_counts", "read_counts"):
+        if pk[k].size == 0:
+            pk[k] = np.zeros(1, np.uint64)
+    A = _rle_records(pk["allele_bases"], pk["allele_counts"], pk["allele_off"])
+    R = _rle_records(pk["read_bases"], pk["read_counts"], pk["read_off"])
+    a_st, r_st = pk["allele_start"], pk["read_start"]
+    n = a_st.size - 1
+    out = np.zeros(max(int((np.diff(a_st) * np.diff(r_st)).sum()), 1), dtype=np.float32)
+    npairs = C.c_int64()
+    pc = p.c()
+    check(lib().mp_bubble_allele_supports(engine.handle, machines.handle, C.byref(pc), n, a_st.ctypes.data,
+                                          A.ctypes.data, r_st.ctypes.data, R.ctypes.data,
+                                          pk["read_strand"].ctypes.data, int(bool(include_repeat_counts)),
+                                          int(max_repeat_count_exclusive), out.ctypes.data, C.byref(npairs)))
+    return out, npairs.value
+
+
+def bubble_allele_supports(machines, p, bubbles, include_repeat_counts=True,
+                           max_repeat_count_exclusive=MAX_REPEAT_EXCLUSIVE, engine=None):
+    """The per-read x per-allele forward log-likelihoods of bubbleGraph.c:895-928
+    for many bubbles in one forward-only launch (mp_bubble_allele_supports).
+
+    bubbles: list of (alleles, reads): alleles a list of run-length strings
+    (bases, repeatCounts), reads a list of ((bases, repeatCounts), strand).
+    Returns one float32 array (alleleNo, readNo) per bubble, and the number of
+    forward computations run (repeated read substrings of a bubble reuse the
+    first one's values, as the reference does)."""
+    pk = pack_bubbles(bubbles)
+    flat, npairs = bubble_allele_supports_packed(machines, p, pk, include_repeat_counts,
+                                                 max_repeat_count_exclusive, engine)
+    a_st, r_st = pk["allele_start"], pk["read_start"]
+    res, at = [], 0
+    for b in range(len(bubbles)):
+        na, nr = int(a_st[b + 1] - a_st[b]), int(r_st[b + 1] - r_st[b])
+        res.append(flat[at:at + na * nr].reshape(na, nr).copy())
+        at += na * nr
+    return res, npairs

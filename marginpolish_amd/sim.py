@@ -259,3 +259,57 @@ def config4(n_pairs=256, length=100_000, seed=4, err=0.12, sv_frac=0.005, W=128,
         keep = (src >= 0) & (pos >= 0)
         pairs.append(_pair(ref, read, src[keep], pos[keep], W, strands[r % len(strands)], r, r))
     return Workload("config4", pairs, W, length, n_pairs)
+
+
+def _rle_pair(bases):
+    rb, runs, _ = _rle(bases)
+    return (_BASE_CHARS[rb].tobytes(), runs.astype(np.uint64))
+
+
+_BASE_CHARS = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+def bubbles(n_bubbles=4000, depth=60, seed=5, err=0.09, region=(8, 40), flank=6, max_alleles=4, dup_frac=0.2):
+    """Bubble-graph likelihood workload (bubbleGraph.c:895-928 shapes): per
+    bubble, a reference region of `region` bases plus `flank` bases each
+    side (runs ~ Geometric(0.45), GC 45%), the reference allele and 1..3
+    alternatives (a substitution, a +-1 run-length change or a 1-3 base
+    indel inside the region), and `depth` read substrings, each a copy of one
+    allele with `err` per-base errors; a `dup_frac` share of reads repeats an
+    earlier read's substring exactly (identical substrings are common in
+    short windows).  Returns [(alleles, reads)] with run-length strings
+    (bases, repeatCounts) and reads ((bases, counts), strand)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_bubbles):
+        L = int(rng.integers(region[0], region[1] + 1)) + 2 * flank
+        ref = _ref_runs(rng, L, 0.45, 0.45)
+        alleles = [ref]
+        for _ in range(int(rng.integers(1, max_alleles))):
+            a = ref.copy()
+            pos = int(rng.integers(flank, L - flank))
+            kind = int(rng.integers(0, 4))
+            if kind == 0:
+                a[pos] = (a[pos] + 1 + int(rng.integers(0, 3))) % 4
+            elif kind == 1:
+                a = np.insert(a, pos, a[pos])
+            elif kind == 2:
+                a = np.delete(a, pos)
+            else:
+                k = int(rng.integers(1, 4))
+                a = np.insert(a, pos, rng.integers(0, 4, size=k).astype(a.dtype))
+            if not any(a.size == b.size and np.array_equal(a, b) for b in alleles):
+                alleles.append(a)
+        reads = []
+        for r in range(depth):
+            if reads and rng.random() < dup_frac:
+                reads.append(reads[int(rng.integers(0, len(reads)))])
+                continue
+            src = alleles[int(rng.integers(0, len(alleles)))]
+            mut, _ = _mutate(rng, src, err)
+            if mut.size == 0:
+                mut = src.copy()
+            reads.append((_rle_pair(mut), int(rng.integers(0, 2))))
+        out.append(([_rle_pair(a) for a in alleles], reads))
+    return out
+
