@@ -134,12 +134,22 @@ def main():
             aug1 = time.time() - tb
             gsz = [g.sizes() for g in graphs]
             del g1
+            # the polished run lengths of every graph (poa_estimateRepeatCountsUsingBayesianModel)
+            by_chunk = [[] for _ in refs]
+            for i in range(len(pairs)):
+                by_chunk[pair_poa[i]].append(i)
+            tr = time.time()
+            for k, g in enumerate(graphs):
+                ids = by_chunk[k]
+                g.estimate_repeat_counts(pp.machines, [rle_reads[i] for i in ids], [w.pairs[i].strand for i in ids])
+            t_rc = time.time() - tr
         bb.close()
         del graphs
     t_aug, t_poa_iter = poa_iters[1]
     log("POA iteration: %.3f s (augment %.3f s on all cores, %.3f s on one thread; %d graphs, %d inserts, "
-        "%d deletes)" % (t_poa_iter, t_aug, aug1, len(refs), sum(s["inserts"] for s in gsz),
-                         sum(s["deletes"] for s in gsz)))
+        "%d deletes); repeat-count estimation %.3f s" % (t_poa_iter, t_aug, aug1, len(refs),
+                                                         sum(s["inserts"] for s in gsz),
+                                                         sum(s["deletes"] for s in gsz), t_rc))
 
     # hardware tiling picked by the autotuner on this batch (launch geometry only; results unchanged)
     tuned = b.autotune(3)
@@ -178,7 +188,7 @@ def main():
         "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),
         "first_iteration_s": round(iters[0][1], 4),
         "poa_iteration_s": round(t_poa_iter, 4), "poa_reads_per_s": round(len(pairs) / t_poa_iter, 1),
-        "augment_s": round(t_aug, 4), "augment_1thread_s": round(aug1, 4),
+        "augment_s": round(t_aug, 4), "augment_1thread_s": round(aug1, 4), "repeat_count_estimation_s": round(t_rc, 4),
         "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
         "plain_sample_pairs": len(samp), "autotuned_narrow_threads": tuned[0],
     }

@@ -88,6 +88,21 @@ mp_status mp_batch_poa_augment(mp_batch *b, mp_poa *const *poas, int64_t nPoas, 
                                const int64_t *pairReadNo, const mp_rle_string *reads, int32_t compareRepeatCounts,
                                int32_t threads);
 
+/* poa_estimateRepeatCountsUsingBayesianModel (polisher.c:1557-1566) with
+   repeatSubMatrix_getMLRepeatCount (1846-1865): for every node after the
+   prefix node, the maximum-likelihood run length of its base given its base
+   observations, each contributing weight / 1e7 times the model's repeat-count
+   log-probability (repeatCountSubstitutionMatrix of the read's strand; an N
+   node uses the A row) of the read's run length at the observed offset,
+   capped at MAXIMUM_REPEAT_LENGTH - 1; candidates run from the smallest to
+   the largest (capped) observed run length, the first maximum wins, and 0 (no
+   observation) becomes 1.  reads[readNo] / readStrand[readNo] are the reads
+   the graph was augmented with (uncapped run lengths).  Writes nodes - 1
+   counts (the polished reference's run lengths). */
+mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_model *m, const mp_rle_string *reads,
+                                        const int32_t *readStrand, int64_t nReads, uint64_t *repeatCounts,
+                                        int32_t threads);
+
 mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *sizes);
 
 /* Per node: base, repeatCount, baseWeights[5] (A C G T N) and

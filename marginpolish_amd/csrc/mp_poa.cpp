@@ -683,3 +683,103 @@ extern "C" mp_status mp_poa_get_observations(const mp_poa *poa, int32_t kind, in
         for (size_t i = 0; i < ob.size(); i++) obs[cnt[rank[ent[i]]]++] = ob[i];
     return MP_OK;
 }
+
+extern "C" mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_model *m, const mp_rle_string *reads,
+                                                   const int32_t *readStrand, int64_t nReads, uint64_t *repeatCounts,
+                                                   int32_t threads) {
+    if (!poa || !m || !repeatCounts || nReads < 0 || (nReads > 0 && (!reads || !readStrand))) {
+        mp::set_error("mp_poa_estimate_repeat_counts: bad argument");
+        return MP_EINVAL;
+    }
+    constexpr int64_t kRL = mp::kMaxRepeat;  // MAXIMUM_REPEAT_LENGTH: the repeat table's bound (exclusive)
+    if ((int64_t)m->repeatLog10.size() < 4 * kRL * kRL) {
+        mp::set_error("mp_poa_estimate_repeat_counts: the model has no repeat-count table");
+        return MP_EINVAL;
+    }
+    const int64_t n = poa->nodes();
+    const std::vector<mp_aligned_pair> &op = poa->obsPairs;
+    // every observation's read must exist and cover its offset
+    for (size_t sg = 0; sg < poa->obsSeg.size(); sg++) {
+        const int64_t rn = poa->obsSeg[sg].first;
+        if (rn < 0 || rn >= nReads || !valid_string(&reads[rn])) {
+            mp::set_error("mp_poa_estimate_repeat_counts: observation of read " + std::to_string(rn) +
+                          " outside the read array");
+            return MP_EINVAL;
+        }
+        const int64_t e = sg + 1 < poa->obsSeg.size() ? poa->obsSeg[sg + 1].second : (int64_t)op.size();
+        for (int64_t i = poa->obsSeg[sg].second; i < e; i++)
+            if (op[i].y < 0 || op[i].y >= reads[rn].length) {
+                mp::set_error("mp_poa_estimate_repeat_counts: observation offset outside read " + std::to_string(rn));
+                return MP_EINVAL;
+            }
+    }
+    try {
+        // each node's observations in append order: (observed run length, strand, weight)
+        struct Obs {
+            uint64_t observed;
+            int32_t strand;
+            double weight;
+        };
+        std::vector<int64_t> start(n + 1, 0);
+        for (const mp_aligned_pair &t : op) start[(int64_t)t.x + 2]++;
+        for (int64_t k = 0; k < n; k++) start[k + 1] += start[k];
+        std::vector<Obs> ob(op.size());
+        for (size_t sg = 0; sg < poa->obsSeg.size(); sg++) {
+            const int64_t rn = poa->obsSeg[sg].first;
+            const int64_t e = sg + 1 < poa->obsSeg.size() ? poa->obsSeg[sg + 1].second : (int64_t)op.size();
+            for (int64_t i = poa->obsSeg[sg].second; i < e; i++)
+                ob[start[(int64_t)op[i].x + 1]++] =
+                    Obs{reads[rn].repeatCounts[op[i].y], readStrand[rn] ? 1 : 0, (double)op[i].weight};
+        }
+        // start[k] is now the end of node k's observations; node k's begin is start[k - 1] (0 for node 0)
+        const double *tab = m->repeatLog10.data();
+        auto node_count = [&](int64_t k) -> uint64_t {
+            const int64_t a = k > 0 ? start[k - 1] : 0, e = start[k];
+            int b = base_symbol(poa->nodeBase[k]);
+            if (b == 4) b = 0;  // N reads the A row (polisher.c:1758)
+            int64_t lo = kRL, hi = 0;
+            for (int64_t i = a; i < e; i++) {
+                lo = std::min<int64_t>(lo, (int64_t)std::min<uint64_t>(ob[i].observed, (uint64_t)INT64_MAX));
+                hi = std::max<int64_t>(hi, (int64_t)std::min<uint64_t>(ob[i].observed, (uint64_t)INT64_MAX));
+            }
+            if (hi >= kRL) hi = kRL - 1;
+            int64_t c;
+            if (lo == kRL) {
+                c = 0;  // no observation below the bound (otherwise lo <= hi)
+            } else {
+                double best = 0.0;
+                c = lo;
+                for (int64_t u = lo; u <= hi; u++) {
+                    double lp = 0.0;
+                    for (int64_t i = a; i < e; i++) {
+                        const int64_t o = ob[i].observed >= (uint64_t)kRL ? kRL - 1 : (int64_t)ob[i].observed;
+                        const int row = ob[i].strand ? b : 3 - b;
+                        lp += tab[(row * kRL + u) * kRL + o] * ob[i].weight;
+                    }
+                    lp /= 10000000.0;
+                    if (u == lo || lp > best) {
+                        best = lp;
+                        c = u;
+                    }
+                }
+            }
+            return c == 0 ? 1 : (uint64_t)c;
+        };
+        if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+        threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, (n + 1023) / 1024));
+        std::atomic<int64_t> next{1};
+        auto work = [&]() {
+            for (int64_t k0; (k0 = next.fetch_add(1024)) < n;)
+                for (int64_t k = k0; k < std::min<int64_t>(n, k0 + 1024); k++) repeatCounts[k - 1] = node_count(k);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < threads; t++) th.emplace_back(work);
+        work();
+        for (auto &t : th) t.join();
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poa_estimate_repeat_counts: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+

@@ -126,6 +126,22 @@ class Poa:
         check(lib().mp_poa_get_observations(self._h, int(kind), start.ctypes.data, obs.ctypes.data))
         return start, obs[:tot]
 
+    def estimate_repeat_counts(self, machines, reads, read_strands, threads=0):
+        """poa_estimateRepeatCountsUsingBayesianModel (polisher.c:1557-1566): the
+        maximum-likelihood run length of every reference node (nodes 1..n) from
+        its base observations; reads[readNo] = (bases, repeatCounts) with
+        uncapped counts, read_strands[readNo] = 1 for forward reads."""
+        keep = [_as_rle(r) for r in reads]
+        rs = np.empty(max(len(keep), 1), dtype=_lib.RLE_STRING_DTYPE)
+        for i, (b, c) in enumerate(keep):
+            rs[i] = (b.ctypes.data, c.ctypes.data, b.size)
+        st = np.ascontiguousarray(np.asarray(list(read_strands) or [0], dtype=np.int32))
+        out = np.zeros(max(self.sizes()["nodes"] - 1, 1), dtype=np.uint64)
+        check(lib().mp_poa_estimate_repeat_counts(self._h, machines.handle, rs.ctypes.data, st.ctypes.data,
+                                                  len(keep), out.ctypes.data, int(threads)))
+        del keep
+        return out[: self.sizes()["nodes"] - 1]
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mp_poa_free(self._h)
@@ -144,6 +160,10 @@ def poa_getReferenceGraph(reference, maxRepeatCount=MAXIMUM_REPEAT_LENGTH):
 
 def poa_augment(poa, read, readStrand, readNo, matches, inserts, deletes, poaConstructCompareRepeatCounts=True):
     poa.augment(read, readStrand, readNo, matches, inserts, deletes, poaConstructCompareRepeatCounts)
+
+
+def poa_estimateRepeatCountsUsingBayesianModel(poa, machines, reads, read_strands):
+    return poa.estimate_repeat_counts(machines, reads, read_strands)
 
 
 def augment_batch(batch, poas, pair_poa, reads, pair_read_no=None, compare_repeat_counts=True, threads=0):

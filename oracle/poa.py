@@ -16,6 +16,8 @@ complete-insert/delete enumeration against a direct definition).
 """
 from __future__ import annotations
 
+import numpy as np
+
 NUC_SYMBOL = {"A": 0, "a": 0, "C": 1, "c": 1, "G": 2, "g": 2, "T": 3, "t": 3}
 ALPHABET_SIZE = 5  # A C G T N (stateMachine.c:63-71)
 
@@ -259,3 +261,47 @@ def realign_augment(ref, max_rc, reads, lists, cmp_counts=True):
     for n, ((r, strand), (m, ins, dels)) in enumerate(zip(reads, lists)):
         augment(poa, r, strand, n, m, ins, dels, cmp_counts)
     return poa
+
+
+MAXIMUM_REPEAT_LENGTH = 51  # margin.h:123
+
+
+def estimate_repeat_counts(poa, reads, strands, repeat, max_rl=MAXIMUM_REPEAT_LENGTH):
+    """poa_estimateRepeatCountsUsingBayesianModel (polisher.c:1557-1566) with
+    repeatSubMatrix_getMLRepeatCount / getLogProbForGivenRepeatCount /
+    getMinAndMaxRepeatCountObservations (1796-1865) and repeatSubMatrix_setLogProb
+    (1756-1770): per node after the prefix, over candidate run lengths from the
+    smallest observed to the largest observed (capped at max_rl - 1), the sum
+    of weight * logProb[strand row][candidate][min(observed, max_rl - 1)] / 1e7,
+    first maximum; no observation -> 0 -> 1.  reads[readNo] = (bases, counts),
+    strands[readNo] true for forward; repeat = [5 x max_rl x max_rl] rows by
+    base (forward strand), [underlying][observed]."""
+    tab = np.asarray(repeat, dtype=np.float64).reshape(5, max_rl, max_rl).tolist()
+    out = []
+    for node in poa.nodes[1:]:
+        b = char_to_symbol(node.base)
+        if b == 4:
+            b = 0
+        obs = node.observations
+        lo, hi = max_rl, 0
+        for rn, off, _ in obs:
+            o = reads[rn][1][off]
+            lo, hi = min(lo, o), max(hi, o)
+        if hi >= max_rl:
+            hi = max_rl - 1
+        if lo == max_rl:
+            c = 0  # no observation below the bound (lo <= hi otherwise)
+        else:
+            best, c = None, lo
+            for u in range(lo, hi + 1):
+                lp = 0.0
+                for rn, off, w in obs:
+                    o = min(reads[rn][1][off], max_rl - 1)
+                    row = b if strands[rn] else 3 - b
+                    lp += tab[row][u][o] * w
+                lp /= 10000000.0
+                if best is None or lp > best:
+                    best, c = lp, u
+        out.append(1 if c == 0 else c)
+    return out
+

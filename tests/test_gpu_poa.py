@@ -95,6 +95,17 @@ def test_batch_augment_matches_oracle(config1_batch):
                   oracle_lists(gx), True)
         g.augment(r, w.pairs[i].strand, read_no[i], m, gy, gx, True)
     assert_same_graph(native_as_oracle(g), o)
+    # the maximum-likelihood run lengths of the graph (poa_estimateRepeatCountsUsingBayesianModel)
+    from oracle.oracle import PolishParams as OP
+    reads_by_no = {int(read_no[i]): reads[i] for i in idx}
+    strands_by_no = {int(read_no[i]): w.pairs[i].strand for i in idx}
+    n = max(reads_by_no) + 1
+    rl = [reads_by_no.get(k, (b"", np.zeros(0, np.uint64))) for k in range(n)]
+    st = [strands_by_no.get(k, 1) for k in range(n)]
+    got = g.estimate_repeat_counts(ph.PolishParams(PARAMS).machines, rl, st)
+    rl_o = [(bytes(r[0]).decode(), tuple(int(c) for c in r[1])) for r in rl]
+    want = P.estimate_repeat_counts(o, rl_o, [bool(v) for v in st], OP(PARAMS).repeat)
+    assert got.tolist() == want
 
 
 def test_batch_augment_refuses_bad_input(config1_batch):

@@ -159,3 +159,43 @@ def test_realistic_lists_match_oracle():
     assert_same_graph(nodes, o)
     assert total > 3000
     assert sum(len(n.inserts) for n in nodes) > 10 and sum(len(n.deletes) for n in nodes) > 10
+
+
+@pytest.mark.parametrize("table", ["params", "random"])
+def test_repeat_count_estimates_match_oracle(table):
+    """poa_estimateRepeatCountsUsingBayesianModel: library vs oracle, bit for
+    bit, on graphs built from FP64-oracle posterior lists of simulated reads
+    (uncapped run lengths, both strands)."""
+    from marginpolish_amd import pairhmm as ph
+    pp = O.PolishParams(PARAMS)
+    mach = ph.PolishParams(PARAMS).machines
+    rep = np.asarray(pp.repeat, dtype=np.float64).reshape(5, 51, 51).copy()
+    if table == "random":
+        rep[:4] = -np.abs(np.random.default_rng(3).normal(2.0, 1.0, size=(4, 51, 51)))
+        mach.set_repeat_log10(rep[:4])
+    w = sim.config0(n_reads=5, ref_len=800, seed=13, strands=(1, 0))
+    ref = mpoa.rle_from_symbols(w.pairs[0].ref_sym, w.pairs[0].ref_runs)
+    ref_o = (ref[0].tobytes().decode(), tuple(int(c) for c in ref[1]))
+    g, o = mpoa.Poa(ref), P.Poa(ref_o, 51)
+    reads, reads_o, strands = [], [], []
+    for r, pr in enumerate(w.pairs):
+        sx, sy, an, first = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        m, gx, gy, _, _ = O.posteriors(pp.forward if pr.strand else pp.reverse, sx, sy, an, pp.p)
+        lists = []
+        for t in (m, gy, gx):
+            t = np.asarray(t, dtype=np.int64).reshape(-1, 3).copy()
+            t[:, 1] += first
+            lists.append([tuple(int(v) for v in row) for row in t])
+        runs = pr.read_runs.copy()
+        runs[::37] += 60  # some overlong observations
+        read = mpoa.rle_from_symbols(pr.read_sym, runs)
+        reads.append(read)
+        reads_o.append((read[0].tobytes().decode(), tuple(int(c) for c in read[1])))
+        strands.append(pr.strand)
+        g.augment(read, pr.strand, r, *lists, True)
+        P.augment(o, reads_o[-1], bool(pr.strand), r, *lists, True)
+    got = g.estimate_repeat_counts(mach, reads, strands)
+    want = P.estimate_repeat_counts(o, reads_o, [bool(s) for s in strands], rep)
+    assert got.tolist() == want
+    assert len(set(want)) > 3
+

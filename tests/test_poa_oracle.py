@@ -154,3 +154,38 @@ def test_weights_are_run_minima_and_observations_follow():
         tot = sum(sum(n.base_weights) for n in poa.nodes)
         assert tot == sum(float(w) for w, _, _ in matches)
         assert tot == sum(sum(n.rc_weights) for n in poa.nodes)
+
+
+def _peaked_table():
+    """log P(observed | underlying) = -|observed - underlying| for every base:
+    the maximum-likelihood run length is then a weighted median."""
+    import numpy as np
+    u = np.arange(51)
+    t = -np.abs(u[None, :] - u[:, None]).astype(float)  # [underlying][observed]
+    return np.stack([t] * 5)
+
+
+def test_repeat_count_estimate_known_answers():
+    """repeatSubMatrix_getMLRepeatCount (polisher.c:1846-1865) on a table whose
+    answer is a weighted median; no observation -> 1; an overlong observation
+    is capped at 50; observations that are all overlong leave the smallest
+    observed count at its initial bound, which the reference treats as "no
+    valid observation" (-> 1)."""
+    ref = ("ACGT", (3, 1, 2, 1))
+    reads = [("A", (2,)), ("A", (3,)), ("A", (3,)), ("A", (5,)), ("C", (80,)), ("G", (60,)), ("G", (70,)),
+             ("C", (3,))]
+    poa = P.Poa(ref, 51)
+    for rn in range(4):
+        P.augment(poa, reads[rn], True, rn, [(10, 0, 0)], [], [])
+    P.augment(poa, reads[4], False, 4, [(5, 1, 0)], [], [])      # node 2: overlong (-> 50) with weight 5
+    P.augment(poa, reads[7], True, 7, [(1, 1, 0)], [], [])       # and 3 with weight 1
+    P.augment(poa, reads[5], True, 5, [(7, 2, 0)], [], [])       # node 3: all overlong
+    P.augment(poa, reads[6], True, 6, [(7, 2, 0)], [], [])
+    strands = [True, True, True, True, False, True, True, True]
+    got = P.estimate_repeat_counts(poa, reads, strands, _peaked_table())
+    assert got == [3, 50, 1, 1]
+    # weights move the median
+    poa = P.Poa(ref, 51)
+    for rn, w in enumerate([1, 1, 1, 100]):
+        P.augment(poa, reads[rn], True, rn, [(w, 0, 0)], [], [])
+    assert P.estimate_repeat_counts(poa, reads, strands, _peaked_table())[0] == 5
