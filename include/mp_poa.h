@@ -103,6 +103,21 @@ mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_model *m, co
                                         const int32_t *readStrand, int64_t nReads, uint64_t *repeatCounts,
                                         int32_t threads);
 
+/* poa_getAnchorAlignments (polisher.c:536-592): anchor pairs for the next
+   realignment from the graph's base observations.  Nodes in order (node i
+   maps to consensus position poaToConsensusMap[i - 1], or i - 1 when the map
+   is null; -1 = unaligned), each node's observations in order: an
+   observation with weight / 1e7 > anchorParams[0] becomes (consensus
+   position, read offset, expansion) of its read, expansion = anchorParams[1]
+   raised to anchorParams[k + 1] for every further threshold anchorParams[k]
+   (k = 2, 4, ...) the weight reaches, appended only when both coordinates
+   exceed the read's previous anchor.  counts[r] receives read r's anchor
+   count (r < nReads); when anchors is not null the anchors (3 int64 each)
+   are written read after read, and capacity (in anchors) must hold them all. */
+mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *poaToConsensusMap, int64_t nReads,
+                                   const double *anchorParams, int64_t nAnchorParams, int64_t *counts,
+                                   int64_t *anchors, int64_t capacity);
+
 mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *sizes);
 
 /* Per node: base, repeatCount, baseWeights[5] (A C G T N) and

@@ -783,3 +783,97 @@ extern "C" mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_m
     return MP_OK;
 }
 
+extern "C" mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *poaToConsensusMap, int64_t nReads,
+                                              const double *anchorParams, int64_t nAnchorParams, int64_t *counts,
+                                              int64_t *anchors, int64_t capacity) {
+    if (!poa || nReads < 0 || !counts || !anchorParams || nAnchorParams < 2 || (nAnchorParams & 1)) {
+        mp::set_error("mp_poa_anchor_alignments: bad argument (anchor parameters come in threshold/expansion pairs)");
+        return MP_EINVAL;
+    }
+    const std::vector<mp_aligned_pair> &op = poa->obsPairs;
+    const size_t nseg = poa->obsSeg.size();
+    for (size_t sg = 0; sg < nseg; sg++)
+        if (poa->obsSeg[sg].first < 0 || poa->obsSeg[sg].first >= nReads) {
+            mp::set_error("mp_poa_anchor_alignments: an observation's read number is not below nReads");
+            return MP_EINVAL;
+        }
+    try {
+        // every read's observations in the reference's visiting order: node
+        // ascending, then the node's observation order (append order)
+        std::vector<int64_t> rStart(nReads + 1, 0);
+        for (size_t sg = 0; sg < nseg; sg++) {
+            const int64_t e = sg + 1 < nseg ? poa->obsSeg[sg + 1].second : (int64_t)op.size();
+            rStart[poa->obsSeg[sg].first + 1] += e - poa->obsSeg[sg].second;
+        }
+        for (int64_t r = 0; r < nReads; r++) rStart[r + 1] += rStart[r];
+        std::vector<int64_t> idx(op.size());  // tuple indices grouped by read, append order within a read
+        {
+            std::vector<int64_t> at(rStart.begin(), rStart.end() - 1);
+            for (size_t sg = 0; sg < nseg; sg++) {
+                const int64_t e = sg + 1 < nseg ? poa->obsSeg[sg + 1].second : (int64_t)op.size();
+                int64_t &w = at[poa->obsSeg[sg].first];
+                for (int64_t i = poa->obsSeg[sg].second; i < e; i++) idx[w++] = i;
+            }
+        }
+        std::vector<std::vector<int64_t>> out((size_t)nReads);
+        const int64_t nNodes = poa->nodes();
+        const double thr0 = anchorParams[0];
+        auto per_read = [&](int64_t r) {
+            int64_t *a = idx.data() + rStart[r], *e = idx.data() + rStart[r + 1];
+            std::stable_sort(a, e, [&](int64_t u, int64_t v) { return op[u].x < op[v].x; });
+            std::vector<int64_t> &lst = out[r];
+            for (int64_t *q = a; q < e; q++) {
+                const mp_aligned_pair &t = op[*q];
+                const int64_t node = (int64_t)t.x + 1;
+                if (node < 1 || node >= nNodes) continue;
+                const int64_t ci = poaToConsensusMap ? poaToConsensusMap[node - 1] : node - 1;
+                if (ci == -1) continue;
+                const double nw = (double)t.weight / 10000000.0;
+                if (!(nw > thr0)) continue;
+                int64_t expansion = (int64_t)anchorParams[1];
+                for (int64_t k = 2; k < nAnchorParams; k += 2) {
+                    if (nw >= anchorParams[k])
+                        expansion = (int64_t)anchorParams[k + 1];
+                    else
+                        break;
+                }
+                const size_t L = lst.size();
+                if (L == 0 || (lst[L - 3] < ci && lst[L - 2] < (int64_t)t.y)) {
+                    lst.push_back(ci);
+                    lst.push_back(t.y);
+                    lst.push_back(expansion);
+                }
+            }
+        };
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), nReads));
+        std::atomic<int64_t> next{0};
+        auto work = [&]() {
+            for (int64_t r; (r = next.fetch_add(1)) < nReads;) per_read(r);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; t++) th.emplace_back(work);
+        work();
+        for (auto &t : th) t.join();
+        int64_t total = 0;
+        for (int64_t r = 0; r < nReads; r++) {
+            counts[r] = (int64_t)out[r].size() / 3;
+            total += counts[r];
+        }
+        if (anchors) {
+            if (capacity < total) {
+                mp::set_error("mp_poa_anchor_alignments: capacity below the " + std::to_string(total) + " anchors");
+                return MP_EINVAL;
+            }
+            int64_t w = 0;
+            for (int64_t r = 0; r < nReads; r++) {
+                std::copy(out[r].begin(), out[r].end(), anchors + w);
+                w += (int64_t)out[r].size();
+            }
+        }
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poa_anchor_alignments: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+

@@ -142,6 +142,25 @@ class Poa:
         del keep
         return out[: self.sizes()["nodes"] - 1]
 
+    def anchor_alignments(self, n_reads, anchor_params, poa_to_consensus=None):
+        """poa_getAnchorAlignments (polisher.c:536-592): per read an (n, 3)
+        int64 array of (consensus position, read offset, expansion) anchors.
+        anchor_params = minPosteriorProbForAlignmentAnchors (threshold,
+        expansion pairs); poa_to_consensus maps node i to consensus position
+        poa_to_consensus[i - 1] (-1 = unaligned), None = identity."""
+        ap = np.ascontiguousarray(np.asarray(anchor_params, dtype=np.float64))
+        cm = None if poa_to_consensus is None else np.ascontiguousarray(np.asarray(poa_to_consensus, np.int64))
+        counts = np.zeros(max(n_reads, 1), dtype=np.int64)
+        cmp_ = None if cm is None else cm.ctypes.data
+        check(lib().mp_poa_anchor_alignments(self._h, cmp_, int(n_reads), ap.ctypes.data, ap.size,
+                                             counts.ctypes.data, None, 0))
+        tot = int(counts[:n_reads].sum())
+        buf = np.zeros((max(tot, 1), 3), dtype=np.int64)
+        check(lib().mp_poa_anchor_alignments(self._h, cmp_, int(n_reads), ap.ctypes.data, ap.size,
+                                             counts.ctypes.data, buf.ctypes.data, tot))
+        ends = np.cumsum(counts[:n_reads])
+        return [buf[e - c:e] for e, c in zip(ends, counts[:n_reads])]
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mp_poa_free(self._h)

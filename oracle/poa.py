@@ -305,3 +305,30 @@ def estimate_repeat_counts(poa, reads, strands, repeat, max_rl=MAXIMUM_REPEAT_LE
         out.append(1 if c == 0 else c)
     return out
 
+
+def anchor_alignments(poa, n_reads, params, poa_to_consensus=None):
+    """poa_getAnchorAlignments (polisher.c:536-592): nodes in order, each
+    node's observations in order; an observation above params[0] (weight /
+    1e7) anchors its read at (consensus position, offset, expansion), the
+    expansion stepping through the (threshold, expansion) pairs it reaches;
+    kept only when strictly after the read's previous anchor."""
+    out = [[] for _ in range(n_reads)]
+    for i in range(1, len(poa.nodes)):
+        ci = i - 1 if poa_to_consensus is None else poa_to_consensus[i - 1]
+        if ci == -1:
+            continue
+        for rn, off, w in poa.nodes[i].observations:
+            nw = w / 10000000.0
+            if not nw > params[0]:
+                continue
+            exp = int(params[1])
+            for k in range(2, len(params), 2):
+                if nw >= params[k]:
+                    exp = int(params[k + 1])
+                else:
+                    break
+            lst = out[rn]
+            if not lst or (lst[-1][0] < ci and lst[-1][1] < off):
+                lst.append((ci, off, exp))
+    return out
+

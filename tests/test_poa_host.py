@@ -199,3 +199,36 @@ def test_repeat_count_estimates_match_oracle(table):
     assert got.tolist() == want
     assert len(set(want)) > 3
 
+
+def test_anchor_alignments_match_oracle():
+    """poa_getAnchorAlignments: library vs oracle on a graph of several
+    reads built from FP64-oracle lists, identity map and a consensus map with
+    unaligned nodes and shifts."""
+    pp = O.PolishParams(PARAMS)
+    w = sim.config0(n_reads=6, ref_len=900, seed=17, strands=(1, 0))
+    ref = mpoa.rle_from_symbols(w.pairs[0].ref_sym, w.pairs[0].ref_runs)
+    ref_o = (ref[0].tobytes().decode(), tuple(int(c) for c in ref[1]))
+    g, o = mpoa.Poa(ref), P.Poa(ref_o, 51)
+    for r, pr in enumerate(w.pairs):
+        sx, sy, an, first = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        m, gx, gy, _, _ = O.posteriors(pp.forward if pr.strand else pp.reverse, sx, sy, an, pp.p)
+        lists = []
+        for t in (m, gy, gx):
+            t = np.asarray(t, dtype=np.int64).reshape(-1, 3).copy()
+            t[:, 1] += first
+            lists.append([tuple(int(v) for v in row) for row in t])
+        read = mpoa.rle_from_symbols(pr.read_sym, pr.read_runs)
+        read_o = (read[0].tobytes().decode(), tuple(int(c) for c in read[1]))
+        g.augment(read, pr.strand, r, *lists, True)
+        P.augment(o, read_o, bool(pr.strand), r, *lists, True)
+    params = [0.9, 10, 0.95, 4, 0.99, 0]
+    n = len(ref[0])
+    rng = np.random.default_rng(1)
+    cmap = np.cumsum(rng.integers(0, 3, size=n)) - 1
+    cmap[rng.random(n) < 0.1] = -1
+    for mp_ in (None, cmap):
+        got = g.anchor_alignments(len(w.pairs), params, mp_)
+        want = P.anchor_alignments(o, len(w.pairs), params, None if mp_ is None else [int(v) for v in mp_])
+        assert [a.tolist() for a in got] == [[list(t) for t in lst] for lst in want]
+        assert sum(len(a) for a in got) > 1000
+

@@ -189,3 +189,18 @@ def test_repeat_count_estimate_known_answers():
     for rn, w in enumerate([1, 1, 1, 100]):
         P.augment(poa, reads[rn], True, rn, [(w, 0, 0)], [], [])
     assert P.estimate_repeat_counts(poa, reads, strands, _peaked_table())[0] == 5
+
+
+def test_anchor_alignments_known_answer():
+    """poa_getAnchorAlignments (polisher.c:536-592) on a hand-built graph:
+    thresholds 0.9 -> 10, 0.95 -> 4, 0.99 -> 0 (allParams.np.json); an
+    observation at or below 0.9 is skipped, a non-increasing one is dropped,
+    unaligned nodes (-1) are skipped."""
+    poa = P.Poa(P.rle_no_rle("ACGTA"), 51)
+    read = P.rle_no_rle("ACGTA")
+    m = [(9_500_000, 0, 0), (9_000_000, 1, 1), (9_990_000, 2, 2), (9_999_999, 2, 1), (9_200_000, 3, 3),
+         (9_600_000, 4, 4)]
+    P.augment(poa, read, True, 0, m, [], [])
+    params = [0.9, 10, 0.95, 4, 0.99, 0]
+    assert P.anchor_alignments(poa, 1, params) == [[(0, 0, 4), (2, 2, 0), (3, 3, 10), (4, 4, 4)]]
+    assert P.anchor_alignments(poa, 1, params, [0, 1, -1, 5, 6]) == [[(0, 0, 4), (5, 3, 10), (6, 4, 4)]]
