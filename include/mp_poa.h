@@ -118,6 +118,22 @@ mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *poaToConsen
                                    const double *anchorParams, int64_t nAnchorParams, int64_t *counts,
                                    int64_t *anchors, int64_t capacity);
 
+/* poa_getConsensus (polisher.c:1040-1275) with getMaxWeight (1025-1038):
+   forward log-probabilities through the graph (match / insert / delete
+   transitions weighted by the node's base and indel weights, the reference's
+   polynomial logAdd), then a greedy traceback from the end; each visited
+   node contributes its heaviest base (and, with run-length encoding, its
+   heaviest run length) unless the reference's scaled by
+   referenceBasePenalty still beats it.  Writes the consensus as an expanded
+   string (not terminated) when outCap suffices (*outLen always; MP_EINVAL
+   when out is too small) and, when poaToConsensusMap is not null, nodes - 1
+   entries mapping node i to the position of its base in the run-length (or
+   plain) consensus, -1 where none.  A reference run length at or beyond the
+   graph's maxRepeatCount reads as weight 0 (the reference reads past its
+   array there). */
+mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEncoding, double referenceBasePenalty, char *out,
+                           int64_t outCap, int64_t *outLen, int64_t *poaToConsensusMap);
+
 mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *sizes);
 
 /* Per node: base, repeatCount, baseWeights[5] (A C G T N) and

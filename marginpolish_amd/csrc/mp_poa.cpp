@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cctype>
+#include <cmath>
+#include <limits>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -872,6 +874,201 @@ extern "C" mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *
         }
     } catch (const std::bad_alloc &) {
         mp::set_error("mp_poa_anchor_alignments: out of host memory");
+        return MP_ENOMEM;
+    }
+    return MP_OK;
+}
+
+namespace {
+
+// The reference's logAdd (pairwiseAligner.c:282-299): log(e^x + e^y) through a
+// 4-piece cubic of log(1 + e^d), coefficients written as float literals (so
+// rounded to float, then used in double arithmetic), cut at d >= 7.5.
+inline double log1pexp_cubic(double d) {
+    if (d <= 1.0f) return ((-0.009350833524763f * d + 0.130659527668286f) * d + 0.498799810682272f) * d + 0.693203116424741f;
+    if (d <= 2.5f) return ((-0.014532321752540f * d + 0.139942324101744f) * d + 0.495635523139337f) * d + 0.692140569840976f;
+    if (d <= 4.5f) return ((-0.004605031767994f * d + 0.063427417320019f) * d + 0.695956496475118f) * d + 0.514272634594009f;
+    return ((-0.000458661602210f * d + 0.009695946122598f) * d + 0.930734667215156f) * d + 0.168037164329057f;
+}
+
+inline double log_add(double x, double y) {
+    const double kNeg = -std::numeric_limits<double>::infinity();
+    if (x < y) return (x == kNeg || y - x >= 7.5) ? y : log1pexp_cubic(y - x) + x;
+    return (y == kNeg || x - y >= 7.5) ? x : log1pexp_cubic(x - y) + y;
+}
+
+// getMaxWeight (polisher.c:1025-1038): the heaviest non-reference entry
+// (last of equals), unless the reference entry scaled by the penalty beats it.
+int64_t max_weight(const double *w, int64_t n, int64_t ref, double penalty) {
+    double mw = 0.0;
+    int64_t mi = -1;
+    for (int64_t j = 0; j < n; j++)
+        if (j != ref && w[j] >= mw) {
+            mw = w[j];
+            mi = j;
+        }
+    const double wr = ref >= 0 && ref < n ? w[ref] : 0.0;
+    return (wr * penalty > mw || mi < 0) ? ref : mi;
+}
+
+}  // namespace
+
+extern "C" mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEncoding, double referenceBasePenalty,
+                                      char *out, int64_t outCap, int64_t *outLen, int64_t *poaToConsensusMap) {
+    if (!poa || !outLen) {
+        mp::set_error("mp_poa_consensus: null argument");
+        return MP_EINVAL;
+    }
+    const bool rle = useRunLengthEncoding != 0;
+    const double kNeg = -std::numeric_limits<double>::infinity();
+    try {
+        const int64_t n = poa->nodes();
+        const int64_t mrc = poa->maxRC;
+        auto ins_weight = [&](int32_t q) { return poa->ins[q].wF + poa->ins[q].wR; };
+        auto del_weight = [&](int32_t q) { return poa->del[q].wF + poa->del[q].wR; };
+        std::vector<double> totalOut(n, 0.0), fwd(n + 1, kNeg), matchFwd(n, 0.0);
+        fwd[0] = 0.0;
+        // deletions arriving at each node index (from node i, length L -> i + L + 1), in node then list order
+        std::vector<std::vector<int32_t>> incoming(n + 1);
+        for (int64_t i = 0; i < n; i++)
+            for (int32_t q = poa->delHead[i]; q >= 0; q = poa->del[q].next)
+                if (i + poa->del[q].len + 1 <= n) incoming[i + poa->del[q].len + 1].push_back(q);
+        std::vector<int64_t> delFrom(poa->del.size(), 0);
+        for (int64_t i = 0; i < n; i++)
+            for (int32_t q = poa->delHead[i]; q >= 0; q = poa->del[q].next) delFrom[q] = i;
+        // forward pass (polisher.c:1073-1136)
+        for (int64_t i = 0; i < n; i++) {
+            double indel = 0.0;
+            for (int32_t q = poa->insHead[i]; q >= 0; q = poa->ins[q].next) indel += ins_weight(q);
+            for (int32_t q = poa->delHead[i]; q >= 0; q = poa->del[q].next) indel += del_weight(q);
+            double mt = 0.0;
+            if (i == 0) {
+                if (n == 1) {
+                    mt = 1.0;
+                } else {
+                    for (int64_t j = 1; j < n; j++)
+                        for (int k = 0; k < kAlphabet; k++) mt += poa->baseW[j * kAlphabet + k];
+                    mt /= (double)(n - 1);
+                    mt -= indel;
+                }
+            } else {
+                for (int k = 0; k < kAlphabet; k++) mt += poa->baseW[i * kAlphabet + k];
+                mt -= indel;
+            }
+            mt = mt <= 0.0 ? 0.0001 : mt;
+            totalOut[i] = mt + indel;
+            for (int32_t q = poa->insHead[i]; q >= 0; q = poa->ins[q].next)
+                fwd[i + 1] = log_add(fwd[i + 1], fwd[i] + std::log(ins_weight(q) / totalOut[i]));
+            for (int32_t q = poa->delHead[i]; q >= 0; q = poa->del[q].next) {
+                const int64_t t = i + poa->del[q].len + 1;
+                if (t <= n) fwd[t] = log_add(fwd[t], fwd[i] + std::log(del_weight(q) / totalOut[i]));
+            }
+            matchFwd[i] = fwd[i] + std::log(mt / totalOut[i]);
+            fwd[i + 1] = log_add(fwd[i + 1], matchFwd[i]);
+        }
+        // greedy traceback from the end (polisher.c:1138-1245)
+        std::vector<int64_t> map((size_t)std::max<int64_t>(n - 1, 0), -1);
+        std::vector<std::pair<const char *, int64_t>> pieces;  // (base run or insert) in reverse order
+        std::string store;                                    // backing for expanded runs, filled later
+        std::vector<std::pair<char, int64_t>> runs;           // one entry per piece: base and count, or insert
+        std::vector<int32_t> pieceIns;                        // insert index of a piece, -1 for a node base
+        int64_t running = 0;
+        char prev = '-';
+        for (int64_t i = n; i > 0;) {
+            if (i < n) {
+                const int bsym = base_symbol(poa->nodeBase[i]);
+                const int64_t mb = max_weight(&poa->baseW[i * kAlphabet], kAlphabet, bsym, referenceBasePenalty);
+                const char base = "ACGTN"[mb];
+                int64_t rc = 1;
+                if (rle) {
+                    rc = max_weight(&poa->rcW[i * mrc], mrc, (int64_t)std::min<uint64_t>(poa->nodeRC[i], INT64_MAX),
+                                    referenceBasePenalty);
+                    rc = rc == 0 ? 1 : rc;
+                    if (prev != base) map[i - 1] = running++;
+                    prev = base;
+                } else {
+                    map[i - 1] = running++;
+                }
+                runs.emplace_back(base, rc);
+                pieceIns.push_back(-1);
+            }
+            double maxIns = kNeg, totIns = kNeg;
+            int32_t bestIns = -1;
+            for (int32_t q = poa->insHead[i - 1]; q >= 0; q = poa->ins[q].next) {
+                const double pr = std::log(ins_weight(q) / totalOut[i - 1]) + fwd[i - 1];
+                if (pr > maxIns) {
+                    maxIns = pr;
+                    bestIns = q;
+                }
+                totIns = log_add(totIns, pr);
+            }
+            double maxDel = kNeg, totDel = kNeg;
+            int32_t bestDel = -1;
+            for (int32_t q : incoming[i]) {
+                const int64_t src = delFrom[q];
+                const double pr = std::log(del_weight(q) / totalOut[src]) + fwd[src];
+                if (pr > maxDel) {
+                    maxDel = pr;
+                    bestDel = q;
+                }
+                totDel = log_add(totDel, pr);
+            }
+            if (matchFwd[i - 1] >= totDel && matchFwd[i - 1] >= totIns) {
+                i--;
+            } else if (totIns >= totDel) {
+                const InsEntry &E = poa->ins[bestIns];
+                runs.emplace_back('\0', 0);
+                pieceIns.push_back(bestIns);
+                if (rle) {
+                    const char last = poa->insB[E.sym + E.len - 1];
+                    running += E.len + (last != prev ? 0 : -1);
+                    prev = poa->insB[E.sym];
+                } else {
+                    running += E.len;
+                }
+                i--;
+            } else {
+                i -= poa->del[bestDel].len + 1;
+            }
+        }
+        // the pieces were collected from the end: expand them front to back
+        int64_t total = 0;
+        for (size_t q = 0; q < runs.size(); q++) {
+            if (pieceIns[q] < 0) {
+                total += runs[q].second;
+            } else {
+                const InsEntry &E = poa->ins[pieceIns[q]];
+                for (int64_t t = 0; t < E.len; t++) total += (int64_t)poa->insC[E.sym + t];
+            }
+        }
+        *outLen = total;
+        std::string expanded;
+        expanded.reserve((size_t)total);
+        for (size_t q = runs.size(); q-- > 0;) {
+            if (pieceIns[q] < 0) {
+                expanded.append((size_t)runs[q].second, runs[q].first);
+            } else {
+                const InsEntry &E = poa->ins[pieceIns[q]];
+                for (int64_t t = 0; t < E.len; t++)
+                    expanded.append((size_t)poa->insC[E.sym + t], poa->insB[E.sym + t]);
+            }
+        }
+        // consensus length in run-length (or plain) symbols, then the map from the string's start
+        int64_t clen = (int64_t)expanded.size();
+        if (rle) {
+            clen = 0;
+            for (size_t t = 0; t < expanded.size(); t++)
+                if (t == 0 || expanded[t] != expanded[t - 1]) clen++;
+        }
+        if (poaToConsensusMap)
+            for (int64_t i = 0; i + 1 < n; i++) poaToConsensusMap[i] = map[i] == -1 ? -1 : clen - 1 - map[i];
+        if (!out || outCap < total) {
+            mp::set_error("mp_poa_consensus: output buffer too small (need " + std::to_string(total) + ")");
+            return MP_EINVAL;
+        }
+        std::memcpy(out, expanded.data(), (size_t)total);
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poa_consensus: out of host memory");
         return MP_ENOMEM;
     }
     return MP_OK;

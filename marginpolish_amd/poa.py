@@ -161,6 +161,20 @@ class Poa:
         ends = np.cumsum(counts[:n_reads])
         return [buf[e - c:e] for e, c in zip(ends, counts[:n_reads])]
 
+    def consensus(self, use_run_length_encoding=True, reference_base_penalty=0.5):
+        """poa_getConsensus (polisher.c:1040-1275): the expanded consensus string
+        and poaToConsensusMap (node i -> position in the run-length consensus,
+        -1 where none)."""
+        n = self.sizes()["nodes"]
+        total = C.c_int64()
+        cmap = np.zeros(max(n - 1, 1), dtype=np.int64)
+        lib().mp_poa_consensus(self._h, int(bool(use_run_length_encoding)), float(reference_base_penalty), None, 0,
+                               C.byref(total), cmap.ctypes.data)
+        buf = C.create_string_buffer(max(total.value, 1))
+        check(lib().mp_poa_consensus(self._h, int(bool(use_run_length_encoding)), float(reference_base_penalty), buf,
+                                     total.value, C.byref(total), cmap.ctypes.data))
+        return buf.raw[:total.value].decode(), cmap[: n - 1]
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mp_poa_free(self._h)

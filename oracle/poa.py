@@ -332,3 +332,140 @@ def anchor_alignments(poa, n_reads, params, poa_to_consensus=None):
                 lst.append((ci, off, exp))
     return out
 
+
+def _f32(v):
+    return float(np.float32(v))
+
+
+_CUBIC = [(_f32(1.0), [_f32(c) for c in (-0.009350833524763, 0.130659527668286, 0.498799810682272, 0.693203116424741)]),
+          (_f32(2.5), [_f32(c) for c in (-0.014532321752540, 0.139942324101744, 0.495635523139337, 0.692140569840976)]),
+          (_f32(4.5), [_f32(c) for c in (-0.004605031767994, 0.063427417320019, 0.695956496475118, 0.514272634594009)]),
+          (float("inf"), [_f32(c) for c in (-0.000458661602210, 0.009695946122598, 0.930734667215156,
+                                            0.168037164329057)])]
+LOG_ZERO = float("-inf")
+
+
+def log_add(x, y):
+    """logAdd (pairwiseAligner.c:282-299): the 4-piece cubic of log(1 + e^d)
+    with float-rounded coefficients, in double arithmetic, cut at 7.5."""
+    if x < y:
+        x, y = y, x
+        if y == LOG_ZERO or x - y >= 7.5:
+            return x
+    elif y == LOG_ZERO or x - y >= 7.5:
+        return x
+    d = x - y
+    for lim, (a, b, c, e) in _CUBIC:
+        if d <= lim:
+            return ((a * d + b) * d + c) * d + e + y
+
+
+def max_weight(w, ref, penalty):
+    """getMaxWeight (polisher.c:1025-1038)."""
+    mw, mi = 0.0, -1
+    for j, v in enumerate(w):
+        if j != ref and v >= mw:
+            mw, mi = v, j
+    wr = w[ref] if 0 <= ref < len(w) else 0.0
+    return ref if (wr * penalty > mw or mi < 0) else mi
+
+
+def consensus(poa, rle=True, penalty=0.5):
+    """poa_getConsensus (polisher.c:1040-1275): forward pass over the graph,
+    greedy traceback; returns (expanded consensus, poaToConsensusMap)."""
+    import math
+    nodes = poa.nodes
+    n = len(nodes)
+    tot_out = [0.0] * n
+    fwd = [LOG_ZERO] * (n + 1)
+    fwd[0] = 0.0
+    match_fwd = [0.0] * n
+    incoming = [[] for _ in range(n + 1)]
+    for i, node in enumerate(nodes):
+        for e in node.deletes:
+            incoming[i + e[0] + 1].append((i, e))
+    for i, node in enumerate(nodes):
+        indel = 0.0
+        for e in node.inserts:
+            indel += e[1] + e[2]
+        for e in node.deletes:
+            indel += e[1] + e[2]
+        mt = 0.0
+        if i == 0:
+            if n == 1:
+                mt = 1.0
+            else:
+                for j in range(1, n):
+                    for v in nodes[j].base_weights:
+                        mt += v
+                mt /= n - 1
+                mt -= indel
+        else:
+            for v in node.base_weights:
+                mt += v
+            mt -= indel
+        mt = 0.0001 if mt <= 0.0 else mt
+        tot_out[i] = mt + indel
+        for e in node.inserts:
+            fwd[i + 1] = log_add(fwd[i + 1], fwd[i] + math.log((e[1] + e[2]) / tot_out[i]))
+        for e in node.deletes:
+            t = i + e[0] + 1
+            fwd[t] = log_add(fwd[t], fwd[i] + math.log((e[1] + e[2]) / tot_out[i]))
+        match_fwd[i] = fwd[i] + math.log(mt / tot_out[i])
+        fwd[i + 1] = log_add(fwd[i + 1], match_fwd[i])
+    cmap = [-1] * (n - 1)
+    pieces = []
+    running = 0
+    prev = "-"
+    i = n
+    while i > 0:
+        if i < n:
+            node = nodes[i]
+            mb = max_weight(node.base_weights, char_to_symbol(node.base), penalty)
+            base = "ACGTN"[mb]
+            if rle:
+                rc = max_weight(node.rc_weights, node.repeat_count, penalty)
+                rc = 1 if rc == 0 else rc
+                pieces.append(base * rc)
+                if prev != base:
+                    cmap[i - 1] = running
+                    running += 1
+                prev = base
+            else:
+                pieces.append(base)
+                cmap[i - 1] = running
+                running += 1
+        max_ins, tot_ins, best_ins = LOG_ZERO, LOG_ZERO, None
+        for e in nodes[i - 1].inserts:
+            pr = math.log((e[1] + e[2]) / tot_out[i - 1]) + fwd[i - 1]
+            if pr > max_ins:
+                max_ins, best_ins = pr, e
+            tot_ins = log_add(tot_ins, pr)
+        max_del, tot_del, best_del = LOG_ZERO, LOG_ZERO, None
+        for src, e in incoming[i]:
+            pr = math.log((e[1] + e[2]) / tot_out[src]) + fwd[src]
+            if pr > max_del:
+                max_del, best_del = pr, e
+            tot_del = log_add(tot_del, pr)
+        if match_fwd[i - 1] >= tot_del and match_fwd[i - 1] >= tot_ins:
+            i -= 1
+        elif tot_ins >= tot_del:
+            s = best_ins[0]
+            pieces.append(expand(s))
+            if rle:
+                running += len(s[0]) + (0 if s[0][-1] != prev else -1)
+                prev = s[0][0]
+            else:
+                running += len(s[0])
+            i -= 1
+        else:
+            i -= best_del[0] + 1
+    expanded = "".join(reversed(pieces))
+    clen = len(rle_string(expanded)[0]) if rle else len(expanded)
+    assert running == clen
+    return expanded, [-1 if v == -1 else clen - 1 - v for v in cmap]
+
+
+def rle_string(s):
+    return rle(s)
+

@@ -232,3 +232,69 @@ def test_anchor_alignments_match_oracle():
         assert [a.tolist() for a in got] == [[list(t) for t in lst] for lst in want]
         assert sum(len(a) for a in got) > 1000
 
+
+def _edit_distance(a, b):
+    prev = list(range(len(b) + 1))
+    for i, ca in enumerate(a, 1):
+        cur = [i] + [0] * len(b)
+        for j, cb in enumerate(b, 1):
+            cur[j] = min(prev[j] + 1, cur[j - 1] + 1, prev[j - 1] + (ca != cb))
+        prev = cur
+    return prev[-1]
+
+
+@pytest.mark.parametrize("rle,penalty", [(True, 0.5), (False, 0.5), (True, 1.0)])
+def test_consensus_matches_oracle_and_polishes(rle, penalty):
+    """poa_getConsensus: library vs oracle bit for bit (string and map) on a
+    graph of reads simulated from a truth sequence and aligned (FP64 oracle)
+    to a draft with errors; the consensus is closer to the truth than the
+    draft.  Without run-length encoding the graph is built from plain
+    strings and the no-RLE params, as the reference requires."""
+    from tests.conftest import PARAMS_NO_RLE
+    pp = O.PolishParams(PARAMS if rle else PARAMS_NO_RLE)
+    rng = np.random.default_rng(23)
+    truth = sim._ref_runs(rng, 600, 0.45, 0.45)
+    draft, dpos = sim._mutate(rng, truth, 0.04)
+    reads = [sim._mutate(rng, truth, 0.06) for _ in range(16)]
+    chars = np.frombuffer(b"ACGT", np.uint8)
+
+    def plain(bases):  # symbols without run lengths (useRepeatCountsInAlignment off)
+        return bases.astype(np.uint16), np.ones(bases.size, np.uint64)
+
+    if rle:
+        rb, rr, _ = sim._rle(draft)
+        ref_rle = mpoa.rle_from_symbols(sim._symbols(rb, rr), rr)
+    else:
+        ref_rle = (chars[draft], np.ones(draft.size, np.uint64))
+    ref_o = (bytes(ref_rle[0]).decode(), tuple(int(c) for c in ref_rle[1]))
+    g, o = mpoa.Poa(ref_rle, 51 if rle else 2), P.Poa(ref_o, 51 if rle else 2)
+    for r, (rd, pos) in enumerate(reads):
+        t_idx = np.flatnonzero(pos >= 0)
+        keep = dpos[t_idx] >= 0
+        if rle:
+            pr = sim._pair(draft, rd, dpos[t_idx][keep], pos[t_idx][keep], 10, r % 2, 0, r)
+            ref_sym, read_sym, anchors, read = pr.ref_sym, pr.read_sym, pr.anchors, \
+                mpoa.rle_from_symbols(pr.read_sym, pr.read_runs)
+        else:
+            X, Y = dpos[t_idx][keep], pos[t_idx][keep]
+            ok = np.r_[True, (X[1:] > X[:-1]) & (Y[1:] > Y[:-1])]
+            anchors = np.stack([X[ok], Y[ok], np.full(int(ok.sum()), 10)], axis=1).astype(np.int64)
+            ref_sym, read_sym = plain(draft)[0], plain(rd)[0]
+            read = (chars[rd], np.ones(rd.size, np.uint64))
+        sx, sy, an, first = crop.crop_pair(ref_sym, read_sym, anchors)
+        m, gx, gy, _, _ = O.posteriors(pp.forward if r % 2 else pp.reverse, sx, sy, an, pp.p)
+        lists = []
+        for t in (m, gy, gx):
+            t = np.asarray(t, dtype=np.int64).reshape(-1, 3).copy()
+            t[:, 1] += first
+            lists.append([tuple(int(v) for v in row) for row in t])
+        g.augment(read, r % 2, r, *lists, True)
+        P.augment(o, (bytes(read[0]).decode(), tuple(int(c) for c in read[1])), bool(r % 2), r, *lists, True)
+    s, cmap = g.consensus(rle, penalty)
+    s_o, cmap_o = P.consensus(o, rle, penalty)
+    assert s == s_o
+    assert cmap.tolist() == cmap_o
+    if penalty == 0.5:
+        t_str = bytes(chars[truth]).decode()
+        d_str = bytes(chars[draft]).decode()
+        assert _edit_distance(s, t_str) < _edit_distance(d_str, t_str) / 2

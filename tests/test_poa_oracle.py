@@ -204,3 +204,54 @@ def test_anchor_alignments_known_answer():
     params = [0.9, 10, 0.95, 4, 0.99, 0]
     assert P.anchor_alignments(poa, 1, params) == [[(0, 0, 4), (2, 2, 0), (3, 3, 10), (4, 4, 4)]]
     assert P.anchor_alignments(poa, 1, params, [0, 1, -1, 5, 6]) == [[(0, 0, 4), (5, 3, 10), (6, 4, 4)]]
+
+
+def test_log_add_matches_log_sum_exp():
+    """logAdd (pairwiseAligner.c:282-299): within the cubic's error of the
+    exact log-sum-exp below the 7.5 cut, the larger argument above it."""
+    import math
+    for x in [0.0, -0.3, -1.0, -2.2, -4.0, -7.4]:
+        for y in [0.0, -0.5, -3.0]:
+            exact = max(x, y) + math.log1p(math.exp(-abs(x - y)))
+            got = P.log_add(x, y)
+            if abs(x - y) >= 7.5:
+                assert got == max(x, y)
+            else:
+                assert abs(got - exact) < 2e-3, (x, y)
+    assert P.log_add(float("-inf"), -2.0) == -2.0 and P.log_add(-2.0, -12.0) == -2.0
+
+
+def _graph(ref, events):
+    """A graph with hand-set weights: events = [(kind, node, payload, weight)]."""
+    poa = P.Poa(P.rle_no_rle(ref), 51)
+    for node in poa.nodes[1:]:
+        node.base_weights[P.char_to_symbol(node.base)] = 100.0
+        node.rc_weights[1] = 100.0
+    for kind, i, payload, w in events:
+        if kind == "sub":
+            poa.nodes[i].base_weights = [0.0] * 5
+            poa.nodes[i].base_weights[P.char_to_symbol(payload)] = w
+        elif kind == "ins":
+            P._add_insert(poa.nodes[i], P.rle_no_rle(payload), w, True, (0, 0, w))
+        else:
+            P._add_delete(poa.nodes[i], payload, w, True, (0, 0, w))
+    return poa
+
+
+def test_consensus_known_answers():
+    """poa_getConsensus (polisher.c:1040-1275) on hand-weighted graphs: the
+    reference when it is supported, a heavier substitution, an insert or a
+    delete when the reads carry it, nothing when they do not."""
+    assert P.consensus(_graph("GATTACA", []), rle=False)[0] == "GATTACA"
+    assert P.consensus(_graph("GATTACA", []), rle=False)[1] == list(range(7))
+    assert P.consensus(_graph("GATTACA", [("sub", 4, "G", 100.0)]), rle=False)[0] == "GATGACA"
+    # the reference base keeps a position unless the alternative beats it after the 0.5 discount
+    g = _graph("GATTACA", [])
+    g.nodes[4].base_weights = [0.0, 0.0, 60.0, 100.0, 0.0]
+    assert P.consensus(g, rle=False)[0] == "GATGACA"
+    g.nodes[4].base_weights = [0.0, 0.0, 40.0, 100.0, 0.0]
+    assert P.consensus(g, rle=False)[0] == "GATTACA"
+    assert P.consensus(_graph("GATTACA", [("ins", 3, "GG", 90.0)]), rle=False)[0] == "GATGGTACA"
+    assert P.consensus(_graph("GATTACA", [("ins", 3, "GG", 5.0)]), rle=False)[0] == "GATTACA"
+    s, m = P.consensus(_graph("GATTACA", [("del", 2, 2, 90.0)]), rle=False)
+    assert s == "GAACA" and m == [0, 1, -1, -1, 2, 3, 4]
