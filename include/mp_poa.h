@@ -134,6 +134,13 @@ mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *poaToConsen
 mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEncoding, double referenceBasePenalty, char *out,
                            int64_t outCap, int64_t *outLen, int64_t *poaToConsensusMap);
 
+/* The graph's weight totals (polisher.c:706-758): scores[0] =
+   poa_getReferenceNodeTotalMatchWeight, [1] =
+   poa_getReferenceNodeTotalDisagreementWeight, [2] = poa_getInsertTotalWeight
+   (weight x run count), [3] = poa_getDeleteTotalWeight (weight x length);
+   poa_realignIterative's score is [0] - ([1] + [2] + [3]). */
+mp_status mp_poa_scores(const mp_poa *poa, double *scores);
+
 mp_status mp_poa_get_sizes(const mp_poa *poa, mp_poa_sizes *sizes);
 
 /* Per node: base, repeatCount, baseWeights[5] (A C G T N) and

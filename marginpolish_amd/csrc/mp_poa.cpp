@@ -1074,3 +1074,26 @@ extern "C" mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEnc
     return MP_OK;
 }
 
+extern "C" mp_status mp_poa_scores(const mp_poa *poa, double *scores) {
+    if (!poa || !scores) {
+        mp::set_error("mp_poa_scores: null argument");
+        return MP_EINVAL;
+    }
+    double match = 0.0, other = 0.0, ins = 0.0, del = 0.0;
+    for (int64_t i = 0; i < poa->nodes(); i++) {
+        const int rs = base_symbol(poa->nodeBase[i]);
+        match += poa->baseW[i * kAlphabet + rs];
+        for (int j = 0; j < kAlphabet; j++)
+            if (j != rs) other += poa->baseW[i * kAlphabet + j];
+        for (int32_t q = poa->insHead[i]; q >= 0; q = poa->ins[q].next)
+            ins += (poa->ins[q].wF + poa->ins[q].wR) * (double)poa->ins[q].len;
+        for (int32_t q = poa->delHead[i]; q >= 0; q = poa->del[q].next)
+            del += (poa->del[q].wF + poa->del[q].wR) * (double)poa->del[q].len;
+    }
+    scores[0] = match;
+    scores[1] = other;
+    scores[2] = ins;
+    scores[3] = del;
+    return MP_OK;
+}
+

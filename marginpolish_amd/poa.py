@@ -175,6 +175,17 @@ class Poa:
                                      total.value, C.byref(total), cmap.ctypes.data))
         return buf.raw[:total.value].decode(), cmap[: n - 1]
 
+    def scores(self):
+        """(reference match weight, disagreement weight, insert weight, delete
+        weight) as poa_realignIterative scores the graph (polisher.c:706-758)."""
+        out = np.zeros(4)
+        check(lib().mp_poa_scores(self._h, out.ctypes.data))
+        return tuple(float(v) for v in out)
+
+    def score(self):
+        m, d, i, x = self.scores()
+        return m - (x + i + d)
+
     def close(self):
         if getattr(self, "_h", None):
             lib().mp_poa_free(self._h)

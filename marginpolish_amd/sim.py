@@ -313,3 +313,68 @@ def bubbles(n_bubbles=4000, depth=60, seed=5, err=0.09, region=(8, 40), flank=6,
         out.append(([_rle_pair(a) for a in alleles], reads))
     return out
 
+
+@dataclasses.dataclass
+class PolishChunk:
+    start: int               # draft coordinates of the chunk (with its boundaries)
+    end: int
+    ref: tuple               # run-length draft segment (bases, counts)
+    reads: list              # [((bases, counts), strand)]
+    anchors: list            # per read: (n, 3) RLE anchors (draft x, read y, expansion)
+
+
+def _plain(bases):
+    """Every base its own run (rleString_construct_no_rle)."""
+    return bases, np.ones(bases.size, np.int64), np.arange(bases.size)
+
+
+def polish_workload(seed=7, genome_len=30000, draft_err=0.03, read_err=0.08, depth=20, mean=3000, sd=1500,
+                    chunk_size=10000, boundary=250, W=10, rle=True):
+    """A draft to polish: a truth genome (runs ~ Geometric(0.45), GC 45%), a
+    draft = truth with `draft_err` errors, reads sampled from the truth at
+    `read_err`; chunks of the draft with `boundary` bases each side, every
+    read clipped to each chunk it overlaps, RLE anchors from the reads' true
+    placement on the draft (rle=False: plain strings, one symbol per base).
+    Returns (truth bases, draft bases, chunks)."""
+    enc = _rle if rle else _plain
+    rng = np.random.default_rng(seed)
+    truth = _ref_runs(rng, genome_len, 0.45, 0.45)
+    draft, dpos = _mutate(rng, truth, draft_err)   # dpos: draft position of each truth base
+    nreads = int(round(depth * genome_len / mean))
+    lens = np.minimum(_lognormal_lengths(rng, nreads, mean, sd, 500, genome_len), genome_len)
+    starts = rng.integers(0, np.maximum(1, genome_len - lens + 1))
+    strands = rng.integers(0, 2, size=nreads)
+    chars = np.frombuffer(b"ACGT", np.uint8)
+    dlen = draft.size
+    nchunks = (dlen + chunk_size - 1) // chunk_size
+    chunks = []
+    for c in range(nchunks):
+        cs, ce = max(0, c * chunk_size - boundary), min(dlen, (c + 1) * chunk_size + boundary)
+        rb, rr, rid = enc(draft[cs:ce])
+        chunks.append(PolishChunk(cs, ce, (chars[rb].tobytes(), rr.astype(np.uint64)), [], []))
+        chunks[-1]._rid = rid
+    for r in range(nreads):
+        s, e = int(starts[r]), int(starts[r] + lens[r])
+        read, pos = _mutate(rng, truth[s:e], read_err)
+        dp = dpos[s:e]
+        ok = (pos >= 0) & (dp >= 0)
+        D, Y = dp[ok], pos[ok]
+        if D.size < 2:
+            continue
+        for c, ch in enumerate(chunks):
+            sel = (D >= ch.start) & (D < ch.end)
+            if sel.sum() < 2:
+                continue
+            d0, y0, y1 = D[sel], int(Y[sel][0]), int(Y[sel][-1])
+            seg = read[y0:y1 + 1]
+            qb, qr, qid = enc(seg)
+            X = ch._rid[d0 - ch.start]
+            Yr = qid[Y[sel] - y0]
+            keep = np.r_[True, (X[1:] > X[:-1]) & (Yr[1:] > Yr[:-1])]
+            X, Yr = X[keep], Yr[keep]
+            ch.reads.append(((chars[qb].tobytes(), qr.astype(np.uint64)), int(strands[r])))
+            ch.anchors.append(np.stack([X, Yr, np.full(X.size, W)], axis=1).astype(np.int64))
+    for ch in chunks:
+        del ch._rid
+    return truth, draft, chunks
+

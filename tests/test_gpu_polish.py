@@ -1,0 +1,100 @@
+"""The whole realignment pipeline on the GPU: initial batch, POA graphs,
+consensus iterations, run lengths, contig merge — a simulated draft gets
+closer to the truth it was made from."""
+import numpy as np
+import pytest
+
+from marginpolish_amd import pairhmm as ph
+from marginpolish_amd import polish, sim
+from tests.conftest import PARAMS
+
+pytestmark = pytest.mark.gpu
+
+
+def banded_edit_distance(a, b, band=400):
+    """Levenshtein distance restricted to |i - j| <= band (exact when the
+    optimal path stays inside the band, as it does for these near-equal strings)."""
+    a = np.frombuffer(a.encode(), np.uint8)
+    b = np.frombuffer(b.encode(), np.uint8)
+    n, m = a.size, b.size
+    big = 1 << 30
+    prev = np.full(m + 1, big, np.int64)
+    prev[: min(m, band) + 1] = np.arange(min(m, band) + 1)
+    for i in range(1, n + 1):
+        lo, hi = max(1, i - band), min(m, i + band)
+        cur = np.full(m + 1, big, np.int64)
+        if i <= band:
+            cur[0] = i
+        if lo <= hi:
+            sub = prev[lo - 1:hi] + (a[i - 1] != b[lo - 1:hi])
+            dele = prev[lo:hi + 1] + 1
+            best = np.minimum(sub, dele)
+            # insertions run along the row: a prefix-minimum of best - j, plus j
+            j = np.arange(lo, hi + 1)
+            left = cur[lo - 1] + 1
+            run = np.minimum.accumulate(np.minimum(best - j, np.r_[left - lo, np.full(hi - lo, big)]))
+            cur[lo:hi + 1] = np.minimum(best, run + j)
+        prev = cur
+    return int(prev[m])
+
+
+def _strings(truth, draft):
+    chars = np.frombuffer(b"ACGT", np.uint8)
+    return chars[truth].tobytes().decode(), chars[draft].tobytes().decode()
+
+
+def test_banded_edit_distance():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a = "".join(rng.choice(list("ACGT"), size=rng.integers(0, 40)))
+        b = "".join(rng.choice(list("ACGT"), size=rng.integers(0, 40)))
+        # full DP for reference
+        d = list(range(len(b) + 1))
+        for i, ca in enumerate(a, 1):
+            nd = [i] + [0] * len(b)
+            for j, cb in enumerate(b, 1):
+                nd[j] = min(d[j] + 1, nd[j - 1] + 1, d[j - 1] + (ca != cb))
+            d = nd
+        assert banded_edit_distance(a, b, band=100) == d[-1]
+
+
+def _run(params_path, rle, iters, seed=7):
+    truth, draft, chunks = sim.polish_workload(seed=seed, rle=rle)
+    params = ph.PolishParams(params_path)
+    cfg = polish.PolishConfig(params_path)
+    cfg.maxRealignmentPolishIterations = 0
+    cfg.maxPoaConsensusIterations = iters
+    cfg.chunkBoundary = 250
+    refs, reads, anchors = [c.ref for c in chunks], [c.reads for c in chunks], [c.anchors for c in chunks]
+    return truth, draft, polish.polish_contig(params, cfg, refs, reads, anchors), (params, cfg, refs, reads, anchors)
+
+
+def test_polish_draft_towards_truth():
+    """Plain-string params (no run-length encoding): the consensus rounds move
+    a 3%-error draft most of the way to the truth its 8%-error reads came from."""
+    from tests.conftest import PARAMS_NO_RLE
+    truth, draft, merged, args = _run(PARAMS_NO_RLE, False, 3)
+    t_str, d_str = _strings(truth, draft)
+    d0 = banded_edit_distance(d_str, t_str)
+    d1 = banded_edit_distance(merged, t_str)
+    print("draft edit distance %d, polished %d" % (d0, d1), flush=True)
+    assert d1 < 0.4 * d0  # measured: 808 -> 248
+    assert polish.polish_contig(*args) == merged  # deterministic
+
+
+def test_polish_rle_pipeline_runs():
+    """Run-length params: the whole pipeline (consensus rounds with the
+    maximum-likelihood run lengths) runs and is deterministic.  Accuracy is
+    not asserted: these simulated reads do not follow the ONT run-length
+    error model the params' table was trained on (the estimates lengthen
+    runs that the simulator does not shorten)."""
+    truth, draft, merged, args = _run(PARAMS, True, 2)
+    assert abs(len(merged) - len(truth)) < 0.05 * len(truth)
+    assert polish.polish_contig(*args) == merged
+
+
+def test_default_params_refuse_the_unbuilt_polish_path():
+    cfg = polish.PolishConfig(PARAMS)  # maxRealignmentPolishIterations = 1 in these params
+    truth, draft, chunks = sim.polish_workload(seed=1, genome_len=3000, chunk_size=3000)
+    with pytest.raises(NotImplementedError):
+        polish.realign_all(ph.PolishParams(PARAMS), cfg, [chunks[0].ref], [chunks[0].reads], [chunks[0].anchors])
