@@ -118,6 +118,7 @@ def main():
         read_no[i] = seen.get(p.chunk, 0)
         seen[p.chunk] = read_no[i] + 1
     rle_reads = [mpoa.rle_from_symbols(p.read_sym, p.read_runs) for p in w.pairs]
+    packed_reads = mpoa.PackedReads(rle_reads)  # the reads' RleStrings, laid out once
     poa_iters = []
     for it in range(2):
         t1 = time.time()
@@ -125,12 +126,12 @@ def main():
         bb.run()
         ta = time.time()
         graphs = [mpoa.Poa(r) for r in refs]
-        mpoa.augment_batch(bb, graphs, pair_poa, rle_reads, read_no)
+        mpoa.augment_batch(bb, graphs, pair_poa, packed_reads, read_no)
         poa_iters.append((time.time() - ta, time.time() - t1))
         if it == 1:
             tb = time.time()
             g1 = [mpoa.Poa(r) for r in refs]
-            mpoa.augment_batch(bb, g1, pair_poa, rle_reads, read_no, threads=1)
+            mpoa.augment_batch(bb, g1, pair_poa, packed_reads, read_no, threads=1)
             aug1 = time.time() - tb
             gsz = [g.sizes() for g in graphs]
             del g1

@@ -16,6 +16,7 @@
 #include <limits>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -430,67 +431,119 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
 
     std::vector<std::vector<InsEvent>> iev(npairs);
     std::vector<std::vector<DelEvent>> dev(npairs);
-    std::atomic<int64_t> nextPair{0}, nextPoa{0};
-    std::atomic<int64_t> badPair{-1};
-    // phase 1: events of every read (graphs read-only)
-    auto phase1 = [&]() {
-        Scratch S;
-        for (int64_t i; (i = nextPair.fetch_add(1)) < npairs;) {
-            const mp_poa *g = poas[pairPoa[i]];
-            const Str ref = g->ref();
+    auto pair_lists = [&](int64_t i, const mp_aligned_pair *&m, int64_t &nm, const mp_aligned_pair *&gx, int64_t &nx,
+                          const mp_aligned_pair *&gy, int64_t &ny) {
+        m = lists[0] + starts[0][i];
+        gx = lists[1] + starts[1][i];
+        gy = lists[2] + starts[2][i];
+        nm = starts[0][i + 1] - starts[0][i];
+        nx = starts[1][i + 1] - starts[1][i];
+        ny = starts[2][i + 1] - starts[2][i];
+    };
+    // Work items, one pool of threads:
+    //  - events of a read (graphs read-only), reads grouped by graph, larger graphs first;
+    //  - a graph's insert task and delete task, ready once all its reads' events exist;
+    //  - a graph's match task (no dependency).
+    // Each task applies the graph's reads in batch order, so lists and sums come
+    // out in the reference's order.
+    std::vector<int64_t> evOrder;
+    evOrder.reserve((size_t)npairs);
+    for (int64_t p : poaOrder) evOrder.insert(evOrder.end(), byPoa[p].begin(), byPoa[p].end());
+    std::unique_ptr<std::atomic<int64_t>[]> pending(new std::atomic<int64_t>[std::max<int64_t>(nPoas, 1)]);
+    for (int64_t p = 0; p < nPoas; p++) pending[p].store((int64_t)byPoa[p].size());
+    std::mutex readyMu;
+    std::vector<std::pair<int64_t, int>> ready;
+    size_t readyHead = 0;
+    std::atomic<int64_t> nextEv{0}, nextMatch{0}, done{0};
+    const int64_t totalTasks = 3 * (int64_t)poaOrder.size();
+    auto run_task = [&](int64_t p, int kind) {
+        mp_poa *g = poas[p];
+        if (kind == 0) {
+            int64_t nmTot = 0;
+            for (int64_t i : byPoa[p]) nmTot += starts[0][i + 1] - starts[0][i];
+            g->obsPairs.reserve(g->obsPairs.size() + nmTot);
+        }
+        for (int64_t i : byPoa[p]) {
             const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
-            const mp_aligned_pair *m = lists[0] + starts[0][i], *gx = lists[1] + starts[1][i],
-                                  *gy = lists[2] + starts[2][i];
-            const int64_t nm = starts[0][i + 1] - starts[0][i], nx = starts[1][i + 1] - starts[1][i],
-                          ny = starts[2][i + 1] - starts[2][i];
-            if (!check_read(ref, rd, m, nm, gy, ny, gx, nx)) {
-                int64_t none = -1;
-                badPair.compare_exchange_strong(none, i);
+            const int64_t rn = pairReadNo ? pairReadNo[i] : i;
+            if (kind == 0) {
+                apply_matches(g, rd, rn, lists[0] + starts[0][i], starts[0][i + 1] - starts[0][i]);
+            } else if (kind == 1) {
+                apply_inserts(g, rd, strand[i] != 0, rn, iev[i]);
+                std::vector<InsEvent>().swap(iev[i]);
+            } else {
+                apply_deletes(g, strand[i] != 0, rn, dev[i]);
+                std::vector<DelEvent>().swap(dev[i]);
+            }
+        }
+        done.fetch_add(1);
+    };
+    auto worker = [&]() {
+        Scratch S;
+        for (;;) {
+            std::pair<int64_t, int> task{-1, 0};
+            {
+                std::lock_guard<std::mutex> lk(readyMu);
+                if (readyHead < ready.size()) task = ready[readyHead++];
+            }
+            if (task.first >= 0) {
+                run_task(task.first, task.second);
                 continue;
             }
-            read_events(ref, rd, m, nm, gy, ny, gx, nx, cmp, S, iev[i], dev[i]);
-        }
-    };
-    // phase 2: per graph, its matches, its inserts and its deletes (three
-    // independent tasks), each applying the graph's reads in batch order
-    std::vector<std::pair<int64_t, int>> tasks;
-    for (int kind : {1, 0, 2})
-        for (int64_t p : poaOrder) tasks.emplace_back(p, kind);
-    auto phase2 = [&]() {
-        for (int64_t k; (k = nextPoa.fetch_add(1)) < (int64_t)tasks.size();) {
-            const int64_t p = tasks[k].first;
-            mp_poa *g = poas[p];
-            if (tasks[k].second == 0) {
-                int64_t nmTot = 0;
-                for (int64_t i : byPoa[p]) nmTot += starts[0][i + 1] - starts[0][i];
-                g->obsPairs.reserve(g->obsPairs.size() + nmTot);
-            }
-            for (int64_t i : byPoa[p]) {
+            const int64_t q = nextEv.fetch_add(1);
+            if (q < (int64_t)evOrder.size()) {
+                const int64_t i = evOrder[q];
+                const mp_poa *g = poas[pairPoa[i]];
                 const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
-                const int64_t rn = pairReadNo ? pairReadNo[i] : i;
-                if (tasks[k].second == 0)
-                    apply_matches(g, rd, rn, lists[0] + starts[0][i], starts[0][i + 1] - starts[0][i]);
-                else if (tasks[k].second == 1)
-                    apply_inserts(g, rd, strand[i] != 0, rn, iev[i]);
-                else
-                    apply_deletes(g, strand[i] != 0, rn, dev[i]);
+                const mp_aligned_pair *m, *gx, *gy;
+                int64_t nm, nx, ny;
+                pair_lists(i, m, nm, gx, nx, gy, ny);
+                read_events(g->ref(), rd, m, nm, gy, ny, gx, nx, cmp, S, iev[i], dev[i]);
+                if (pending[pairPoa[i]].fetch_sub(1) == 1) {
+                    std::lock_guard<std::mutex> lk(readyMu);
+                    ready.emplace_back(pairPoa[i], 1);
+                    ready.emplace_back(pairPoa[i], 2);
+                }
+                continue;
             }
+            const int64_t mt = nextMatch.fetch_add(1);
+            if (mt < (int64_t)poaOrder.size()) {
+                run_task(poaOrder[mt], 0);
+                continue;
+            }
+            if (done.load() >= totalTasks) break;
+            std::this_thread::yield();
         }
     };
     try {
+        // every pair's coordinates are checked before any graph changes
+        std::atomic<int64_t> nextCheck{0}, badPair{INT64_MAX};
+        auto checker = [&]() {
+            for (int64_t i0; (i0 = nextCheck.fetch_add(64)) < npairs;)
+                for (int64_t i = i0; i < std::min<int64_t>(npairs, i0 + 64); i++) {
+                    const mp_aligned_pair *m, *gx, *gy;
+                    int64_t nm, nx, ny;
+                    pair_lists(i, m, nm, gx, nx, gy, ny);
+                    const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
+                    if (!check_read(poas[pairPoa[i]]->ref(), rd, m, nm, gy, ny, gx, nx)) {
+                        int64_t cur = badPair.load();
+                        while (i < cur && !badPair.compare_exchange_weak(cur, i)) {
+                        }
+                    }
+                }
+        };
         std::vector<std::thread> th;
-        for (int t = 1; t < threads; t++) th.emplace_back(phase1);
-        phase1();
+        for (int t = 1; t < threads; t++) th.emplace_back(checker);
+        checker();
         for (auto &t : th) t.join();
-        if (badPair.load() >= 0) {
+        if (badPair.load() != INT64_MAX) {
             set_error("mp_batch_poa_augment: pair " + std::to_string(badPair.load()) +
                       " has list coordinates outside its graph's reference or its read");
             return MP_EINVAL;
         }
         th.clear();
-        const int t2 = (int)std::min<int64_t>(threads, (int64_t)tasks.size());
-        for (int t = 1; t < t2; t++) th.emplace_back(phase2);
-        phase2();
+        for (int t = 1; t < threads; t++) th.emplace_back(worker);
+        worker();
         for (auto &t : th) t.join();
     } catch (const std::bad_alloc &) {
         set_error("mp_batch_poa_augment: out of host memory");

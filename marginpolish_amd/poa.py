@@ -43,6 +43,22 @@ def _as_rle(r):
     return b, c
 
 
+class PackedReads:
+    """Run-length reads laid out once as the library's mp_rle_string array
+    (the reads do not change between realignment iterations)."""
+
+    def __init__(self, reads):
+        self._keep = [_as_rle(r) for r in reads]
+        self.records = np.empty(max(len(self._keep), 1), dtype=_lib.RLE_STRING_DTYPE)
+        if self._keep:
+            self.records["bases"][: len(self._keep)] = [b.ctypes.data for b, _ in self._keep]
+            self.records["repeatCounts"][: len(self._keep)] = [c.ctypes.data for _, c in self._keep]
+            self.records["length"][: len(self._keep)] = [b.size for b, _ in self._keep]
+
+    def __len__(self):
+        return len(self._keep)
+
+
 def _rle_struct(b, c):
     return _lib.RleString(b.ctypes.data, c.ctypes.data, b.size)
 
@@ -218,13 +234,11 @@ def augment_batch(batch, poas, pair_poa, reads, pair_read_no=None, compare_repea
     n = batch.npairs
     if len(reads) != n or len(pair_poa) != n:
         raise ValueError("one graph index and one read string per pair")
-    keep = [_as_rle(r) for r in reads]
-    rs = np.empty(n, dtype=_lib.RLE_STRING_DTYPE)
-    for i, (b, c) in enumerate(keep):
-        rs[i] = (b.ctypes.data, c.ctypes.data, b.size)
+    packed = reads if isinstance(reads, PackedReads) else PackedReads(reads)
+    rs = packed.records
     pp = np.ascontiguousarray(np.asarray(pair_poa, dtype=np.int64))
     rn = np.ascontiguousarray(np.asarray(pair_read_no if pair_read_no is not None else np.arange(n), dtype=np.int64))
     handles = (C.c_void_p * max(len(poas), 1))(*[p.handle for p in poas])
     check(lib().mp_batch_poa_augment(batch.handle, handles, len(poas), pp.ctypes.data, rn.ctypes.data,
                                      rs.ctypes.data, int(bool(compare_repeat_counts)), int(threads)))
-    del keep
+    del packed
