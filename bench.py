@@ -139,10 +139,10 @@ def main():
             by_chunk = [[] for _ in refs]
             for i in range(len(pairs)):
                 by_chunk[pair_poa[i]].append(i)
+            chunk_reads = [mpoa.PackedReads([rle_reads[i] for i in ids]) for ids in by_chunk]
+            chunk_strands = [[w.pairs[i].strand for i in ids] for ids in by_chunk]
             tr = time.time()
-            for k, g in enumerate(graphs):
-                ids = by_chunk[k]
-                g.estimate_repeat_counts(pp.machines, [rle_reads[i] for i in ids], [w.pairs[i].strand for i in ids])
+            mpoa.estimate_repeat_counts_many(graphs, pp.machines, chunk_reads, chunk_strands)
             t_rc = time.time() - tr
         bb.close()
         del graphs

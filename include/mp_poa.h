@@ -103,6 +103,13 @@ mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_model *m, co
                                         const int32_t *readStrand, int64_t nReads, uint64_t *repeatCounts,
                                         int32_t threads);
 
+/* The same for nPoas graphs at once (graphs processed in parallel): graph g's
+   reads are reads[readStart[g] .. readStart[g + 1]) (indexed by readNo from
+   readStart[g]), its counts go to repeatCounts[g] (nodes - 1 entries). */
+mp_status mp_poas_estimate_repeat_counts(const mp_poa *const *poas, int64_t nPoas, const mp_model *m,
+                                         const mp_rle_string *reads, const int32_t *readStrand,
+                                         const int64_t *readStart, uint64_t *const *repeatCounts, int32_t threads);
+
 /* poa_getAnchorAlignments (polisher.c:536-592): anchor pairs for the next
    realignment from the graph's base observations.  Nodes in order (node i
    maps to consensus position poaToConsensusMap[i - 1], or i - 1 when the map

@@ -84,7 +84,7 @@ EXPORTS = [
     "mp_batch_autotune", "mp_poa_reference_graph", "mp_poa_free", "mp_poa_augment", "mp_batch_poa_augment",
     "mp_poa_get_sizes", "mp_poa_get_nodes", "mp_poa_get_inserts", "mp_poa_get_deletes", "mp_poa_get_observations",
     "mp_bubble_allele_supports", "mp_remove_overlaps", "mp_merge_chunks", "mp_poa_estimate_repeat_counts",
-    "mp_poa_anchor_alignments", "mp_poa_consensus", "mp_poa_scores",
+    "mp_poa_anchor_alignments", "mp_poa_consensus", "mp_poa_scores", "mp_poas_estimate_repeat_counts",
 ]
 
 _lib = None
@@ -155,6 +155,7 @@ def lib():
         "mp_poa_anchor_alignments": (C.c_int, [vp, vp, i64, vp, i64, vp, vp, i64]),
         "mp_poa_consensus": (C.c_int, [vp, i32, dbl, vp, i64, P(i64), vp]),
         "mp_poa_scores": (C.c_int, [vp, vp]),
+        "mp_poas_estimate_repeat_counts": (C.c_int, [P(vp), i64, vp, vp, vp, vp, P(vp), i32]),
         "mp_merge_chunks": (C.c_int, [vp, P(BandParams), i64, vp, vp, i64, C.c_char_p, vp, i64, P(i64)]),
     }
     for name, (res, args) in sig.items():

@@ -1150,3 +1150,45 @@ extern "C" mp_status mp_poa_scores(const mp_poa *poa, double *scores) {
     return MP_OK;
 }
 
+extern "C" mp_status mp_poas_estimate_repeat_counts(const mp_poa *const *poas, int64_t nPoas, const mp_model *m,
+                                                    const mp_rle_string *reads, const int32_t *readStrand,
+                                                    const int64_t *readStart, uint64_t *const *repeatCounts,
+                                                    int32_t threads) {
+    if (!poas || nPoas < 0 || !m || (nPoas > 0 && (!readStart || !repeatCounts))) {
+        mp::set_error("mp_poas_estimate_repeat_counts: bad argument");
+        return MP_EINVAL;
+    }
+    for (int64_t g = 0; g < nPoas; g++)
+        if (!poas[g] || !repeatCounts[g] || readStart[g + 1] < readStart[g]) {
+            mp::set_error("mp_poas_estimate_repeat_counts: bad graph " + std::to_string(g));
+            return MP_EINVAL;
+        }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, nPoas));
+    std::atomic<int64_t> next{0};
+    std::atomic<int64_t> badGraph{-1};
+    std::vector<std::string> errs((size_t)std::max<int64_t>(nPoas, 1));
+    std::vector<mp_status> sts((size_t)std::max<int64_t>(nPoas, 1), MP_OK);
+    auto work = [&]() {
+        for (int64_t g; (g = next.fetch_add(1)) < nPoas;) {
+            const int64_t r0 = readStart[g];
+            sts[g] = mp_poa_estimate_repeat_counts(poas[g], m, reads + r0, readStrand + r0, readStart[g + 1] - r0,
+                                                   repeatCounts[g], 1);
+            if (sts[g] != MP_OK) {
+                errs[g] = mp_last_error();
+                int64_t none = -1;
+                badGraph.compare_exchange_strong(none, g);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < threads; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    if (badGraph.load() >= 0) {
+        mp::set_error("graph " + std::to_string(badGraph.load()) + ": " + errs[badGraph.load()]);
+        return sts[badGraph.load()];
+    }
+    return MP_OK;
+}
+

@@ -242,3 +242,33 @@ def augment_batch(batch, poas, pair_poa, reads, pair_read_no=None, compare_repea
     check(lib().mp_batch_poa_augment(batch.handle, handles, len(poas), pp.ctypes.data, rn.ctypes.data,
                                      rs.ctypes.data, int(bool(compare_repeat_counts)), int(threads)))
     del packed
+
+
+def estimate_repeat_counts_many(graphs, machines, reads_per_graph, strands_per_graph, threads=0):
+    """poa_estimateRepeatCountsUsingBayesianModel for many graphs in one
+    call (graphs in parallel): reads_per_graph[g] = graph g's reads by readNo
+    (a list of (bases, counts) or a PackedReads), strands_per_graph[g] their
+    strands.  Returns one uint64 array of run lengths per graph."""
+    recs, strands, start = [], [], [0]
+    keep = []
+    for rs, st in zip(reads_per_graph, strands_per_graph):
+        pk = rs if isinstance(rs, PackedReads) else PackedReads(rs)
+        keep.append(pk)
+        recs.append(pk.records[: len(pk)])
+        strands.append(np.asarray(list(st), dtype=np.int32))
+        start.append(start[-1] + len(pk))
+    allrec = np.concatenate(recs) if recs else np.zeros(1, _lib.RLE_STRING_DTYPE)
+    if allrec.size == 0:
+        allrec = np.zeros(1, _lib.RLE_STRING_DTYPE)
+    allst = np.concatenate(strands).astype(np.int32) if strands else np.zeros(1, np.int32)
+    if allst.size == 0:
+        allst = np.zeros(1, np.int32)
+    st_arr = np.asarray(start, dtype=np.int64)
+    outs = [np.zeros(max(g.sizes()["nodes"] - 1, 1), dtype=np.uint64) for g in graphs]
+    hs = (C.c_void_p * max(len(graphs), 1))(*[g.handle for g in graphs])
+    ptrs = (C.c_void_p * max(len(graphs), 1))(*[o.ctypes.data for o in outs])
+    check(lib().mp_poas_estimate_repeat_counts(hs, len(graphs), machines.handle, allrec.ctypes.data, allst.ctypes.data,
+                                               st_arr.ctypes.data, ptrs, int(threads)))
+    del keep
+    return [o[: g.sizes()["nodes"] - 1] for o, g in zip(outs, graphs)]
+

@@ -197,6 +197,10 @@ def test_repeat_count_estimates_match_oracle(table):
     got = g.estimate_repeat_counts(mach, reads, strands)
     want = P.estimate_repeat_counts(o, reads_o, [bool(s) for s in strands], rep)
     assert got.tolist() == want
+    # several graphs in one call (the same graph twice, and an empty one)
+    empty = mpoa.Poa(ref)
+    many = mpoa.estimate_repeat_counts_many([g, empty, g], mach, [reads, [], reads], [strands, [], strands])
+    assert many[0].tolist() == want and many[2].tolist() == want and set(many[1].tolist()) == {1}
     assert len(set(want)) > 3
 
 
