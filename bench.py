@@ -44,10 +44,11 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=1200)
     ap.add_argument("--err-pairs", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("config1", "config2", "config4", "bubbles"), default="config1",
+    ap.add_argument("--workload", choices=("config1", "config2", "config4", "bubbles", "polish"), default="config1",
                     help="config1 (the headline batch), config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks), "
                          "config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at one --band) or "
-                         "bubbles (bubble-graph per-read x per-allele forward likelihoods)")
+                         "bubbles (bubble-graph per-read x per-allele forward likelihoods) or polish (a simulated "
+                         "draft polished end to end: realign, consensus rounds, run lengths, merge)")
     ap.add_argument("--band", type=int, default=128, help="config4: anchor expansion W (128, 256, 512, 1024)")
     ap.add_argument("--pairs", type=int, default=256, help="config4: number of 100 kb pairs")
     ap.add_argument("--max-chunks", type=int, default=None, help="limit the number of chunks (config2)")
@@ -67,6 +68,8 @@ def main():
         return run_config4(args)
     if args.workload == "bubbles":
         return run_bubbles(args)
+    if args.workload == "polish":
+        return run_polish(args)
 
     if args.copy_calib_mib > 0:
         import torch
@@ -510,6 +513,47 @@ def run_bubbles(args):
                                          "(FP64 log space, polynomial logAdd), one thread, called per pair"
                                          % (len(samp), npairs, c_cells),
                                "gcups": round(c_cells / tc / 1e9, 5)}
+    print(json.dumps(out), flush=True)
+
+
+def run_polish(args):
+    """A 400 kb draft (3% error) polished end to end from 60x reads at 8%
+    error (plain-string params, 3 consensus rounds): realign batch + POA
+    graphs, consensus rounds, merge.  value = reads/s of the whole run."""
+    import numpy as np
+    from marginpolish_amd import pairhmm, polish, sim
+    from tests.test_gpu_polish import banded_edit_distance
+    params_path = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.no_rle.json")
+    t0 = time.time()
+    truth, draft, chunks = sim.polish_workload(seed=11, genome_len=400_000, depth=60, mean=8000, sd=6000,
+                                               chunk_size=10_000, boundary=250, rle=False)
+    nreads = sum(len(c.reads) for c in chunks)
+    log("polish: %d chunks, %d chunk reads, built in %.1f s" % (len(chunks), nreads, time.time() - t0))
+    params = pairhmm.PolishParams(params_path)
+    cfg = polish.PolishConfig(params_path)
+    cfg.maxRealignmentPolishIterations = 0
+    cfg.maxPoaConsensusIterations = 3
+    cfg.chunkBoundary = 250
+    refs, reads, anchors = [c.ref for c in chunks], [c.reads for c in chunks], [c.anchors for c in chunks]
+    eng = pairhmm.default_engine()
+    runs = []
+    for _ in range(2):
+        t1 = time.perf_counter()
+        merged = polish.polish_contig(params, cfg, refs, reads, anchors, engine=eng)
+        runs.append(time.perf_counter() - t1)
+    chars = np.frombuffer(b"ACGT", np.uint8)
+    t_str, d_str = chars[truth].tobytes().decode(), chars[draft].tobytes().decode()
+    d0 = banded_edit_distance(d_str, t_str, 1500)
+    d1 = banded_edit_distance(merged, t_str, 1500)
+    log("polish: %.2f s (first %.2f s); edit distance to truth: draft %d, polished %d" % (runs[1], runs[0], d0, d1))
+    out = {"metric": "end-to-end polishing reads/s (realign + consensus rounds + merge)",
+           "value": round(nreads / runs[1], 1), "unit": "reads/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(runs[1] * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "polish: 400 kb draft at 3% error, 60x reads (mean 8 kb) at 8%, 40 chunks of 10 kb "
+                                  "(+-250 bp), plain-string params (allParams.np.no_rle.json), 3 consensus rounds",
+                      "chunk_reads": nreads},
+           "draft_edit_distance": d0, "polished_edit_distance": d1}
     print(json.dumps(out), flush=True)
 
 
