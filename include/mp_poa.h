@@ -137,7 +137,9 @@ mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *poaToConsen
    entries mapping node i to the position of its base in the run-length (or
    plain) consensus, -1 where none.  A reference run length at or beyond the
    graph's maxRepeatCount reads as weight 0 (the reference reads past its
-   array there). */
+   array there).  The result is kept with the graph until it changes, so a
+   size query followed by the fill computes it once (calls on one graph must
+   not overlap). */
 mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEncoding, double referenceBasePenalty, char *out,
                            int64_t outCap, int64_t *outLen, int64_t *poaToConsensusMap);
 

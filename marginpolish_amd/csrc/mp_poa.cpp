@@ -106,6 +106,14 @@ struct mp_poa {
     // insert / delete observations in append order, with the entry each belongs to
     std::vector<int32_t> insObsEnt, delObsEnt;
     std::vector<mp_poa_observation> insObs, delObs;
+    // last consensus (a size query followed by the fill does not recompute it);
+    // `gen` counts changes to the graph
+    std::atomic<uint64_t> gen{0};
+    mutable uint64_t consGen = ~0ull;
+    mutable int32_t consRle = -1;
+    mutable double consPenalty = 0.0;
+    mutable std::string consString;
+    mutable std::vector<int64_t> consMap;
 
     Str ref() const { return Str{refB.data(), refC.data(), (int64_t)refB.size()}; }
     int64_t nodes() const { return (int64_t)nodeBase.size(); }
@@ -297,6 +305,7 @@ void read_events(const Str &ref, const Str &read, const mp_aligned_pair *m, int6
 // deletes.  The three touch disjoint parts of the graph, so for a sequence
 // of reads each part may be brought up to date on its own.
 void apply_matches(mp_poa *g, const Str &read, int64_t readNo, const mp_aligned_pair *m, int64_t nm) {
+    g->gen.fetch_add(1, std::memory_order_relaxed);
     const int64_t mrc = g->maxRC;
     if (nm > 0) {
         g->obsSeg.emplace_back(readNo, (int64_t)g->obsPairs.size());
@@ -313,6 +322,7 @@ void apply_matches(mp_poa *g, const Str &read, int64_t readNo, const mp_aligned_
 }
 
 void apply_inserts(mp_poa *g, const Str &read, bool strand, int64_t readNo, const std::vector<InsEvent> &iev) {
+    g->gen.fetch_add(1, std::memory_order_relaxed);
     for (const InsEvent &e : iev) {
         const char *sb = read.b + e.start;
         const uint64_t *sc = read.c + e.start;
@@ -366,6 +376,7 @@ void apply_inserts(mp_poa *g, const Str &read, bool strand, int64_t readNo, cons
 }
 
 void apply_deletes(mp_poa *g, bool strand, int64_t readNo, const std::vector<DelEvent> &dev) {
+    g->gen.fetch_add(1, std::memory_order_relaxed);
     for (const DelEvent &e : dev) {
         int32_t hit = -1;
         for (int32_t q = g->delHead[e.pos]; q >= 0; q = g->del[q].next)
@@ -976,6 +987,19 @@ extern "C" mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEnc
     const double kNeg = -std::numeric_limits<double>::infinity();
     try {
         const int64_t n = poa->nodes();
+        if (poa->consGen == poa->gen.load() && poa->consRle == (rle ? 1 : 0) &&
+            poa->consPenalty == referenceBasePenalty) {  // computed for this graph and these arguments
+            const int64_t total = (int64_t)poa->consString.size();
+            *outLen = total;
+            if (poaToConsensusMap && n > 1)
+                std::memcpy(poaToConsensusMap, poa->consMap.data(), sizeof(int64_t) * (size_t)(n - 1));
+            if (!out || outCap < total) {
+                mp::set_error("mp_poa_consensus: output buffer too small (need " + std::to_string(total) + ")");
+                return MP_EINVAL;
+            }
+            if (total) std::memcpy(out, poa->consString.data(), (size_t)total);
+            return MP_OK;
+        }
         const int64_t mrc = poa->maxRC;
         auto ins_weight = [&](int32_t q) { return poa->ins[q].wF + poa->ins[q].wR; };
         auto del_weight = [&](int32_t q) { return poa->del[q].wF + poa->del[q].wR; };
@@ -1113,13 +1137,18 @@ extern "C" mp_status mp_poa_consensus(const mp_poa *poa, int32_t useRunLengthEnc
             for (size_t t = 0; t < expanded.size(); t++)
                 if (t == 0 || expanded[t] != expanded[t - 1]) clen++;
         }
-        if (poaToConsensusMap)
-            for (int64_t i = 0; i + 1 < n; i++) poaToConsensusMap[i] = map[i] == -1 ? -1 : clen - 1 - map[i];
+        for (int64_t i = 0; i + 1 < n; i++) map[i] = map[i] == -1 ? -1 : clen - 1 - map[i];
+        if (poaToConsensusMap && n > 1) std::memcpy(poaToConsensusMap, map.data(), sizeof(int64_t) * (size_t)(n - 1));
+        poa->consString.swap(expanded);
+        poa->consMap.swap(map);
+        poa->consGen = poa->gen.load();
+        poa->consRle = rle ? 1 : 0;
+        poa->consPenalty = referenceBasePenalty;
         if (!out || outCap < total) {
             mp::set_error("mp_poa_consensus: output buffer too small (need " + std::to_string(total) + ")");
             return MP_EINVAL;
         }
-        std::memcpy(out, expanded.data(), (size_t)total);
+        if (total) std::memcpy(out, poa->consString.data(), (size_t)total);
     } catch (const std::bad_alloc &) {
         mp::set_error("mp_poa_consensus: out of host memory");
         return MP_ENOMEM;

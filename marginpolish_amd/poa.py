@@ -168,12 +168,10 @@ class Poa:
         cm = None if poa_to_consensus is None else np.ascontiguousarray(np.asarray(poa_to_consensus, np.int64))
         counts = np.zeros(max(n_reads, 1), dtype=np.int64)
         cmp_ = None if cm is None else cm.ctypes.data
+        cap = max(self.sizes()["nodeObservations"], 1)  # every anchor comes from one base observation
+        buf = np.empty((cap, 3), dtype=np.int64)
         check(lib().mp_poa_anchor_alignments(self._h, cmp_, int(n_reads), ap.ctypes.data, ap.size,
-                                             counts.ctypes.data, None, 0))
-        tot = int(counts[:n_reads].sum())
-        buf = np.zeros((max(tot, 1), 3), dtype=np.int64)
-        check(lib().mp_poa_anchor_alignments(self._h, cmp_, int(n_reads), ap.ctypes.data, ap.size,
-                                             counts.ctypes.data, buf.ctypes.data, tot))
+                                             counts.ctypes.data, buf.ctypes.data, cap))
         ends = np.cumsum(counts[:n_reads])
         return [buf[e - c:e] for e, c in zip(ends, counts[:n_reads])]
 

@@ -11,6 +11,8 @@ built here and is refused rather than skipped.
 from __future__ import annotations
 
 import json
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -76,7 +78,7 @@ def realign(params, cfg, refs, reads, anchors, engine=None):
 def _estimate(params, cfg, graph, ref, reads):
     if not cfg.useRunLengthEncoding:
         return ref
-    counts = graph.estimate_repeat_counts(params.machines, [r for r, _ in reads], [s for _, s in reads])
+    counts = graph.estimate_repeat_counts(params.machines, [r for r, _ in reads], [s for _, s in reads], threads=1)
     return np.asarray(ref[0], np.uint8), counts
 
 
@@ -92,39 +94,53 @@ def realign_all(params, cfg, refs, reads, anchors, engine=None):
     graphs = realign(params, cfg, refs, reads, anchors, engine)
     ref_strings = [(np.frombuffer(bytes(r[0]), np.uint8) if isinstance(r[0], (bytes, bytearray))
                     else np.asarray(r[0], np.uint8), np.asarray(r[1], np.uint64)) for r in refs]
-    scores = [g.score() for g in graphs]
-    active = [True] * n
-    it = 0
-    while it < cfg.maxPoaConsensusIterations and any(active):
-        it += 1
-        todo, new_refs, new_anchors = [], [], []
-        for c in range(n):
-            if not active[c]:
-                continue
-            cons, cmap = graphs[c].consensus(cfg.useRunLengthEncoding, cfg.referenceBasePenalty)
-            cons_rle = _rle_of(cons) if cfg.useRunLengthEncoding else (
-                np.frombuffer(cons.encode(), np.uint8).copy(), np.ones(len(cons), np.uint64))
-            if _same_rle(cons_rle, ref_strings[c]):  # greedy convergence
-                active[c] = False
-                continue
-            todo.append(c)
-            new_refs.append(cons_rle)
-            new_anchors.append(graphs[c].anchor_alignments(len(reads[c]), cfg.minPosteriorProbForAlignmentAnchors,
-                                                           cmap))
-        if not todo:
-            break
-        new_graphs = realign(params, cfg, new_refs, [reads[c] for c in todo], new_anchors, engine)
-        for c, g2, ref2 in zip(todo, new_graphs, new_refs):
-            ref2 = _estimate(params, cfg, g2, ref2, reads[c])
-            s2 = g2.score()
-            if s2 <= scores[c] and it > cfg.minPoaConsensusIterations:
-                active[c] = False  # the score did not improve: keep the previous graph
-                continue
-            graphs[c], ref_strings[c], scores[c] = g2, ref2, s2
+    # per-chunk host steps run on a thread pool (the library calls release the interpreter)
+    pool = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    try:
+        scores = list(pool.map(lambda g: g.score(), graphs))
+        active = [True] * n
+        it = 0
+        while it < cfg.maxPoaConsensusIterations and any(active):
+            it += 1
+
+            def next_round(c):
+                cons, cmap = graphs[c].consensus(cfg.useRunLengthEncoding, cfg.referenceBasePenalty)
+                cons_rle = _rle_of(cons) if cfg.useRunLengthEncoding else (
+                    np.frombuffer(cons.encode(), np.uint8).copy(), np.ones(len(cons), np.uint64))
+                if _same_rle(cons_rle, ref_strings[c]):  # greedy convergence
+                    return c, None, None
+                return c, cons_rle, graphs[c].anchor_alignments(len(reads[c]),
+                                                                cfg.minPosteriorProbForAlignmentAnchors, cmap)
+
+            todo, new_refs, new_anchors = [], [], []
+            for c, cons_rle, an in pool.map(next_round, [c for c in range(n) if active[c]]):
+                if cons_rle is None:
+                    active[c] = False
+                else:
+                    todo.append(c)
+                    new_refs.append(cons_rle)
+                    new_anchors.append(an)
+            if not todo:
+                break
+            new_graphs = realign(params, cfg, new_refs, [reads[c] for c in todo], new_anchors, engine)
+
+            def judge(k):
+                c, g2, ref2 = todo[k], new_graphs[k], new_refs[k]
+                return _estimate(params, cfg, g2, ref2, reads[c]), g2.score()
+
+            for k, (ref2, s2) in enumerate(pool.map(judge, range(len(todo)))):
+                c = todo[k]
+                if s2 <= scores[c] and it > cfg.minPoaConsensusIterations:
+                    active[c] = False  # the score did not improve: keep the previous graph
+                    continue
+                graphs[c], ref_strings[c], scores[c] = new_graphs[k], ref2, s2
+        final = list(pool.map(lambda c: _estimate(params, cfg, graphs[c], ref_strings[c], reads[c]), range(n)))
+    finally:
+        pool.shutdown()
     out = []
     for c in range(n):
-        b, cnt = _estimate(params, cfg, graphs[c], ref_strings[c], reads[c])
-        out.append((graphs[c], "".join(chr(x) * int(k) for x, k in zip(b.tolist(), cnt.tolist()))))
+        b, cnt = final[c]
+        out.append((graphs[c], np.repeat(b, cnt.astype(np.int64)).tobytes().decode()))
     return out
 
 
