@@ -378,3 +378,75 @@ def polish_workload(seed=7, genome_len=30000, draft_err=0.03, read_err=0.08, dep
         del ch._rid
     return truth, draft, chunks
 
+
+def polish_workload_rle(repeat_log10, seed=9, runs=12000, draft_err=0.03, read_err=0.05, depth=20, mean_runs=1500,
+                        chunk_runs=4000, boundary_runs=100, W=10):
+    """A run-length draft to polish, with reads drawn from the run-length
+    error model the params encode: truth = `runs` runs (GC 45%, lengths ~
+    Geometric(0.45)); the draft = truth with `draft_err` of its runs given a
+    length off by one (so draft run i is truth run i); every read covers a
+    span of truth runs, each run's observed length drawn from
+    P(observed | true length) = 10^repeat_log10[base row][true][observed]
+    (the reverse strand reads the complement's row) and, at `read_err`, a run
+    dropped or an extra run inserted.  Chunks of the draft in run coordinates.
+    Returns (truth (bases, counts), draft (bases, counts), chunks)."""
+    rng = np.random.default_rng(seed)
+    tab = np.asarray(repeat_log10, dtype=np.float64).reshape(-1, MAX_REPEAT_EXCLUSIVE, MAX_REPEAT_EXCLUSIVE)[:4]
+    prob = 10.0 ** tab
+    prob /= prob.sum(axis=2, keepdims=True)
+    cdf = np.cumsum(prob, axis=2)
+    pa = 0.275
+    p4 = np.array([pa, 0.225, 0.225, pa])
+    bases = rng.choice(4, size=runs, p=p4).astype(np.uint8)
+    for _ in range(64):
+        eq = np.flatnonzero(bases[1:] == bases[:-1]) + 1
+        if eq.size == 0:
+            break
+        bases[eq] = rng.choice(4, size=eq.size, p=p4)
+    lens = np.minimum(rng.geometric(0.45, size=runs), MAX_REPEAT_EXCLUSIVE - 1)
+    dlens = lens.copy()
+    flip = rng.random(runs) < draft_err
+    dlens[flip] = np.maximum(1, dlens[flip] + np.where(rng.random(int(flip.sum())) < 0.5, -1, 1))
+    chars = np.frombuffer(b"ACGT", np.uint8)
+    nreads = int(round(depth * runs / mean_runs))
+    chunks = []
+    for c in range((runs + chunk_runs - 1) // chunk_runs):
+        cs, ce = max(0, c * chunk_runs - boundary_runs), min(runs, (c + 1) * chunk_runs + boundary_runs)
+        chunks.append(PolishChunk(cs, ce, (chars[bases[cs:ce]].tobytes(), dlens[cs:ce].astype(np.uint64)), [], []))
+    for _ in range(nreads):
+        n = int(min(runs, max(50, rng.normal(mean_runs, mean_runs / 3))))
+        s0 = int(rng.integers(0, runs - n + 1))
+        strand = int(rng.integers(0, 2))
+        rb, rl, rx = [], [], []  # read runs and the truth run each came from (-1: inserted)
+        for t in range(s0, s0 + n):
+            u = rng.random()
+            if u < read_err / 2:
+                continue  # run dropped
+            if u < read_err:
+                nb = int(rng.integers(0, 4))
+                if (not rb or nb != rb[-1]) and nb != bases[t]:
+                    rb.append(nb)
+                    rl.append(1)
+                    rx.append(-1)
+            b = int(bases[t])
+            row = b if strand else 3 - b
+            o = int(np.searchsorted(cdf[row, lens[t]], rng.random() * cdf[row, lens[t], -1]))
+            o = max(1, min(o, MAX_REPEAT_EXCLUSIVE - 1))
+            if rb and rb[-1] == b:  # a dropped run joined two equal runs: merge them
+                rl[-1] += o
+                continue
+            rb.append(b)
+            rl.append(o)
+            rx.append(t)
+        rb, rl, rx = np.array(rb, np.uint8), np.array(rl, np.uint64), np.array(rx, np.int64)
+        for ch in chunks:
+            sel = np.flatnonzero((rx >= ch.start) & (rx < ch.end))
+            if sel.size < 2:
+                continue
+            a, e = sel[0], sel[-1] + 1
+            X, Y = rx[sel] - ch.start, sel - a
+            keep = np.r_[True, (X[1:] > X[:-1]) & (Y[1:] > Y[:-1])]
+            ch.reads.append(((chars[rb[a:e]].tobytes(), rl[a:e].copy()), strand))
+            ch.anchors.append(np.stack([X[keep], Y[keep], np.full(int(keep.sum()), W)], axis=1).astype(np.int64))
+    return (chars[bases].tobytes(), lens.astype(np.uint64)), (chars[bases].tobytes(), dlens.astype(np.uint64)), chunks
+

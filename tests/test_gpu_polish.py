@@ -98,3 +98,38 @@ def test_default_params_refuse_the_unbuilt_polish_path():
     truth, draft, chunks = sim.polish_workload(seed=1, genome_len=3000, chunk_size=3000)
     with pytest.raises(NotImplementedError):
         polish.realign_all(ph.PolishParams(PARAMS), cfg, [chunks[0].ref], [chunks[0].reads], [chunks[0].anchors])
+
+
+def test_polish_rle_fixes_run_lengths():
+    """Reads drawn from the params' own run-length error model: on covered
+    short runs (true length <= 3, where P(observed | length) separates the
+    lengths) the maximum-likelihood estimate repairs most of the draft's
+    off-by-one runs.  Longer runs stay hard for the reference's estimator:
+    its candidates only span the observed lengths, and the model under-calls
+    long runs (measured on a CPU-built graph: 96 wrong of 2891 covered runs,
+    mostly of length >= 5, against 51 in the draft)."""
+    from oracle import oracle as O
+    truth, draft, chunks = sim.polish_workload_rle(O.PolishParams(PARAMS).repeat)
+    params = ph.PolishParams(PARAMS)
+    cfg = polish.PolishConfig(PARAMS)
+    cfg.maxRealignmentPolishIterations = 0
+    cfg.chunkBoundary = 0
+    cfg.maxPoaConsensusIterations = 0
+    refs, reads, anchors = [c.ref for c in chunks], [c.reads for c in chunks], [c.anchors for c in chunks]
+    res = polish.realign_all(params, cfg, refs, reads, anchors)
+    est_wrong = draft_wrong = short = 0
+    for c, (g, s) in zip(chunks, res):
+        b = np.frombuffer(s.encode(), np.uint8)
+        lens = np.diff(np.r_[np.flatnonzero(np.r_[True, b[1:] != b[:-1]]), b.size])
+        st, _ = g.observations(0)
+        covered = np.diff(st)[1:] > 0
+        tl = truth[1][c.start:c.end].astype(np.int64)
+        dl = draft[1][c.start:c.end].astype(np.int64)
+        assert lens.size == tl.size  # the bases are kept: one polished run per draft run
+        sel = covered & (tl <= 3)
+        short += int(sel.sum())
+        est_wrong += int((lens[sel] != tl[sel]).sum())
+        draft_wrong += int((dl[sel] != tl[sel]).sum())
+    print("short covered runs %d: draft wrong %d, estimate wrong %d" % (short, draft_wrong, est_wrong), flush=True)
+    assert short > 0.8 * int((truth[1] <= 3).sum())
+    assert est_wrong < draft_wrong / 2
