@@ -230,19 +230,13 @@ def _as_anchors(a):
     return arr
 
 
-class Batch:
-    """A device-resident batch of read/reference pairs (mp_batch)."""
+class PairArray:
+    """Pairs laid out once as the library's mp_pair array (what a C caller
+    hands to mp_batch_create): build it once and create batches from it
+    without re-marshalling the Python tuples."""
 
-    def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None,
-                 fp64=False, graph=True):
-        """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight]).
-        fp64: FP64 arithmetic on the per-cell-exponent kernel (the precision
-        comparison path; 3-state machines run with the long gaps unreachable).
-        graph: replay the batch's captured hipGraph on every run (False:
-        enqueue the launches directly)."""
-        self.engine = engine or default_engine()
+    def __init__(self, pairs):
         n = len(pairs)
-        # the mp_pair array as a numpy record array of the same layout, filled column by column
         arr = np.zeros(max(n, 1), dtype=_lib.PAIR_DTYPE)
         sxs = [_as_u16(pr[0]) for pr in pairs]
         sys_ = [_as_u16(pr[1]) for pr in pairs]
@@ -258,8 +252,30 @@ class Batch:
             arr["strand"][:n] = [int(pr[3]) for pr in pairs]
             arr["raggedLeft"][:n] = [int(bool(pr[4])) if len(pr) > 4 else 0 for pr in pairs]
             arr["raggedRight"][:n] = [int(bool(pr[5])) if len(pr) > 5 else 0 for pr in pairs]
-        self._arr = arr
-        arr = arr.ctypes.data_as(C.POINTER(_lib.Pair))
+        self.array = arr
+        self.n = n
+
+    def __len__(self):
+        return self.n
+
+
+class Batch:
+    """A device-resident batch of read/reference pairs (mp_batch)."""
+
+    def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None,
+                 fp64=False, graph=True):
+        """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight]),
+        or a PairArray built from one.
+        fp64: FP64 arithmetic on the per-cell-exponent kernel (the precision
+        comparison path; 3-state machines run with the long gaps unreachable).
+        graph: replay the batch's captured hipGraph on every run (False:
+        enqueue the launches directly)."""
+        self.engine = engine or default_engine()
+        # the mp_pair array (a numpy record array of the same layout)
+        pa = pairs if isinstance(pairs, PairArray) else PairArray(pairs)
+        n = len(pa)
+        self._pairs = pa
+        arr = pa.array.ctypes.data_as(C.POINTER(_lib.Pair))
         h = C.c_void_p()
         pc = p.c()
         check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
