@@ -120,3 +120,38 @@ def test_batch_augment_refuses_bad_input(config1_batch):
     short = [mpoa.Poa((r[0][:10], r[1][:10])) for r in refs]
     with pytest.raises(Exception):
         mpoa.augment_batch(b, short, pair_poa, reads, read_no)
+
+
+def test_batch_augment_edge_cases(config1_batch):
+    """Pairs of several graphs interleaved in the batch, a graph no pair
+    uses, and read numbers given out of order: each graph still sees its
+    pairs in batch order, as the per-read calls would."""
+    w = sim.config1(max_chunks=2)
+    pp = ph.PolishParams(PARAMS)
+    idx = list(range(len(w.pairs)))
+    rng = np.random.default_rng(4)
+    rng.shuffle(idx)  # chunks interleaved
+    idx = idx[:120]
+    pairs = [(w.pairs[i].ref_sym, w.pairs[i].read_sym, w.pairs[i].anchors, w.pairs[i].strand) for i in idx]
+    b = ph.Batch(pp.machines, pairs, pp.p, crop_reference=True)
+    b.run()
+    lists = b.fetch()
+    chunks = sorted(set(p.chunk for p in w.pairs))
+    first = {}
+    for p in w.pairs:
+        first.setdefault(p.chunk, p)
+    refs = [mpoa.rle_from_symbols(first[c].ref_sym, first[c].ref_runs) for c in chunks]
+    refs.append(refs[0])  # a graph no pair uses
+    pair_poa = np.array([chunks.index(w.pairs[i].chunk) for i in idx], dtype=np.int64)
+    read_no = rng.permutation(len(idx)).astype(np.int64)
+    reads = [mpoa.rle_from_symbols(w.pairs[i].read_sym, w.pairs[i].read_runs) for i in idx]
+    g_batch = [mpoa.Poa(r) for r in refs]
+    mpoa.augment_batch(b, g_batch, pair_poa, reads, read_no, threads=5)
+    g_read = [mpoa.Poa(r) for r in refs]
+    for k, i in enumerate(idx):
+        m, gx, gy, _ = lists[k]
+        g_read[pair_poa[k]].augment(reads[k], w.pairs[i].strand, read_no[k], m, gy, gx, True)
+    for a, c in zip(g_batch, g_read):
+        assert_same_graph(native_as_oracle(a), _AsOracle(native_as_oracle(c)))
+    assert g_batch[-1].sizes()["nodeObservations"] == 0
+    b.close()
