@@ -375,6 +375,122 @@ void apply_inserts(mp_poa *g, const Str &read, bool strand, int64_t readNo, cons
     }
 }
 
+// One node range's share of a graph's insert task.  Entries the graph held
+// before this call are updated in place (their nodes belong to this range
+// only); entries new in this call are built here and merged into the graph
+// afterwards, in this range's creation order, so every node's list and every
+// entry's sums and observations come out as the sequential loop makes them.
+struct InsShard {
+    int64_t n0 = 0, n1 = 0;
+    std::vector<InsEntry> ins;
+    std::vector<char> b;
+    std::vector<uint64_t> c;
+    std::vector<int32_t> tab;
+    std::vector<int32_t> obsEnt;  // >= 0: local entry; < 0: the graph's entry -(value + 1)
+    std::vector<mp_poa_observation> obs;
+};
+
+inline bool same_insert(const char *eb, const uint64_t *ec, const InsEntry &E, const InsEvent &e, const char *sb,
+                        const uint64_t *sc) {
+    if (E.hash != e.hash || E.node != e.pos || E.len != e.len) return false;
+    for (int64_t t = 0; t < e.len; t++) {
+        const int64_t src = (t - e.rot + e.len) % e.len;
+        if (eb[E.sym + t] != sb[src] || ec[E.sym + t] != sc[src]) return false;
+    }
+    return true;
+}
+
+void shard_inserts(mp_poa *g, InsShard &S, const Str &read, bool strand, int64_t readNo,
+                   const std::vector<InsEvent> &iev) {
+    for (const InsEvent &e : iev) {
+        if (e.pos < S.n0 || e.pos >= S.n1) continue;
+        const char *sb = read.b + e.start;
+        const uint64_t *sc = read.c + e.start;
+        int32_t hit = -1;
+        if (!g->insTab.empty()) {  // the graph's own entries (index read-only here)
+            const size_t mask = g->insTab.size() - 1;
+            for (size_t slot = slot_key(e.hash, e.pos) & mask; g->insTab[slot] >= 0; slot = (slot + 1) & mask)
+                if (same_insert(g->insB.data(), g->insC.data(), g->ins[g->insTab[slot]], e, sb, sc)) {
+                    hit = g->insTab[slot];
+                    break;
+                }
+        }
+        if (hit >= 0) {
+            InsEntry &E = g->ins[hit];
+            (strand ? E.wF : E.wR) += e.w;
+            S.obsEnt.push_back(-(hit + 1));
+            S.obs.push_back(mp_poa_observation{readNo, e.start, e.w});
+            continue;
+        }
+        if (2 * (S.ins.size() + 1) > S.tab.size()) {
+            const size_t cap = std::max<size_t>(256, 2 * S.tab.size());
+            S.tab.assign(cap, -1);
+            for (size_t q = 0; q < S.ins.size(); q++) {
+                size_t h = slot_key(S.ins[q].hash, S.ins[q].node) & (cap - 1);
+                while (S.tab[h] >= 0) h = (h + 1) & (cap - 1);
+                S.tab[h] = (int32_t)q;
+            }
+        }
+        const size_t mask = S.tab.size() - 1;
+        size_t slot = slot_key(e.hash, e.pos) & mask;
+        for (; S.tab[slot] >= 0; slot = (slot + 1) & mask)
+            if (same_insert(S.b.data(), S.c.data(), S.ins[S.tab[slot]], e, sb, sc)) {
+                hit = S.tab[slot];
+                break;
+            }
+        if (hit < 0) {
+            hit = (int32_t)S.ins.size();
+            S.tab[slot] = hit;
+            S.ins.push_back(InsEntry{e.hash, e.pos, (int64_t)S.b.size(), e.len, 0.0, 0.0, -1});
+            for (int64_t t = 0; t < e.len; t++) {
+                const int64_t src = (t - e.rot + e.len) % e.len;
+                S.b.push_back(sb[src]);
+                S.c.push_back(sc[src]);
+            }
+        }
+        InsEntry &E = S.ins[hit];
+        (strand ? E.wF : E.wR) += e.w;
+        S.obsEnt.push_back(hit);
+        S.obs.push_back(mp_poa_observation{readNo, e.start, e.w});
+    }
+}
+
+// The shards' new entries into the graph (shards in node order), then the index.
+void merge_shards(mp_poa *g, std::vector<InsShard> &shards) {
+    g->gen.fetch_add(1, std::memory_order_relaxed);
+    for (InsShard &S : shards) {
+        const int32_t base = (int32_t)g->ins.size();
+        const int64_t symBase = (int64_t)g->insB.size();
+        g->insB.insert(g->insB.end(), S.b.begin(), S.b.end());
+        g->insC.insert(g->insC.end(), S.c.begin(), S.c.end());
+        for (const InsEntry &E : S.ins) {
+            const int32_t q = (int32_t)g->ins.size();
+            InsEntry F = E;
+            F.sym += symBase;
+            F.next = -1;
+            g->ins.push_back(F);
+            if (g->insTail[E.node] >= 0)
+                g->ins[g->insTail[E.node]].next = q;
+            else
+                g->insHead[E.node] = q;
+            g->insTail[E.node] = q;
+        }
+        for (size_t i = 0; i < S.obsEnt.size(); i++) {
+            g->insObsEnt.push_back(S.obsEnt[i] >= 0 ? base + S.obsEnt[i] : -S.obsEnt[i] - 1);
+            g->insObs.push_back(S.obs[i]);
+        }
+        std::vector<char>().swap(S.b);
+    }
+    size_t cap = 1024;
+    while (cap < 2 * (g->ins.size() + 1)) cap *= 2;
+    g->insTab.assign(cap, -1);
+    for (size_t q = 0; q < g->ins.size(); q++) {
+        size_t h = slot_key(g->ins[q].hash, g->ins[q].node) & (cap - 1);
+        while (g->insTab[h] >= 0) h = (h + 1) & (cap - 1);
+        g->insTab[h] = (int32_t)q;
+    }
+}
+
 void apply_deletes(mp_poa *g, bool strand, int64_t readNo, const std::vector<DelEvent> &dev) {
     g->gen.fetch_add(1, std::memory_order_relaxed);
     for (const DelEvent &e : dev) {
@@ -467,8 +583,54 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
     size_t readyHead = 0;
     std::atomic<int64_t> nextEv{0}, nextMatch{0}, done{0};
     const int64_t totalTasks = 3 * (int64_t)poaOrder.size();
+    // a graph's insert task with many events runs as node-range shards; the
+    // last shard to finish merges them (kind 10 + shard index)
+    struct ShardSet {
+        std::vector<InsShard> s;
+        std::atomic<int> left{0};
+    };
+    std::vector<std::unique_ptr<ShardSet>> shardSets((size_t)std::max<int64_t>(nPoas, 1));
+    constexpr int64_t kEventsPerShard = 20000;
+    constexpr int kMaxShards = 8;
+    auto push_ready = [&](int64_t p, int kind) {
+        std::lock_guard<std::mutex> lk(readyMu);
+        ready.emplace_back(p, kind);
+    };
     auto run_task = [&](int64_t p, int kind) {
         mp_poa *g = poas[p];
+        if (kind == 1) {
+            int64_t nev = 0;
+            for (int64_t i : byPoa[p]) nev += (int64_t)iev[i].size();
+            const int ns = (int)std::min<int64_t>(kMaxShards, nev / kEventsPerShard);
+            if (ns >= 2) {
+                shardSets[p].reset(new ShardSet);
+                ShardSet &ss = *shardSets[p];
+                ss.s.resize((size_t)ns);
+                const int64_t n = g->nodes();
+                for (int k = 0; k < ns; k++) {
+                    ss.s[k].n0 = n * k / ns;
+                    ss.s[k].n1 = n * (k + 1) / ns;
+                }
+                ss.left.store(ns);
+                for (int k = 0; k < ns; k++) push_ready(p, 10 + k);
+                return;  // counted done when the last shard merges
+            }
+        }
+        if (kind >= 10) {
+            ShardSet &ss = *shardSets[p];
+            InsShard &S = ss.s[kind - 10];
+            for (int64_t i : byPoa[p]) {
+                const Str rd{reads[i].bases, reads[i].repeatCounts, reads[i].length};
+                shard_inserts(g, S, rd, strand[i] != 0, pairReadNo ? pairReadNo[i] : i, iev[i]);
+            }
+            if (ss.left.fetch_sub(1) == 1) {
+                merge_shards(g, ss.s);
+                for (int64_t i : byPoa[p]) std::vector<InsEvent>().swap(iev[i]);
+                shardSets[p].reset();
+                done.fetch_add(1);
+            }
+            return;
+        }
         if (kind == 0) {
             int64_t nmTot = 0;
             for (int64_t i : byPoa[p]) nmTot += starts[0][i + 1] - starts[0][i];
@@ -511,9 +673,8 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
                 pair_lists(i, m, nm, gx, nx, gy, ny);
                 read_events(g->ref(), rd, m, nm, gy, ny, gx, nx, cmp, S, iev[i], dev[i]);
                 if (pending[pairPoa[i]].fetch_sub(1) == 1) {
-                    std::lock_guard<std::mutex> lk(readyMu);
-                    ready.emplace_back(pairPoa[i], 1);
-                    ready.emplace_back(pairPoa[i], 2);
+                    push_ready(pairPoa[i], 1);
+                    push_ready(pairPoa[i], 2);
                 }
                 continue;
             }
