@@ -57,8 +57,8 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (chunks shard across GPUs "
-              "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here (chunks would shard across "
+              "GPUs with no exchange between them). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

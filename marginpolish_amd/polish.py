@@ -124,11 +124,11 @@ def realign_all(params, cfg, refs, reads, anchors, engine=None):
                 break
             new_graphs = realign(params, cfg, new_refs, [reads[c] for c in todo], new_anchors, engine)
 
-            def judge(k):
+            def rescore(k):
                 c, g2, ref2 = todo[k], new_graphs[k], new_refs[k]
                 return _estimate(params, cfg, g2, ref2, reads[c]), g2.score()
 
-            for k, (ref2, s2) in enumerate(pool.map(judge, range(len(todo)))):
+            for k, (ref2, s2) in enumerate(pool.map(rescore, range(len(todo)))):
                 c = todo[k]
                 if s2 <= scores[c] and it > cfg.minPoaConsensusIterations:
                     active[c] = False  # the score did not improve: keep the previous graph
