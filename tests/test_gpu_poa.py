@@ -61,7 +61,12 @@ def test_batch_augment_equals_per_read_augment(config1_batch, cmp):
     for a, c in zip(g_batch, g_read):
         assert_same_graph(native_as_oracle(a), _AsOracle(native_as_oracle(c)))
     s = g_batch[0].sizes()
+    print("graph 0:", s, flush=True)
     assert s["inserts"] > 100 and s["deletes"] > 100 and s["nodeObservations"] > 10000
+    # one insert observation per insert event, and a graph with 40000 or more
+    # events runs its insert task as node-range shards: the comparison above
+    # then covers the shard merge
+    assert s["insertObservations"] >= 40000
 
 
 def test_batch_augment_thread_counts_agree(config1_batch):
