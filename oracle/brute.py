@@ -125,3 +125,92 @@ def enumerate_paths(model, sX, sY, ragged_left=False, ragged_right=False):
         for s, x, y in cells:
             post[s, x, y] += w
     return total, post
+
+
+def traceback_windows(L, R, D, min_diags, traceback_diagonals, expansion):
+    """The traceback points of getPosteriorProbsWithBanding
+    (pairwiseAligner.c:715-783) as [(tb, lo, hi, at_end)]: a window fires at
+    the first anti-diagonal xay >= lo + minDiagsBetweenTraceBack whose band is
+    at most 2*diagonalExpansion+1 cells wide, or at the last one (D); it emits
+    the posteriors of anti-diagonals in (lo, hi] with hi = tb - (traceBack + 1)
+    (hi = D at the end), and the next window starts at lo = hi."""
+    out, lo = [], 0
+    for xay in range(1, D + 1):
+        at_end = xay == D
+        width = (R[xay] - L[xay]) // 2 + 1
+        if at_end or (xay >= lo + min_diags and width <= 2 * expansion + 1):
+            hi = D if at_end else xay - (traceback_diagonals + 1)
+            out.append((xay, lo, hi, at_end))
+            lo = hi
+            if at_end:
+                break
+    return out
+
+
+def enumerate_windowed(model, sX, sY, L, R, min_diags, traceback_diagonals, expansion,
+                       ragged_left=False, ragged_right=False):
+    """Exact windowed posteriors by path enumeration inside a band.
+
+    L[d], R[d] bound x - y on anti-diagonal d = x + y (the band of
+    pairwiseAligner.c:120-226; a path may only visit band cells).  For the
+    window ending at traceback anti-diagonal tb (end vector = endStateProb, or
+    raggedEndStateProb for the final window of a ragged-right alignment), the
+    posterior of state s at cell (x, y) is
+
+        sum of path weight x end[last state] over band paths from the start
+            that visit (x, y) in state s and stop ON anti-diagonal tb
+        / the same sum over all band paths that stop on tb
+
+    -- a match stepping from tb-1 to tb+1 never reaches tb and is not counted,
+    which is what initialising only B[tb] (and zeroing B[tb-1]) does in the
+    reference (pairwiseAligner.c:729-750).  A cell gets the posterior of the
+    window whose emitted range (lo, hi] holds its anti-diagonal.
+
+    Returns (windows, totals, post) with post[s][x][y] and totals[k] the log
+    of window k's path sum."""
+    lX, lY = len(sX), len(sY)
+    D = lX + lY
+    T, start_v, end_v = machine(model)
+    ns = T.shape[0]
+    start = start_v[1 if ragged_left else 0]
+    wins = traceback_windows(L, R, D, min_diags, traceback_diagonals, expansion)
+
+    def in_band(x, y):
+        d = x + y
+        return 0 <= x <= lX and 0 <= y <= lY and L[d] <= x - y <= R[d]
+
+    post = np.zeros((ns, lX + 1, lY + 1))
+    totals = []
+    for tb, lo, hi, at_end in wins:
+        end = end_v[1 if (at_end and ragged_right) else 0]
+        paths = []
+
+        def rec(x, y, s, lp, cells):
+            if x + y == tb:
+                paths.append((lp + end[s], list(cells)))
+                return
+            for nxt in range(ns):
+                k = kind(nxt)
+                dx, dy = STEP[k]
+                nx, ny = x + dx, y + dy
+                if nx + ny > tb or not in_band(nx, ny):
+                    continue
+                e = _emit(model, k, int(sX[nx - 1]) if dx else 4, int(sY[ny - 1]) if dy else 4)
+                step = lp + T[s, nxt] + e
+                if step == -math.inf:
+                    continue
+                cells.append((nxt, nx, ny))
+                rec(nx, ny, nxt, step, cells)
+                cells.pop()
+
+        for s0 in range(ns):
+            if start[s0] != -math.inf:
+                rec(0, 0, s0, start[s0], [])
+        total = _lse([p for p, _ in paths])
+        totals.append(total)
+        for lp, cells in paths:
+            w = math.exp(lp - total)
+            for s, x, y in cells:
+                if lo < x + y <= hi:
+                    post[s, x, y] += w
+    return wins, totals, post

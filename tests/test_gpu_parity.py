@@ -105,6 +105,13 @@ def _oracle_jobs(op, jobs, oparams, mode=O.LOGADD_EXACT):
     return out
 
 
+# Log-likelihood bar: FP32 rounding compounds along the anti-diagonals at
+# about 1e-7 relative per step, i.e. ~1e-3 nats over 1e4 anti-diagonals; the
+# bar is 1e-2 nats absolute.
+LL_TOL = 1e-2
+LL_DRIFT = {"max": 0.0}
+
+
 def _check(gres, ores, thr, check_order=True):
     worst = 0.0
     for (gm, gx, gy, glp), (om, ox, oy, olp, _) in zip(gres, ores):
@@ -115,7 +122,9 @@ def _check(gres, ores, thr, check_order=True):
             worst = max(worst, err)
             if check_order:
                 assert order_consistent(g, o)
-        assert abs(glp - olp) <= 1e-3 * max(1.0, abs(olp)), (glp, olp)
+        assert abs(glp - olp) <= LL_TOL, (glp, olp)
+        LL_DRIFT["max"] = max(LL_DRIFT["max"], abs(glp - olp))
+    say("max log-likelihood drift vs O_exact so far: %.3g nats" % LL_DRIFT["max"])
     return worst
 
 
@@ -453,3 +462,96 @@ def test_autotune_keeps_results(ph, params, config1):
     b2 = ph.Batch(pp.machines, jobs, pp.p).run()
     for r1, r2 in zip(b2.fetch(), ref):
         assert np.array_equal(r1[0], r2[0])
+
+
+def test_windowed_brute_force_tiny(ph, params):
+    """Multi-window posteriors of every cell-state (threshold 0) vs exact
+    enumeration of the band paths that stop on each window's traceback
+    anti-diagonal (2-4 windows per pair: traceBackDiagonals 1,
+    minDiagsBetweenTraceBack 3-4), random anchors with dynamic and static
+    bands, N symbols, all ragged combinations; the default machine and both
+    run-length strands."""
+    from tests.brute_cases import brute_posteriors, dense, emitted, windowed_cases
+    pp, op = params
+    dm = ph.StateMachines.default_nucleotide()
+    odm = O.default_nucleotide_model()
+    worst = 0.0
+    for mach, models in ((dm, (odm, odm)), (pp.machines, (op.reverse, op.forward))):
+        for strand in (0, 1):
+            om = models[strand]
+            cases = windowed_cases(200 + strand + 2 * om.rle, 24, om)
+            groups = {}
+            for c in cases:
+                p = c[3]
+                key = (p.minDiagsBetweenTraceBack, p.diagonalExpansion, p.dynamicAnchorExpansion)
+                groups.setdefault(key, []).append(c)
+            for key, grp in sorted(groups.items()):
+                say("case: windowed brute force rle=%s strand=%d params=%s, %d pairs"
+                    % (om.rle, strand, key, len(grp)))
+                bp = _bp(ph, grp[0][3])
+                jobs = [(sx, sy, np.array(an, np.int64).reshape(-1, 3) if an else None, strand, rl, rr)
+                        for sx, sy, an, _, rl, rr in grp]
+                b = ph.Batch(mach, jobs, bp)
+                b.run()
+                for case, (gm, gx, gy, lp) in zip(grp, b.fetch()):
+                    _, totals, post = brute_posteriors(om, case)
+                    assert abs(lp - totals[-1]) < 1e-4 * max(1.0, abs(totals[-1])), (lp, totals[-1])
+                    for st, lst in ((0, gm), (1, gx), (2, gy)):
+                        err = float(np.max(np.abs(dense(as_rows(lst), post[st].shape) - emitted(post, st))))
+                        assert err <= TOL + 1e-7, (key, st, err)
+                        worst = max(worst, err)
+    say("windowed brute force max-abs posterior err: %.3g" % worst)
+
+
+def _with_n(rng, syms, frac, rle):
+    """Replace a fraction of the symbols' bases by N (4), keeping run lengths."""
+    s = np.array(syms, dtype=np.uint16).copy()
+    idx = rng.sample(range(len(s)), max(1, int(frac * len(s))))
+    for i in idx:
+        s[i] = ((int(s[i]) >> 8) << 8 | 4) if rle else 4
+    return s
+
+
+@pytest.mark.parametrize("rle", [True, False])
+def test_n_symbols_parity(ph, params, rle):
+    """Ambiguous bases (N = symbol 4) in both sequences: the N emissions
+    (match -2.772588722, gap -1.386294361, stateMachine.c:360-380) and, with
+    run lengths, N read through the repeat table's A row (polisher.c:1758),
+    on both strands, vs O_exact; plus tiny all-N-heavy pairs vs exact
+    enumeration."""
+    from marginpolish_amd import crop, sim
+    from tests.conftest import PARAMS_NO_RLE
+    pp = ph.PolishParams(PARAMS if rle else PARAMS_NO_RLE)
+    op = O.PolishParams(PARAMS if rle else PARAMS_NO_RLE)
+    rng = random.Random(77 + rle)
+    w = sim.config0(n_reads=8, ref_len=2000, seed=21, W=16, strands=(1, 0))
+    jobs = []
+    for pr in w.pairs:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        if not rle:
+            sx, sy = (np.asarray(sx) & 255).astype(np.uint16), (np.asarray(sy) & 255).astype(np.uint16)
+        jobs.append((_with_n(rng, sx, 0.03, rle), _with_n(rng, sy, 0.03, rle), an, pr.strand))
+    say("case: N symbols config0-like rle=%s, 8 pairs" % rle)
+    b = ph.Batch(pp.machines, jobs, pp.p)
+    b.run()
+    worst = _check(b.fetch(), _oracle_jobs(op, jobs, op.p), op.p.threshold)
+    say("N symbols rle=%s max-abs posterior err vs O_exact: %.3g" % (rle, worst))
+    cases = []
+    for i in range(32):
+        lX, lY = rng.randint(1, 5), rng.randint(1, 5)
+        strand = i & 1
+        from tests.brute_cases import rand_syms
+        cases.append((rand_syms(rng, lX, rle, 0.4), rand_syms(rng, lY, rle, 0.4), None, strand,
+                      bool(i & 2), bool(i & 4)))
+    p = ph.PairwiseAlignmentParameters(threshold=0.0, diagonalExpansion=20, dynamicAnchorExpansion=True)
+    say("case: N symbols tiny enumeration rle=%s, 32 pairs" % rle)
+    b = ph.Batch(pp.machines, cases, p)
+    b.run()
+    for (sx, sy, _, strand, rl, rr), (gm, gx, gy, lp) in zip(cases, b.fetch()):
+        om = op.forward if strand else op.reverse
+        total, post = brute.enumerate_paths(om, sx, sy, rl, rr)
+        assert abs(lp - total) < 1e-4 * max(1.0, abs(total))
+        for st, lst in ((0, gm), (1, gx), (2, gy)):
+            from tests.brute_cases import dense, emitted
+            err = float(np.max(np.abs(dense(as_rows(lst), post[st].shape) - emitted(post, st))))
+            assert err <= TOL + 1e-7, (st, err)

@@ -373,3 +373,60 @@ def test_emission_counts_vs_brute_force(ragged):
             assert abs(got[0].sum() + got[2].sum() - lY) < 1e-9
             if m.rle:
                 assert abs(got[3].sum() - got[0].sum()) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["default", "rle_fwd", "rle_rev"])
+def test_windowed_posteriors_vs_brute_force(name):
+    """Multi-window posteriors (pairwiseAligner.c:722-792: backward
+    re-initialised with endStateProb at each traceback anti-diagonal, only the
+    anti-diagonals up to tb - (traceBackDiagonals + 1) emitted per window) vs
+    exact enumeration of the band paths that stop on each window's traceback
+    anti-diagonal.  2-4 windows per pair, random anchors, N symbols, all four
+    ragged combinations; every cell-state (threshold 0).  O_exact within the
+    1e-7 weight quantum; O_faithful (polynomial logAdd, the same control flow)
+    within 5e-3, its own approximation error on these tiny DPs."""
+    from tests.brute_cases import brute_posteriors, dense, emitted, windowed_cases
+    model = dict(_models())[name]
+    cases = windowed_cases(41 + len(name), 24, model)
+    nwin = []
+    for case in cases:
+        sx, sy, anchors, p, rl, rr = case
+        wins, totals, post = brute_posteriors(model, case)
+        nwin.append(len(wins))
+        for mode, tol in ((O.LOGADD_EXACT, 1.01e-7), (O.LOGADD_POLY, 5e-3)):
+            mm, gx, gy, tot, _ = O.posteriors(model, sx, sy, anchors, p, rl, rr, mode=mode)
+            # the pair's reported total is the final window's
+            assert abs(tot - totals[-1]) < (1e-9 if mode == O.LOGADD_EXACT else 1e-2), (name, tot, totals[-1])
+            for st, lst in ((0, mm), (1, gx), (2, gy)):
+                got = dense(lst, post[st].shape)
+                want = emitted(post, st)
+                if mode == O.LOGADD_EXACT:
+                    want = np.floor(want * 1e7) / 1e7
+                err = np.max(np.abs(got - want))
+                assert err <= tol, (name, mode, st, err, wins)
+    assert max(nwin) >= 3 and min(nwin) >= 2
+
+
+def test_windowed_brute_force_catches_wrong_end_vector():
+    """The windowed enumeration distinguishes the reference's backward
+    initialisation from plausible mistakes: a flat end vector (log 0 for
+    every state) at intermediate traceback points, or one global window over
+    the whole matrix, changes some posterior by more than 1e-2."""
+    from tests.brute_cases import brute_posteriors, windowed_cases
+    model = O.default_nucleotide_model()
+    T, start, end = brute.machine(model)
+    worst_flat, worst_global = 0.0, 0.0
+    for case in windowed_cases(7, 12, model, p_n=0.0):
+        sx, sy, anchors, p, rl, rr = case
+        _, _, post = brute_posteriors(model, case)
+        saved = brute.machine
+        try:
+            brute.machine = lambda m: (T, start, [[0.0, 0.0, 0.0], end[1]])
+            _, _, post_flat = brute_posteriors(model, case)
+        finally:
+            brute.machine = saved
+        worst_flat = max(worst_flat, float(np.max(np.abs(post_flat - post))))
+        if not anchors:
+            _, post_global = brute.enumerate_paths(model, sx, sy, rl, rr)
+            worst_global = max(worst_global, float(np.max(np.abs(post_global - post))))
+    assert worst_flat > 1e-2 and worst_global > 1e-2, (worst_flat, worst_global)
