@@ -55,8 +55,15 @@ enum {
     MP_BATCH_CROP_REFERENCE = 1, /* crop sX per pair as poa_realign's caller does (polisher.c:605-654) */
     MP_BATCH_FP64 = 2,           /* FP64 arithmetic on the per-cell-exponent kernel (no reference counterpart:
                                     the precision comparison path); bands up to 1216 cells, no expectations */
-    MP_BATCH_NO_GRAPH = 4        /* enqueue each run's launches directly instead of replaying the batch's
-                                    hipGraph (captured at the first run, re-captured after mp_batch_set_model) */
+    MP_BATCH_NO_GRAPH = 4,       /* enqueue each run's launches directly instead of replaying the batch's
+                                    hipGraphs (captured at the first run, re-captured after mp_batch_set_model) */
+    MP_BATCH_EXACT_CHAIN = 8     /* posterior modes: run the traceback windows of a sub-problem one after the
+                                    other on one forward chain, as the reference does.  Without it a band of at
+                                    most 64 cells with several windows runs each window as a job of its own,
+                                    whose forward sweep restarts 512 anti-diagonals before the window from a
+                                    flat vector (posteriors are invariant to the forward's scale and its
+                                    direction forgets the start within ~100 anti-diagonals of an anchored
+                                    band); the log-likelihood is chained in FP64 across the windows */
 };
 
 typedef struct mp_model mp_model;          /* both strands' state machines + tables */
@@ -218,6 +225,14 @@ mp_status mp_batch_autotune(mp_batch *b, int32_t repeats, int32_t *narrowThreads
 mp_status mp_batch_run_plain(mp_batch *b);
 /* Device time of the last run's pair-HMM kernel launch(es), milliseconds (HIP events on the engine's stream). */
 double mp_batch_kernel_ms(const mp_batch *b);
+/* Device time of the last run's list assembly and log-likelihood pass, which
+ * follows the kernels in every run (the reference's lists per pair, built on
+ * the device; milliseconds). */
+double mp_batch_list_ms(const mp_batch *b);
+/* Sum of the pairs' log-likelihoods of the last run, accumulated in FP64 on
+ * the device in a fixed order (each pair's own value as mp_batch_fetch_counts
+ * reports it). */
+mp_status mp_batch_fetch_total_log_prob(mp_batch *b, double *total);
 int64_t mp_batch_cells(const mp_batch *b);          /* band cells over all pairs (integer band geometry) */
 int64_t mp_batch_npairs(const mp_batch *b);
 /* Copy results to the host.  counts: npairs*3 list lengths; logProb: npairs

@@ -22,6 +22,7 @@ MODE_POSTERIORS, MODE_MATCH_ONLY, MODE_FORWARD_ONLY, MODE_EXPECTATIONS = 0, 1, 2
 BATCH_CROP_REFERENCE = 1
 BATCH_FP64 = 2
 BATCH_NO_GRAPH = 4
+BATCH_EXACT_CHAIN = 8
 
 
 class MPError(RuntimeError):
@@ -85,6 +86,7 @@ EXPORTS = [
     "mp_poa_get_sizes", "mp_poa_get_nodes", "mp_poa_get_inserts", "mp_poa_get_deletes", "mp_poa_get_observations",
     "mp_bubble_allele_supports", "mp_remove_overlaps", "mp_merge_chunks", "mp_poa_estimate_repeat_counts",
     "mp_poa_anchor_alignments", "mp_poa_consensus", "mp_poa_scores", "mp_poas_estimate_repeat_counts",
+    "mp_batch_list_ms", "mp_batch_fetch_total_log_prob",
 ]
 
 _lib = None
@@ -119,6 +121,8 @@ def lib():
         "mp_batch_run": (C.c_int, [vp]),
         "mp_batch_run_plain": (C.c_int, [vp]),
         "mp_batch_kernel_ms": (dbl, [vp]),
+        "mp_batch_list_ms": (dbl, [vp]),
+        "mp_batch_fetch_total_log_prob": (C.c_int, [vp, P(dbl)]),
         "mp_batch_cells": (i64, [vp]),
         "mp_batch_npairs": (i64, [vp]),
         "mp_batch_fetch_counts": (C.c_int, [vp, P(i64), P(dbl)]),

@@ -1,4 +1,6 @@
-// Posterior lists assembled on the device.
+// Posterior lists and log-likelihoods assembled on the device, inside the
+// batch's run (every size is read from device memory, so the whole pass is
+// captured in the run's second hipGraph and needs no host round trip).
 //
 // The pair-HMM kernels leave per job two record streams (forward side,
 // backward side), each a run of anti-diagonal blocks in sweep order with the
@@ -11,12 +13,21 @@
 // x > 0 and the gap-Y list at p_Y >= threshold with y > 0 (586-664), weight
 // floor(min(p, 1) * 1e7).
 //
-// 1. `order_records`: one workgroup per job side places every record at its
-//    emission index (pairs in order, a pair's jobs in split order, then the
-//    order above) and decodes it: coordinates, three weights, list flags.
-// 2. three exclusive scans of the list flags give each tuple's list index.
-// 3. `emit_lists` writes the (weight, x, y) tuples; `pair_list_starts` the
-//    per-pair list offsets.
+// Jobs sit in pair-major "slots": a pair's sub-problems in split order, the
+// window jobs of one sub-problem last window first (so the slot order is the
+// emission order).
+// 1. `slot_record_scan`: record count per slot, exclusive scan -> each job's
+//    first emission index; flags an overflowing record region.
+// 2. `order_records`: one workgroup per job side places every record at its
+//    emission index and decodes it: coordinates, three weights, list flags,
+//    and a mark on the first record of each pair.
+// 3. `list_flag_sums` / `list_block_offsets` / `list_scan_emit`: a scan of the
+//    three list flags in emission order that writes the (weight, x, y) tuples
+//    and, at each pair's first record, the three list positions.
+// 4. `pair_list_starts`: per-pair list offsets.
+// 5. `pair_log_likelihood` / `batch_log_likelihood`: FP64 log-likelihood per
+//    pair (window jobs chained through their traceback anti-diagonals) and
+//    their sum, in a fixed order.
 #pragma once
 
 #include "mp_device.hpp"
@@ -24,36 +35,101 @@
 namespace mpk {
 
 struct JobLists {
-    int64_t outF, outB;   // record regions: forward side, backward side
-    int64_t base;         // emission index of the job's first record
-    int64_t bandOff;      // the job's band entries
-    int32_t nF, nB;       // records written by each side
+    int64_t outF, outB;     // record regions: forward side, backward side
+    int64_t bandOff;        // the job's band entries
+    int32_t capF, capB;     // record capacities of the two sides
     int32_t winOff, nWin;
-    int32_t xAdd, yAdd;   // output coordinate shifts (split offset + reference crop - 1, split offset - 1)
+    int32_t xAdd, yAdd;     // output coordinate shifts (split offset + reference crop - 1, split offset - 1)
+    int32_t slot;           // the job's slot
+    int32_t pairSlot;       // first slot of the job's pair
 };
 
-struct ListTuple {        // the C ABI's mp_aligned_pair
+struct ListTuple {          // the C ABI's mp_aligned_pair
     int64_t weight;
     int32_t x, y;
 };
 
-__global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobLists *jl, const int32_t *winEnd,
-                                                     const int *band, float thr, int withIndels, int nseg,
-                                                     int2 *xy, int4 *wf) {
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;                              // records per thread
+constexpr int kScanTile = kScanThreads * kScanItems;       // records per workgroup
+constexpr int kPairStartBit = 8;                           // wf.w: first record of a pair
+
+// Exclusive scan over a workgroup of 64-lane waves; returns this thread's
+// prefix and writes the workgroup total to *total (every thread).
+template <class T, int NT>
+__device__ __forceinline__ T block_excl_scan(T v, T *sh, T *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    T waveOff = 0, tot = 0;
+    for (int i = 0; i < NT / 64; i++) {
+        if (i < w) waveOff += sh[i];
+        tot += sh[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return waveOff + x - v;
+}
+
+// Record count per slot and its exclusive scan (one workgroup).  A slot whose
+// job wrote more records than its region holds sets *overflow (the host then
+// sizes the regions exactly and runs the batch again).
+__global__ __launch_bounds__(1024) void slot_record_scan(const int32_t *slotJob, int32_t nslots,
+                                                         const int32_t *counts, const JobLists *jl,
+                                                         const int32_t *error, int64_t *slotBase,
+                                                         int32_t *overflow) {
+    __shared__ int64_t sh[16];
+    int64_t carry = 0;
+    bool ovf = false;
+    const bool bad = *error != 0;
+    for (int32_t s0 = 0; s0 < nslots; s0 += 1024) {
+        const int32_t s = s0 + (int32_t)threadIdx.x;
+        int64_t v = 0;
+        if (s < nslots && !bad) {
+            const int32_t q = slotJob[s];
+            if (q >= 0) {
+                const int32_t cF = counts[2 * q], cB = counts[2 * q + 1];
+                const JobLists J = jl[q];
+                ovf |= cF > J.capF || cB > J.capB || cF < 0 || cB < 0;
+                v = (int64_t)min(max(cF, 0), J.capF) + min(max(cB, 0), J.capB);
+            }
+        }
+        int64_t tot;
+        const int64_t pre = block_excl_scan<int64_t, 1024>(v, sh, &tot);
+        if (s < nslots) slotBase[s] = carry + pre;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) slotBase[nslots] = carry;
+    if (__any(ovf) && (threadIdx.x & 63) == 0) atomicOr(overflow, 1);
+}
+
+__global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobLists *jl, const int32_t *counts,
+                                                     const int64_t *slotBase, const int32_t *winEnd,
+                                                     const int *band, const int32_t *skip, const int32_t *error,
+                                                     float thr, int withIndels, int nseg, int2 *xy, int4 *wf) {
+    if (*skip || *error) return;
     for (int seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
         const JobLists J = jl[seg >> 1];
         const int side = seg & 1;  // 0 forward, 1 backward
-        const int n = side ? J.nB : J.nF;
+        const int nF = counts[2 * (seg >> 1)], nB = counts[2 * (seg >> 1) + 1];
+        const int n = side ? nB : nF;
         const int4 *r = rec + (side ? J.outB : J.outF);
         const int32_t *we = winEnd + 2 * J.winOff;
+        const int64_t base = slotBase[J.slot], pairBase = slotBase[J.pairSlot];
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             int w = 0;
             while (w + 1 < J.nWin && t >= we[2 * w + side]) w++;
             const int fBeg = w ? we[2 * (w - 1)] : 0, fEnd = we[2 * w];
             const int bBeg = w ? we[2 * (w - 1) + 1] : 0, bEnd = we[2 * w + 1];
-            const int beg = side ? bBeg : fBeg, end = side ? bEnd : fEnd;
+            const int beg = side ? bBeg : fBeg, end = min(side ? bEnd : fEnd, n);
             // windows after this one come first
-            const int64_t off = (int64_t)(J.nF - fEnd) + (J.nB - bEnd);
+            const int64_t off = (int64_t)(nF - fEnd) + (nB - bEnd);
             const int4 me = r[t];
             const uint32_t d = (uint32_t)me.w >> 11, cell = (uint32_t)me.w & 2047u;
             int u = t, v = t + 1;
@@ -63,11 +139,12 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
             for (int i = u; i < v; i++) rank += ((uint32_t)r[i].w & 2047u) > cell;
             // backward blocks are stored anti-diagonals descending: walk them from the end
             const int64_t pos = side ? (int64_t)(bEnd - v) + rank : (int64_t)(bEnd - bBeg) + (u - fBeg) + rank;
-            const int64_t g = J.base + off + pos;
+            const int64_t g = base + off + pos;
             const int x = band_xL(band[J.bandOff + d]) + (int)cell, y = (int)d - x;
             const float pM = __int_as_float(me.x), pX = __int_as_float(me.y), pY = __int_as_float(me.z);
             const int flags = ((pM >= thr) & (x > 0) & (y > 0)) | (((withIndels != 0) & (pX >= thr) & (x > 0)) << 1) |
-                              (((withIndels != 0) & (pY >= thr) & (y > 0)) << 2);
+                              (((withIndels != 0) & (pY >= thr) & (y > 0)) << 2) |
+                              (g == pairBase ? kPairStartBit : 0);
             xy[g] = make_int2(x + J.xAdd, y + J.yAdd);
             wf[g] = make_int4((int)floorf(fminf(pM, 1.f) * 1e7f), (int)floorf(fminf(pX, 1.f) * 1e7f),
                               (int)floorf(fminf(pY, 1.f) * 1e7f), flags);
@@ -75,34 +152,150 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
     }
 }
 
-struct ListBit {
-    int l;
-    __host__ __device__ int operator()(const int4 &f) const { return (f.w >> l) & 1; }
-};
+__device__ __forceinline__ int4 flag_counts(int f) { return make_int4(f & 1, (f >> 1) & 1, (f >> 2) & 1, 0); }
+__device__ __forceinline__ int4 add4(int4 a, int4 b) { return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
 
-__global__ __launch_bounds__(256) void emit_lists(const int2 *xy, const int4 *wf, const int32_t *pos0,
-                                                  const int32_t *pos1, const int32_t *pos2, int64_t n,
-                                                  ListTuple *out0, ListTuple *out1, ListTuple *out2) {
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
-        const int4 f = wf[g];
-        const int2 c = xy[g];
-        if (f.w & 1) out0[pos0[g]] = ListTuple{f.x, c.x, c.y};
-        if (f.w & 2) out1[pos1[g]] = ListTuple{f.y, c.x, c.y};
-        if (f.w & 4) out2[pos2[g]] = ListTuple{f.z, c.x, c.y};
+// Per workgroup tile: the number of tuples of each list.
+__global__ __launch_bounds__(kScanThreads) void list_flag_sums(const int4 *wf, const int64_t *nPtr,
+                                                               const int32_t *skip, int4 *tileSums) {
+    __shared__ int sh[3][kScanThreads / 64];
+    const int64_t n = *skip ? 0 : *nPtr;
+    const int64_t t0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int4 c = make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (t0 + i < n) c = add4(c, flag_counts(wf[t0 + i].w));
+    int v[3] = {c.x, c.y, c.z};
+    for (int l = 0; l < 3; l++)
+        for (int o = 32; o > 0; o >>= 1) v[l] += __shfl_xor(v[l], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int l = 0; l < 3; l++) sh[l][threadIdx.x >> 6] = v[l];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int4 s = make_int4(0, 0, 0, 0);
+        for (int w = 0; w < kScanThreads / 64; w++) s = add4(s, make_int4(sh[0][w], sh[1][w], sh[2][w], 0));
+        tileSums[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of the tile sums in place (one workgroup); entry nTiles
+// receives the list lengths.
+__global__ __launch_bounds__(1024) void list_block_offsets(int4 *tileSums, int64_t nTiles) {
+    __shared__ int sh[3][16];
+    int carry[3] = {0, 0, 0};
+    for (int64_t b0 = 0; b0 < nTiles; b0 += 1024) {
+        const int64_t b = b0 + threadIdx.x;
+        const int4 s = b < nTiles ? tileSums[b] : make_int4(0, 0, 0, 0);
+        int vs[3] = {s.x, s.y, s.z}, pre[3], tot[3];
+        for (int l = 0; l < 3; l++) pre[l] = block_excl_scan<int, 1024>(vs[l], sh[l], &tot[l]);
+        if (b < nTiles) tileSums[b] = make_int4(carry[0] + pre[0], carry[1] + pre[1], carry[2] + pre[2], 0);
+        for (int l = 0; l < 3; l++) carry[l] += tot[l];
+    }
+    if (threadIdx.x == 0) tileSums[nTiles] = make_int4(carry[0], carry[1], carry[2], 0);
+}
+
+// Scan inside each tile, then write every tuple at its list position; the
+// first record of a pair leaves the three positions in pairPos.
+__global__ __launch_bounds__(kScanThreads) void list_scan_emit(const int2 *xy, const int4 *wf, const int64_t *nPtr,
+                                                               const int32_t *skip, const int4 *tileOff,
+                                                               ListTuple *out0, ListTuple *out1, ListTuple *out2,
+                                                               int4 *pairPos) {
+    __shared__ int sh[3][kScanThreads / 64];
+    const int64_t n = *skip ? 0 : *nPtr;
+    const int64_t tile0 = (int64_t)blockIdx.x * kScanTile;
+    if (tile0 >= n) return;  // workgroup-uniform
+    const int64_t t0 = tile0 + (int64_t)threadIdx.x * kScanItems;
+    int f[kScanItems];
+    int4 c = make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        f[i] = t0 + i < n ? wf[t0 + i].w : 0;
+        c = add4(c, flag_counts(f[i]));
+    }
+    int tot[3];
+    const int4 o = tileOff[blockIdx.x];
+    int p0 = o.x + block_excl_scan<int, kScanThreads>(c.x, sh[0], &tot[0]);
+    int p1 = o.y + block_excl_scan<int, kScanThreads>(c.y, sh[1], &tot[1]);
+    int p2 = o.z + block_excl_scan<int, kScanThreads>(c.z, sh[2], &tot[2]);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (!f[i]) continue;
+        const int64_t g = t0 + i;
+        const int4 w = wf[g];
+        const int2 xyg = xy[g];
+        if (f[i] & kPairStartBit) pairPos[g] = make_int4(p0, p1, p2, 0);
+        if (f[i] & 1) out0[p0++] = ListTuple{w.x, xyg.x, xyg.y};
+        if (f[i] & 2) out1[p1++] = ListTuple{w.y, xyg.x, xyg.y};
+        if (f[i] & 4) out2[p2++] = ListTuple{w.z, xyg.x, xyg.y};
     }
 }
 
 // starts[l * (np + 1) + i] = index in list l of pair i's first tuple; entry np = list length
-__global__ __launch_bounds__(256) void pair_list_starts(const int64_t *pairStart, int64_t np, int64_t n,
-                                                        const int4 *wf, const int32_t *pos0, const int32_t *pos1,
-                                                        const int32_t *pos2, int64_t *starts) {
+__global__ __launch_bounds__(256) void pair_list_starts(const int32_t *pairSlot, int64_t np, const int64_t *slotBase,
+                                                        int32_t nslots, const int4 *pairPos, const int4 *totals,
+                                                        const int32_t *skip, int64_t *starts) {
+    const int64_t n = *skip ? 0 : slotBase[nslots];
+    const int4 tot = *skip ? make_int4(0, 0, 0, 0) : *totals;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= np; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t g = pairStart[i];
-        const int32_t *pos[3] = {pos0, pos1, pos2};
-        for (int l = 0; l < 3; l++)
-            starts[l * (np + 1) + i] =
-                g < n ? pos[l][g] : (n > 0 ? (int64_t)pos[l][n - 1] + ((wf[n - 1].w >> l) & 1) : 0);
+        const int64_t g = *skip ? 0 : slotBase[pairSlot[i]];
+        const int4 p = g < n ? pairPos[g] : tot;
+        starts[i] = p.x;
+        starts[(np + 1) + i] = p.y;
+        starts[2 * (np + 1) + i] = p.z;
     }
+}
+
+// FP64 log-likelihood of every pair: the sum over its sub-problems of the
+// last window's total.  A sub-problem run as window jobs (its slots hold the
+// windows last first, the first slot carries their number K) chains the
+// scales: C_0 = 0, C_k = C_{k-1} + log t_{k-1} - log u_k for a restarted
+// window k (u_k = its forward . endV at tb_{k-1}, t_{k-1} = window k-1's
+// total, both over the paths that end on tb_{k-1}), C_k = 0 when window k's
+// forward starts at the matrix start; log P = log t_{K-1} + C_{K-1}.
+__device__ __forceinline__ double log_t(int2 v) {
+    return log((double)__int_as_float(v.x)) + (double)v.y * 0.69314718055994530942;
+}
+__global__ __launch_bounds__(256) void pair_log_likelihood(const int32_t *pairSlot, int64_t np,
+                                                           const int32_t *slotJob, const int32_t *slotK,
+                                                           const DevJob *jobs, const int2 *logTotal,
+                                                           const int2 *chain, const int32_t *error, double *out) {
+    if (*error) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        double ll = 0.0;
+        for (int32_t s = pairSlot[i]; s < pairSlot[i + 1];) {
+            const int32_t K = max(slotK[s], 1);
+            if (slotJob[s] < 0) {
+                s += K;
+                continue;
+            }
+            double C = 0.0, lt = 0.0;
+            for (int32_t k = 0; k < K; k++) {
+                const int32_t q = slotJob[s + K - 1 - k];
+                const double ltk = log_t(logTotal[q]);
+                if (k > 0) C = (jobs[q].flags & kJobRestart) ? C + lt - log_t(chain[q]) : 0.0;
+                lt = ltk;
+            }
+            ll += lt + C;
+            s += K;
+        }
+        out[i] = ll;
+    }
+}
+
+// Sum of the pairs' log-likelihoods in a fixed order (one workgroup).
+__global__ __launch_bounds__(1024) void batch_log_likelihood(const double *pairLL, int64_t np,
+                                                             const int32_t *error, double *out) {
+    __shared__ double sh[1024];
+    double v = 0.0;
+    if (!*error)
+        for (int64_t i = threadIdx.x; i < np; i += 1024) v += pairLL[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 }  // namespace mpk

@@ -65,6 +65,7 @@ struct HostJob {
     int32_t cap[2];
     int64_t outOff;
     int32_t ringNeed;
+    int32_t fstart = 0;  // forward start anti-diagonal (window jobs)
 };
 
 template <class T>
@@ -103,7 +104,7 @@ struct mp_engine {
     int cus = 0;
     int splitBlocksPerCU = 1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     // pinned double buffer for large device-to-host result copies (allocated at first use)
     static constexpr size_t kStageBytes = (size_t)64 << 20;
     void *stage[2] = {nullptr, nullptr};
@@ -123,9 +124,11 @@ struct mp_batch {
     std::vector<HostJob> jobs;          // in device order
     std::vector<int64_t> pairCrop;      // reference crop offset per pair
     std::vector<uint8_t> pairStrand;    // 1 = forward-strand read
-    std::vector<int64_t> pairFirstJob;  // index into `order` of a pair's first job
-    std::vector<int32_t> order;         // pair-major list of device job indices (split order; -1 = empty)
-    std::vector<int32_t> pairNJobs;
+    std::vector<int64_t> pairFirstJob;  // index into `order` of a pair's first slot
+    std::vector<int32_t> order;         // slots: pair-major device job indices (split order, a sub-problem's
+                                        // window jobs last window first; -1 = empty sub-problem)
+    std::vector<int32_t> slotK;         // per slot: window jobs of its sub-problem at its first slot, else 0
+    std::vector<int32_t> pairNJobs;     // slots per pair
     std::vector<DevWin> winHost;
     int64_t cells = 0;
     int32_t ringLen = kK;
@@ -139,6 +142,7 @@ struct mp_batch {
     DevBuf<int32_t> dCounts;
     DevBuf<int32_t> dWinEnd;
     DevBuf<int2> dLogTotal;
+    DevBuf<int2> dChain;
     DevBuf<float> dRing;
     DevBuf<int32_t> dRingExp;
     DevBuf<float> dTables;
@@ -150,6 +154,7 @@ struct mp_batch {
     int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
     int32_t modelStates = 3;
     hipGraphExec_t graph = nullptr;  // the run's resets and launches, captured once per parameter block
+    hipGraphExec_t graphLists = nullptr;  // the run's list assembly and log-likelihoods
     int32_t narrowBlocks = 0;
     int32_t narrowThreads = 512;  // workgroup size of the narrow kernels (the autotuner may pick 256)
     int32_t narrowWaves = 0;      // resident narrow wavefronts the rings are sized for
@@ -157,13 +162,23 @@ struct mp_batch {
     uint64_t tuneKey = 0;
     int64_t outTotal = 0;
     double kernelMs = 0.0;
+    double listMs = 0.0;
     bool ran = false;
     std::vector<int32_t> hCounts;  // 2 per job, fetched after each run
-    // materialized lists
-    bool listsReady = false;
-    DevBuf<mpk::ListTuple> dList[3];      // the lists on the device (copied out by fetch)
+    // lists and log-likelihoods, assembled on the device by every run (mp_lists.hpp)
+    bool listsReady = false;            // listStart / pairLogProb copied to the host since the last run
+    std::vector<mpk::JobLists> jlHost;  // per device job
+    DevBuf<mpk::JobLists> dJL;
+    DevBuf<int32_t> dSlotJob, dSlotK, dPairSlot, dOverflow;
+    DevBuf<int64_t> dSlotBase, dStarts;
+    DevBuf<int2> dXY;
+    DevBuf<int4> dWF, dTileSums, dPairPos;
+    DevBuf<double> dPairLL, dBatchLL;
+    int64_t nTiles = 0;
+    DevBuf<mpk::ListTuple> dList[3];    // the lists on the device (copied out by fetch)
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
+    double batchLogProb = 0.0;
 };
 
 namespace {
@@ -199,6 +214,49 @@ mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
 mp_status reset_i32(mp_batch *b, int32_t *q, int n) {
     hipLaunchKernelGGL(fill_i32, dim3((n + 255) / 256), dim3(256), 0, b->eng->stream, q, 0, n);
     HIP_TRY(hipGetLastError());
+    return MP_OK;
+}
+
+bool lists_mode(const mp_batch *b) { return b->mode == MP_MODE_POSTERIORS || b->mode == MP_MODE_MATCH_ONLY; }
+
+// Per-job list metadata (record regions, capacities, slots) and the device
+// buffers of the list assembly, sized by the record capacity b->outTotal.
+mp_status alloc_lists(mp_batch *b) {
+    const int64_t nj = (int64_t)b->jobs.size(), np = b->npairs, ns = (int64_t)b->order.size();
+    std::vector<int32_t> jobSlot((size_t)std::max<int64_t>(nj, 1), -1);
+    for (int64_t sl = 0; sl < ns; sl++)
+        if (b->order[sl] >= 0) jobSlot[b->order[sl]] = (int32_t)sl;
+    std::vector<int32_t> slotPair((size_t)std::max<int64_t>(ns, 1), 0);
+    for (int64_t i = 0; i < np; i++)
+        for (int32_t k = 0; k < b->pairNJobs[i]; k++) slotPair[b->pairFirstJob[i] + k] = (int32_t)i;
+    b->jlHost.resize((size_t)nj);
+    for (int64_t q = 0; q < nj; q++) {
+        const HostJob &hj = b->jobs[q];
+        mpk::JobLists &J = b->jlHost[q];
+        J.outF = hj.outOff;
+        J.outB = hj.outOff + hj.cap[0];
+        J.bandOff = hj.bandOff;
+        J.capF = hj.cap[0];
+        J.capB = hj.cap[1];
+        J.winOff = hj.winOff;
+        J.nWin = hj.nWin;
+        J.xAdd = (int32_t)(hj.xOff + b->pairCrop[hj.pair] - 1);
+        J.yAdd = (int32_t)(hj.yOff - 1);
+        J.slot = jobSlot[q];
+        J.pairSlot = (int32_t)b->pairFirstJob[hj.pair];
+    }
+    hipStream_t s = b->eng->stream;
+    mp_status st;
+    if ((st = b->dJL.upload(b->jlHost, s))) return st;
+    if (!lists_mode(b)) return MP_OK;
+    const int64_t cap = std::max<int64_t>(b->outTotal, 1);
+    b->nTiles = (cap + mpk::kScanTile - 1) / mpk::kScanTile;
+    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) || (st = b->dPairPos.alloc((size_t)cap)) ||
+        (st = b->dTileSums.alloc((size_t)b->nTiles + 1)))
+        return st;
+    for (int l = 0; l < 3; l++)
+        if ((st = b->dList[l].alloc((size_t)cap))) return st;
+    HIP_TRY(hipStreamSynchronize(s));
     return MP_OK;
 }
 
@@ -342,7 +400,8 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
     e->cus = prop.multiProcessorCount;
     e->splitBlocksPerCU = std::max(1, bpc);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipEventCreate(&e->ev2) != hipSuccess) {
         delete e;
         mp::set_error("cannot create HIP stream/events");
         return MP_EHIP;
@@ -403,6 +462,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     for (int64_t i = 0; i < npairs; i++) b->pairStrand[i] = pairs[i].strand ? 1 : 0;
     b->pairNJobs.assign(npairs, 0);
     b->pairFirstJob.assign(npairs, 0);
+    std::vector<int64_t> pairFirstSub(npairs, 0);  // a pair's first sub-problem in `jobs`
     const bool split = use_split(mode);
 
     // ---- 1. encoded symbols, cropping, split points -> jobs (pair order) ----
@@ -497,8 +557,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     int64_t njobs = 0;
     for (int64_t i = 0; i < npairs; i++) {
         symOff[i + 1] = symOff[i] + 2 + (prep[i].end - prep[i].first) + pairs[i].lY;
-        b->pairFirstJob[i] = njobs;
-        b->pairNJobs[i] = prep[i].nsp;
+        pairFirstSub[i] = njobs;
         b->pairCrop[i] = prep[i].first;
         njobs += prep[i].nsp;
     }
@@ -532,7 +591,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         int64_t j = 0;
         for (int64_t s = 0; s < ns; s++) {
             const int64_t x1 = spp[4 * s], y1 = spp[4 * s + 1], x2 = spp[4 * s + 2], y2 = spp[4 * s + 3];
-            std::vector<int64_t> &sub = jobAnchors[b->pairFirstJob[i] + s];
+            std::vector<int64_t> &sub = jobAnchors[pairFirstSub[i] + s];
             const int64_t j0 = j;
             while (j < pr.nAnchors && (pr.anchors[3 * j] - pp.first) + pr.anchors[3 * j + 1] < x2 + y2) j++;
             if (j > j0) sub.resize(3 * (size_t)(j - j0));
@@ -541,7 +600,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                 sub[3 * (a - j0) + 1] = pr.anchors[3 * a + 1] - y1;
                 sub[3 * (a - j0) + 2] = pr.anchors[3 * a + 2];
             }
-            HostJob &hj = jobs[b->pairFirstJob[i] + s];
+            HostJob &hj = jobs[pairFirstSub[i] + s];
             hj.pair = i;
             hj.xOff = x1;
             hj.yOff = y1;
@@ -693,20 +752,76 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
 
+    // ---- 2b. window jobs (kWarmup): in the posterior modes a narrow sub-problem with
+    // several traceback windows runs each window as a job of its own ----
+    int64_t subCells = 0;
+    for (int64_t j = 0; j < nj; j++) subCells += jobs[j].cells;
+    std::vector<int64_t> subFirstX(nj + 1, 0);
+    {
+        const bool winJobs = split && (flags & MP_BATCH_EXACT_CHAIN) == 0;
+        int64_t nx = 0;
+        for (int64_t j = 0; j < nj; j++) {
+            subFirstX[j] = nx;
+            nx += (winJobs && jobs[j].cls < 3 && winRef[j].n > 1) ? winRef[j].n : 1;
+        }
+        subFirstX[nj] = nx;
+        if (nx >= ((int64_t)1 << 30)) {
+            delete b;
+            mp::set_error("batch too large: more than 2^30 jobs (split it into several batches)");
+            return MP_EUNSUPPORTED;
+        }
+        if (nx != nj) {
+            std::vector<HostJob> xj((size_t)nx);
+            std::vector<WinRef> xw((size_t)nx);
+            parallel_for(nj, [&](int64_t j) {
+                const int64_t x0 = subFirstX[j], n = subFirstX[j + 1] - x0;
+                if (n == 1) {
+                    xj[x0] = jobs[j];
+                    xw[x0] = winRef[j];
+                    return;
+                }
+                const WinRef &wr = winRef[j];
+                DevWin *wv = winPool[wr.t].data() + wr.off;
+                const int32_t *bj = band + jobs[j].bandOff;
+                for (int64_t k = 0; k < n; k++) {
+                    DevWin &w = wv[k];
+                    HostJob h = jobs[j];
+                    const int64_t fs = k == 0 ? 0 : (std::max<int64_t>(0, w.lo - kWarmup) & ~(int64_t)(kK - 1));
+                    h.fstart = (int32_t)fs;
+                    if (fs > 0) h.flags |= kJobRestart;
+                    // crossing: the forward row (from fs) and the backward row (from tb) meet
+                    // halfway, at 7 (mod 8), inside [lo, tb - 1]
+                    int64_t c = (fs + w.tb - 1) / 2;
+                    c = ((c + 4) & ~(int64_t)(kK - 1)) - 1;
+                    w.cx = (int32_t)std::min<int64_t>(std::max<int64_t>(c, w.lo), w.tb - 1);
+                    int64_t cells = 0;  // the job's work, for the device order
+                    for (int64_t d = fs; d <= w.tb; d++) cells += band_wm1_narrow(bj[d]) + 1;
+                    h.cells = cells;
+                    h.ringNeed = (int32_t)((w.tb - w.lo + 4 * kK - 1) / kK * kK);
+                    xj[x0 + k] = h;
+                    xw[x0 + k] = WinRef{wr.t, 1, wr.off + k};
+                }
+            });
+            jobs.swap(xj);
+            winRef.swap(xw);
+        }
+    }
+    const int64_t nxj = (int64_t)jobs.size();
+
     // ---- 3. device order: by class, then band area descending ----
     // (class ascending, cells descending, job index ascending): a stable LSD
     // radix sort of the jobs on (class, maxCells - cells)
     std::vector<int32_t> ord;
     {
         int64_t maxCells = 0;
-        for (int64_t j = 0; j < nj; j++)
+        for (int64_t j = 0; j < nxj; j++)
             if (jobs[j].lX + jobs[j].lY > 0) {
                 ord.push_back((int32_t)j);
                 maxCells = std::max(maxCells, jobs[j].cells);
             }
         int bitsC = 1;
         while (bitsC < 60 && ((int64_t)1 << bitsC) <= maxCells) bitsC++;
-        std::vector<uint64_t> key((size_t)nj);
+        std::vector<uint64_t> key((size_t)nxj);
         for (int32_t j : ord) key[j] = ((uint64_t)jobs[j].cls << bitsC) | (uint64_t)(maxCells - jobs[j].cells);
         std::vector<int32_t> tmp(ord.size());
         std::vector<size_t> cnt(2049);
@@ -719,7 +834,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
     const int64_t nq = (int64_t)ord.size();
-    std::vector<int32_t> devIndex(nj, -1);
+    std::vector<int32_t> devIndex(nxj, -1);
     std::vector<DevJob> dj((size_t)nq);
     std::vector<DevWin> &dw = b->winHost;
     const bool post = mode <= MP_MODE_MATCH_ONLY;
@@ -771,8 +886,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
             b->classBegin[sl.cls] = std::min<int32_t>(b->classBegin[sl.cls], (int32_t)q);
             b->classEnd[sl.cls] = (int32_t)q + 1;
         }
-        b->cells += sl.cells;
     }
+    b->cells = subCells;
     for (int c = 0; c < kNumClasses; c++)
         if (b->classBegin[c] > b->classEnd[c]) b->classBegin[c] = b->classEnd[c];
     dw.resize((size_t)winTotal);
@@ -789,10 +904,31 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         d.lX = hj.lX; d.lY = hj.lY; d.winOff = hj.winOff; d.nWin = hj.nWin;
         d.cap[0] = hj.cap[0]; d.cap[1] = hj.cap[1];
         d.flags = hj.flags;
-        d.pad = 0;
+        d.fstart = hj.fstart;
         b->jobs[q] = hj;
     });
-    b->order.assign(devIndex.begin(), devIndex.end());
+    // ---- 3b. slots: per pair its sub-problems in split order, a sub-problem's window
+    // jobs last window first (the order of the reference's lists) ----
+    {
+        std::vector<int32_t> slots, slotK;
+        slots.reserve((size_t)nxj);
+        slotK.reserve((size_t)nxj);
+        for (int64_t i = 0; i < npairs; i++) {
+            const int64_t f = (int64_t)slots.size();
+            for (int32_t sp = 0; sp < prep[i].nsp; sp++) {
+                const int64_t j = pairFirstSub[i] + sp;
+                const int64_t x0 = subFirstX[j], n = subFirstX[j + 1] - x0;
+                for (int64_t k = n - 1; k >= 0; k--) {
+                    slots.push_back(devIndex[x0 + k]);
+                    slotK.push_back(k == n - 1 ? (int32_t)n : 0);
+                }
+            }
+            b->pairFirstJob[i] = f;
+            b->pairNJobs[i] = (int32_t)(slots.size() - f);
+        }
+        b->order.swap(slots);
+        b->slotK.swap(slotK);
+    }
     b->ringLen = ringLen;
     b->outTotal = outOff;
 
@@ -810,6 +946,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
+        (st = b->dChain.alloc(std::max<size_t>(dj.size(), 1))) || (st = b->dSlotJob.upload(b->order, s)) ||
+        (st = b->dSlotK.upload(b->slotK, s)) || (st = b->dSlotBase.alloc(b->order.size() + 1)) ||
+        (st = b->dOverflow.alloc(1)) || (st = b->dStarts.alloc(3 * ((size_t)npairs + 1))) ||
+        (st = b->dPairLL.alloc(std::max<size_t>((size_t)npairs, 1))) || (st = b->dBatchLL.alloc(1)) ||
         (st = b->dQueue.alloc(kNumClasses + 1)) || (st = b->dError.alloc(1)) ||
         (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1)) ||
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
@@ -879,6 +1019,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.counts = b->dCounts.p;
     kp.winEnd = b->dWinEnd.p;
     kp.logTotal = b->dLogTotal.p;
+    kp.chain = b->dChain.p;
+    kp.tbd = (int32_t)p->traceBackDiagonals;
     kp.ring = b->dRing.p;
     kp.ringExp = b->dRingExp.p;
     kp.ringWide = b->dRingWide.p;
@@ -899,6 +1041,15 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         kp.jobBegin[c] = b->classBegin[c];
         kp.jobEnd[c] = b->classEnd[c];
     }
+    {
+        std::vector<int32_t> pairSlot((size_t)npairs + 1);
+        for (int64_t i = 0; i < npairs; i++) pairSlot[i] = (int32_t)b->pairFirstJob[i];
+        pairSlot[npairs] = (int32_t)b->order.size();
+        if ((st = b->dPairSlot.upload(pairSlot, s)) || (st = alloc_lists(b))) {
+            delete b;
+            return st;
+        }
+    }
     HIP_TRY(hipStreamSynchronize(s));
     *out = b;
     return MP_OK;
@@ -906,7 +1057,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
 
 static void drop_graph(mp_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graphLists) (void)hipGraphExecDestroy(b->graphLists);
     b->graph = nullptr;
+    b->graphLists = nullptr;
 }
 
 // The device work of one batched run: clear the counters, then the narrow
@@ -928,13 +1081,51 @@ static mp_status enqueue_run(mp_batch *b) {
     return MP_OK;
 }
 
-// hipGraph of enqueue_run (the per-iteration graph: capture once, replay every
-// run until the parameter block changes).  The graph holds kernel nodes only;
-// counters are cleared by fill kernels, not memset nodes.
-static mp_status capture_run(mp_batch *b) {
+// The run's list assembly and log-likelihoods (mp_lists.hpp): every size is
+// read from device memory, so the same launches serve every run.
+static mp_status enqueue_lists(mp_batch *b) {
+    hipStream_t s = b->eng->stream;
+    const int64_t np = b->npairs;
+    const int32_t ns = (int32_t)b->order.size(), nj = (int32_t)b->jobs.size();
+    mp_status st;
+    if (lists_mode(b)) {
+        if ((st = reset_i32(b, b->dOverflow.p, 1))) return st;
+        hipLaunchKernelGGL(mpk::slot_record_scan, dim3(1), dim3(1024), 0, s, b->dSlotJob.p, ns, b->dCounts.p,
+                           b->dJL.p, b->dError.p, b->dSlotBase.p, b->dOverflow.p);
+        hipLaunchKernelGGL(mpk::order_records, dim3((unsigned)std::max(1, std::min(2 * nj, 65535))), dim3(256), 0, s,
+                           b->dOut.p, b->dJL.p, b->dCounts.p, b->dSlotBase.p, b->dWinEnd.p, b->dBand.p,
+                           b->dOverflow.p, b->dError.p, b->kp.threshold, b->kp.withIndels, 2 * nj, b->dXY.p,
+                           b->dWF.p);
+        const int64_t *nPtr = b->dSlotBase.p + ns;
+        hipLaunchKernelGGL(mpk::list_flag_sums, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dWF.p,
+                           nPtr, b->dOverflow.p, b->dTileSums.p);
+        hipLaunchKernelGGL(mpk::list_block_offsets, dim3(1), dim3(1024), 0, s, b->dTileSums.p, b->nTiles);
+        hipLaunchKernelGGL(mpk::list_scan_emit, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dXY.p,
+                           b->dWF.p, nPtr, b->dOverflow.p, b->dTileSums.p, b->dList[0].p, b->dList[1].p,
+                           b->dList[2].p, b->dPairPos.p);
+        const unsigned gs = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
+        hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase.p, ns,
+                           b->dPairPos.p, b->dTileSums.p + b->nTiles, b->dOverflow.p, b->dStarts.p);
+        HIP_TRY(hipGetLastError());
+    }
+    if (np > 0) {
+        const unsigned gp = (unsigned)std::min<int64_t>((np + 255) / 256, 65535);
+        hipLaunchKernelGGL(mpk::pair_log_likelihood, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotJob.p,
+                           b->dSlotK.p, b->dJobs.p, b->dLogTotal.p, b->dChain.p, b->dError.p, b->dPairLL.p);
+        hipLaunchKernelGGL(mpk::batch_log_likelihood, dim3(1), dim3(1024), 0, s, b->dPairLL.p, np, b->dError.p,
+                           b->dBatchLL.p);
+        HIP_TRY(hipGetLastError());
+    }
+    return MP_OK;
+}
+
+// hipGraph of enqueue_run or enqueue_lists (the per-iteration graphs: captured
+// once, replayed every run until the parameter block changes).  The graphs hold
+// kernel nodes only; counters are cleared by fill kernels, not memset nodes.
+static mp_status capture(mp_batch *b, mp_status (*enqueue)(mp_batch *), hipGraphExec_t *exec) {
     hipStream_t s = b->eng->stream;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const mp_status st = enqueue_run(b);
+    const mp_status st = enqueue(b);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(s, &g);
     if (st) {
@@ -945,10 +1136,10 @@ static mp_status capture_run(mp_batch *b) {
         mp::set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
         return MP_EHIP;
     }
-    const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
-        b->graph = nullptr;
+        *exec = nullptr;
         mp::set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
         return MP_EHIP;
     }
@@ -962,16 +1153,21 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     const int32_t nj = (int32_t)b->jobs.size();
     b->listsReady = false;
     if (nj == 0) {
-        b->kernelMs = 0.0;
+        b->kernelMs = b->listMs = 0.0;
         b->ran = true;
         b->hCounts.clear();
+        for (int l = 0; l < 3; l++) b->listStart[l].assign(b->npairs + 1, 0);
+        b->pairLogProb.assign(b->npairs, 0.0);
+        b->batchLogProb = 0.0;
+        b->listsReady = true;
         return MP_OK;
     }
     mp_status st;
-    float ms = 0.f;
+    float ms = 0.f, msLists = 0.f;
+    const bool useGraph = !plain && (b->flags & MP_BATCH_NO_GRAPH) == 0;
+    if (useGraph && !b->graph && (st = capture(b, enqueue_run, &b->graph))) return st;
+    if (useGraph && !b->graphLists && (st = capture(b, enqueue_lists, &b->graphLists))) return st;
     if (!plain) {
-        const bool useGraph = (b->flags & MP_BATCH_NO_GRAPH) == 0;
-        if (useGraph && !b->graph && (st = capture_run(b))) return st;
         HIP_TRY(hipEventRecord(e->ev0, s));
         if (useGraph) {
             HIP_TRY(hipGraphLaunch(b->graph, s));
@@ -979,8 +1175,6 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
             return st;
         }
         HIP_TRY(hipEventRecord(e->ev1, s));
-        HIP_TRY(hipEventSynchronize(e->ev1));
-        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     } else {
         if (b->mode == MP_MODE_EXPECTATIONS &&
             (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
@@ -1010,24 +1204,30 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
             }
         }
         HIP_TRY(hipEventRecord(e->ev1, s));
-        HIP_TRY(hipEventSynchronize(e->ev1));
-        HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     }
+    // list assembly and log-likelihoods on the device
+    if (useGraph) {
+        HIP_TRY(hipGraphLaunch(b->graphLists, s));
+    } else if ((st = enqueue_lists(b))) {
+        return st;
+    }
+    HIP_TRY(hipEventRecord(e->ev2, s));
+    HIP_TRY(hipEventSynchronize(e->ev2));
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    HIP_TRY(hipEventElapsedTime(&msLists, e->ev1, e->ev2));
     b->kernelMs = ms;
-    int32_t kerr = 0;
+    b->listMs = msLists;
+    int32_t kerr = 0, ovf = 0;
     HIP_TRY(hipMemcpy(&kerr, b->dError.p, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (kerr) {
         mp::set_error("pair-HMM kernel: forward/backward sweeps lost step (code " + std::to_string(kerr) + ")");
         return MP_EHIP;
     }
+    if (lists_mode(b)) HIP_TRY(hipMemcpy(&ovf, b->dOverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost));
     // Overflowing record regions: size them exactly and run the batch again.
-    b->hCounts.resize((size_t)nj * 2);
-    HIP_TRY(hipMemcpy(b->hCounts.data(), b->dCounts.p, sizeof(int32_t) * 2 * nj, hipMemcpyDeviceToHost));
-    bool overflow = false;
-    for (int32_t j = 0; j < nj && !overflow; j++)
-        for (int l = 0; l < 2; l++)
-            if (b->hCounts[2 * j + l] > b->jobs[j].cap[l]) overflow = true;
-    if (overflow) {
+    if (ovf) {
+        b->hCounts.resize((size_t)nj * 2);
+        HIP_TRY(hipMemcpy(b->hCounts.data(), b->dCounts.p, sizeof(int32_t) * 2 * nj, hipMemcpyDeviceToHost));
         std::vector<DevJob> dj(nj);
         HIP_TRY(hipMemcpy(dj.data(), b->dJobs.p, sizeof(DevJob) * nj, hipMemcpyDeviceToHost));
         int64_t off = 0;
@@ -1043,8 +1243,9 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
         if ((st = b->dOut.alloc(std::max<int64_t>(off, 1)))) return st;
         b->kp.out = b->dOut.p;
-        drop_graph(b);
         b->outTotal = off;
+        if ((st = alloc_lists(b))) return st;
+        drop_graph(b);
         return batch_run_impl(b, plain);
     }
     b->ran = true;
@@ -1143,122 +1344,33 @@ extern "C" mp_status mp_batch_run_plain(mp_batch *b) {
 }
 
 extern "C" double mp_batch_kernel_ms(const mp_batch *b) { return b ? b->kernelMs : 0.0; }
+extern "C" double mp_batch_list_ms(const mp_batch *b) { return b ? b->listMs : 0.0; }
 extern "C" int64_t mp_batch_cells(const mp_batch *b) { return b ? b->cells : 0; }
 extern "C" int64_t mp_batch_npairs(const mp_batch *b) { return b ? b->npairs : 0; }
 
-// Records -> the reference's three lists per pair.  Per job the reference
-// emits, window by window, the posterior anti-diagonals from `from` down to
-// lo+1 (cells ascending within one), and the caller appends each split's list
-// reversed (pairwiseAligner.c:1072-1074); so a job contributes, for windows
-// last to first, its backward-side records reversed followed by its
-// forward-side records as written.
-// Exclusive scan of one list flag over the emission-ordered records.
-static mp_status scan_list_flag(const int4 *wf, int32_t *pos, int64_t n, int l, hipStream_t s) {
-    hipcub::TransformInputIterator<int32_t, mpk::ListBit, const int4 *> it(wf, mpk::ListBit{l});
-    size_t tmpBytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, it, pos, (int)n, s));
-    DevBuf<char> tmp;
-    mp_status st;
-    if ((st = tmp.alloc(std::max<size_t>(tmpBytes, 1)))) return st;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmpBytes, it, pos, (int)n, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return MP_OK;
-}
-
-// Per-pair log-likelihoods and, in posterior modes, the three lists in the
-// reference order (built on the device, mp_lists.hpp).
+// The run assembled the lists and the log-likelihoods on the device
+// (mp_lists.hpp); bring the per-pair list offsets and log-likelihoods to the
+// host once per run.
 static mp_status materialize(mp_batch *b) {
     if (b->listsReady) return MP_OK;
-    const int32_t nj = (int32_t)b->jobs.size();
     const int64_t np = b->npairs;
-    for (int l = 0; l < 3; l++) {
-        b->dList[l].alloc(0);
-        b->listStart[l].assign(np + 1, 0);
-    }
+    HIP_TRY(hipSetDevice(b->eng->device));
+    hipStream_t s = b->eng->stream;
+    std::vector<int64_t> starts(3 * (np + 1), 0);
     b->pairLogProb.assign(np, 0.0);
-    std::vector<double> lp(std::max(nj, 1), 0.0);
-    std::vector<int2> lt(std::max(nj, 1));
-    if (nj > 0) {
-        HIP_TRY(hipSetDevice(b->eng->device));
-        HIP_TRY(hipMemcpy(lt.data(), b->dLogTotal.p, sizeof(int2) * nj, hipMemcpyDeviceToHost));
-        for (int32_t j = 0; j < nj; j++) {
-            float t;
-            std::memcpy(&t, &lt[j].x, sizeof(float));
-            lp[j] = std::log((double)t) + (double)lt[j].y * 0.69314718055994530942;
-        }
+    b->batchLogProb = 0.0;
+    if (lists_mode(b))
+        HIP_TRY(hipMemcpyAsync(starts.data(), b->dStarts.p, sizeof(int64_t) * starts.size(), hipMemcpyDeviceToHost,
+                               s));
+    if (np > 0) {
+        HIP_TRY(hipMemcpyAsync(b->pairLogProb.data(), b->dPairLL.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&b->batchLogProb, b->dBatchLL.p, sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    for (int64_t i = 0; i < np; i++)
-        for (int32_t sj = 0; sj < b->pairNJobs[i]; sj++) {
-            const int32_t q = b->order[b->pairFirstJob[i] + sj];
-            if (q >= 0) b->pairLogProb[i] += lp[q];
-        }
-    if (b->mode <= MP_MODE_MATCH_ONLY && nj > 0) {
-        // emission index base of every job: pairs in order, a pair's jobs in split order
-        std::vector<mpk::JobLists> jl(nj);
-        std::vector<int64_t> pairStart(np + 1, 0);
-        int64_t total = 0;
-        for (int64_t i = 0; i < np; i++) {
-            pairStart[i] = total;
-            for (int32_t sj = 0; sj < b->pairNJobs[i]; sj++) {
-                const int32_t q = b->order[b->pairFirstJob[i] + sj];
-                if (q < 0) continue;
-                const HostJob &hj = b->jobs[q];
-                mpk::JobLists &J = jl[q];
-                J.outF = hj.outOff;
-                J.outB = hj.outOff + hj.cap[0];
-                J.base = total;
-                J.bandOff = hj.bandOff;
-                J.nF = b->hCounts[2 * q];
-                J.nB = b->hCounts[2 * q + 1];
-                J.winOff = hj.winOff;
-                J.nWin = hj.nWin;
-                J.xAdd = (int32_t)(hj.xOff + b->pairCrop[i] - 1);
-                J.yAdd = (int32_t)(hj.yOff - 1);
-                total += (int64_t)J.nF + J.nB;
-            }
-        }
-        pairStart[np] = total;
-        if (total >= INT32_MAX) {
-            mp::set_error("batch too large: more than 2^31 posterior records (split it into several batches)");
-            return MP_EUNSUPPORTED;
-        }
-        if (total > 0) {
-            hipStream_t s = b->eng->stream;
-            DevBuf<mpk::JobLists> dJL;
-            DevBuf<int64_t> dPairStart, dStarts;
-            DevBuf<int2> dXY;
-            DevBuf<int4> dWF;
-            DevBuf<int32_t> dPos[3];
-            mp_status st;
-            if ((st = dJL.upload(jl, s)) || (st = dPairStart.upload(pairStart, s)) || (st = dXY.alloc(total)) ||
-                (st = dWF.alloc(total)) || (st = dPos[0].alloc(total)) || (st = dPos[1].alloc(total)) ||
-                (st = dPos[2].alloc(total)) || (st = dStarts.alloc(3 * (np + 1))))
-                return st;
-            hipLaunchKernelGGL(mpk::order_records, dim3((unsigned)std::min<int64_t>(2 * (int64_t)nj, 65535)),
-                               dim3(256), 0, s, b->dOut.p, dJL.p, b->dWinEnd.p, b->dBand.p, b->kp.threshold,
-                               b->kp.withIndels, 2 * nj, dXY.p, dWF.p);
-            HIP_TRY(hipGetLastError());
-            for (int l = 0; l < 3; l++)
-                if ((st = scan_list_flag(dWF.p, dPos[l].p, total, l, s))) return st;
-            const unsigned gs = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
-            hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, dPairStart.p, np, total, dWF.p,
-                               dPos[0].p, dPos[1].p, dPos[2].p, dStarts.p);
-            HIP_TRY(hipGetLastError());
-            std::vector<int64_t> starts(3 * (np + 1));
-            HIP_TRY(hipMemcpyAsync(starts.data(), dStarts.p, sizeof(int64_t) * starts.size(), hipMemcpyDeviceToHost,
-                                   s));
-            HIP_TRY(hipStreamSynchronize(s));
-            DevBuf<mpk::ListTuple> *dList = b->dList;
-            for (int l = 0; l < 3; l++) {
-                for (int64_t i = 0; i <= np; i++) b->listStart[l][i] = starts[l * (np + 1) + i];
-                if ((st = dList[l].alloc(std::max<int64_t>(b->listStart[l][np], 1)))) return st;
-            }
-            const unsigned ge = (unsigned)std::min<int64_t>((total + 255) / 256, 65535 * 16);
-            hipLaunchKernelGGL(mpk::emit_lists, dim3(ge), dim3(256), 0, s, dXY.p, dWF.p, dPos[0].p, dPos[1].p,
-                               dPos[2].p, total, dList[0].p, dList[1].p, dList[2].p);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipStreamSynchronize(s));
-        }
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int l = 0; l < 3; l++) b->listStart[l].assign(starts.begin() + l * (np + 1), starts.begin() + (l + 1) * (np + 1));
+    if (b->listStart[0][np] >= INT32_MAX || b->listStart[1][np] >= INT32_MAX || b->listStart[2][np] >= INT32_MAX) {
+        mp::set_error("batch too large: a list holds more than 2^31 tuples (split it into several batches)");
+        return MP_EUNSUPPORTED;
     }
     b->listsReady = true;
     return MP_OK;
@@ -1332,6 +1444,21 @@ extern "C" mp_status mp_batch_fetch_counts(mp_batch *b, int64_t *counts, double 
         for (int l = 0; l < 3; l++) counts[3 * i + l] = b->listStart[l][i + 1] - b->listStart[l][i];
         if (logProb) logProb[i] = b->pairLogProb[i];
     }
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_fetch_total_log_prob(mp_batch *b, double *total) {
+    if (!b || !total) {
+        mp::set_error("mp_batch_fetch_total_log_prob: null argument");
+        return MP_EINVAL;
+    }
+    if (!b->ran) {
+        mp::set_error("mp_batch_fetch_total_log_prob: batch has not run");
+        return MP_EINVAL;
+    }
+    mp_status st = materialize(b);
+    if (st) return st;
+    *total = b->batchLogProb;
     return MP_OK;
 }
 

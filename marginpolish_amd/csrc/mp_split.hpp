@@ -126,6 +126,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int32_t *const gCounts = kp.counts;
     int32_t *const gWinEnd = kp.winEnd;
     int2 *const gLogTotal = kp.logTotal;
+    int2 *const gChain = kp.chain;
+    const int gTbd = kp.tbd;
     int32_t *const gQueue = kp.queue + cls;
     int32_t *const gError = kp.error;
     const int gJobBegin = kp.jobBegin[cls], gJobEnd = kp.jobEnd[cls];
@@ -170,6 +172,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     uint32_t xrow = 0, yrow = 0, bandRow = 0;  // byte offsets: x/y symbol bases, the job's band
     int64_t outBase = 0;
     int tb = 0, lo = 0, from = 0, cx = 0, postFrom = -1;
+    int capD = -1;  // restarted window job: the previous window's traceback anti-diagonal (chain value)
     int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, cpos = 0;
     bool post = false, init = false, reload = true;
     float totalInv = 0.f;
@@ -259,18 +262,21 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         win = 0;
         loadWin();
         if (!bwd) {
-            d = 0;
-            gs = 0;
+            d = jb.fstart;
+            gs = d;
             S = 0;
             init = true;
             post = false;
             postFrom = -1;
             reload = true;
             resetExchange();
+            const bool restart = (jflags & kJobRestart) != 0;
             const bool rg = (jflags >> 1) & 1;
-            iv0 = rg ? sv10 : sv00;
-            iv1 = rg ? sv11 : sv01;
-            iv2 = rg ? sv12 : sv02;
+            // a restarted window job starts from a flat vector (kWarmup before lo)
+            iv0 = restart ? 1.f : (rg ? sv10 : sv00);
+            iv1 = restart ? 1.f : (rg ? sv11 : sv01);
+            iv2 = restart ? 1.f : (rg ? sv12 : sv02);
+            capD = restart ? lo + gTbd + 1 : -1;
         } else {
             startBwdWindow();
         }
@@ -365,7 +371,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                                           sh->xv[pr][sl][0][3][lane] * sh->xv[pr][sl][1][3][lane]);
                 Stot = sh->xexp[pr][sl][0][row] + sh->xexp[pr][sl][1][row];
                 totalInv = t > 0.f ? 1.f / t : 0.f;
-                if (!bwd && tb == D && k == 0) gLogTotal[job] = make_int2(__float_as_int(t), Stot);
+                if (!bwd && win + 1 == nWin && k == 0) gLogTotal[job] = make_int2(__float_as_int(t), Stot);
                 xn += 1;
                 post = true;
                 postFrom = from;
@@ -440,8 +446,10 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         };
 
         const bool slowRow = (bC[0] & (bwd ? kSlowB : kSlowF)) != 0;
-        const bool fullRow =
-            !init && !slowRow && (bwd ? (d == gs + K - 1 && stop <= gs) : (d == gs && stop >= gs + K - 1));
+        // (the macro step holding a restarted job's chain anti-diagonal takes the general body)
+        const bool fullRow = !init && !slowRow &&
+                             (bwd ? (d == gs + K - 1 && stop <= gs) : (d == gs && stop >= gs + K - 1)) &&
+                             (capD < gs || capD >= gs + K);
         const bool full = __all(fullRow);
         const bool anyPost = __any(post);
         const bool slowWave = __any(slowRow);
@@ -493,10 +501,20 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 *expSlot = S;
             }
             // kept for the backward row (before the crossing) and for the next
-            // window's lower posterior diagonals (above `from`)
+            // window's lower posterior diagonals (above `from`); a window job's
+            // warm-up (at or below lo) is never read
             // (whole rows: a full 192-byte row segment makes whole-line writes;
             // lanes outside the band store values that are never read)
-            buf_store3(ringRs, (!post | (dj > from)) ? cbase + j * kRingStepBytes : kBufOff, p0, p1, p2);
+            buf_store3(ringRs, ((!post | (dj > from)) & (dj > lo)) ? cbase + j * kRingStepBytes : kBufOff, p0, p1,
+                       p2);
+            if constexpr (!FULL) {
+                // chain value of a restarted window job: F . endV at the previous window's tb
+                const bool capHere = dj == capD;
+                if (__any(capHere)) {
+                    const float u = rsum<SEG>(fmaf(ev02, p2, fmaf(ev01, p1, ev00 * p0)));
+                    if (capHere && k == 0) gChain[job] = make_int2(__float_as_int(u), S);
+                }
+            }
             if constexpr (POST) {
                 if (!FULL || j == 0) sc = ldexpf(totalInv, S + eC - Stot);
                 const F3 pb = rC[j];

@@ -263,13 +263,15 @@ class Batch:
     """A device-resident batch of read/reference pairs (mp_batch)."""
 
     def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None,
-                 fp64=False, graph=True):
+                 fp64=False, graph=True, exact_chain=False):
         """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight]),
         or a PairArray built from one.
         fp64: FP64 arithmetic on the per-cell-exponent kernel (the precision
         comparison path; 3-state machines run with the long gaps unreachable).
-        graph: replay the batch's captured hipGraph on every run (False:
-        enqueue the launches directly)."""
+        graph: replay the batch's captured hipGraphs on every run (False:
+        enqueue the launches directly).
+        exact_chain: run a sub-problem's traceback windows on one forward
+        chain (the reference's order) instead of as window jobs."""
         self.engine = engine or default_engine()
         # the mp_pair array (a numpy record array of the same layout)
         pa = pairs if isinstance(pairs, PairArray) else PairArray(pairs)
@@ -280,7 +282,8 @@ class Batch:
         pc = p.c()
         check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
                                     (_lib.BATCH_CROP_REFERENCE if crop_reference else 0) |
-                                    (_lib.BATCH_FP64 if fp64 else 0) | (0 if graph else _lib.BATCH_NO_GRAPH),
+                                    (_lib.BATCH_FP64 if fp64 else 0) | (0 if graph else _lib.BATCH_NO_GRAPH) |
+                                    (_lib.BATCH_EXACT_CHAIN if exact_chain else 0),
                                     C.byref(h)))
         self._h = h
         self.npairs = n
@@ -311,8 +314,19 @@ class Batch:
         return lib().mp_batch_kernel_ms(self._h)
 
     @property
+    def list_ms(self):
+        """Device time of the last run's list assembly (ms)."""
+        return lib().mp_batch_list_ms(self._h)
+
+    @property
     def cells(self):
         return lib().mp_batch_cells(self._h)
+
+    def total_log_prob(self):
+        """Sum of the pairs' log-likelihoods, accumulated in FP64 on the device."""
+        v = C.c_double()
+        check(lib().mp_batch_fetch_total_log_prob(self._h, C.byref(v)))
+        return v.value
 
     def counts(self):
         c = np.zeros((max(self.npairs, 1), 3), dtype=np.int64)

@@ -132,10 +132,18 @@ def _symbols(rbases, runs):
     return ((rl << 8) | rbases.astype(np.uint32)).astype(np.uint16)
 
 
-def _pair(ref_seg, read_seg, ref_idx, read_idx, W, strand, chunk, read_id):
-    """RLE both segments and derive strictly increasing RLE anchors from the
-    aligned base pairs (ref_idx[k], read_idx[k])."""
+def _ref_rle(ref_seg):
+    """Run-length form of a reference segment: (symbols, run lengths, run id
+    of every base), shared by every pair of the segment."""
     rb, rr, rid = _rle(ref_seg)
+    return _symbols(rb, rr), rr.astype(np.uint64), rid
+
+
+def _pair(ref_seg, read_seg, ref_idx, read_idx, W, strand, chunk, read_id, ref_rle=None):
+    """RLE both segments and derive strictly increasing RLE anchors from the
+    aligned base pairs (ref_idx[k], read_idx[k]).  ref_rle: the segment's
+    _ref_rle, when several pairs share it."""
+    rsym, rruns, rid = ref_rle if ref_rle is not None else _ref_rle(ref_seg)
     qb, qr, qid = _rle(read_seg)
     X = rid[ref_idx]
     Y = qid[read_idx]
@@ -143,8 +151,7 @@ def _pair(ref_seg, read_seg, ref_idx, read_idx, W, strand, chunk, read_id):
         keep = np.r_[True, (X[1:] > X[:-1]) & (Y[1:] > Y[:-1])]
         X, Y = X[keep], Y[keep]
     anchors = np.stack([X, Y, np.full(X.size, W, dtype=np.int64)], axis=1).astype(np.int64)
-    return Pair(_symbols(rb, rr), _symbols(qb, qr), anchors, strand, chunk, read_id, rr.astype(np.uint64),
-                qr.astype(np.uint64))
+    return Pair(rsym, _symbols(qb, qr), anchors, strand, chunk, read_id, rruns, qr.astype(np.uint64))
 
 
 def config0(n_reads=20, ref_len=5000, seed=0, err=0.10, W=32, strands=(1,)):
@@ -192,6 +199,7 @@ def chunked(name, seed, ref_len, gc, p_geom, tandem, chunk_size, boundary, depth
         cs = max(0, c * chunk_size - boundary)
         ce = min(ref_len, (c + 1) * chunk_size + boundary)
         ref_seg = ref[cs:ce]
+        ref_rle = _ref_rle(ref_seg)
         for (r, s, read, pos) in chunk_reads[c]:
             # clip the read to the reference bases it aligns inside the chunk
             lo_i, hi_i = max(cs - s, 0), min(ce - s, pos.size)
@@ -205,7 +213,7 @@ def chunked(name, seed, ref_len, gc, p_geom, tandem, chunk_size, boundary, depth
             read_seg = read[q0:q1 + 1]
             ref_idx = al + (lo_i + s - cs)
             read_idx = p[al] - q0
-            pairs.append(_pair(ref_seg, read_seg, ref_idx, read_idx, W, int(strands[r]), c, r))
+            pairs.append(_pair(ref_seg, read_seg, ref_idx, read_idx, W, int(strands[r]), c, r, ref_rle))
     return Workload(name, pairs, W, ref_len, nchunks)
 
 

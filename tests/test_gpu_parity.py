@@ -5,6 +5,7 @@ O_exact (exact log-sum-exp, same bands, windows and splits); an entry may be
 present on one side only when its posterior is within 1e-4 of the threshold.
 Output order equals the reference order for the entries both sides hold.
 """
+import math
 import os
 import random
 import sys
@@ -555,3 +556,40 @@ def test_n_symbols_parity(ph, params, rle):
             from tests.brute_cases import dense, emitted
             err = float(np.max(np.abs(dense(as_rows(lst), post[st].shape) - emitted(post, st))))
             assert err <= TOL + 1e-7, (st, err)
+
+
+def test_window_jobs_equal_exact_chain(ph, params):
+    """Window jobs (each traceback window its own job, the forward restarted
+    512 anti-diagonals early from a flat vector) against the reference's
+    single forward chain (MP_BATCH_EXACT_CHAIN) and O_exact: 6 pairs of ~8 k
+    anti-diagonals with a window every 1000 (restarts from the third window
+    on), both strands, ragged ends.  The two runs round differently (their
+    FP32 scale trajectories differ), so the bar is the FP32 level: posteriors
+    within 2e-5 of each other (each within 1e-4 of O_exact), log-likelihoods
+    (chained in FP64 across the windows) within 1e-3 nats; the device's FP64
+    batch total equals the sum of the pairs'."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(n_reads=6, ref_len=6000, seed=23, W=10, strands=(1, 0))
+    jobs = []
+    for i, pr in enumerate(w.pairs):
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand, bool(i & 1), bool(i & 2)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    say("case: window jobs vs exact chain, 6 pairs")
+    bw = ph.Batch(pp.machines, jobs, _bp(ph, opar)).run()
+    be = ph.Batch(pp.machines, jobs, _bp(ph, opar), exact_chain=True).run()
+    gw, ge = bw.fetch(), be.fetch()
+    worst, drift = 0.0, 0.0
+    for a, c in zip(gw, ge):
+        for l in range(3):
+            err, _, only = compare(as_rows(a[l]), as_rows(c[l]), opar.threshold, 2e-5)
+            worst = max(worst, err)
+        drift = max(drift, abs(a[3] - c[3]))
+    say("window jobs vs exact chain: max posterior diff %.3g, max log-likelihood diff %.3g" % (worst, drift))
+    assert worst <= 2e-5 and drift <= 1e-3
+    _check(gw, _oracle_jobs(op, jobs, opar), opar.threshold)
+    for b, g in ((bw, gw), (be, ge)):
+        tot = b.total_log_prob()
+        assert abs(tot - math.fsum(r[3] for r in g)) <= 1e-9 * abs(tot)
