@@ -1,12 +1,15 @@
 """Benchmark: batched pair-HMM forward-backward + posteriors on one MI355X.
 
-Workload (the batched-uneven-pairs configuration): 40 chunks of 10 kb (500 bp
-boundaries) from a seeded 400 kb reference (GC 45%, runs ~ Geometric(0.45)),
-60x log-normal reads (mean 8 kb, sd 6 kb, 0.5-40 kb), 9% error, RLE symbols,
-anchors = true RLE match pairs with expansion 10, allParams.np.json
-3-state RLE model.  One step = one POA realignment iteration: every
-read/chunk pair (~5.7k) through forward, backward and posterior extraction
-in one persistent kernel launch, inputs resident in HBM.
+Default workload (the full-polish-loop configuration, one realignment
+iteration): a seeded 5 Mb reference (GC 50%, runs ~ Geometric(0.5), 1% in
+2-6-mer tandem repeats), 50x log-normal reads (mean 20 kb, sd 15 kb, max
+100 kb) at 8% error, 50 chunks of 100 kb with 5 kb boundaries, RLE symbols,
+anchors = true RLE match pairs with expansion 10, allParams.np.json 3-state
+RLE model.  One step = one POA realignment iteration's device work: every
+read/chunk pair (~16k) through forward, backward and posterior extraction
+(the persistent kernels), then the reference's three lists per pair and the
+FP64 log-likelihoods assembled on the device; inputs resident in HBM.
+`--workload config1` runs the 400 kb / 40-chunk batch instead.
 
 Prints one JSON line (rank 0, N=1)."""
 import argparse
@@ -35,20 +38,56 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _rows(arr):
+    """(weight, x, y) list of one pair (numpy structured or int rows) -> {(x, y): weight}."""
+    if getattr(arr, "dtype", None) is not None and arr.dtype.names:
+        return {(int(x), int(y)): int(w) for w, x, y in zip(arr["weight"], arr["x"], arr["y"])}
+    return {(int(r[1]), int(r[2])): int(r[0]) for r in arr}
+
+
+def _posterior_diff(a, b, threshold, tol):
+    """Max |p_a - p_b| over the (x, y) both lists hold, or 1.0 when an entry
+    held by one side only lies more than tol above the threshold (the two
+    sides may decide p >= threshold differently only near it)."""
+    da, db = _rows(a), _rows(b)
+    worst = 0.0
+    for k in set(da) & set(db):
+        worst = max(worst, abs(da[k] - db[k]) / 1e7)
+    edge = (threshold + tol) * 1e7 + 1
+    for d, other in ((da, db), (db, da)):
+        for k in set(d) - set(other):
+            if d[k] > edge:
+                return 1.0
+    return worst
+
+
+def _host_threads():
+    """Host threads for the CPU baseline: the cores this process may use
+    (affinity), capped by OMP_NUM_THREADS when the machine sets it (the GPU
+    box allots 16 per GPU)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(omp) if omp.isdigit() and int(omp) > 0 else aff
+    return max(1, min(aff, cap)), aff, os.cpu_count() or 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-sample", type=int, default=64)
-    ap.add_argument("--cpu-pairs", type=int, default=1200)
-    ap.add_argument("--err-pairs", type=int, default=60)
+    ap.add_argument("--cpu-pairs", type=int, default=2000)
+    ap.add_argument("--err-pairs", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("config1", "config2", "config4", "bubbles", "polish"), default="config1",
-                    help="config1 (the headline batch), config2 (5 Mb reference, 50x 20 kb reads, 100 kb chunks), "
-                         "config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at one --band) or "
-                         "bubbles (bubble-graph per-read x per-allele forward likelihoods) or polish (a simulated "
-                         "draft polished end to end: realign, consensus rounds, run lengths, merge)")
+    ap.add_argument("--no-poa", action="store_true", help="skip the POA-augmentation timing")
+    ap.add_argument("--exact-chain", action="store_true",
+                    help="run each sub-problem's traceback windows on one forward chain (no window jobs)")
+    ap.add_argument("--workload", choices=("config1", "config2", "config4", "bubbles", "polish"), default="config2",
+                    help="config2 (the headline: 5 Mb reference, 50x 20 kb reads, 100 kb chunks), config1 (400 kb, "
+                         "40 chunks of 10 kb), config4 (five-state long-band stress: FP32 vs FP64 GPU vs CPU FP64 at "
+                         "one --band) or bubbles (bubble-graph per-read x per-allele forward likelihoods) or polish "
+                         "(a simulated draft polished end to end: realign, consensus rounds, run lengths, merge)")
     ap.add_argument("--band", type=int, default=128, help="config4: anchor expansion W (128, 256, 512, 1024)")
     ap.add_argument("--pairs", type=int, default=256, help="config4: number of 100 kb pairs")
     ap.add_argument("--max-chunks", type=int, default=None, help="limit the number of chunks (config2)")
@@ -88,13 +127,14 @@ def main():
     log("workload: %d pairs built in %.1f s" % (len(pairs), time.time() - t0))
     pp = pairhmm.PolishParams(PARAMS)
     eng = pairhmm.default_engine()
+    exact = args.exact_chain
 
-    # reads/s: one POA iteration end to end (host prep + upload + kernel + result handoff), timed on the
-    # second iteration of the engine (the first also pays one-time allocations: pinned staging, code load)
+    # reads/s: one POA iteration end to end (host prep + upload + device work + lists to the host), timed on
+    # the second iteration of the engine (the first also pays one-time allocations: pinned staging, code load)
     iters = []
     for it in range(2):
         t1 = time.time()
-        b = pairhmm.Batch(pp.machines, pair_array, pp.p, crop_reference=True, engine=eng)
+        b = pairhmm.Batch(pp.machines, pair_array, pp.p, crop_reference=True, engine=eng, exact_chain=exact)
         t_prep = time.time() - t1
         b.run()
         res = b.fetch()
@@ -106,8 +146,77 @@ def main():
     cells = b.cells
     log("iteration: first %.3f s; second: prep %.3f s, total %.3f s, %d cells" % (iters[0][1], t_prep, t_iter, cells))
 
-    # the same iteration through the POA augmentation: every chunk's graph built from the batch's lists inside
-    # the library (no list handoff to Python); timed on the second run, plus the augment alone on one thread
+    poa = {}
+    if not args.no_poa:
+        poa = _poa_iteration(w, pairs, pair_array, pp, eng, exact)
+
+    # hardware tiling picked by the autotuner on this batch (launch geometry only; results unchanged)
+    tuned = b.autotune(3)
+    log("autotune: %d-thread narrow workgroups, best %.3f ms" % (tuned[0], tuned[2]))
+    for _ in range(args.warmup):
+        b.run()
+    kms, lms = [], []
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        kms.append(b.kernel_ms)
+        lms.append(b.list_ms)
+    wall = time.perf_counter() - t2
+    ms_per_step = wall * 1e3 / args.steps
+    kernel_ms = float(np.mean(kms))
+    list_ms = float(np.mean(lms))
+    gcups = cells / (ms_per_step * 1e-3) / 1e9
+    kernel_gcups = cells / (kernel_ms * 1e-3) / 1e9
+    total_ll = b.total_log_prob()
+    njobs, rec_cap, recs = b.stats()
+    log("batched: %.3f ms/step wall, %.3f ms kernel, %.3f ms list assembly, %.1f GCUPS"
+        % (ms_per_step, kernel_ms, list_ms, gcups))
+
+    # plain per-read kernel (one launch per pair) on a sample of pairs
+    order = np.argsort([-len(p[0]) for p in pairs])
+    samp = [pairs[i] for i in order[:: max(1, len(pairs) // args.plain_sample)][: args.plain_sample]]
+    bp = pairhmm.Batch(pp.machines, samp, pp.p, crop_reference=True, engine=eng, exact_chain=exact)
+    bp.run_plain()
+    plain_gcups = bp.cells / (bp.kernel_ms * 1e-3) / 1e9
+    log("plain: %d pairs, %.3f ms, %.2f GCUPS" % (len(samp), bp.kernel_ms, plain_gcups))
+    bp.close()
+
+    out = {
+        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.workload] % w.chunks,
+                   "pairs": len(pairs), "cells": int(cells), "band_half_width": 10,
+                   "traceback_windows": "exact chain" if exact else "window jobs (512 anti-diagonal warm-up)"},
+        "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
+        "list_assembly_ms": round(list_ms, 4), "device_jobs": njobs, "posterior_records": recs,
+        "record_capacity": rec_cap,
+        "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4), "iteration_prep_s": round(t_prep, 4),
+        "first_iteration_s": round(iters[0][1], 4), "total_log_likelihood_fp64": total_ll,
+        "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
+        "plain_sample_pairs": len(samp), "autotuned_narrow_threads": tuned[0],
+    }
+    out.update(poa)
+    achieved = FLOPS_PER_CELL * cells / (kernel_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_split_kernel_%s.json" % args.workload)
+    if os.path.exists(tf) and not exact:  # measured on this workload's launch (window jobs)
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic}
+
+    if not args.no_cpu:
+        out.update(_cpu_leg(args, pairs, res, pp))
+    print(json.dumps(out), flush=True)
+
+
+def _poa_iteration(w, pairs, pair_array, pp, eng, exact):
+    """The same iteration through the POA augmentation: every chunk's graph
+    built from the batch's lists inside the library (no list handoff to
+    Python); timed on the second run, plus the augment alone on one thread
+    and the run-length estimation of every graph."""
+    import numpy as np
+    from marginpolish_amd import pairhmm
     from marginpolish_amd import poa as mpoa
     chunk_ids = sorted(set(p.chunk for p in w.pairs))
     first_pair = {}
@@ -126,7 +235,7 @@ def main():
     poa_iters = []
     for it in range(2):
         t1 = time.time()
-        bb = pairhmm.Batch(pp.machines, pair_array, pp.p, crop_reference=True, engine=eng)
+        bb = pairhmm.Batch(pp.machines, pair_array, pp.p, crop_reference=True, engine=eng, exact_chain=exact)
         tc = time.time()
         bb.run()
         ta = time.time()
@@ -157,126 +266,85 @@ def main():
         "%d deletes); repeat-count estimation %.3f s" % (t_poa_iter, t_aug, aug1, len(refs),
                                                          sum(s["inserts"] for s in gsz),
                                                          sum(s["deletes"] for s in gsz), t_rc))
+    return {"poa_iteration_s": round(t_poa_iter, 4), "poa_reads_per_s": round(len(pairs) / t_poa_iter, 1),
+            "augment_s": round(t_aug, 4), "augment_1thread_s": round(aug1, 4),
+            "repeat_count_estimation_s": round(t_rc, 4)}
 
-    # hardware tiling picked by the autotuner on this batch (launch geometry only; results unchanged)
-    tuned = b.autotune(3)
-    log("autotune: %d-thread narrow workgroups, best %.3f ms" % (tuned[0], tuned[2]))
-    for _ in range(args.warmup):
-        b.run()
-    kms = []
-    t2 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run()
-        kms.append(b.kernel_ms)
-    wall = time.perf_counter() - t2
-    ms_per_step = wall * 1e3 / args.steps
-    kernel_ms = float(np.mean(kms))
-    gcups = cells / (ms_per_step * 1e-3) / 1e9
-    kernel_gcups = cells / (kernel_ms * 1e-3) / 1e9
-    log("batched: %.3f ms/step wall, %.3f ms kernel, %.1f GCUPS" % (ms_per_step, kernel_ms, gcups))
 
-    # plain per-read kernel (one launch per pair) on a sample of pairs
-    order = np.argsort([-len(p[0]) for p in pairs])
-    samp = [pairs[i] for i in order[:: max(1, len(pairs) // args.plain_sample)][: args.plain_sample]]
-    bp = pairhmm.Batch(pp.machines, samp, pp.p, crop_reference=True, engine=eng)
-    bp.run_plain()
-    plain_gcups = bp.cells / (bp.kernel_ms * 1e-3) / 1e9
-    bs = pairhmm.Batch(pp.machines, samp, pp.p, crop_reference=True, engine=eng)
-    bs.run()
-    log("plain: %d pairs, %.3f ms, %.2f GCUPS" % (len(samp), bp.kernel_ms, plain_gcups))
+def _cpu_leg(args, pairs, res, pp):
+    """CPU baseline (the oracle's C restatement of the reference algorithm,
+    FP64 log space with the polynomial logAdd, built -march=native on this
+    host, OpenMP over pairs) on a bounded sample, and the GPU's posterior
+    error and log-likelihood drift against the exact FP64 oracle."""
+    from concurrent.futures import ThreadPoolExecutor
 
-    out = {
-        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload] % w.chunks,
-                   "pairs": len(pairs), "cells": int(cells), "band_half_width": 10},
-        "kernel_ms": round(kernel_ms, 4), "kernel_gcups": round(kernel_gcups, 3),
-        "reads_per_s": round(reads_per_s, 1), "iteration_s": round(t_iter, 4),
-        "first_iteration_s": round(iters[0][1], 4),
-        "poa_iteration_s": round(t_poa_iter, 4), "poa_reads_per_s": round(len(pairs) / t_poa_iter, 1),
-        "augment_s": round(t_aug, 4), "augment_1thread_s": round(aug1, 4), "repeat_count_estimation_s": round(t_rc, 4),
-        "plain_gcups": round(plain_gcups, 4), "batched_over_plain": round(kernel_gcups / plain_gcups, 2),
-        "plain_sample_pairs": len(samp), "autotuned_narrow_threads": tuned[0],
-    }
-    achieved = FLOPS_PER_CELL * cells / (kernel_ms * 1e-3) / 1e12
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_pairhmm_split_kernel_config1.json")
-    if os.path.exists(tf) and args.workload == "config1":  # measured on the config1 launch
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS,
-                       "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": traffic}
+    import numpy as np
+    from marginpolish_amd import crop
+    from oracle import oracle as O
+    native = O.use_native()
+    op = O.PolishParams(PARAMS)
+    nthr, aff, nproc = _host_threads()
+    rng = np.random.default_rng(0)
+    pick = rng.choice(len(pairs), size=min(args.cpu_pairs, len(pairs)), replace=False)
+    cpu_jobs = []
+    for i in pick:
+        sx, sy, an, _ = crop.crop_pair(*pairs[i][:3])
+        cpu_jobs.append((sx, sy, an, pairs[i][3]))
+    t3 = time.perf_counter()
+    ccells, _ = O.batch_cells((op.reverse, op.forward), cpu_jobs, op.p, O.LOGADD_POLY, nthr)
+    tc = time.perf_counter() - t3
+    one = cpu_jobs[: max(1, len(cpu_jobs) // 16)]
+    t4 = time.perf_counter()
+    c1, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_POLY, 1)
+    t1c = time.perf_counter() - t4
+    # the exact-arithmetic oracle (O_exact) on the single-thread sample, for reference
+    t5 = time.perf_counter()
+    c2, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_EXACT, 1)
+    t2c = time.perf_counter() - t5
+    log("cpu: %d pairs on %d threads %.2f s; %d pairs on 1 thread %.2f s" % (len(cpu_jobs), nthr, tc, len(one), t1c))
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    out = {"cpu_baseline": {"value": round(ccells / tc / 1e9, 5), "unit": "GCUPS", "cores": nthr, "kind": "port",
+                            "sample": "%d of %d %s pairs (%d cells), reference algorithm restated in C (FP64 log "
+                                      "space, polynomial logAdd) built -march=native on this host, OpenMP over pairs"
+                                      % (len(cpu_jobs), len(pairs), args.workload, ccells),
+                            "reads_per_s": round(len(cpu_jobs) / tc, 2),
+                            "single_core_gcups": round(c1 / t1c / 1e9, 5),
+                            "single_core_reads_per_s": round(len(one) / t1c, 3),
+                            "o_exact_single_core_gcups": round(c2 / t2c / 1e9, 5),
+                            "cpu_model": cpu_model, "nproc": nproc, "affinity_cores": aff,
+                            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                            "library": os.path.relpath(native, ROOT)}}
 
-    if not args.no_cpu:
-        from oracle import oracle as O
-        from tests.parity import as_rows, compare
-        op = O.PolishParams(PARAMS)
-        ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        ncores = max(1, min(ncores, 16))
-        rng = np.random.default_rng(0)
-        pick = rng.choice(len(pairs), size=min(args.cpu_pairs, len(pairs)), replace=False)
-        cpu_jobs = []
-        for i in pick:
-            sx, sy, an, _ = crop.crop_pair(*pairs[i][:3])
-            cpu_jobs.append((sx, sy, an, pairs[i][3]))
-        t3 = time.perf_counter()
-        ccells, _ = O.batch_cells((op.reverse, op.forward), cpu_jobs, op.p, O.LOGADD_POLY, ncores)
-        tc = time.perf_counter() - t3
-        one = cpu_jobs[: max(1, len(cpu_jobs) // 10)]
-        t4 = time.perf_counter()
-        c1, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_POLY, 1)
-        t1c = time.perf_counter() - t4
-        log("cpu: %d pairs on %d threads %.2f s; %d pairs on 1 thread %.2f s" % (len(cpu_jobs), ncores, tc,
-                                                                             len(one), t1c))
-        # the exact-arithmetic oracle (O_exact) on the single-thread sample, for reference
-        t5 = time.perf_counter()
-        c2, _ = O.batch_cells((op.reverse, op.forward), one, op.p, O.LOGADD_EXACT, 1)
-        t2c = time.perf_counter() - t5
-        cpu_model = ""
-        try:
-            with open("/proc/cpuinfo") as fh:
-                cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
-        except OSError:
-            pass
-        out["cpu_baseline"] = {"value": round(ccells / tc / 1e9, 5), "unit": "GCUPS", "cores": ncores,
-                               "kind": "port",
-                               "sample": "%d of %d config1 pairs (%d cells), reference algorithm restated in C "
-                                         "(FP64 log space, polynomial logAdd), OpenMP over pairs"
-                                         % (len(cpu_jobs), len(pairs), ccells),
-                               "reads_per_s": round(len(cpu_jobs) / tc, 2),
-                               "single_core_gcups": round(c1 / t1c / 1e9, 5),
-                               "single_core_reads_per_s": round(len(one) / t1c, 3),
-                               "o_exact_single_core_gcups": round(c2 / t2c / 1e9, 5),
-                               "cpu_model": cpu_model}
-        # max-abs posterior error vs the exact FP64 oracle on a sample
-        worst = 0.0
-        for i in pick[: args.err_pairs]:
-            sx, sy, an, off = crop.crop_pair(*pairs[i][:3])
-            om, ox, oy, _, _ = O.posteriors(op.forward if pairs[i][3] else op.reverse, sx, sy, an, op.p)
-            gm, gx, gy, _ = res[i]
-            for g, o in ((gm, om), (gx, ox), (gy, oy)):
-                o = o.copy()
-                o[:, 1] += off
-                try:
-                    worst = max(worst, compare(as_rows(g), o, op.p.threshold, 1e-4)[0])
-                except AssertionError:
-                    worst = 1.0  # an entry far above threshold on one side only
-        out["max_abs_posterior_err"] = worst
-        out["posterior_err_sample_pairs"] = int(min(args.err_pairs, len(pick)))
-        # the reference arithmetic (polynomial logAdd) against exact log-sum-exp on the same sample
-        fe = 0.0
-        for i in pick[: args.err_pairs]:
-            sx, sy, an, _ = crop.crop_pair(*pairs[i][:3])
-            m = op.forward if pairs[i][3] else op.reverse
-            ex = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_EXACT)
-            fa = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_POLY)
-            for a, b_ in zip(fa[:3], ex[:3]):
-                try:
-                    fe = max(fe, compare(a, b_, op.p.threshold, 1.0)[0])
-                except AssertionError:
-                    fe = 1.0
-        out["o_faithful_vs_o_exact_max_abs"] = fe
-    print(json.dumps(out), flush=True)
+    # max-abs posterior error and log-likelihood drift vs the exact FP64 oracle, and the reference
+    # arithmetic (polynomial logAdd) against exact log-sum-exp, on a sample (pairs in parallel)
+    def one_pair(i):
+        sx, sy, an, off = crop.crop_pair(*pairs[i][:3])
+        m = op.forward if pairs[i][3] else op.reverse
+        ex = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_EXACT)
+        fa = O.posteriors(m, sx, sy, an, op.p, mode=O.LOGADD_POLY)
+        gm, gx, gy, glp = res[i]
+        err = 0.0
+        for g, o in ((gm, ex[0]), (gx, ex[1]), (gy, ex[2])):
+            o = o.copy()
+            o[:, 1] += off
+            err = max(err, _posterior_diff(g, o, op.p.threshold, 1e-4))
+        fe = max(_posterior_diff(a, b_, op.p.threshold, 1.0) for a, b_ in zip(fa[:3], ex[:3]))
+        return err, abs(glp - ex[3]), fe
+
+    t6 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nthr) as pool:
+        got = list(pool.map(one_pair, pick[: args.err_pairs]))
+    log("posterior error sample: %d pairs in %.1f s" % (len(got), time.perf_counter() - t6))
+    out["max_abs_posterior_err"] = max(g[0] for g in got)
+    out["max_log_likelihood_drift"] = max(g[1] for g in got)
+    out["posterior_err_sample_pairs"] = len(got)
+    out["o_faithful_vs_o_exact_max_abs"] = max(g[2] for g in got)
+    return out
 
 
 FLOPS_PER_CELL_5 = 120  # five states: forward 55 (5 emission muls, 25 FMA sends), backward 55, posterior 10
@@ -525,7 +593,7 @@ def run_polish(args):
     graphs, consensus rounds, merge.  value = reads/s of the whole run."""
     import numpy as np
     from marginpolish_amd import pairhmm, polish, sim
-    from tests.test_gpu_polish import banded_edit_distance
+    from marginpolish_amd.sim import banded_edit_distance
     params_path = os.path.join(ROOT, "tests", "golden", "params", "allParams.np.no_rle.json")
     t0 = time.time()
     truth, draft, chunks = sim.polish_workload(seed=11, genome_len=400_000, depth=60, mean=8000, sd=6000,

@@ -233,6 +233,11 @@ double mp_batch_list_ms(const mp_batch *b);
  * the device in a fixed order (each pair's own value as mp_batch_fetch_counts
  * reports it). */
 mp_status mp_batch_fetch_total_log_prob(mp_batch *b, double *total);
+/* Diagnostics (no reference counterpart): device jobs of the batch (split
+ * sub-problems; a sub-problem run as window jobs counts once per window),
+ * posterior-record capacity, and the records the last run wrote.  Outputs
+ * may be NULL. */
+mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *recordCapacity, int64_t *records);
 int64_t mp_batch_cells(const mp_batch *b);          /* band cells over all pairs (integer band geometry) */
 int64_t mp_batch_npairs(const mp_batch *b);
 /* Copy results to the host.  counts: npairs*3 list lengths; logProb: npairs

@@ -19,11 +19,9 @@
 // 1. `slot_record_scan`: record count per slot, exclusive scan -> each job's
 //    first emission index; flags an overflowing record region.
 // 2. `order_records`: one workgroup per job side places every record at its
-//    emission index and decodes it: coordinates, three weights, list flags,
-//    and a mark on the first record of each pair.
+//    emission index and decodes it: coordinates, three weights, list flags.
 // 3. `list_flag_sums` / `list_block_offsets` / `list_scan_emit`: a scan of the
-//    three list flags in emission order that writes the (weight, x, y) tuples
-//    and, at each pair's first record, the three list positions.
+//    three list flags in emission order that writes the (weight, x, y) tuples.
 // 4. `pair_list_starts`: per-pair list offsets.
 // 5. `pair_log_likelihood` / `batch_log_likelihood`: FP64 log-likelihood per
 //    pair (window jobs chained through their traceback anti-diagonals) and
@@ -52,7 +50,6 @@ struct ListTuple {          // the C ABI's mp_aligned_pair
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;                              // records per thread
 constexpr int kScanTile = kScanThreads * kScanItems;       // records per workgroup
-constexpr int kPairStartBit = 8;                           // wf.w: first record of a pair
 
 // Exclusive scan over a workgroup of 64-lane waves; returns this thread's
 // prefix and writes the workgroup total to *total (every thread).
@@ -121,7 +118,7 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
         const int n = side ? nB : nF;
         const int4 *r = rec + (side ? J.outB : J.outF);
         const int32_t *we = winEnd + 2 * J.winOff;
-        const int64_t base = slotBase[J.slot], pairBase = slotBase[J.pairSlot];
+        const int64_t base = slotBase[J.slot];
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             int w = 0;
             while (w + 1 < J.nWin && t >= we[2 * w + side]) w++;
@@ -143,8 +140,7 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
             const int x = band_xL(band[J.bandOff + d]) + (int)cell, y = (int)d - x;
             const float pM = __int_as_float(me.x), pX = __int_as_float(me.y), pY = __int_as_float(me.z);
             const int flags = ((pM >= thr) & (x > 0) & (y > 0)) | (((withIndels != 0) & (pX >= thr) & (x > 0)) << 1) |
-                              (((withIndels != 0) & (pY >= thr) & (y > 0)) << 2) |
-                              (g == pairBase ? kPairStartBit : 0);
+                              (((withIndels != 0) & (pY >= thr) & (y > 0)) << 2);
             xy[g] = make_int2(x + J.xAdd, y + J.yAdd);
             wf[g] = make_int4((int)floorf(fminf(pM, 1.f) * 1e7f), (int)floorf(fminf(pX, 1.f) * 1e7f),
                               (int)floorf(fminf(pY, 1.f) * 1e7f), flags);
@@ -152,96 +148,174 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
     }
 }
 
-__device__ __forceinline__ int4 flag_counts(int f) { return make_int4(f & 1, (f >> 1) & 1, (f >> 2) & 1, 0); }
-__device__ __forceinline__ int4 add4(int4 a, int4 b) { return make_int4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
+// The three list scans read the records in emission order in chunks of 64
+// (one wavefront, coalesced); a chunk's list prefixes come from ballots.
+// Tile t covers records [t * kScanTile, (t + 1) * kScanTile): chunk (i, w) =
+// records tile0 + i * 256 + w * 64 + lane, i = 0..7, wave w = 0..3.
+__device__ __forceinline__ uint64_t ballot_bits(bool b) { return __ballot(b); }
+__device__ __forceinline__ int prefix_in_wave(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
-// Per workgroup tile: the number of tuples of each list.
+// Per tile: the number of tuples of each list.
 __global__ __launch_bounds__(kScanThreads) void list_flag_sums(const int4 *wf, const int64_t *nPtr,
                                                                const int32_t *skip, int4 *tileSums) {
     __shared__ int sh[3][kScanThreads / 64];
     const int64_t n = *skip ? 0 : *nPtr;
-    const int64_t t0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    int4 c = make_int4(0, 0, 0, 0);
+    const int64_t tile0 = (int64_t)blockIdx.x * kScanTile;
+    if (tile0 >= n) return;  // workgroup-uniform
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int c0 = 0, c1 = 0, c2 = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; i++)
-        if (t0 + i < n) c = add4(c, flag_counts(wf[t0 + i].w));
-    int v[3] = {c.x, c.y, c.z};
-    for (int l = 0; l < 3; l++)
-        for (int o = 32; o > 0; o >>= 1) v[l] += __shfl_xor(v[l], o, 64);
-    if ((threadIdx.x & 63) == 0)
-        for (int l = 0; l < 3; l++) sh[l][threadIdx.x >> 6] = v[l];
+    for (int i = 0; i < kScanItems; i++) {
+        const int64_t g = tile0 + i * kScanThreads + threadIdx.x;
+        const int f = g < n ? wf[g].w : 0;
+        c0 += __popcll(ballot_bits(f & 1));
+        c1 += __popcll(ballot_bits(f & 2));
+        c2 += __popcll(ballot_bits(f & 4));
+    }
+    if (lane == 0) {
+        sh[0][w] = c0;
+        sh[1][w] = c1;
+        sh[2][w] = c2;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int4 s = make_int4(0, 0, 0, 0);
-        for (int w = 0; w < kScanThreads / 64; w++) s = add4(s, make_int4(sh[0][w], sh[1][w], sh[2][w], 0));
+        for (int k = 0; k < kScanThreads / 64; k++) {
+            s.x += sh[0][k];
+            s.y += sh[1][k];
+            s.z += sh[2][k];
+        }
         tileSums[blockIdx.x] = s;
     }
 }
 
-// Exclusive scan of the tile sums in place (one workgroup); entry nTiles
-// receives the list lengths.
-__global__ __launch_bounds__(1024) void list_block_offsets(int4 *tileSums, int64_t nTiles) {
+// Exclusive scan of the tile sums of the ceil(n / kScanTile) tiles in place
+// (one workgroup, four tiles per thread per pass); the list lengths go to
+// *totals.
+__global__ __launch_bounds__(1024) void list_block_offsets(int4 *tileSums, const int64_t *nPtr,
+                                                           const int32_t *skip, int4 *totals) {
     __shared__ int sh[3][16];
+    const int64_t n = *skip ? 0 : *nPtr;
+    const int64_t nTiles = (n + kScanTile - 1) / kScanTile;
     int carry[3] = {0, 0, 0};
-    for (int64_t b0 = 0; b0 < nTiles; b0 += 1024) {
-        const int64_t b = b0 + threadIdx.x;
-        const int4 s = b < nTiles ? tileSums[b] : make_int4(0, 0, 0, 0);
-        int vs[3] = {s.x, s.y, s.z}, pre[3], tot[3];
-        for (int l = 0; l < 3; l++) pre[l] = block_excl_scan<int, 1024>(vs[l], sh[l], &tot[l]);
-        if (b < nTiles) tileSums[b] = make_int4(carry[0] + pre[0], carry[1] + pre[1], carry[2] + pre[2], 0);
+    for (int64_t b0 = 0; b0 < nTiles; b0 += 4 * 1024) {
+        const int64_t b = b0 + 4 * (int64_t)threadIdx.x;
+        int4 v[4];
+        int mine[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = b + k < nTiles ? tileSums[b + k] : make_int4(0, 0, 0, 0);
+            mine[0] += v[k].x;
+            mine[1] += v[k].y;
+            mine[2] += v[k].z;
+        }
+        int pre[3], tot[3];
+        for (int l = 0; l < 3; l++) pre[l] = carry[l] + block_excl_scan<int, 1024>(mine[l], sh[l], &tot[l]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (b + k < nTiles) {
+                tileSums[b + k] = make_int4(pre[0], pre[1], pre[2], 0);
+                pre[0] += v[k].x;
+                pre[1] += v[k].y;
+                pre[2] += v[k].z;
+            }
         for (int l = 0; l < 3; l++) carry[l] += tot[l];
     }
-    if (threadIdx.x == 0) tileSums[nTiles] = make_int4(carry[0], carry[1], carry[2], 0);
+    if (threadIdx.x == 0) *totals = make_int4(carry[0], carry[1], carry[2], 0);
 }
 
-// Scan inside each tile, then write every tuple at its list position; the
-// first record of a pair leaves the three positions in pairPos.
+// Every tuple written at its list position (positions at or beyond a list's
+// capacity are dropped: the host then grows that list and assembles again).
 __global__ __launch_bounds__(kScanThreads) void list_scan_emit(const int2 *xy, const int4 *wf, const int64_t *nPtr,
                                                                const int32_t *skip, const int4 *tileOff,
-                                                               ListTuple *out0, ListTuple *out1, ListTuple *out2,
-                                                               int4 *pairPos) {
-    __shared__ int sh[3][kScanThreads / 64];
+                                                               int4 cap, ListTuple *out0, ListTuple *out1,
+                                                               ListTuple *out2) {
+    __shared__ int cnt[3][kScanItems * kScanThreads / 64];
     const int64_t n = *skip ? 0 : *nPtr;
     const int64_t tile0 = (int64_t)blockIdx.x * kScanTile;
     if (tile0 >= n) return;  // workgroup-uniform
-    const int64_t t0 = tile0 + (int64_t)threadIdx.x * kScanItems;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NC = kScanItems * kScanThreads / 64;  // chunks per tile
     int f[kScanItems];
-    int4 c = make_int4(0, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
-        f[i] = t0 + i < n ? wf[t0 + i].w : 0;
-        c = add4(c, flag_counts(f[i]));
+        const int64_t g = tile0 + i * kScanThreads + threadIdx.x;
+        f[i] = g < n ? wf[g].w : 0;
+        const int c = i * (kScanThreads / 64) + w;
+        const int a = __popcll(ballot_bits(f[i] & 1)), b = __popcll(ballot_bits(f[i] & 2)),
+                  d = __popcll(ballot_bits(f[i] & 4));
+        if (lane == 0) {
+            cnt[0][c] = a;
+            cnt[1][c] = b;
+            cnt[2][c] = d;
+        }
     }
-    int tot[3];
+    __syncthreads();
+    if (threadIdx.x < 3) {  // exclusive prefix over the tile's chunks, per list
+        int run = 0;
+        for (int c = 0; c < NC; c++) {
+            const int v = cnt[threadIdx.x][c];
+            cnt[threadIdx.x][c] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
     const int4 o = tileOff[blockIdx.x];
-    int p0 = o.x + block_excl_scan<int, kScanThreads>(c.x, sh[0], &tot[0]);
-    int p1 = o.y + block_excl_scan<int, kScanThreads>(c.y, sh[1], &tot[1]);
-    int p2 = o.z + block_excl_scan<int, kScanThreads>(c.z, sh[2], &tot[2]);
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
+        const int c = i * (kScanThreads / 64) + w;
+        const uint64_t m0 = ballot_bits(f[i] & 1), m1 = ballot_bits(f[i] & 2), m2 = ballot_bits(f[i] & 4);
         if (!f[i]) continue;
-        const int64_t g = t0 + i;
-        const int4 w = wf[g];
-        const int2 xyg = xy[g];
-        if (f[i] & kPairStartBit) pairPos[g] = make_int4(p0, p1, p2, 0);
-        if (f[i] & 1) out0[p0++] = ListTuple{w.x, xyg.x, xyg.y};
-        if (f[i] & 2) out1[p1++] = ListTuple{w.y, xyg.x, xyg.y};
-        if (f[i] & 4) out2[p2++] = ListTuple{w.z, xyg.x, xyg.y};
+        const int64_t g = tile0 + i * kScanThreads + threadIdx.x;
+        const int4 wv = wf[g];
+        const int2 c2 = xy[g];
+        const int p0 = o.x + cnt[0][c] + prefix_in_wave(m0);
+        const int p1 = o.y + cnt[1][c] + prefix_in_wave(m1);
+        const int p2 = o.z + cnt[2][c] + prefix_in_wave(m2);
+        if ((f[i] & 1) && p0 < cap.x) out0[p0] = ListTuple{wv.x, c2.x, c2.y};
+        if ((f[i] & 2) && p1 < cap.y) out1[p1] = ListTuple{wv.y, c2.x, c2.y};
+        if ((f[i] & 4) && p2 < cap.z) out2[p2] = ListTuple{wv.z, c2.x, c2.y};
     }
 }
 
-// starts[l * (np + 1) + i] = index in list l of pair i's first tuple; entry np = list length
+// starts[l * (np + 1) + i] = index in list l of pair i's first tuple; entry np
+// = list length.  One wavefront per pair: the tile's offset plus the list
+// flags of the tile's records before the pair's first record.
 __global__ __launch_bounds__(256) void pair_list_starts(const int32_t *pairSlot, int64_t np, const int64_t *slotBase,
-                                                        int32_t nslots, const int4 *pairPos, const int4 *totals,
-                                                        const int32_t *skip, int64_t *starts) {
+                                                        int32_t nslots, const int4 *wf, const int4 *tileOff,
+                                                        const int4 *totals, const int32_t *skip, int64_t *starts) {
+    const int lane = threadIdx.x & 63;
     const int64_t n = *skip ? 0 : slotBase[nslots];
-    const int4 tot = *skip ? make_int4(0, 0, 0, 0) : *totals;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= np; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i <= np;
+         i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int64_t g = *skip ? 0 : slotBase[pairSlot[i]];
-        const int4 p = g < n ? pairPos[g] : tot;
-        starts[i] = p.x;
-        starts[(np + 1) + i] = p.y;
-        starts[2 * (np + 1) + i] = p.z;
+        int4 p;
+        if (g >= n) {
+            p = *skip ? make_int4(0, 0, 0, 0) : *totals;
+        } else {
+            const int64_t t = g / kScanTile, t0 = t * kScanTile;
+            int c0 = 0, c1 = 0, c2 = 0;
+            for (int64_t k = t0 + lane; k < g; k += 64) {
+                const int f = wf[k].w;
+                c0 += f & 1;
+                c1 += (f >> 1) & 1;
+                c2 += (f >> 2) & 1;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                c0 += __shfl_xor(c0, o, 64);
+                c1 += __shfl_xor(c1, o, 64);
+                c2 += __shfl_xor(c2, o, 64);
+            }
+            const int4 b = tileOff[t];
+            p = make_int4(b.x + c0, b.y + c1, b.z + c2, 0);
+        }
+        if (lane == 0) {
+            starts[i] = p.x;
+            starts[(np + 1) + i] = p.y;
+            starts[2 * (np + 1) + i] = p.z;
+        }
     }
 }
 

@@ -172,7 +172,9 @@ struct mp_batch {
     DevBuf<int32_t> dSlotJob, dSlotK, dPairSlot, dOverflow;
     DevBuf<int64_t> dSlotBase, dStarts;
     DevBuf<int2> dXY;
-    DevBuf<int4> dWF, dTileSums, dPairPos;
+    DevBuf<int4> dWF, dTileSums, dListTotals;
+    int64_t listCap[3] = {0, 0, 0};     // tuples the device lists hold (grown when a run needs more)
+    int64_t records = 0;                // posterior records of the last run
     DevBuf<double> dPairLL, dBatchLL;
     int64_t nTiles = 0;
     DevBuf<mpk::ListTuple> dList[3];    // the lists on the device (copied out by fetch)
@@ -251,11 +253,18 @@ mp_status alloc_lists(mp_batch *b) {
     if (!lists_mode(b)) return MP_OK;
     const int64_t cap = std::max<int64_t>(b->outTotal, 1);
     b->nTiles = (cap + mpk::kScanTile - 1) / mpk::kScanTile;
-    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) || (st = b->dPairPos.alloc((size_t)cap)) ||
-        (st = b->dTileSums.alloc((size_t)b->nTiles + 1)))
+    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) ||
+        (st = b->dTileSums.alloc((size_t)b->nTiles + 1)) || (st = b->dListTotals.alloc(1)))
         return st;
-    for (int l = 0; l < 3; l++)
-        if ((st = b->dList[l].alloc((size_t)cap))) return st;
+    // first guess of the list lengths (a record holds at most one tuple of each list; the
+    // capacities count two records per posterior anti-diagonal and side): half the records
+    for (int l = 0; l < 3; l++) {
+        const int64_t want = std::max<int64_t>(cap / 2, 1024);
+        if (b->listCap[l] < want) {
+            if ((st = b->dList[l].alloc((size_t)want))) return st;
+            b->listCap[l] = want;
+        }
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return MP_OK;
 }
@@ -838,7 +847,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     std::vector<DevJob> dj((size_t)nq);
     std::vector<DevWin> &dw = b->winHost;
     const bool post = mode <= MP_MODE_MATCH_ONLY;
-    // record capacities (two per posterior anti-diagonal of each side; exact re-run on overflow),
+    // record capacities (1.5 per posterior anti-diagonal of each side; exact re-run on overflow),
     // gathered per device slot so the offset pass below reads memory in order
     struct Slot {
         int64_t cells, out;
@@ -859,8 +868,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                 nB += w.from - w.lo;
             }
         }
-        hj.cap[0] = post && hj.cls < 3 ? (int32_t)std::min<int64_t>(2 * nF + 64, INT32_MAX / 4) : 0;
-        hj.cap[1] = post ? (int32_t)std::min<int64_t>(2 * nB + 64, INT32_MAX / 4) : 0;
+        // (measured on 100 kb chunks: 0.9 records per posterior anti-diagonal; a job that writes
+        // more than its region holds makes the run size every region exactly and run again)
+        hj.cap[0] = post && hj.cls < 3 ? (int32_t)std::min<int64_t>(nF + nF / 2 + 64, INT32_MAX / 4) : 0;
+        hj.cap[1] = post ? (int32_t)std::min<int64_t>(nB + nB / 2 + 64, INT32_MAX / 4) : 0;
         hj.nWin = wr.n;
         slot[q] = Slot{hj.cells, (int64_t)hj.cap[0] + hj.cap[1], wr.n, hj.cls, hj.ringNeed, hj.wmax};
     });
@@ -1099,13 +1110,17 @@ static mp_status enqueue_lists(mp_batch *b) {
         const int64_t *nPtr = b->dSlotBase.p + ns;
         hipLaunchKernelGGL(mpk::list_flag_sums, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dWF.p,
                            nPtr, b->dOverflow.p, b->dTileSums.p);
-        hipLaunchKernelGGL(mpk::list_block_offsets, dim3(1), dim3(1024), 0, s, b->dTileSums.p, b->nTiles);
+        hipLaunchKernelGGL(mpk::list_block_offsets, dim3(1), dim3(1024), 0, s, b->dTileSums.p, nPtr, b->dOverflow.p,
+                           b->dListTotals.p);
+        const int4 cap = make_int4((int)std::min<int64_t>(b->listCap[0], INT32_MAX),
+                                   (int)std::min<int64_t>(b->listCap[1], INT32_MAX),
+                                   (int)std::min<int64_t>(b->listCap[2], INT32_MAX), 0);
         hipLaunchKernelGGL(mpk::list_scan_emit, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dXY.p,
-                           b->dWF.p, nPtr, b->dOverflow.p, b->dTileSums.p, b->dList[0].p, b->dList[1].p,
-                           b->dList[2].p, b->dPairPos.p);
-        const unsigned gs = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
+                           b->dWF.p, nPtr, b->dOverflow.p, b->dTileSums.p, cap, b->dList[0].p, b->dList[1].p,
+                           b->dList[2].p);
+        const unsigned gs = (unsigned)std::min<int64_t>((np + 1 + 3) / 4, 65535);  // one wavefront per pair
         hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase.p, ns,
-                           b->dPairPos.p, b->dTileSums.p + b->nTiles, b->dOverflow.p, b->dStarts.p);
+                           b->dWF.p, b->dTileSums.p, b->dListTotals.p, b->dOverflow.p, b->dStarts.p);
         HIP_TRY(hipGetLastError());
     }
     if (np > 0) {
@@ -1248,7 +1263,42 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         drop_graph(b);
         return batch_run_impl(b, plain);
     }
+    if (lists_mode(b)) {
+        // lists longer than their device buffers: grow them and assemble the lists again
+        // (the kernels' records stay valid)
+        int4 tot;
+        int64_t recs = 0;
+        HIP_TRY(hipMemcpy(&tot, b->dListTotals.p, sizeof(int4), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&recs, b->dSlotBase.p + b->order.size(), sizeof(int64_t), hipMemcpyDeviceToHost));
+        b->records = recs;
+        const int64_t need[3] = {tot.x, tot.y, tot.z};
+        bool grow = false;
+        for (int l = 0; l < 3; l++)
+            if (need[l] > b->listCap[l]) {
+                const int64_t want = need[l] + need[l] / 16 + 1024;
+                if ((st = b->dList[l].alloc((size_t)want))) return st;
+                b->listCap[l] = want;
+                grow = true;
+            }
+        if (grow) {
+            if (b->graphLists) (void)hipGraphExecDestroy(b->graphLists);
+            b->graphLists = nullptr;
+            if ((st = enqueue_lists(b))) return st;
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+    }
     b->ran = true;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *recordCapacity, int64_t *records) {
+    if (!b) {
+        mp::set_error("mp_batch_stats: null batch");
+        return MP_EINVAL;
+    }
+    if (jobs) *jobs = (int64_t)b->jobs.size();
+    if (recordCapacity) *recordCapacity = b->outTotal;
+    if (records) *records = b->records;
     return MP_OK;
 }
 

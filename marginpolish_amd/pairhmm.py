@@ -322,6 +322,12 @@ class Batch:
     def cells(self):
         return lib().mp_batch_cells(self._h)
 
+    def stats(self):
+        """(device jobs, posterior-record capacity, records of the last run)."""
+        j, c, r = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().mp_batch_stats(self._h, C.byref(j), C.byref(c), C.byref(r)))
+        return j.value, c.value, r.value
+
     def total_log_prob(self):
         """Sum of the pairs' log-likelihoods, accumulated in FP64 on the device."""
         v = C.c_double()

@@ -57,6 +57,18 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def use_native():
+    """Build the oracle for the host it runs on (-march=native, into
+    _build_native/) and use that build from here on: the CPU baseline is timed
+    with it.  Returns its path."""
+    global _lib, LIB_PATH
+    subprocess.check_call(["make", "-s", "-C", HERE, "native"])
+    LIB_PATH = os.path.join(HERE, "_build_native", "liboracle.so")
+    _lib = None
+    lib()
+    return LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:

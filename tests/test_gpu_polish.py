@@ -6,36 +6,10 @@ import pytest
 
 from marginpolish_amd import pairhmm as ph
 from marginpolish_amd import polish, sim
+from marginpolish_amd.sim import banded_edit_distance
 from tests.conftest import PARAMS
 
 pytestmark = pytest.mark.gpu
-
-
-def banded_edit_distance(a, b, band=400):
-    """Levenshtein distance restricted to |i - j| <= band (exact when the
-    optimal path stays inside the band, as it does for these near-equal strings)."""
-    a = np.frombuffer(a.encode(), np.uint8)
-    b = np.frombuffer(b.encode(), np.uint8)
-    n, m = a.size, b.size
-    big = 1 << 30
-    prev = np.full(m + 1, big, np.int64)
-    prev[: min(m, band) + 1] = np.arange(min(m, band) + 1)
-    for i in range(1, n + 1):
-        lo, hi = max(1, i - band), min(m, i + band)
-        cur = np.full(m + 1, big, np.int64)
-        if i <= band:
-            cur[0] = i
-        if lo <= hi:
-            sub = prev[lo - 1:hi] + (a[i - 1] != b[lo - 1:hi])
-            dele = prev[lo:hi + 1] + 1
-            best = np.minimum(sub, dele)
-            # insertions run along the row: a prefix-minimum of best - j, plus j
-            j = np.arange(lo, hi + 1)
-            left = cur[lo - 1] + 1
-            run = np.minimum.accumulate(np.minimum(best - j, np.r_[left - lo, np.full(hi - lo, big)]))
-            cur[lo:hi + 1] = np.minimum(best, run + j)
-        prev = cur
-    return int(prev[m])
 
 
 def _strings(truth, draft):
