@@ -268,6 +268,17 @@ void mp_batch_free(mp_batch *b);
 /* computeForwardProbability (pairwiseAligner.c:813-865) for one pair. */
 mp_status mp_pairhmm_forward_log_prob(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,
                                       double *logProb);
+/* No counterpart in the reference: computeForwardProbability for one pair
+ * with no device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * result is mp_pairhmm_forward_log_prob's, from the same kernel.  Its device
+ * memory is a 16 MiB scratch array per GPU, held until the process ends:
+ * pairs whose tables, band entries and symbols exceed it (about 1.9 M
+ * anti-diagonals), bands wider than 64 cells and five-state models return
+ * MP_EUNSUPPORTED.  Calls on one GPU share that scratch and the engine's
+ * stream: calls that overlap on one GPU race. */
+mp_status mp_pairhmm_forward_log_prob_noalloc(mp_engine *e, const mp_model *m, const mp_pair *pair,
+                                              const mp_band_params *p, double *logProb);
 /* getExpectationsUsingAnchors (pairwiseAligner.c:1151-1156) for one pair:
  * trans9 receives the expected transition counts, likelihood the hmm->likelihood increment. */
 mp_status mp_pairhmm_expectations(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,

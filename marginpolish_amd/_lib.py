@@ -86,7 +86,7 @@ EXPORTS = [
     "mp_poa_get_sizes", "mp_poa_get_nodes", "mp_poa_get_inserts", "mp_poa_get_deletes", "mp_poa_get_observations",
     "mp_bubble_allele_supports", "mp_remove_overlaps", "mp_merge_chunks", "mp_poa_estimate_repeat_counts",
     "mp_poa_anchor_alignments", "mp_poa_consensus", "mp_poa_scores", "mp_poas_estimate_repeat_counts",
-    "mp_batch_list_ms", "mp_batch_fetch_total_log_prob", "mp_batch_stats",
+    "mp_batch_list_ms", "mp_batch_fetch_total_log_prob", "mp_batch_stats", "mp_pairhmm_forward_log_prob_noalloc",
 ]
 
 _lib = None
@@ -134,6 +134,7 @@ def lib():
         "mp_traceback_windows": (i64, [P(i32), P(i32), i64, P(BandParams), P(i32), i64]),
         "mp_batch_fetch_expectations": (C.c_int, [vp, P(dbl), P(dbl)]),
         "mp_pairhmm_forward_log_prob": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl)]),
+        "mp_pairhmm_forward_log_prob_noalloc": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl)]),
         "mp_pairhmm_expectations": (C.c_int, [vp, vp, P(Pair), P(BandParams), P(dbl), P(dbl)]),
         "mp_model_set_transitions": (C.c_int, [vp, P(dbl), i32]),
         "mp_model_set_transitions5": (C.c_int, [vp, P(dbl)]),

@@ -99,8 +99,15 @@ struct DevBuf {
 
 }  // namespace
 
+// Device scratch of the allocation-free entry (mp_pairhmm_forward_log_prob_noalloc):
+// one copy per GPU, held until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 struct mp_engine {
     int device = 0;
+    unsigned char *scratch = nullptr;  // this device's copy of `scratch`
     int cus = 0;
     int splitBlocksPerCU = 1;
     hipStream_t stream = nullptr;
@@ -367,6 +374,95 @@ void choose_crossings(std::vector<DevWin> &w) {
     }
 }
 
+// One job's band entries (bj: padEnd entries, the band padded to whole cells),
+// band class, cell count and traceback windows with their crossings and ring
+// need: the planner's per-job step (pairwiseAligner.c:120-226, 715-783).
+// Returns false when the anchors do not fit the sequences.
+struct PlanScratch {
+    std::vector<int32_t> L, R, XL;
+    std::vector<mp::Window> w;
+};
+bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_band_params &bp, int32_t mode,
+              bool five, int32_t *bj, int64_t padEnd, PlanScratch &sc, std::vector<DevWin> &wj) {
+    const mp_band_params *p = &bp;
+    const bool split = use_split(mode);
+    std::vector<int32_t> &L = sc.L, &R = sc.R, &XL = sc.XL;
+    std::vector<mp::Window> &w = sc.w;
+    const int64_t D = hj.lX + hj.lY;
+    L.resize(D + 1);
+    R.resize(D + 1);
+    const bool dyn = mode != MP_MODE_FORWARD_ONLY && p->dynamicAnchorExpansion != 0;
+    if (!mp::build_band(anchors, nAnchors, hj.lX, hj.lY, p->diagonalExpansion, dyn, L.data(), R.data()))
+        return false;
+    int32_t wmax = 0;
+    int64_t cells = 0;
+    for (int64_t d = 0; d <= D; d++) {
+        const int32_t wd = (R[d] - L[d]) / 2 + 1;
+        wmax = std::max(wmax, wd);
+        cells += wd;
+    }
+    hj.wmax = wmax;
+    hj.cells = D > 0 ? cells : 0;
+    hj.cls = five ? 3 : (wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
+    const bool narrow = hj.cls < 3;
+    // xL = x of the band's first cell: (d + xmyL) / 2
+    XL.resize(D + 1);
+    for (int64_t d = 0; d <= D; d++) XL[d] = (int32_t)((d + L[d]) / 2);
+    if (hj.cls == 0) {
+        // rows of 16 x-aligned lanes: the cells of three consecutive
+        // anti-diagonals (and their neighbours) must span at most 16 x
+        auto xr = [&](int64_t d) { return XL[d] + (R[d] - L[d]) / 2; };
+        bool fits = true;
+        for (int64_t d = 0; d <= D && fits; d++) {
+            int64_t lo = XL[d] - 1, hi = xr(d);
+            for (int64_t e = d - 2; e < d; e++)
+                if (e >= 0) { lo = std::min<int64_t>(lo, XL[e]); hi = std::max<int64_t>(hi, xr(e)); }
+            int64_t lo2 = XL[d], hi2 = xr(d) + 1;
+            for (int64_t e = d + 1; e <= d + 2; e++)
+                if (e <= D) { lo2 = std::min<int64_t>(lo2, XL[e]); hi2 = std::max<int64_t>(hi2, xr(e)); }
+            fits = hi - lo <= 15 && hi2 - lo2 <= 15;
+        }
+        if (!fits) hj.cls = 1;
+    }
+    for (int64_t d = 0; d <= D; d++) {
+        const int32_t wd = (R[d] - L[d]) / 2 + 1;
+        bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
+    }
+    for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
+    if (narrow && hj.cls > 0) {
+        // cells whose neighbour exchange is not a one-lane shift
+        auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
+        for (int64_t d = 0; d <= D; d++) {
+            const bool fastF =
+                d == 0 || (in(XL[d] - XL[d - 1], 0, 1) && (d < 2 || in(XL[d] - XL[d - 2], 0, 2)));
+            const bool fastB =
+                d == D || (in(XL[d] - XL[d + 1], -1, 0) && (d + 2 > D || in(XL[d] - XL[d + 2], -2, 0)));
+            const int64_t c0 = d & ~(int64_t)(kK - 1);
+            if (!fastF)
+                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowF;
+            if (!fastB)
+                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowB;
+        }
+    }
+    w.clear();
+    if (D > 0 && mode == MP_MODE_FORWARD_ONLY) w.push_back(mp::Window{(int32_t)D, 0, (int32_t)D});
+    else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, w);
+    wj.clear();
+    for (auto &x : w) wj.push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
+    if (split && narrow) choose_crossings(wj);
+    // live ring entries.  Narrow kernels (concurrent sweeps): forward and
+    // backward values of (lo_k, tb_{k+1}].  Wide kernels (the backward of
+    // window k runs after its forward): forward values of [lo_k, tb_k].
+    int64_t need = 1;
+    for (size_t k = 0; k < wj.size(); k++) {
+        const int64_t hi = narrow ? wj[std::min(k + 1, wj.size() - 1)].tb : wj[k].tb;
+        need = std::max<int64_t>(need, hi - wj[k].lo + (narrow ? 3 * kK : 2));
+    }
+    hj.ringNeed = (int32_t)((need + kK - 1) / kK * kK);
+    return true;
+}
+
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -404,8 +500,11 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
     int bpc = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void *)pairhmm_split_kernel, 512,
                                                          kSplitLdsBytes));
+    void *scratchPtr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&scratchPtr, HIP_SYMBOL(scratch)));
     mp_engine *e = new mp_engine();
     e->device = device;
+    e->scratch = static_cast<unsigned char *>(scratchPtr);
     e->cus = prop.multiProcessorCount;
     e->splitBlocksPerCU = std::max(1, bpc);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -650,8 +749,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         std::atomic<int64_t> next{0};
         for (int t = 0; t < nth; t++)
             th.emplace_back([&, t]() {
-                std::vector<int32_t> L, R, XL;
-                std::vector<mp::Window> w;
+                PlanScratch sc;
                 std::vector<DevWin> wj;
                 std::vector<DevWin> &pool = winPool[t];
                 for (int64_t j = 0, jEnd = 0;; j++) {
@@ -661,86 +759,15 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     }
                     if (j >= nj) break;
                     HostJob &hj = jobs[j];
-                    const int64_t D = hj.lX + hj.lY;
-                    L.resize(D + 1);
-                    R.resize(D + 1);
+                    hj.bandOff = bandOff[j];
                     const std::vector<int64_t> &a = jobAnchors[j];
-                    const bool dyn = mode != MP_MODE_FORWARD_ONLY && p->dynamicAnchorExpansion != 0;
-                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion, dyn,
-                                        L.data(), R.data())) {
+                    if (!plan_job(hj, a.data(), (int64_t)a.size() / 3, *p, mode, five, band + bandOff[j],
+                                  bandOff[j + 1] - bandOff[j], sc, wj)) {
                         err = 1;
                         continue;
                     }
-                    int32_t wmax = 0;
-                    int64_t cells = 0;
-                    for (int64_t d = 0; d <= D; d++) {
-                        const int32_t wd = (R[d] - L[d]) / 2 + 1;
-                        wmax = std::max(wmax, wd);
-                        cells += wd;
-                    }
-                    hj.wmax = wmax;
-                    hj.cells = D > 0 ? cells : 0;
-                    hj.cls = five ? 3 : (wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
-                    hj.bandOff = bandOff[j];
-                    int32_t *bj = band + bandOff[j];
-                    const bool narrow = hj.cls < 3;
-                    // xL = x of the band's first cell: (d + xmyL) / 2
-                    XL.resize(D + 1);
-                    for (int64_t d = 0; d <= D; d++) XL[d] = (int32_t)((d + L[d]) / 2);
-                    if (hj.cls == 0) {
-                        // rows of 16 x-aligned lanes: the cells of three consecutive
-                        // anti-diagonals (and their neighbours) must span at most 16 x
-                        auto xr = [&](int64_t d) { return XL[d] + (R[d] - L[d]) / 2; };
-                        bool fits = true;
-                        for (int64_t d = 0; d <= D && fits; d++) {
-                            int64_t lo = XL[d] - 1, hi = xr(d);
-                            for (int64_t e = d - 2; e < d; e++)
-                                if (e >= 0) { lo = std::min<int64_t>(lo, XL[e]); hi = std::max<int64_t>(hi, xr(e)); }
-                            int64_t lo2 = XL[d], hi2 = xr(d) + 1;
-                            for (int64_t e = d + 1; e <= d + 2; e++)
-                                if (e <= D) { lo2 = std::min<int64_t>(lo2, XL[e]); hi2 = std::max<int64_t>(hi2, xr(e)); }
-                            fits = hi - lo <= 15 && hi2 - lo2 <= 15;
-                        }
-                        if (!fits) hj.cls = 1;
-                    }
-                    for (int64_t d = 0; d <= D; d++) {
-                        const int32_t wd = (R[d] - L[d]) / 2 + 1;
-                        bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
-                    }
-                    const int64_t padEnd = bandOff[j + 1] - bandOff[j];
-                    for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
-                    if (narrow && hj.cls > 0) {
-                        // cells whose neighbour exchange is not a one-lane shift
-                        auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
-                        for (int64_t d = 0; d <= D; d++) {
-                            const bool fastF =
-                                d == 0 || (in(XL[d] - XL[d - 1], 0, 1) && (d < 2 || in(XL[d] - XL[d - 2], 0, 2)));
-                            const bool fastB =
-                                d == D || (in(XL[d] - XL[d + 1], -1, 0) && (d + 2 > D || in(XL[d] - XL[d + 2], -2, 0)));
-                            const int64_t c0 = d & ~(int64_t)(kK - 1);
-                            if (!fastF)
-                                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowF;
-                            if (!fastB)
-                                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowB;
-                        }
-                    }
-                    w.clear();
-                    if (D > 0 && mode == MP_MODE_FORWARD_ONLY) w.push_back(mp::Window{(int32_t)D, 0, (int32_t)D});
-                    else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, w);
-                    wj.clear();
-                    for (auto &x : w) wj.push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
-                    if (split && narrow) choose_crossings(wj);
                     winRef[j] = WinRef{t, (int32_t)wj.size(), (int64_t)pool.size()};
                     pool.insert(pool.end(), wj.begin(), wj.end());
-                    // live ring entries.  Narrow kernels (concurrent sweeps): forward and
-                    // backward values of (lo_k, tb_{k+1}].  Wide kernels (the backward of
-                    // window k runs after its forward): forward values of [lo_k, tb_k].
-                    int64_t need = 1;
-                    for (size_t k = 0; k < wj.size(); k++) {
-                        const int64_t hi = narrow ? wj[std::min(k + 1, wj.size() - 1)].tb : wj[k].tb;
-                        need = std::max<int64_t>(need, hi - wj[k].lo + (narrow ? 3 * kK : 2));
-                    }
-                    hj.ringNeed = (int32_t)((need + kK - 1) / kK * kK);
                 }
             });
         for (auto &t : th) t.join();
@@ -1650,6 +1677,139 @@ extern "C" mp_status mp_pairhmm_forward_log_prob(mp_engine *e, const mp_model *m
     if (!(st = mp_batch_run_plain(b))) st = mp_batch_fetch_counts(b, counts, logProb);
     mp_batch_free(b);
     return st;
+}
+
+// computeForwardProbability for one pair with no device allocation: the
+// encoded tables, the job and its window, the band entries, the symbols, the
+// counters and the value ring of the forward-only kernel live in this device's
+// `scratch`; the same kernel as mp_pairhmm_forward_log_prob runs on the
+// engine's stream.  Calls on one device share the scratch (and the engine's
+// stream): calls that overlap on one device race.
+extern "C" mp_status mp_pairhmm_forward_log_prob_noalloc(mp_engine *e, const mp_model *m, const mp_pair *pair,
+                                                         const mp_band_params *p, double *logProb) {
+    if (!e || !m || !pair || !p || !logProb) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: null argument");
+        return MP_EINVAL;
+    }
+    const mp_pair &pr = *pair;
+    if (pr.lX < 0 || pr.lY < 0 || (pr.lX > 0 && !pr.sX) || (pr.lY > 0 && !pr.sY) || pr.nAnchors < 0 ||
+        (pr.nAnchors > 0 && !pr.anchors)) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: bad sequence or anchor arguments");
+        return MP_EINVAL;
+    }
+    if (m->mach[1].nStates != 3) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: three-state machines only");
+        return MP_EUNSUPPORTED;
+    }
+    if (p->diagonalExpansion < 0 || (p->diagonalExpansion & 1)) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: diagonalExpansion must be even and non-negative");
+        return MP_EINVAL;
+    }
+    if (!mp::anchors_valid(pr.anchors, pr.nAnchors, pr.lX, pr.lY)) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: anchors must be strictly increasing, inside the "
+                      "sequences, with even non-negative expansion");
+        return MP_EANCHORS;
+    }
+    const int64_t D = pr.lX + pr.lY;
+    if (D == 0) {
+        *logProb = 0.0;
+        return MP_OK;
+    }
+    // scratch layout (256-byte aligned regions)
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const int64_t padEnd = (D / kK + 2) * kK;
+    const int64_t nsym = kSymPad + 2 + pr.lX + pr.lY + kSymPad;
+    const size_t oTab = 0, oJob = up(oTab + sizeof(float) * kTabFloats), oWin = up(oJob + sizeof(DevJob)),
+                 oCnt = up(oWin + sizeof(DevWin)), oRing = up(oCnt + 64),
+                 oBand = up(oRing + sizeof(float4) * (size_t)(4 * kK) * 64),
+                 oSym = up(oBand + sizeof(int32_t) * (size_t)padEnd), oEnd = oSym + sizeof(uint32_t) * (size_t)nsym;
+    if (D >= kMaxDiagonals || oEnd > kNoAllocScratchBytes) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: the pair needs " + std::to_string(oEnd) +
+                      " bytes of device scratch; the allocation-free entry holds " +
+                      std::to_string(kNoAllocScratchBytes) + " (use mp_pairhmm_forward_log_prob)");
+        return MP_EUNSUPPORTED;
+    }
+    // host staging, reused across calls on this thread
+    thread_local std::vector<uint32_t> sym;
+    thread_local std::vector<int32_t> band;
+    thread_local std::vector<float> tab;
+    thread_local std::vector<DevWin> wj;
+    thread_local PlanScratch sc;
+    const int strand = pr.strand ? 1 : 0;
+    sym.assign((size_t)nsym, enc_x(4, 0));
+    sym[0] = kZeroSym;
+    const int64_t baseX = kSymPad + 1, baseY = baseX + pr.lX + 1;
+    sym[kSymPad] = enc_x(4, strand);
+    for (int64_t x = 0; x < pr.lX; x++) sym[baseX + x] = enc_x(pr.sX[x], strand);
+    sym[baseY - 1] = enc_y(4, strand);
+    for (int64_t y = 0; y < pr.lY; y++) sym[baseY + y] = enc_y(pr.sY[y], strand);
+    HostJob hj{};
+    hj.lX = (int32_t)pr.lX;
+    hj.lY = (int32_t)pr.lY;
+    hj.symX = baseX;
+    hj.symY = baseY;
+    hj.flags = strand | (pr.raggedLeft ? 2 : 0) | (pr.raggedRight ? 4 : 0);
+    band.resize((size_t)padEnd);
+    if (!plan_job(hj, pr.anchors, pr.nAnchors, *p, MP_MODE_FORWARD_ONLY, false, band.data(), padEnd, sc, wj)) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: invalid band (anchors inconsistent with the sequences)");
+        return MP_EANCHORS;
+    }
+    if (hj.cls > 2) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: band width " + std::to_string(hj.wmax) +
+                      " exceeds the 64 cells of the allocation-free entry (use mp_pairhmm_forward_log_prob)");
+        return MP_EUNSUPPORTED;
+    }
+    KParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    build_tables(m, false, tab, kp);
+    DevJob dj{};
+    dj.symX = hj.symX;
+    dj.symY = hj.symY;
+    dj.bandOff = 0;
+    dj.lX = hj.lX;
+    dj.lY = hj.lY;
+    dj.winOff = 0;
+    dj.nWin = 1;
+    dj.flags = hj.flags;
+    unsigned char *sp = e->scratch;
+    kp.tables = reinterpret_cast<const float *>(sp + oTab);
+    kp.jobs = reinterpret_cast<const DevJob *>(sp + oJob);
+    kp.win = reinterpret_cast<const DevWin *>(sp + oWin);
+    int32_t *cnt = reinterpret_cast<int32_t *>(sp + oCnt);  // queue[4], error, counts[2], pad, logTotal
+    kp.queue = cnt;
+    kp.error = cnt + 4;
+    kp.counts = cnt + 5;
+    kp.logTotal = reinterpret_cast<int2 *>(cnt + 8);
+    kp.ring = reinterpret_cast<float *>(sp + oRing);
+    kp.ringLen = 4 * kK;
+    kp.band = reinterpret_cast<const int *>(sp + oBand);
+    kp.sym = reinterpret_cast<const uint32_t *>(sp + oSym);
+    kp.mode = MP_MODE_FORWARD_ONLY;
+    kp.threshold = (float)p->threshold;
+    for (int c = 0; c < kNumClasses; c++) kp.jobBegin[c] = kp.jobEnd[c] = 1;
+    kp.jobBegin[hj.cls] = 0;
+    kp.wideBegin = kp.wideEnd = 1;
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    HIP_TRY(hipMemcpyAsync(sp + oTab, tab.data(), sizeof(float) * kTabFloats, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sp + oJob, &dj, sizeof(DevJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sp + oWin, wj.data(), sizeof(DevWin), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sp + oBand, band.data(), sizeof(int32_t) * (size_t)padEnd, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sp + oSym, sym.data(), sizeof(uint32_t) * (size_t)nsym, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, cnt, 0, 10);  // counters, error, total
+    hipLaunchKernelGGL(pairhmm_seq_kernel, dim3(1), dim3(64), kSeqLdsBytes, s, kp);
+    HIP_TRY(hipGetLastError());
+    int32_t back[10];
+    HIP_TRY(hipMemcpyAsync(back, cnt, sizeof(back), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (back[4]) {
+        mp::set_error("mp_pairhmm_forward_log_prob_noalloc: kernel error code " + std::to_string(back[4]));
+        return MP_EHIP;
+    }
+    float t;
+    std::memcpy(&t, &back[8], sizeof(float));
+    *logProb = std::log((double)t) + (double)back[9] * 0.69314718055994530942;
+    return MP_OK;
 }
 
 extern "C" mp_status mp_pairhmm_expectations(mp_engine *e, const mp_model *m, const mp_pair *pair,
