@@ -63,7 +63,13 @@ enum {
                                     whose forward sweep restarts 512 anti-diagonals before the window from a
                                     flat vector (posteriors are invariant to the forward's scale and its
                                     direction forgets the start within ~100 anti-diagonals of an anchored
-                                    band); the log-likelihood is chained in FP64 across the windows */
+                                    band); the log-likelihood is chained in FP64 across the windows */,
+    MP_BATCH_REFERENCE_ARITHMETIC = 16 /* the reference's own arithmetic on the GPU: FP64 log space with its
+                                    polynomial logAdd (pairwiseAligner.c:279-299), its operation order, windows and
+                                    every-10th-diagonal total (687-808), so weights are the reference's (up to the
+                                    last bit of exp); three-state machines, posterior and match-only modes, any
+                                    band width.  For callers whose integer decisions must be the reference's
+                                    (chunk stitching, consensus identity); one workgroup per sub-problem */
 };
 
 typedef struct mp_model mp_model;          /* both strands' state machines + tables */

@@ -23,6 +23,7 @@ BATCH_CROP_REFERENCE = 1
 BATCH_FP64 = 2
 BATCH_NO_GRAPH = 4
 BATCH_EXACT_CHAIN = 8
+BATCH_REFERENCE_ARITHMETIC = 16
 
 
 class MPError(RuntimeError):

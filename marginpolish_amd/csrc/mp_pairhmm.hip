@@ -28,6 +28,7 @@
 
 #include "mp_internal.hpp"
 #include "mp_lists.hpp"
+#include "mp_ref.hpp"
 #include "mp_seq.hpp"
 #include "mp_split.hpp"
 #include "mp_wide.hpp"
@@ -188,6 +189,17 @@ struct mp_batch {
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
     double batchLogProb = 0.0;
+    // reference-arithmetic batches (MP_BATCH_REFERENCE_ARITHMETIC, mp_ref.hpp)
+    bool ref = false;
+    std::vector<mpk::RefJob> refJobs;
+    mpk::RefParams rp{};
+    int32_t refBlocks = 0;
+    DevBuf<mpk::RefJob> dRefJobs;
+    DevBuf<uint16_t> dRawSym;
+    DevBuf<int2> dLR;
+    DevBuf<double> dRepeat, dRefF, dRefB, dRefDots, dRefLogTotal;
+    DevBuf<int64_t> dRefOffs, dSlotBase3;
+    DevBuf<mpk::ListTuple> dRaw;
 };
 
 namespace {
@@ -272,6 +284,201 @@ mp_status alloc_lists(mp_batch *b) {
             b->listCap[l] = want;
         }
     }
+    HIP_TRY(hipStreamSynchronize(s));
+    return MP_OK;
+}
+
+// Reference-arithmetic batch (mp_ref.hpp): raw symbols, the full band of every
+// sub-problem as (xmyL, xmyR) per anti-diagonal, per-workgroup working memory
+// sized by the largest job, tuple regions per job and list.
+mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *pairs, const mp_band_params *p,
+                                 const std::vector<int64_t> &symOff, const std::vector<int64_t> &pairFirstSub,
+                                 std::vector<HostJob> &jobs, std::vector<std::vector<int64_t>> &jobAnchors) {
+    mp_engine *e = b->eng;
+    const int64_t np = b->npairs, nj = (int64_t)jobs.size();
+    b->ref = true;
+    // raw symbols at the same pool positions as the encoded ones (N before each sequence)
+    std::vector<uint16_t> raw((size_t)(symOff[np] + kSymPad), (uint16_t)4);
+    for (int64_t i = 0; i < np; i++) {
+        const mp_pair &pr = pairs[i];
+        const int64_t baseX = symOff[i] + 1, lXc = symOff[i + 1] - symOff[i] - 2 - pr.lY, baseY = baseX + lXc + 1;
+        for (int64_t x = 0; x < lXc; x++) raw[baseX + x] = pr.sX[b->pairCrop[i] + x];
+        for (int64_t y = 0; y < pr.lY; y++) raw[baseY + y] = pr.sY[y];
+    }
+    // bands
+    std::vector<int64_t> lrOff(nj + 1, 0);
+    for (int64_t j = 0; j < nj; j++) lrOff[j + 1] = lrOff[j] + jobs[j].lX + jobs[j].lY + 1;
+    std::vector<int2> lr((size_t)std::max<int64_t>(lrOff[nj], 1));
+    std::vector<int64_t> cells(nj, 0);
+    std::vector<int32_t> wmax(nj, 0);
+    std::atomic<int> bad{0};
+    {
+        const int nth = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::atomic<int64_t> next{0};
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&]() {
+                std::vector<int32_t> L, R;
+                for (int64_t j; (j = next.fetch_add(1)) < nj;) {
+                    const HostJob &hj = jobs[j];
+                    const int64_t D = hj.lX + hj.lY;
+                    L.resize(D + 1);
+                    R.resize(D + 1);
+                    const std::vector<int64_t> &a = jobAnchors[j];
+                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion,
+                                        p->dynamicAnchorExpansion != 0, L.data(), R.data())) {
+                        bad = 1;
+                        continue;
+                    }
+                    int64_t c = 0;
+                    int32_t wm = 0;
+                    for (int64_t d = 0; d <= D; d++) {
+                        lr[lrOff[j] + d] = make_int2(L[d], R[d]);
+                        const int32_t wd = (R[d] - L[d]) / 2 + 1;
+                        c += wd;
+                        wm = std::max(wm, wd);
+                    }
+                    cells[j] = D > 0 ? c : 0;
+                    wmax[j] = wm;
+                }
+            });
+        for (auto &t : th) t.join();
+    }
+    if (bad) {
+        mp::set_error("invalid band (anchors inconsistent with the sequences)");
+        return MP_EANCHORS;
+    }
+    // device order: largest jobs first (the queue is taken in order)
+    std::vector<int32_t> ord(nj);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return cells[a] > cells[c]; });
+    std::vector<int32_t> devIndex(nj);
+    b->refJobs.resize(nj);
+    b->jobs.resize(nj);
+    int64_t maxCells = 1, maxD = 1, outOff = 0, sumD = 0;
+    int32_t maxW = 1;
+    b->cells = 0;
+    for (int64_t q = 0; q < nj; q++) {
+        const int32_t j = ord[q];
+        devIndex[j] = (int32_t)q;
+        const HostJob &hj = jobs[j];
+        const int64_t D = hj.lX + hj.lY;
+        mpk::RefJob &R = b->refJobs[q];
+        R.symX = hj.symX;
+        R.symY = hj.symY;
+        R.lrOff = lrOff[j];
+        R.lX = hj.lX;
+        R.lY = hj.lY;
+        R.flags = hj.flags;
+        R.xAdd = (int32_t)(hj.xOff + b->pairCrop[hj.pair] - 1);
+        R.yAdd = (int32_t)(hj.yOff - 1);
+        for (int l = 0; l < 3; l++) {
+            // (at most one tuple of each list per cell; a job that writes more than its region
+            // holds makes the run size every region exactly and run again)
+            R.cap[l] = (l == 0 || b->mode == MP_MODE_POSTERIORS) ? (int32_t)std::min<int64_t>(D + 64, INT32_MAX / 4) : 0;
+            R.outOff[l] = outOff;
+            outOff += R.cap[l];
+        }
+        b->jobs[q] = hj;
+        maxCells = std::max(maxCells, cells[j]);
+        maxD = std::max(maxD, D);
+        maxW = std::max(maxW, wmax[j]);
+        sumD += D;
+        b->cells += cells[j];
+    }
+    b->outTotal = outOff;
+    {
+        std::vector<int32_t> slots, slotK;
+        for (int64_t i = 0; i < np; i++) {
+            b->pairFirstJob[i] = (int64_t)slots.size();
+            const int32_t nsub = (i + 1 < np ? (int32_t)(pairFirstSub[i + 1] - pairFirstSub[i])
+                                             : (int32_t)(nj - pairFirstSub[i]));
+            for (int32_t sp = 0; sp < nsub; sp++) {
+                const int64_t j = pairFirstSub[i] + sp;
+                slots.push_back(jobs[j].lX + jobs[j].lY > 0 ? devIndex[j] : -1);
+                slotK.push_back(1);
+            }
+            b->pairNJobs[i] = nsub;
+        }
+        b->order.swap(slots);
+        b->slotK.swap(slotK);
+    }
+    // per-workgroup working memory, as many workgroups as half the free memory holds (at most 2 per CU)
+    const size_t perWG = sizeof(double) * (size_t)(3 * maxCells + 9 * (int64_t)maxW + 2 * (int64_t)maxW) +
+                         sizeof(int64_t) * (size_t)(maxD + 2);
+    size_t freeB = 0, totalB = 0;
+    HIP_TRY(hipMemGetInfo(&freeB, &totalB));
+    const int64_t fit = (int64_t)(freeB / 2 / perWG);
+    b->refBlocks = (int32_t)std::min<int64_t>({std::max<int64_t>(nj, 1), (int64_t)e->cus * 2, fit});
+    if (b->refBlocks < 1) {
+        mp::set_error("reference arithmetic: a sub-problem's working memory (" + std::to_string(perWG) +
+                      " bytes) exceeds the free device memory");
+        return MP_ENOMEM;
+    }
+    hipStream_t s = e->stream;
+    mp_status st;
+    std::vector<int32_t> pairSlot((size_t)np + 1);
+    for (int64_t i = 0; i < np; i++) pairSlot[i] = (int32_t)b->pairFirstJob[i];
+    pairSlot[np] = (int32_t)b->order.size();
+    const size_t G = (size_t)b->refBlocks;
+    if ((st = b->dRefJobs.upload(b->refJobs, s)) || (st = b->dRawSym.upload(raw, s)) || (st = b->dLR.upload(lr, s)) ||
+        (st = b->dRefF.alloc(G * 3 * (size_t)maxCells)) || (st = b->dRefB.alloc(G * 9 * (size_t)maxW)) ||
+        (st = b->dRefDots.alloc(G * 2 * (size_t)maxW)) || (st = b->dRefOffs.alloc(G * (size_t)(maxD + 2))) ||
+        (st = b->dRaw.alloc((size_t)std::max<int64_t>(outOff, 1))) ||
+        (st = b->dCounts.alloc(3 * (size_t)std::max<int64_t>(nj, 1))) ||
+        (st = b->dRefLogTotal.alloc((size_t)std::max<int64_t>(nj, 1))) || (st = b->dQueue.alloc(kNumClasses + 1)) ||
+        (st = b->dError.alloc(1)) || (st = b->dOverflow.alloc(1)) || (st = b->dListTotals.alloc(1)) ||
+        (st = b->dSlotJob.upload(b->order, s)) || (st = b->dSlotK.upload(b->slotK, s)) ||
+        (st = b->dPairSlot.upload(pairSlot, s)) || (st = b->dSlotBase3.alloc(3 * (b->order.size() + 1))) ||
+        (st = b->dStarts.alloc(3 * ((size_t)np + 1))) || (st = b->dPairLL.alloc(std::max<size_t>((size_t)np, 1))) ||
+        (st = b->dBatchLL.alloc(1)))
+        return st;
+    const bool rle = m->mach[1].rle != 0;
+    if (rle && (st = b->dRepeat.upload(m->repeatLog10.data(), (size_t)4 * 51 * 51, s))) return st;
+    for (int l = 0; l < 3; l++) {
+        const int64_t want = std::max<int64_t>(sumD / 2, 1024);
+        if ((st = b->dList[l].alloc((size_t)want))) return st;
+        b->listCap[l] = want;
+    }
+    mpk::RefParams &rp = b->rp;
+    std::memset(&rp, 0, sizeof(rp));
+    const mp::Machine &f = m->mach[1];
+    const int ti[9] = {mp::kMatchContinue, mp::kMatchFromGapX, mp::kMatchFromGapY, mp::kGapOpenX, mp::kGapOpenY,
+                       mp::kGapExtendX, mp::kGapExtendY, mp::kGapSwitchToX, mp::kGapSwitchToY};
+    for (int k = 0; k < 9; k++) rp.model.trans[k] = f.trans[ti[k]];
+    for (int sd = 0; sd < 2; sd++) {
+        const mp::Machine &mc = m->mach[sd];
+        for (int k = 0; k < 16; k++) rp.model.match[sd][k] = mc.match[k];
+        for (int k = 0; k < 4; k++) {
+            rp.model.gapX[sd][k] = mc.gapX[k];
+            rp.model.gapY[sd][k] = mc.gapY[k];
+        }
+    }
+    rp.model.rle = rle ? 1 : 0;
+    rp.jobs = b->dRefJobs.p;
+    rp.njobs = (int32_t)nj;
+    rp.sym = b->dRawSym.p;
+    rp.lr = b->dLR.p;
+    rp.repeat = rle ? b->dRepeat.p : nullptr;
+    rp.threshold = p->threshold;
+    rp.withIndels = b->mode == MP_MODE_POSTERIORS;
+    rp.poly = 1;
+    rp.minDiags = p->minDiagsBetweenTraceBack;
+    rp.tbd = p->traceBackDiagonals;
+    rp.expansion = p->diagonalExpansion;
+    rp.F = b->dRefF.p;
+    rp.fStride = 3 * maxCells;
+    rp.B = b->dRefB.p;
+    rp.bStride = 3 * (int64_t)maxW;
+    rp.offs = b->dRefOffs.p;
+    rp.oStride = maxD + 2;
+    rp.dots = b->dRefDots.p;
+    rp.wStride = maxW;
+    rp.out = b->dRaw.p;
+    rp.counts = b->dCounts.p;
+    rp.logTotal = b->dRefLogTotal.p;
+    rp.queue = b->dQueue.p;
+    rp.error = b->dError.p;
     HIP_TRY(hipStreamSynchronize(s));
     return MP_OK;
 }
@@ -552,6 +759,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         return MP_EINVAL;
     }
     const bool fp64 = (flags & MP_BATCH_FP64) != 0;
+    const bool refArith = (flags & MP_BATCH_REFERENCE_ARITHMETIC) != 0;
+    if (refArith && (fp64 || m->mach[1].nStates != 3 || !use_split(mode))) {
+        mp::set_error("mp_batch_create: reference arithmetic runs the three-state machine in the posterior and "
+                      "match-only modes");
+        return MP_EUNSUPPORTED;
+    }
     const bool five = m->mach[1].nStates == 5 || fp64;  // every job on the five-state kernel
     if (five && mode == MP_MODE_EXPECTATIONS) {
         mp::set_error("mp_batch_create: expected transition counts are computed for three-state machines in FP32 "
@@ -720,6 +933,16 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
             hj.flags = strand | (rl ? 2 : 0) | (rr ? 4 : 0);
         }
     });
+
+    if (refArith) {
+        mp_status st = create_reference_batch(b, m, pairs, p, symOff, pairFirstSub, jobs, jobAnchors);
+        if (st) {
+            delete b;
+            return st;
+        }
+        *out = b;
+        return MP_OK;
+    }
 
     // ---- 2. bands, windows, classes (parallel over jobs) ----
     const int64_t nj = (int64_t)jobs.size();
@@ -1105,6 +1328,17 @@ static void drop_graph(mp_batch *b) {
 static mp_status enqueue_run(mp_batch *b) {
     const int32_t nj = (int32_t)b->jobs.size();
     mp_status st;
+    if (b->ref) {
+        if ((st = reset_i32(b, b->dError.p, 1)) || (st = reset_i32(b, b->dQueue.p, 1))) return st;
+        if (b->rp.poly)
+            hipLaunchKernelGGL(mpk::pairhmm_ref_kernel<true>, dim3(b->refBlocks), dim3(mpk::kRefThreads), 0,
+                               b->eng->stream, b->rp);
+        else
+            hipLaunchKernelGGL(mpk::pairhmm_ref_kernel<false>, dim3(b->refBlocks), dim3(mpk::kRefThreads), 0,
+                               b->eng->stream, b->rp);
+        HIP_TRY(hipGetLastError());
+        return MP_OK;
+    }
     if (b->mode == MP_MODE_EXPECTATIONS && (st = reset_i32(b, reinterpret_cast<int32_t *>(b->dExpect.p), 20 * nj)))
         return st;
     if (b->mode == MP_MODE_EXPECTATIONS &&
@@ -1126,6 +1360,28 @@ static mp_status enqueue_lists(mp_batch *b) {
     const int64_t np = b->npairs;
     const int32_t ns = (int32_t)b->order.size(), nj = (int32_t)b->jobs.size();
     mp_status st;
+    if (b->ref) {
+        const unsigned gp = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
+        if ((st = reset_i32(b, b->dOverflow.p, 1))) return st;
+        hipLaunchKernelGGL(mpk::ref_slot_scan, dim3(1), dim3(1024), 0, s, b->dSlotJob.p, ns, b->dCounts.p,
+                           b->dRefJobs.p, b->dError.p, b->dSlotBase3.p, b->dOverflow.p);
+        const int4 cap = make_int4((int)std::min<int64_t>(b->listCap[0], INT32_MAX),
+                                   (int)std::min<int64_t>(b->listCap[1], INT32_MAX),
+                                   (int)std::min<int64_t>(b->listCap[2], INT32_MAX), 0);
+        hipLaunchKernelGGL(mpk::ref_lists_copy, dim3((unsigned)std::max(1, std::min(ns, 65535))), dim3(256), 0, s,
+                           b->dSlotJob.p, ns, b->dCounts.p, b->dRefJobs.p, b->dRaw.p, b->dSlotBase3.p, b->dOverflow.p,
+                           cap, b->dList[0].p, b->dList[1].p, b->dList[2].p);
+        hipLaunchKernelGGL(mpk::ref_pair_starts, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase3.p,
+                           b->dOverflow.p, b->dStarts.p, b->dListTotals.p);
+        if (np > 0) {
+            hipLaunchKernelGGL(mpk::ref_pair_log_likelihood, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np,
+                               b->dSlotJob.p, b->dRefLogTotal.p, b->dError.p, b->dPairLL.p);
+            hipLaunchKernelGGL(mpk::batch_log_likelihood, dim3(1), dim3(1024), 0, s, b->dPairLL.p, np, b->dError.p,
+                               b->dBatchLL.p);
+        }
+        HIP_TRY(hipGetLastError());
+        return MP_OK;
+    }
     if (lists_mode(b)) {
         if ((st = reset_i32(b, b->dOverflow.p, 1))) return st;
         hipLaunchKernelGGL(mpk::slot_record_scan, dim3(1), dim3(1024), 0, s, b->dSlotJob.p, ns, b->dCounts.p,
@@ -1206,6 +1462,7 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
     }
     mp_status st;
     float ms = 0.f, msLists = 0.f;
+    if (b->ref) plain = false;  // reference arithmetic: one workgroup per job either way
     const bool useGraph = !plain && (b->flags & MP_BATCH_NO_GRAPH) == 0;
     if (useGraph && !b->graph && (st = capture(b, enqueue_run, &b->graph))) return st;
     if (useGraph && !b->graphLists && (st = capture(b, enqueue_lists, &b->graphLists))) return st;
@@ -1266,6 +1523,26 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         return MP_EHIP;
     }
     if (lists_mode(b)) HIP_TRY(hipMemcpy(&ovf, b->dOverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ovf && b->ref) {
+        // tuple regions too small: size them exactly and run again
+        std::vector<int32_t> c((size_t)nj * 3);
+        HIP_TRY(hipMemcpy(c.data(), b->dCounts.p, sizeof(int32_t) * 3 * nj, hipMemcpyDeviceToHost));
+        int64_t off = 0;
+        for (int32_t j = 0; j < nj; j++)
+            for (int l = 0; l < 3; l++) {
+                mpk::RefJob &R = b->refJobs[j];
+                R.cap[l] = std::max(R.cap[l], c[3 * j + l]);
+                R.outOff[l] = off;
+                off += R.cap[l];
+            }
+        if ((st = b->dRefJobs.upload(b->refJobs, s)) || (st = b->dRaw.alloc((size_t)std::max<int64_t>(off, 1))))
+            return st;
+        b->rp.jobs = b->dRefJobs.p;
+        b->rp.out = b->dRaw.p;
+        b->outTotal = off;
+        drop_graph(b);
+        return batch_run_impl(b, plain);
+    }
     // Overflowing record regions: size them exactly and run the batch again.
     if (ovf) {
         b->hCounts.resize((size_t)nj * 2);
@@ -1296,7 +1573,10 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         int4 tot;
         int64_t recs = 0;
         HIP_TRY(hipMemcpy(&tot, b->dListTotals.p, sizeof(int4), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&recs, b->dSlotBase.p + b->order.size(), sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (b->ref)
+            recs = (int64_t)tot.x + tot.y + tot.z;
+        else
+            HIP_TRY(hipMemcpy(&recs, b->dSlotBase.p + b->order.size(), sizeof(int64_t), hipMemcpyDeviceToHost));
         b->records = recs;
         const int64_t need[3] = {tot.x, tot.y, tot.z};
         bool grow = false;
@@ -1334,6 +1614,10 @@ extern "C" mp_status mp_batch_set_model(mp_batch *b, const mp_model *m) {
         mp::set_error("mp_batch_set_model: null argument");
         return MP_EINVAL;
     }
+    if (b->ref) {
+        mp::set_error("mp_batch_set_model: not available for reference-arithmetic batches");
+        return MP_EUNSUPPORTED;
+    }
     if (m->mach[1].nStates != b->modelStates) {
         mp::set_error("mp_batch_set_model: the model's state count differs from the batch's");
         return MP_EINVAL;
@@ -1361,6 +1645,10 @@ extern "C" mp_status mp_batch_autotune(mp_batch *b, int32_t repeats, int32_t *na
     if (!b) {
         mp::set_error("mp_batch_autotune: null batch");
         return MP_EINVAL;
+    }
+    if (b->ref) {
+        mp::set_error("mp_batch_autotune: not available for reference-arithmetic batches");
+        return MP_EUNSUPPORTED;
     }
     repeats = std::max(1, std::min(repeats, 16));
     // hardware tiling only: the results do not depend on it (jobs come from a queue, each job's

@@ -67,7 +67,9 @@ mp_status align_overlaps(mp_engine *e, const mp_band_params *p, const std::vecto
     mp_status st = mp_model_default_nucleotide(&m);
     if (st) return st;
     mp_batch *b = nullptr;
-    st = mp_batch_create(e, m, pairs.data(), (int64_t)pairs.size(), p, MP_MODE_MATCH_ONLY, MP_BATCH_NO_GRAPH, &b);
+    // the reference's arithmetic: the cut is an integer decision on near-tied weights
+    st = mp_batch_create(e, m, pairs.data(), (int64_t)pairs.size(), p, MP_MODE_MATCH_ONLY,
+                         MP_BATCH_NO_GRAPH | MP_BATCH_REFERENCE_ARITHMETIC, &b);
     mp_model_free(m);
     if (st) return st;
     std::vector<int64_t> counts(3 * pairs.size());

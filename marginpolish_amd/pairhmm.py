@@ -263,7 +263,7 @@ class Batch:
     """A device-resident batch of read/reference pairs (mp_batch)."""
 
     def __init__(self, machines, pairs, p, mode=_lib.MODE_POSTERIORS, crop_reference=False, engine=None,
-                 fp64=False, graph=True, exact_chain=False):
+                 fp64=False, graph=True, exact_chain=False, reference_arithmetic=False):
         """pairs: iterable of (sX, sY, anchors, strand[, raggedLeft, raggedRight]),
         or a PairArray built from one.
         fp64: FP64 arithmetic on the per-cell-exponent kernel (the precision
@@ -271,7 +271,10 @@ class Batch:
         graph: replay the batch's captured hipGraphs on every run (False:
         enqueue the launches directly).
         exact_chain: run a sub-problem's traceback windows on one forward
-        chain (the reference's order) instead of as window jobs."""
+        chain (the reference's order) instead of as window jobs.
+        reference_arithmetic: the reference's own FP64 log-space arithmetic
+        (polynomial logAdd, its operation order and total cadence), so the
+        weights are the reference's."""
         self.engine = engine or default_engine()
         # the mp_pair array (a numpy record array of the same layout)
         pa = pairs if isinstance(pairs, PairArray) else PairArray(pairs)
@@ -283,7 +286,8 @@ class Batch:
         check(lib().mp_batch_create(self.engine.handle, machines.handle, arr, n, C.byref(pc), int(mode),
                                     (_lib.BATCH_CROP_REFERENCE if crop_reference else 0) |
                                     (_lib.BATCH_FP64 if fp64 else 0) | (0 if graph else _lib.BATCH_NO_GRAPH) |
-                                    (_lib.BATCH_EXACT_CHAIN if exact_chain else 0),
+                                    (_lib.BATCH_EXACT_CHAIN if exact_chain else 0) |
+                                    (_lib.BATCH_REFERENCE_ARITHMETIC if reference_arithmetic else 0),
                                     C.byref(h)))
         self._h = h
         self.npairs = n
