@@ -51,11 +51,12 @@ def _same_rle(a, b):
             np.array_equal(np.asarray(a[1], np.uint64), np.asarray(b[1], np.uint64)))
 
 
-def realign(params, cfg, refs, reads, anchors, engine=None):
+def realign(params, cfg, refs, reads, anchors, engine=None, reference_arithmetic=False):
     """poa_realign (polisher.c:656-703) for several chunks at once: refs[c] =
     (bases, counts), reads[c] = [((bases, counts), strand)], anchors[c][r] =
     RLE anchors of read r.  One batch; every chunk's graph built from the
-    batch's lists (mp_batch_poa_augment)."""
+    batch's lists (mp_batch_poa_augment).  reference_arithmetic: the pair-HMM
+    in the reference's own FP64 arithmetic (its weights exactly)."""
     cap = params.maximumRepeatLength if cfg.useRunLengthEncoding else 2
     pairs, pair_poa, pair_no, rle_reads = [], [], [], []
     for c, (ref, rs, an) in enumerate(zip(refs, reads, anchors)):
@@ -68,7 +69,8 @@ def realign(params, cfg, refs, reads, anchors, engine=None):
             rle_reads.append(read)
     graphs = [mpoa.Poa(ref, cap) for ref in refs]
     if pairs:
-        b = pairhmm.Batch(params.machines, pairs, params.p, crop_reference=True, engine=engine)
+        b = pairhmm.Batch(params.machines, pairs, params.p, crop_reference=True, engine=engine,
+                          reference_arithmetic=reference_arithmetic)
         b.run()
         mpoa.augment_batch(b, graphs, pair_poa, rle_reads, pair_no, cfg.poaConstructCompareRepeatCounts)
         b.close()
@@ -82,7 +84,7 @@ def _estimate(params, cfg, graph, ref, reads):
     return np.asarray(ref[0], np.uint8), counts
 
 
-def realign_all(params, cfg, refs, reads, anchors, engine=None):
+def realign_all(params, cfg, refs, reads, anchors, engine=None, reference_arithmetic=False):
     """poa_realignAll (polisher.c:2109-2129) for every chunk: the initial
     graph, then up to maxPoaConsensusIterations consensus rounds (greedy
     stopping per chunk), then the run lengths of marginPolish.c:557-560.
@@ -91,7 +93,7 @@ def realign_all(params, cfg, refs, reads, anchors, engine=None):
         raise NotImplementedError("the bubble-graph polish iterations (poa_polish) are not built; set "
                                   "maxRealignmentPolishIterations to 0")
     n = len(refs)
-    graphs = realign(params, cfg, refs, reads, anchors, engine)
+    graphs = realign(params, cfg, refs, reads, anchors, engine, reference_arithmetic)
     ref_strings = [(np.frombuffer(bytes(r[0]), np.uint8) if isinstance(r[0], (bytes, bytearray))
                     else np.asarray(r[0], np.uint8), np.asarray(r[1], np.uint64)) for r in refs]
     # per-chunk host steps run on a thread pool (the library calls release the interpreter)
@@ -122,7 +124,8 @@ def realign_all(params, cfg, refs, reads, anchors, engine=None):
                     new_anchors.append(an)
             if not todo:
                 break
-            new_graphs = realign(params, cfg, new_refs, [reads[c] for c in todo], new_anchors, engine)
+            new_graphs = realign(params, cfg, new_refs, [reads[c] for c in todo], new_anchors, engine,
+                                 reference_arithmetic)
 
             def rescore(k):
                 c, g2, ref2 = todo[k], new_graphs[k], new_refs[k]
@@ -144,8 +147,8 @@ def realign_all(params, cfg, refs, reads, anchors, engine=None):
     return out
 
 
-def polish_contig(params, cfg, refs, reads, anchors, engine=None, missing_spacer=""):
+def polish_contig(params, cfg, refs, reads, anchors, engine=None, missing_spacer="", reference_arithmetic=False):
     """Polish every chunk of one contig and merge them (chunker.c:55-118 with
     overlap 2 x chunkBoundary)."""
-    res = realign_all(params, cfg, refs, reads, anchors, engine)
+    res = realign_all(params, cfg, refs, reads, anchors, engine, reference_arithmetic)
     return stitch.mergeContigChunks([s for _, s in res], 2 * cfg.chunkBoundary, params, missing_spacer, engine)

@@ -62,8 +62,10 @@ def rotate(r, k):
 
 
 def substring(r, start, length):
-    """rleString_copySubstring (polisher.c:1433-1453)."""
-    return r[0][start:start + length], r[1][start:start + length]
+    """rleString_copySubstring (polisher.c:1433-1453).  The run lengths come
+    back as a tuple, the form rotate() returns, so that equal strings compare
+    equal in addToInserts whichever of the two made them."""
+    return r[0][start:start + length], tuple(r[1][start:start + length])
 
 
 def _matches_ref(ref, start, s, length, cmp_counts):

@@ -255,3 +255,18 @@ def test_consensus_known_answers():
     assert P.consensus(_graph("GATTACA", [("ins", 3, "GG", 5.0)]), rle=False)[0] == "GATTACA"
     s, m = P.consensus(_graph("GATTACA", [("del", 2, 2, 90.0)]), rle=False)
     assert s == "GAACA" and m == [0, 1, -1, -1, 2, 3, 4]
+
+
+def test_insert_strings_merge_whatever_made_them():
+    """addToInserts (polisher.c:177-205) merges equal run-length strings: a
+    substring of a read and the rotation that makes the same string are one
+    entry (the oracle once kept them apart because one carried its run lengths
+    as a list and the other as a tuple)."""
+    read = ("GAT", [1, 3, 2])
+    s = P.substring(read, 0, 2)
+    r = P.rotate(("AG", (3, 1)), 1)
+    assert s == r == ("GA", (1, 3))
+    node = P.Node("A", 1, 51)
+    P._add_insert(node, s, 5.0, True, (0, 0, 5.0))
+    P._add_insert(node, r, 7.0, True, (1, 0, 7.0))
+    assert len(node.inserts) == 1 and node.inserts[0][1] == 12.0
