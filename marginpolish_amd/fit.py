@@ -243,13 +243,17 @@ class ModelLikelihood:
     parameters (ModelParams), with the gradient in each natural-log
     parameter: expected transition counts and expected emission counts."""
 
-    def __init__(self, machines, pairs, p, rle=False, engine=None):
+    def __init__(self, machines, pairs, p, rle=False, engine=None, fp64_likelihood=False):
+        """fp64_likelihood: log_likelihood() runs the FP64 kernel (the
+        emission tables stay FP32), e.g. for finite differences; the
+        gradient always comes from the FP32 expectation kernel."""
         self.machines = machines
         self.pairs = list(pairs)
         self.rle = bool(rle)
         p = dataclasses.replace(p, minDiagsBetweenTraceBack=1 << 30)
         like_p = dataclasses.replace(p, threshold=2.0)
-        self._like = pairhmm.Batch(machines, self.pairs, like_p, mode=_lib.MODE_MATCH_ONLY, engine=engine)
+        self._like = pairhmm.Batch(machines, self.pairs, like_p, mode=_lib.MODE_MATCH_ONLY, engine=engine,
+                                   fp64=fp64_likelihood)
         self._expect = pairhmm.Batch(machines, self.pairs, p, mode=_lib.MODE_EXPECTATIONS, engine=engine)
         self.evaluations = 0
 

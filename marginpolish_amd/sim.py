@@ -337,19 +337,22 @@ def _plain(bases):
 
 
 def polish_workload(seed=7, genome_len=30000, draft_err=0.03, read_err=0.08, depth=20, mean=3000, sd=1500,
-                    chunk_size=10000, boundary=250, W=10, rle=True):
-    """A draft to polish: a truth genome (runs ~ Geometric(0.45), GC 45%), a
-    draft = truth with `draft_err` errors, reads sampled from the truth at
-    `read_err`; chunks of the draft with `boundary` bases each side, every
-    read clipped to each chunk it overlaps, RLE anchors from the reads' true
-    placement on the draft (rle=False: plain strings, one symbol per base).
-    Returns (truth bases, draft bases, chunks)."""
+                    chunk_size=10000, boundary=250, W=10, rle=True, gc=0.45, p_geom=0.45, tandem=0.0, lo=500,
+                    hi=None):
+    """A draft to polish: a truth genome (GC `gc`, runs ~ Geometric(p_geom),
+    `tandem` of it in 2-6-mer tandem repeats), a draft = truth with
+    `draft_err` errors, reads of log-normal lengths (clipped to lo..hi,
+    default the genome) sampled from the truth at `read_err`; chunks of the
+    draft with `boundary` bases each side, every read clipped to each chunk it
+    overlaps, RLE anchors from the reads' true placement on the draft
+    (rle=False: plain strings, one symbol per base).  Returns (truth bases,
+    draft bases, chunks)."""
     enc = _rle if rle else _plain
     rng = np.random.default_rng(seed)
-    truth = _ref_runs(rng, genome_len, 0.45, 0.45)
+    truth = _ref_runs(rng, genome_len, gc, p_geom, tandem)
     draft, dpos = _mutate(rng, truth, draft_err)   # dpos: draft position of each truth base
     nreads = int(round(depth * genome_len / mean))
-    lens = np.minimum(_lognormal_lengths(rng, nreads, mean, sd, 500, genome_len), genome_len)
+    lens = np.minimum(_lognormal_lengths(rng, nreads, mean, sd, lo, genome_len if hi is None else hi), genome_len)
     starts = rng.integers(0, np.maximum(1, genome_len - lens + 1))
     strands = rng.integers(0, 2, size=nreads)
     chars = np.frombuffer(b"ACGT", np.uint8)
@@ -369,7 +372,11 @@ def polish_workload(seed=7, genome_len=30000, draft_err=0.03, read_err=0.08, dep
         D, Y = dp[ok], pos[ok]
         if D.size < 2:
             continue
-        for c, ch in enumerate(chunks):
+        # only chunks whose span can hold two of the read's draft positions
+        c0 = max(0, (int(D.min()) - boundary) // chunk_size - 1)
+        c1 = min(nchunks, (int(D.max()) + boundary) // chunk_size + 2)
+        for c in range(c0, c1):
+            ch = chunks[c]
             sel = (D >= ch.start) & (D < ch.end)
             if sel.sum() < 2:
                 continue
@@ -484,3 +491,48 @@ def banded_edit_distance(a, b, band=400):
             cur[lo:hi + 1] = np.minimum(best, run + j)
         prev = cur
     return int(prev[m])
+
+
+def edit_distance(a, b):
+    """Levenshtein distance by diagonal transition (Ukkonen; Landau and
+    Vishkin): for e = 0, 1, ... the furthest row reached on every diagonal
+    with e edits, each slide along equal characters done by numpy; O((n + m)
+    d) for d edits, for near-equal strings of any length."""
+    A = np.frombuffer(a.encode(), np.uint8)
+    B = np.frombuffer(b.encode(), np.uint8)
+    n, m = A.size, B.size
+
+    def slide(i, j):
+        step = 256
+        while i < n and j < m:
+            L = min(step, n - i, m - j)
+            neq = np.flatnonzero(A[i:i + L] != B[j:j + L])
+            if neq.size:
+                return i + int(neq[0])
+            i += L
+            j += L
+            step = min(step * 2, 1 << 16)
+        return i
+
+    goal = m - n  # the diagonal k = j - i of the end cell
+    prev = {0: slide(0, 0)}
+    e = 0
+    while not (goal in prev and prev[goal] >= n):
+        e += 1
+        cur = {}
+        for k in range(max(-e, -n), min(e, m) + 1):
+            best = -1
+            if k in prev:
+                best = prev[k] + 1          # substitution
+            if k - 1 in prev:
+                best = max(best, prev[k - 1])  # b's character inserted (j + 1)
+            if k + 1 in prev:
+                best = max(best, prev[k + 1] + 1)  # a's character deleted (i + 1)
+            if best < 0:
+                continue
+            i = min(best, n, m - k)
+            if i < 0 or i + k < 0:
+                continue
+            cur[k] = slide(i, i + k)
+        prev = cur
+    return e

@@ -47,3 +47,40 @@ def as_rows(arr):
         return np.stack([arr["weight"].astype(np.int64), arr["x"].astype(np.int64), arr["y"].astype(np.int64)],
                         axis=1) if len(arr) else np.zeros((0, 3), np.int64)
     return np.asarray(arr, np.int64).reshape(-1, 3)
+
+
+def keyed(res, li):
+    """Every pair's tuples of list li (0 matches, 1 gap-X, 2 gap-Y) from a
+    batch's fetch(): (sorted keys pair << 42 | (x + 1) << 21 | (y + 1),
+    weights in key order); x or y is -1 for a gap before the first base."""
+    keys, ws = [], []
+    for i, r in enumerate(res):
+        a = r[li]
+        keys.append((np.int64(i) << 42) | ((a["x"].astype(np.int64) + 1) << 21) | (a["y"].astype(np.int64) + 1))
+        ws.append(a["weight"].astype(np.int64))
+    k = np.concatenate(keys) if keys else np.zeros(0, np.int64)
+    w = np.concatenate(ws) if ws else np.zeros(0, np.int64)
+    o = np.argsort(k, kind="stable")
+    return k[o], w[o]
+
+
+def batch_list_err(ra, rb):
+    """Two fetch() results of the same pairs: (max |p_a - p_b| over the
+    tuples both hold, the largest probability held by one side only, the
+    number of repeated (pair, x, y) keys, tuples held by both)."""
+    worst, edge, dups, both = 0.0, 0.0, 0, 0
+    for li in range(3):
+        ka, wa = keyed(ra, li)
+        kb, wb = keyed(rb, li)
+        dups += int(np.sum(ka[1:] == ka[:-1]) + np.sum(kb[1:] == kb[:-1]))
+        pa = np.minimum(np.searchsorted(kb, ka), max(kb.size - 1, 0))
+        inA = (kb[pa] == ka) if kb.size else np.zeros(ka.size, bool)
+        pb = np.minimum(np.searchsorted(ka, kb), max(ka.size - 1, 0))
+        inB = (ka[pb] == kb) if ka.size else np.zeros(kb.size, bool)
+        both += int(inA.sum())
+        if inA.any():
+            worst = max(worst, float(np.abs(wa[inA] - wb[pa[inA]]).max()) / PROB1)
+        for w, only in ((wa, ~inA), (wb, ~inB)):
+            if only.any():
+                edge = max(edge, float(w[only].max()) / PROB1)
+    return worst, edge, dups, both

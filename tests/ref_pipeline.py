@@ -209,3 +209,63 @@ def realign_all(op, cfg, refs, reads, anchors, poa="oracle", machines=None, thre
 
 def graphs_counts(graph, reads):
     return graph.repeat_counts(reads)
+
+
+def remove_overlap(prefix, suffix, approx, op):
+    """removeOverlap (polisher.c:1937-2006) on the oracle in the reference's
+    arithmetic: the prefix's last `approx` characters against the suffix's
+    first `approx`, the default nucleotide machine, no anchors, both ends
+    ragged; the first match tuple of maximal weight is the cut.  Returns
+    (weight, prefix crop end, suffix crop start)."""
+    i = max(len(prefix) - approx, 0)
+    j = min(approx, len(suffix))
+    if len(prefix) - i == 0 or j == 0:
+        return 0, len(prefix), 0
+    mm, _, _, _, _ = O.posteriors(O.default_nucleotide_model(), O.symbols(prefix[i:]), O.symbols(suffix[:j]), [],
+                                  op.p, True, True, with_indels=False, mode=O.LOGADD_POLY)
+    best = None
+    for w, x, y in mm:
+        if best is None or w > best[0]:
+            best = (int(w), int(x), int(y))
+    if best is None:
+        return 0, len(prefix), 0
+    return best[0], best[1] + i, best[2]
+
+
+def merge_contig_chunks(chunks, overlap, op, missing="", threads=None):
+    """mergeContigChunks (chunker.c:50-118) with remove_overlap.  The cut of
+    every boundary against the uncropped predecessor is computed up front
+    (in parallel); replaying the sequential rules, a boundary whose
+    predecessor was cropped into its own overlap window is aligned again
+    with the cropped string."""
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        pre = list(pool.map(lambda k: remove_overlap(chunks[k - 1], chunks[k], overlap, op), range(1, len(chunks))))
+    out = []
+    start = 0  # crop start of the last string in `out` within its chunk
+    for k, cur in enumerate(chunks):
+        if not out:
+            out.append(cur)
+            start = 0
+            continue
+        prev = out[-1]
+        if start <= len(chunks[k - 1]) - overlap:
+            w, ce, cs = pre[k - 1]
+            ce -= start
+        else:
+            w, ce, cs = remove_overlap(prev, cur, overlap, op)
+        if w > 0:
+            out[-1] = prev[:ce]
+            out.append(cur[cs:])
+            start = cs
+        elif len(cur) == 0:
+            out.append(missing)
+            start = len(chunks[k]) + 1  # never equal to a chunk's window: always aligned again
+        elif overlap == 0:
+            out.append(cur)
+            start = 0
+        else:
+            out.append("NNNNNNNNNN")
+            out.append(cur)
+            start = 0
+    return "".join(out)

@@ -1,17 +1,23 @@
-"""Parameter-fitting harness on the GPU: the gradient of the summed
-log-likelihood in the nine log transitions (expected transition counts from
-the expectation-mode kernel) against central finite differences of the
-batched likelihood, against the FP64 oracle's expected counts, and recovery of
-the generating transitions by Adam.
+"""Parameter-fitting harness on the GPU.
 
-Bars: gradient vs central differences (h = 1e-3 in log space) within 1e-3
-relative, with an absolute floor of 0.1 counts for the FP32 rounding of the
-likelihood; expected counts vs O_exact within 1e-4 relative (+1e-3 absolute);
-fitted transitions within 5% relative of the generating ones, with the gap
-states tied as in the reference's symmetric machine (untied, the two switch
-transitions X->Y and Y->X trade off along a nearly flat direction of the
-likelihood: a gap-X/gap-Y pair between matches reads the same in either
-order)."""
+Small cases: the gradient of the summed log-likelihood (expected transition
+and emission counts from the expectation-mode kernel) against central finite
+differences of the FP32 batched likelihood (h = 1e-3 in log space, within
+1e-3 relative with an absolute floor of 0.1 counts for the FP32 rounding of
+the likelihood) and against the FP64 oracle's expected counts (O_exact,
+within 1e-4 relative + 1e-3 absolute).
+
+The differentiation workload at its own shape (4096 pairs, 2 kb references,
+band half-width 48, reads from a generating model with transitions, 4 x 4
+substitution, gap emissions and a 51 x 51 run-length confusion): the FP32
+gradient against central differences of the FP64 likelihood (h = 1e-4,
+within 1e-3 relative), counts against O_exact on 16 of the pairs, and 200
+Adam steps from a 20% perturbation recovering the generating parameters
+within 5% (gap states tied as in the reference's symmetric machine: untied,
+the two switch transitions X->Y and Y->X trade off along a nearly flat
+direction of the likelihood, a gap-X/gap-Y pair between matches reading the
+same in either order)."""
+import math
 import sys
 
 import numpy as np
@@ -91,22 +97,6 @@ def test_counts_match_cpu_oracle(fit, setup):
         assert np.allclose(tg, te, rtol=1e-4, atol=1e-3), (tg, te)
 
 
-def test_fit_recovers_generating_transitions(fit, setup):
-    ph, machines, (m16, gx, gy) = setup
-    pairs = fit.simulate_pairs(2048, 1000, T_STAR, m16, gx, gy, 16, seed=13)
-    p = ph.PairwiseAlignmentParameters(threshold=0.01, diagonalExpansion=16, dynamicAnchorExpansion=True)
-    say("case: Adam from a 20% perturbation, 2048 pairs x 1000 bp, gap states tied")
-    like = fit.TransitionLikelihood(machines, pairs, p)
-    rng = np.random.default_rng(5)
-    T0 = T_STAR * (1.0 + 0.2 * rng.choice([-1.0, 1.0], size=(3, 3)))
-    That, hist = fit.fit_transitions(like, T0, steps=200, lr=0.05, symmetric=True)
-    like.close()
-    rel = np.abs(That - T_STAR) / T_STAR
-    say("fitted", That.round(4).tolist(), "max rel err %.4f" % rel.max(), "LL/pair %.3f -> %.3f" % (hist[0], hist[-1]))
-    assert hist[-1] > hist[0]
-    assert rel.max() < 0.05, That
-
-
 # ---------------------------------------------------------------------------
 # emission parameters (substitution, gaps, run-length confusion)
 # ---------------------------------------------------------------------------
@@ -181,24 +171,124 @@ def test_emission_gradient_matches_central_differences(fit, tmp_path_factory):
     like.close()
 
 
-def test_fit_recovers_generating_emissions(fit, tmp_path_factory):
-    """Adam on the transitions (gap states tied), the joint substitution
-    matrix and both gap emissions from a 20% perturbation, 8192 pairs x
-    1000 bp: every parameter within 5% of the generating one.  (With 4096
-    pairs the maximum-likelihood switch transition X<->Y itself sits 5.3% from
-    the generating 0.05 -- Adam started at the generating parameters lands on
-    the same point: a gap-X/gap-Y switch and a substitution explain the same
-    columns, so the switch is the least determined parameter.)"""
+# ---------------------------------------------------------------------------
+# the differentiation workload at its own shape: 4096 pairs, 2 kb references,
+# band half-width 48
+# ---------------------------------------------------------------------------
+
+N_PAIRS, REF_LEN, W = 4096, 2000, 48
+
+
+def _spec_rle_pairs(fit, T, m16, gx, gy, R, seed):
+    """4096 run-length pairs of 2 kb references, half on each strand."""
+    half = N_PAIRS // 2
+    return fit.simulate_rle_pairs(half, REF_LEN, T, m16, gx, gy, R, W, seed=seed, strand=1) + \
+        fit.simulate_rle_pairs(half, REF_LEN, T, m16, gx, gy, R, W, seed=seed + 1, strand=0)
+
+
+def _spec_params(fit):
+    from marginpolish_amd import pairhmm
+    return pairhmm.PairwiseAlignmentParameters(threshold=0.01, diagonalExpansion=W, dynamicAnchorExpansion=True)
+
+
+def _table_step(v, h):
+    """The step in log space that an FP32 table entry exp(v) really takes when
+    v moves by +-h: the engine stores (float)exp(v) (emission and run-length
+    tables; FP64 batches keep the transitions in double)."""
+    up = np.log(np.float64(np.float32(np.exp(v + h))))
+    dn = np.log(np.float64(np.float32(np.exp(v - h))))
+    return up - dn
+
+
+@pytest.fixture(scope="module")
+def spec_rle(fit, tmp_path_factory):
+    """The generating model (3-state transitions, 4 x 4 substitution, gap
+    emissions, 51 x 51 run-length confusion) and 4096 pairs drawn from it."""
+    machines, (m16, gx, gy, R) = _rle_setup(fit, tmp_path_factory, "spec")
+    gx = np.array([0.3, 0.2, 0.2, 0.3])
+    gy = np.array([0.2, 0.3, 0.3, 0.2])
+    machines.set_emissions(np.log(m16), np.log(gx), np.log(gy))
+    pairs = _spec_rle_pairs(fit, T_STAR, m16, gx, gy, R, seed=61)
+    return machines, pairs, (m16, gx, gy, R)
+
+
+def test_spec_shape_gradient_vs_fp64_central_differences(fit, spec_rle):
+    """d(sum LL)/d(natural-log parameter) from the FP32 forward and the
+    hand-written backward (expected counts) on 4096 run-length pairs x 2 kb
+    at W = 48, against central differences (h = 1e-4) of the FP64
+    likelihood, within 1e-3 relative: all nine transitions, all 16
+    substitution entries, all gap emissions and run-length entries.  The
+    divisor is the step each parameter's table entries really take (FP32
+    emission tables: log fl32(e^(v+h)) - log fl32(e^(v-h)); FP64 transitions:
+    2h)."""
+    machines, pairs, (m16, gx, gy, R) = spec_rle
+    say("case: spec-shape gradient vs FP64 central differences, %d RLE pairs x %d, W=%d" % (len(pairs), REF_LEN, W))
+    like = fit.ModelLikelihood(machines, pairs, _spec_params(fit), rle=True, fp64_likelihood=True)
+    base = _true_params(fit, m16, gx, gy, R)
+    _, g = like.evaluate(base)
+    say("cells per evaluation: %d" % like._expect.cells)
+    h = 1e-4
+    probes = [(0, ab) for ab in fit.TRANS_FROM_TO] + [(1, ij) for ij in np.ndindex(4, 4)] + \
+        [(2, (i,)) for i in range(4)] + [(3, (i,)) for i in range(4)] + \
+        [(4, (0, 1, 1)), (4, (1, 2, 2)), (4, (2, 3, 4)), (4, (3, 1, 2)), (4, (0, 2, 1))]
+    worst = 0.0
+    for k, idx in probes:
+        arr = base.arrays()
+        up, dn = [a.copy() for a in arr], [a.copy() for a in arr]
+        up[k][idx] += h
+        dn[k][idx] -= h
+        step = 2 * h if k == 0 else _table_step(arr[k][idx], h)
+        fd = (math.fsum(like.log_likelihood(fit.ModelParams(*up))) -
+              math.fsum(like.log_likelihood(fit.ModelParams(*dn)))) / step
+        rel = abs(fd - g[k][idx]) / abs(fd)
+        worst = max(worst, rel)
+        assert rel <= 1e-3, ((k, idx), fd, g[k][idx])
+    like.close()
+    say("max relative gradient error vs FP64 central differences over %d parameters: %.3g" % (len(probes), worst))
+
+
+def test_spec_shape_counts_match_cpu_oracle(fit, spec_rle):
+    """Expected transition counts (per pair) and emission counts (summed) of
+    16 of the 4096 pairs, 8 per strand, against the FP64 oracle's (O_exact)."""
+    machines, pairs, (m16, gx, gy, R) = spec_rle
+    sample = pairs[:8] + pairs[N_PAIRS // 2: N_PAIRS // 2 + 8]
+    say("case: spec-shape counts vs O_exact, 16 RLE pairs x %d, W=%d" % (REF_LEN, W))
+    like = fit.ModelLikelihood(machines, sample, _spec_params(fit), rle=True)
+    _, grads = like.evaluate(_true_params(fit, m16, gx, gy, R))
+    t_gpu, _ = like._expect.expectations()
+    like.close()
+    op = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1 << 30, traceBackDiagonals=40, diagonalExpansion=W,
+                  splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    rep = np.concatenate([np.log(np.where(R > 0, R, 1e-300)).reshape(-1) / fit.RL_SCALE, np.zeros(51 * 51)])
+    models = {}
+    for st in (0, 1):
+        tab = machines.tables(st)
+        models[st] = O.Model(dict(zip(fit.TRANS_NAMES, tab["trans"])), tab["match"], tab["gapX"], tab["gapY"],
+                             rle=True, strand=st, repeat=rep)
+    want = [np.zeros((4, 4)), np.zeros(4), np.zeros(4), np.zeros((4, 51, 51))]
+    for (sx, sy, an, st), tg in zip(sample, t_gpu):
+        te, _ = O.expectations(models[st], sx, sy, an, op)
+        assert np.allclose(tg, te, rtol=1e-4, atol=1e-3), (tg, te)
+        for acc, v in zip(want, O.emission_counts(models[st], sx, sy, an, op)):
+            acc += v
+    for g, w in zip(grads[1:], want):
+        assert np.allclose(g, w, rtol=1e-4, atol=1e-3), (np.abs(g - w).max(), w.max())
+
+
+def test_spec_shape_fit_recovers_generating_parameters(fit, tmp_path_factory):
+    """200 Adam steps from a 20% perturbation of the transitions (gap states
+    tied), the 4 x 4 substitution matrix and both gap emissions, on 4096
+    pairs x 2 kb at W = 48: every parameter within 5% of the generating
+    one."""
     from marginpolish_amd import pairhmm
     m16, gx, gy = _emissions()
     gx = np.array([0.3, 0.2, 0.2, 0.3])
     gy = np.array([0.2, 0.3, 0.3, 0.2])
-    path = str(tmp_path_factory.mktemp("emitfit") / "generating_model.json")
+    path = str(tmp_path_factory.mktemp("specfit") / "generating_model.json")
     machines = fit.generating_model(PARAMS, T_STAR, m16, gx, gy, path)
-    pairs = fit.simulate_pairs(8192, 1000, T_STAR, m16, gx, gy, 16, seed=41)
-    p = pairhmm.PairwiseAlignmentParameters(threshold=0.01, diagonalExpansion=16, dynamicAnchorExpansion=True)
-    say("case: Adam on transitions + substitution + gap emissions, 8192 pairs x 1000 bp")
-    like = fit.ModelLikelihood(machines, pairs, p, rle=False)
+    pairs = fit.simulate_pairs(N_PAIRS, REF_LEN, T_STAR, m16, gx, gy, W, seed=41)
+    say("case: Adam on transitions + substitution + gap emissions, %d pairs x %d, W=%d" % (N_PAIRS, REF_LEN, W))
+    like = fit.ModelLikelihood(machines, pairs, _spec_params(fit), rle=False)
     truth = fit.ModelParams(np.log(T_STAR), np.log(m16).reshape(4, 4), np.log(gx), np.log(gy))
     rng = np.random.default_rng(8)
     pert = lambda a: a + np.log1p(0.2 * rng.choice([-1.0, 1.0], size=np.shape(a)))
@@ -213,27 +303,26 @@ def test_fit_recovers_generating_emissions(fit, tmp_path_factory):
     assert max(r.max() for r in rel) < 0.05
 
 
-def test_fit_recovers_run_length_confusion(fit, tmp_path_factory):
+def test_spec_shape_fit_recovers_run_length_confusion(fit, tmp_path_factory):
     """The 51 x 51 run-length confusion with the substitution matrix and the
-    gap-X emissions, on run-length symbols from a generating model without
-    insertions (the run-length term scores matched symbols only, so only then
-    is the model a proper distribution of the read's run lengths),
-    transitions held at the generating values; 4096 pairs x 1000 bp, 20%
-    perturbation: substitution and gap-X within 5%, run-length entries with at
-    least 5000 expected observations within 5%."""
-    from marginpolish_amd import pairhmm
+    gap-X emissions, on 4096 run-length pairs x 2 kb at W = 48 from a
+    generating model without insertions (the run-length term scores matched
+    symbols only, so only then is the model a proper distribution of the
+    read's run lengths), transitions held at the generating values, 200 Adam
+    steps from a 20% perturbation: substitution and gap-X within 5%,
+    run-length entries with at least 5000 expected observations within 5%."""
     T_noins = np.array([[0.95, 0.05, 0.0], [0.70, 0.30, 0.0], [0.70, 0.0, 0.30]])
     m16, gx, gy = _emissions()
-    path = str(tmp_path_factory.mktemp("rlfit") / "generating_model.json")
+    path = str(tmp_path_factory.mktemp("specrl") / "generating_model.json")
     machines = fit.generating_model(PARAMS, T_noins, m16, gx, gy, path)
     R = fit.rl_confusion()
     with np.errstate(divide="ignore"):
         logR = np.log(R)
     machines.set_repeat_log10(np.maximum(logR, -690.0) / fit.RL_SCALE, True)
-    pairs = fit.simulate_rle_pairs(4096, 1000, T_noins, m16, gx, gy, R, 16, seed=43)
-    p = pairhmm.PairwiseAlignmentParameters(threshold=0.01, diagonalExpansion=16, dynamicAnchorExpansion=True)
-    say("case: Adam on run-length confusion + substitution + gap X, 4096 RLE pairs x 1000 bp, no insertions")
-    like = fit.ModelLikelihood(machines, pairs, p, rle=True)
+    pairs = _spec_rle_pairs(fit, T_noins, m16, gx, gy, R, seed=43)
+    say("case: Adam on run-length confusion + substitution + gap X, %d RLE pairs x %d, W=%d, no insertions"
+        % (N_PAIRS, REF_LEN, W))
+    like = fit.ModelLikelihood(machines, pairs, _spec_params(fit), rle=True)
     with np.errstate(divide="ignore"):
         truth = fit.ModelParams(np.log(T_noins), np.log(m16).reshape(4, 4), np.log(gx), np.log(gy),
                                 np.maximum(logR, -690.0))

@@ -9,6 +9,7 @@ from marginpolish_amd import pairhmm as ph
 from marginpolish_amd import stitch
 from oracle import oracle as O
 from tests.conftest import PARAMS
+from tests import ref_pipeline as RP
 
 pytestmark = pytest.mark.gpu
 
@@ -28,22 +29,7 @@ def _mutate(rng, s, rate):
     return "".join(out)
 
 
-def _oracle_cut(prefix, suffix, approx, op):
-    """removeOverlap (polisher.c:1937-2006) on the oracle in the reference's
-    arithmetic (polynomial logAdd): (weight, prefix crop end, suffix crop start)."""
-    i = max(len(prefix) - approx, 0)
-    j = min(approx, len(suffix))
-    if len(prefix) - i == 0 or j == 0:
-        return 0, len(prefix), 0
-    mm, _, _, _, _ = O.posteriors(O.default_nucleotide_model(), O.symbols(prefix[i:]), O.symbols(suffix[:j]), [],
-                                  op.p, True, True, with_indels=False, mode=O.LOGADD_POLY)
-    best = None
-    for w, x, y in mm:
-        if best is None or w > best[0]:
-            best = (int(w), int(x), int(y))
-    if best is None:
-        return 0, len(prefix), 0
-    return best[0], best[1] + i, best[2]
+_oracle_cut = RP.remove_overlap
 
 
 def test_remove_overlaps_match_oracle():
@@ -126,11 +112,14 @@ def test_merge_chunks_equals_sequential_merge():
         chunks.append(_mutate(rng, genome[max(0, s - bnd):s + size + bnd], 0.005))
     chunks[5] = ""                      # a missing chunk -> spacer (which the next boundary may crop)
     chunks[8] = chunks[8][:200]         # shorter than the overlap: its crops reach into the next window
+    op = O.PolishParams(PARAMS)
     merged = stitch.mergeContigChunks(chunks, 2 * bnd, pp, "NNNNN")
     assert merged == _merge_reference(chunks, 2 * bnd, pp, "NNNNN")
+    assert merged == RP.merge_contig_chunks(chunks, 2 * bnd, op, "NNNNN")
     # first chunk empty (no overlap to find: a "NNNNNNNNNN" stitch), overlap 0
     odd = ["", chunks[1], chunks[2]]
     assert stitch.mergeContigChunks(odd, 2 * bnd, pp, "") == _merge_reference(odd, 2 * bnd, pp, "")
+    assert stitch.mergeContigChunks(odd, 2 * bnd, pp, "") == RP.merge_contig_chunks(odd, 2 * bnd, op, "")
     assert stitch.mergeContigChunks(chunks[:3], 0, pp, "") == _merge_reference(chunks[:3], 0, pp, "")
     assert stitch.mergeContigChunks([], 10, pp, "") == ""
 

@@ -42,3 +42,43 @@ def test_oracle_and_library_poa_agree():
     b = RP.realign_all(op, Cfg, refs, reads, anchors, poa="library", machines=machines, stats=stats_b)
     assert a == b and stats_a == stats_b
     assert stats_a["realigned"] >= 1, stats_a
+
+
+def _merge_sequential(chunks, overlap, op, missing):
+    """chunker.c:50-118 one boundary at a time on the (cropped) predecessor."""
+    pieces = []
+    for cur in chunks:
+        if not pieces:
+            pieces.append(cur)
+            continue
+        prev = pieces[-1]
+        w, ce, cs = RP.remove_overlap(prev, cur, overlap, op)
+        if w > 0:
+            pieces[-1] = prev[:ce]
+            cur = cur[cs:]
+        elif len(cur) == 0:
+            cur = missing
+        elif overlap != 0:
+            pieces.append("NNNNNNNNNN")
+        pieces.append(cur)
+    return "".join(pieces)
+
+
+def test_merge_replay_equals_sequential_merge():
+    """merge_contig_chunks (cuts against uncropped predecessors, re-aligned
+    where a crop reaches into the window) equals the one-boundary-at-a-time
+    rules: short chunks whose crops reach into the next window, a missing
+    chunk, an empty first chunk, overlap 0."""
+    import random
+    rng = random.Random(4)
+    op = O.PolishParams(PARAMS)
+    genome = "".join(rng.choice("ACGT") for _ in range(4000))
+    size, bnd = 400, 40
+    chunks = [genome[max(0, s - bnd):s + size + bnd] for s in range(0, len(genome), size)]
+    chunks = ["".join(c if rng.random() > 0.01 else rng.choice("ACGT") for c in ch) for ch in chunks]
+    chunks[3] = ""
+    chunks[6] = chunks[6][:50]
+    chunks[7] = chunks[7][:60]
+    for ch, ov, miss in ((chunks, 2 * bnd, "NNNNN"), (chunks, 2 * bnd, ""), (["", chunks[1], chunks[2]], 2 * bnd, ""),
+                         (chunks[:4], 0, "")):
+        assert RP.merge_contig_chunks(ch, ov, op, miss, threads=3) == _merge_sequential(ch, ov, op, miss)
