@@ -244,6 +244,12 @@ mp_status mp_batch_fetch_total_log_prob(mp_batch *b, double *total);
  * posterior-record capacity, and the records the last run wrote.  Outputs
  * may be NULL. */
 mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *recordCapacity, int64_t *records);
+/* The last run's record-region re-runs (no counterpart in the reference): a
+ * job that writes more posterior records than its region holds makes the run
+ * size every region exactly and run the batch again.  reruns: how many times
+ * that happened; overflowJobs: jobs that were over; missingRecords: the
+ * records beyond their regions.  Outputs may be NULL. */
+mp_status mp_batch_rerun_stats(const mp_batch *b, int32_t *reruns, int64_t *overflowJobs, int64_t *missingRecords);
 int64_t mp_batch_cells(const mp_batch *b);          /* band cells over all pairs (integer band geometry) */
 int64_t mp_batch_npairs(const mp_batch *b);
 /* Copy results to the host.  counts: npairs*3 list lengths; logProb: npairs

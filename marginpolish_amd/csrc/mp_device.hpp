@@ -141,6 +141,11 @@ struct KParams {
     int32_t *error;      // set when a wavefront gives up waiting for its partner (never in a correct run)
     float *ringWide;
     int32_t ringWideLen, wideC, wideBegin, wideEnd;
+    int32_t wideWaves;   // ring slices of the 3-state wide kernel (one per wave)
+    // 3-state wide kernel: per wave a ring of ringWideCells cells (a row per anti-diagonal holding
+    // its band's cells only) and ringWideLen row offsets
+    int64_t ringWideCells;
+    int32_t *ringWideIdx;
     int32_t jobBegin[kNumClasses], jobEnd[kNumClasses];
     int32_t ringLen;     // anti-diagonals per ring (multiple of kK)
     int32_t tbd;         // traceBackDiagonals: a window's lo is tb of the previous window - (tbd + 1)

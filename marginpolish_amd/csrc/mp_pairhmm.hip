@@ -160,6 +160,11 @@ struct mp_batch {
     DevBuf<int32_t> dError;
     DevBuf<float> dRingWide;
     int32_t wideBegin = 0, wideEnd = 0, wideC = 0, ringWideLen = 0, wideWaves = 0;
+    int64_t ringWideCells = 0;          // 3-state wide kernel: ring cells per wave
+    // the last run's exact-size re-runs, jobs over their record capacity and their missing records
+    int32_t lastReruns = 0;
+    int64_t lastOvfJobs = 0, lastOvfMissing = 0;
+    DevBuf<int32_t> dRingWideIdx;       // 3-state wide kernel: row offsets, ringWideLen per wave
     int32_t modelStates = 3;
     hipGraphExec_t graph = nullptr;  // the run's resets and launches, captured once per parameter block
     hipGraphExec_t graphLists = nullptr;  // the run's list assembly and log-likelihoods
@@ -226,8 +231,12 @@ mp_status launch_wide(mp_batch *b, int grid, const KParams &kp) {
     else if (kp.nStates == 5)
         hipLaunchKernelGGL(pairhmm_wide5_kernel<float>, dim3(grid), block5, wide5_lds_bytes<float>(kp.wideC),
                            b->eng->stream, kp);
-    else
-        hipLaunchKernelGGL(pairhmm_wide_kernel, dim3(grid), dim3(64), kWideLdsBytes, b->eng->stream, kp);
+    else {
+        // grid = job waves: workgroups of up to wide_group_waves(C) waves
+        const int n = std::max(1, std::min(grid, wide_group_waves(kp.wideC)));
+        hipLaunchKernelGGL(pairhmm_wide_kernel, dim3((grid + n - 1) / n), dim3(64 * n), wide_lds_bytes(kp.wideC, n),
+                           b->eng->stream, kp);
+    }
     HIP_TRY(hipGetLastError());
     return MP_OK;
 }
@@ -581,6 +590,27 @@ void choose_crossings(std::vector<DevWin> &w) {
     }
 }
 
+// Narrow band entries: flag the 8-diagonal cells whose neighbour exchange is
+// not a one-lane shift (going forward, going backward).
+void mark_slow(const int32_t *XL, int64_t D, int32_t *bj) {
+    auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
+    for (int64_t d = 0; d <= D; d++) {
+        const bool fastF = d == 0 || (in(XL[d] - XL[d - 1], 0, 1) && (d < 2 || in(XL[d] - XL[d - 2], 0, 2)));
+        const bool fastB = d == D || (in(XL[d] - XL[d + 1], -1, 0) && (d + 2 > D || in(XL[d] - XL[d + 2], -2, 0)));
+        const int64_t c0 = d & ~(int64_t)(kK - 1);
+        if (!fastF)
+            for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowF;
+        if (!fastB)
+            for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowB;
+    }
+}
+
+// First anti-diagonal of the forward sweep of window job k (window start lo):
+// kWarmup before lo, on a cell boundary; the first window starts at 0.
+inline int64_t window_fstart(size_t k, int64_t lo) {
+    return k == 0 ? 0 : (std::max<int64_t>(0, lo - kWarmup) & ~(int64_t)(kK - 1));
+}
+
 // One job's band entries (bj: padEnd entries, the band padded to whole cells),
 // band class, cell count and traceback windows with their crossings and ring
 // need: the planner's per-job step (pairwiseAligner.c:120-226, 715-783).
@@ -588,6 +618,10 @@ void choose_crossings(std::vector<DevWin> &w) {
 struct PlanScratch {
     std::vector<int32_t> L, R, XL;
     std::vector<mp::Window> w;
+    // prefix sums over the anti-diagonals of the band's width beyond the anchors'
+    // own 2 W + 1 cells (at most 16 per anti-diagonal): ex[d + 1] - ex[d]; and of
+    // the band's width: cw[d + 1] - cw[d]
+    std::vector<int64_t> ex, cw;
 };
 bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_band_params &bp, int32_t mode,
               bool five, int32_t *bj, int64_t padEnd, PlanScratch &sc, std::vector<DevWin> &wj) {
@@ -603,10 +637,16 @@ bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_ba
         return false;
     int32_t wmax = 0;
     int64_t cells = 0;
+    sc.ex.resize(D + 2);
+    sc.cw.resize(D + 2);
+    sc.ex[0] = sc.cw[0] = 0;
+    const int32_t nominal = 2 * p->diagonalExpansion + 1;
     for (int64_t d = 0; d <= D; d++) {
         const int32_t wd = (R[d] - L[d]) / 2 + 1;
         wmax = std::max(wmax, wd);
         cells += wd;
+        sc.ex[d + 1] = sc.ex[d] + std::min(16, std::max(0, wd - nominal));
+        sc.cw[d + 1] = sc.cw[d] + wd;
     }
     hj.wmax = wmax;
     hj.cells = D > 0 ? cells : 0;
@@ -636,21 +676,7 @@ bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_ba
         bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
     }
     for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
-    if (narrow && hj.cls > 0) {
-        // cells whose neighbour exchange is not a one-lane shift
-        auto in = [](int64_t v, int64_t a, int64_t b) { return v >= a && v <= b; };
-        for (int64_t d = 0; d <= D; d++) {
-            const bool fastF =
-                d == 0 || (in(XL[d] - XL[d - 1], 0, 1) && (d < 2 || in(XL[d] - XL[d - 2], 0, 2)));
-            const bool fastB =
-                d == D || (in(XL[d] - XL[d + 1], -1, 0) && (d + 2 > D || in(XL[d] - XL[d + 2], -2, 0)));
-            const int64_t c0 = d & ~(int64_t)(kK - 1);
-            if (!fastF)
-                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowF;
-            if (!fastB)
-                for (int64_t q = c0; q < c0 + kK; q++) bj[q] |= kSlowB;
-        }
-    }
+    if (narrow && hj.cls > 0) mark_slow(XL.data(), D, bj);
     w.clear();
     if (D > 0 && mode == MP_MODE_FORWARD_ONLY) w.push_back(mp::Window{(int32_t)D, 0, (int32_t)D});
     else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, w);
@@ -667,6 +693,40 @@ bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_ba
     }
     hj.ringNeed = (int32_t)((need + kK - 1) / kK * kK);
     return true;
+}
+
+// A wide sub-problem that runs as window jobs: each window's own class from
+// its band over [fstart - 2, tb + 2] (rows of 32 or 64 lanes, or wide; the
+// narrow kernels read the band two anti-diagonals either side of their sweeps)
+// and width maximum; when some window fits a narrow row, the band again in the
+// narrow encoding with the general-exchange flags (widths above 64 clamped: no
+// narrow window's cells reach them), appended to `alt`.  Uses plan_job's
+// L, R, XL.  Returns the copy's offset in `alt`, or -1 without one.
+int64_t window_classes(const HostJob &hj, const PlanScratch &sc, const std::vector<DevWin> &wj, int64_t padEnd,
+                       std::vector<int8_t> &wcls, std::vector<int32_t> &wwm, std::vector<int32_t> &alt) {
+    const int64_t D = hj.lX + hj.lY;
+    bool mixed = false;
+    for (size_t k = 0; k < wj.size(); k++) {
+        const int64_t a = std::max<int64_t>(0, window_fstart(k, wj[k].lo) - 2);
+        const int64_t e = std::min<int64_t>(D, (int64_t)wj[k].tb + 2);
+        int32_t m = 1;
+        for (int64_t d = a; d <= e; d++) m = std::max(m, (sc.R[d] - sc.L[d]) / 2 + 1);
+        const int8_t c = m <= 32 ? 1 : (m <= 64 ? 2 : 3);
+        wcls.push_back(c);
+        wwm.push_back(m);
+        mixed |= c < 3;
+    }
+    if (!mixed) return -1;
+    const int64_t off = (int64_t)alt.size();
+    alt.resize(alt.size() + (size_t)padEnd);
+    int32_t *bj = alt.data() + off;
+    for (int64_t d = 0; d <= D; d++) {
+        const int32_t wd = std::min((sc.R[d] - sc.L[d]) / 2 + 1, 64);
+        bj[d] = (int32_t)((uint32_t)sc.XL[d] << kBandShift) | (wd - 1);
+    }
+    for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
+    mark_slow(sc.XL.data(), D, bj);
+    return off;
 }
 
 
@@ -699,7 +759,7 @@ extern "C" mp_status mp_engine_create(int32_t device, mp_engine **out) {
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_seq_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 kSeqLdsBytesExp));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kWideLdsBytes));
+                                kWideLdsLimit));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel<float>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, wide5_lds_bytes<float>(kWide5MaxC32)));
     HIP_TRY(hipFuncSetAttribute((const void *)pairhmm_wide5_kernel<double>,
@@ -957,14 +1017,22 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // every entry of a job's region is written by its band pass below (anti-diagonals 0..D, then padding)
     std::unique_ptr<int32_t[]> bandBuf(new int32_t[(size_t)bandOff[nj]]);
     int32_t *band = bandBuf.get();
-    // each job's windows live in its thread's pool: (thread, offset, count) per job
+    // each job's windows live in its thread's pool: (thread, offset, count) per job,
+    // with each window's class and width maximum in parallel pools; a wide job whose
+    // windows run as window jobs may keep a narrow copy of its band in the thread's
+    // band pool (alt: offset there, or -1)
     struct WinRef {
         int32_t t, n;
         int64_t off;
+        int64_t alt = -1;
     };
     std::vector<WinRef> winRef(nj);
     const int nthW = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     std::vector<std::vector<DevWin>> winPool(nthW);
+    std::vector<std::vector<int8_t>> clsPool(nthW);
+    std::vector<std::vector<int32_t>> wmPool(nthW), altPool(nthW), exPool(nthW);
+    std::vector<std::vector<int64_t>> rcPool(nthW);  // wide kernel ring cells of each window
+    const bool winJobs = split && (flags & MP_BATCH_EXACT_CHAIN) == 0 && !five;
     std::atomic<int> err{0};
     {
         const int nth = nthW;
@@ -991,9 +1059,35 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     }
                     winRef[j] = WinRef{t, (int32_t)wj.size(), (int64_t)pool.size()};
                     pool.insert(pool.end(), wj.begin(), wj.end());
+                    // the wide kernel's ring holds a window's forward values of [lo, tb], one row
+                    // per anti-diagonal, plus one row lost where the ring wraps
+                    for (const DevWin &x : wj)
+                        rcPool[t].push_back(sc.cw[x.tb + 1] - sc.cw[x.lo] + hj.wmax);
+                    // excess band width over each window's posterior anti-diagonals (lo, from]
+                    for (const DevWin &x : wj)
+                        exPool[t].push_back(five ? 0
+                                                 : (int32_t)std::min<int64_t>(
+                                                       std::max<int64_t>(0, sc.ex[x.from + 1] - sc.ex[x.lo + 1]),
+                                                       INT32_MAX / 8));
+                    if (winJobs && hj.cls == 3 && wj.size() > 1) {
+                        winRef[j].alt = window_classes(hj, sc, wj, bandOff[j + 1] - bandOff[j], clsPool[t], wmPool[t],
+                                                       altPool[t]);
+                    } else {
+                        clsPool[t].insert(clsPool[t].end(), wj.size(), (int8_t)hj.cls);
+                        wmPool[t].insert(wmPool[t].end(), wj.size(), hj.wmax);
+                    }
                 }
             });
         for (auto &t : th) t.join();
+    }
+    // the narrow band copies follow the jobs' bands in the device band array
+    std::vector<int64_t> altBase(nthW + 1, bandOff[nj]);
+    for (int t = 0; t < nthW; t++) altBase[t + 1] = altBase[t] + (int64_t)altPool[t].size();
+    const int64_t bandTotal = altBase[nthW];
+    if (bandTotal >= ((int64_t)1 << 30)) {
+        delete b;
+        mp::set_error("batch too large: more than 2^30 anti-diagonals (split it into several batches)");
+        return MP_EUNSUPPORTED;
     }
     if (err) {
         delete b;
@@ -1011,17 +1105,17 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
 
-    // ---- 2b. window jobs (kWarmup): in the posterior modes a narrow sub-problem with
-    // several traceback windows runs each window as a job of its own ----
+    // ---- 2b. window jobs (kWarmup): in the posterior modes a three-state sub-problem
+    // with several traceback windows runs each window as a job of its own (narrow
+    // ones on the split kernel, wide ones on the wide-band kernel) ----
     int64_t subCells = 0;
     for (int64_t j = 0; j < nj; j++) subCells += jobs[j].cells;
     std::vector<int64_t> subFirstX(nj + 1, 0);
     {
-        const bool winJobs = split && (flags & MP_BATCH_EXACT_CHAIN) == 0;
         int64_t nx = 0;
         for (int64_t j = 0; j < nj; j++) {
             subFirstX[j] = nx;
-            nx += (winJobs && jobs[j].cls < 3 && winRef[j].n > 1) ? winRef[j].n : 1;
+            nx += (winJobs && winRef[j].n > 1) ? winRef[j].n : 1;
         }
         subFirstX[nj] = nx;
         if (nx >= ((int64_t)1 << 30)) {
@@ -1041,22 +1135,33 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                 }
                 const WinRef &wr = winRef[j];
                 DevWin *wv = winPool[wr.t].data() + wr.off;
-                const int32_t *bj = band + jobs[j].bandOff;
                 for (int64_t k = 0; k < n; k++) {
                     DevWin &w = wv[k];
                     HostJob h = jobs[j];
-                    const int64_t fs = k == 0 ? 0 : (std::max<int64_t>(0, w.lo - kWarmup) & ~(int64_t)(kK - 1));
+                    // the window's own class: a narrow window of a wide job reads the narrow copy
+                    h.cls = clsPool[wr.t][wr.off + k];
+                    h.wmax = wmPool[wr.t][wr.off + k];
+                    const bool wide = h.cls == 3;
+                    if (!wide && jobs[j].cls == 3) h.bandOff = altBase[wr.t] + wr.alt;
+                    const int32_t *bj = h.bandOff < bandOff[nj] ? band + h.bandOff
+                                                                 : altPool[wr.t].data() + (h.bandOff - altBase[wr.t]);
+                    const int64_t fs = window_fstart((size_t)k, w.lo);
                     h.fstart = (int32_t)fs;
                     if (fs > 0) h.flags |= kJobRestart;
-                    // crossing: the forward row (from fs) and the backward row (from tb) meet
-                    // halfway, at 7 (mod 8), inside [lo, tb - 1]
-                    int64_t c = (fs + w.tb - 1) / 2;
-                    c = ((c + 4) & ~(int64_t)(kK - 1)) - 1;
-                    w.cx = (int32_t)std::min<int64_t>(std::max<int64_t>(c, w.lo), w.tb - 1);
+                    if (!wide) {
+                        // crossing: the forward row (from fs) and the backward row (from tb) meet
+                        // halfway, at 7 (mod 8), inside [lo, tb - 1]
+                        int64_t c = (fs + w.tb - 1) / 2;
+                        c = ((c + 4) & ~(int64_t)(kK - 1)) - 1;
+                        w.cx = (int32_t)std::min<int64_t>(std::max<int64_t>(c, w.lo), w.tb - 1);
+                    }
                     int64_t cells = 0;  // the job's work, for the device order
-                    for (int64_t d = fs; d <= w.tb; d++) cells += band_wm1_narrow(bj[d]) + 1;
+                    for (int64_t d = fs; d <= w.tb; d++)
+                        cells += (wide ? band_wm1_wide(bj[d]) : band_wm1_narrow(bj[d])) + 1;
                     h.cells = cells;
-                    h.ringNeed = (int32_t)((w.tb - w.lo + 4 * kK - 1) / kK * kK);
+                    // live ring entries: narrow, forward and backward values of (lo, tb]; wide,
+                    // forward values of [lo, tb] (the backward runs after the forward)
+                    h.ringNeed = (int32_t)((w.tb - w.lo + (wide ? 2 + kK - 1 : 4 * kK - 1)) / kK * kK);
                     xj[x0 + k] = h;
                     xw[x0 + k] = WinRef{wr.t, 1, wr.off + k};
                 }
@@ -1100,7 +1205,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // record capacities (1.5 per posterior anti-diagonal of each side; exact re-run on overflow),
     // gathered per device slot so the offset pass below reads memory in order
     struct Slot {
-        int64_t cells, out;
+        int64_t cells, out, ringCells;
         int32_t nWin, cls, ringNeed, wmax;
     };
     std::vector<Slot> slot((size_t)nq);
@@ -1109,13 +1214,17 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         const WinRef &wr = winRef[ord[q]];
         const DevWin *wv = winPool[wr.t].data() + wr.off;
         int64_t nF = 0, nB = 0;
+        // where a three-state band is wider than the anchors' own 2 W + 1 cells the anchors are
+        // sparse and the posteriors spread over several alignments: a window's excess width (at
+        // most 16 per anti-diagonal) is added to each side that holds its records
         for (int32_t k = 0; k < wr.n; k++) {
             const DevWin &w = wv[k];
+            const int64_t ex = exPool[wr.t][wr.off + k];
             if (hj.cls < 3) {
-                nF += std::max(0, w.from - w.cx);
-                nB += std::max(0, std::min(w.cx, w.from) - w.lo);
+                nF += std::max(0, w.from - w.cx) + ex;
+                nB += std::max(0, std::min(w.cx, w.from) - w.lo) + ex;
             } else {
-                nB += w.from - w.lo;
+                nB += w.from - w.lo + ex;
             }
         }
         // (measured on 100 kb chunks: 0.9 records per posterior anti-diagonal; a job that writes
@@ -1123,7 +1232,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         hj.cap[0] = post && hj.cls < 3 ? (int32_t)std::min<int64_t>(nF + nF / 2 + 64, INT32_MAX / 4) : 0;
         hj.cap[1] = post ? (int32_t)std::min<int64_t>(nB + nB / 2 + 64, INT32_MAX / 4) : 0;
         hj.nWin = wr.n;
-        slot[q] = Slot{hj.cells, (int64_t)hj.cap[0] + hj.cap[1], wr.n, hj.cls, hj.ringNeed, hj.wmax};
+        int64_t rc = 0;
+        if (hj.cls == 3)
+            for (int32_t k = 0; k < wr.n; k++) rc = std::max(rc, rcPool[wr.t][wr.off + k]);
+        slot[q] = Slot{hj.cells, (int64_t)hj.cap[0] + hj.cap[1], rc, wr.n, hj.cls, hj.ringNeed, hj.wmax};
     });
     // offsets in device order, batch-wide maxima, class ranges
     std::vector<int64_t> slotOut((size_t)nq), slotWin((size_t)nq);
@@ -1140,7 +1252,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         if (mode != MP_MODE_FORWARD_ONLY && sl.cls < 3) ringLen = std::max(ringLen, sl.ringNeed);
         if (sl.cls == 3) {
             b->wideC = std::max(b->wideC, (sl.wmax + 63) / 64);
-            if (mode != MP_MODE_FORWARD_ONLY) b->ringWideLen = std::max(b->ringWideLen, sl.ringNeed);
+            if (mode != MP_MODE_FORWARD_ONLY) {
+                b->ringWideLen = std::max(b->ringWideLen, sl.ringNeed);
+                b->ringWideCells = std::max(b->ringWideCells, sl.ringCells);
+            }
             b->wideBegin = std::min<int32_t>(b->wideBegin, (int32_t)q);
             b->wideEnd = (int32_t)q + 1;
         } else {
@@ -1203,7 +1318,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
     if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, nsym, s)) ||
-        (st = b->dBand.upload(band, (size_t)bandOff[nj], s)) ||
+        (st = b->dBand.alloc((size_t)bandTotal)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
@@ -1216,6 +1331,21 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
         delete b;
         return st;
+    }
+    {
+        // the jobs' bands, then the narrow band copies of the threads' pools
+        hipError_t he = bandOff[nj] > 0 ? hipMemcpyAsync(b->dBand.p, band, sizeof(int32_t) * (size_t)bandOff[nj],
+                                                         hipMemcpyHostToDevice, s)
+                                        : hipSuccess;
+        for (int t = 0; t < nthW && he == hipSuccess; t++)
+            if (!altPool[t].empty())
+                he = hipMemcpyAsync(b->dBand.p + altBase[t], altPool[t].data(), sizeof(int32_t) * altPool[t].size(),
+                                    hipMemcpyHostToDevice, s);
+        if (he != hipSuccess) {
+            delete b;
+            mp::set_error(std::string("band upload: ") + hipGetErrorString(he));
+            return MP_EHIP;
+        }
     }
     // Rings: one slice per resident wavefront.
     const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
@@ -1262,12 +1392,23 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         // or six values (five-state kernel: values and the cell's exponent, FP32 or FP64); the resident
         // waves' rings take at most ~70% of the free device memory
         const size_t perCell = five ? 6 * (fp64 ? sizeof(double) : sizeof(float)) : sizeof(float4);
-        const size_t perWave = (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * perCell;
+        const size_t perWave = five ? (size_t)std::max(b->ringWideLen, 1) * 64 * b->wideC * perCell
+                                    : (size_t)std::max<int64_t>(b->ringWideCells, 1) * perCell +
+                                          (size_t)std::max(b->ringWideLen, 1) * sizeof(int32_t);
         size_t freeB = 0, totalB = 0;
         HIP_TRY(hipMemGetInfo(&freeB, &totalB));
         const size_t cap = std::max<size_t>(1, freeB / 10 * 7 / perWave);
-        b->wideWaves = (int32_t)std::min<size_t>({(size_t)(b->wideEnd - b->wideBegin), (size_t)e->cus * 4, cap});
-        if ((st = b->dRingWide.alloc((size_t)b->wideWaves * perWave / sizeof(float)))) {
+        // resident waves per CU: five-state, one workgroup of up to 8 waves per job (4 jobs); 3-state, one
+        // job per wave in workgroups of wide_group_waves(C), as many workgroups as the LDS holds
+        size_t perCU = 4;
+        if (!five) {
+            const int n = wide_group_waves(std::max(b->wideC, 1));
+            perCU = (size_t)n * (size_t)std::max(1, kWideLdsLimit / wide_lds_bytes(std::max(b->wideC, 1), n));
+        }
+        b->wideWaves = (int32_t)std::min<size_t>({(size_t)(b->wideEnd - b->wideBegin), (size_t)e->cus * perCU, cap});
+        const size_t ringBytes = five ? perWave : (size_t)std::max<int64_t>(b->ringWideCells, 1) * perCell;
+        if ((st = b->dRingWide.alloc((size_t)b->wideWaves * ringBytes / sizeof(float))) ||
+            (!five && (st = b->dRingWideIdx.alloc((size_t)b->wideWaves * std::max(b->ringWideLen, 1))))) {
             delete b;
             return st;
         }
@@ -1286,7 +1427,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     kp.ringExp = b->dRingExp.p;
     kp.ringWide = b->dRingWide.p;
     kp.ringWideLen = std::max(b->ringWideLen, 1);
+    kp.ringWideCells = std::max<int64_t>(b->ringWideCells, 1);
+    kp.ringWideIdx = b->dRingWideIdx.p;
     kp.wideC = std::max(b->wideC, 1);
+    kp.wideWaves = b->wideWaves;
     kp.wideBegin = b->wideBegin;
     kp.wideEnd = b->wideEnd;
     kp.tables = b->dTables.p;
@@ -1528,6 +1672,7 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         std::vector<int32_t> c((size_t)nj * 3);
         HIP_TRY(hipMemcpy(c.data(), b->dCounts.p, sizeof(int32_t) * 3 * nj, hipMemcpyDeviceToHost));
         int64_t off = 0;
+        b->lastReruns++;
         for (int32_t j = 0; j < nj; j++)
             for (int l = 0; l < 3; l++) {
                 mpk::RefJob &R = b->refJobs[j];
@@ -1550,11 +1695,18 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         std::vector<DevJob> dj(nj);
         HIP_TRY(hipMemcpy(dj.data(), b->dJobs.p, sizeof(DevJob) * nj, hipMemcpyDeviceToHost));
         int64_t off = 0;
+        b->lastReruns++;
         for (int32_t j = 0; j < nj; j++) {
+            bool over = false;
             for (int l = 0; l < 2; l++) {
+                if (b->hCounts[2 * j + l] > b->jobs[j].cap[l]) {
+                    over = true;
+                    b->lastOvfMissing += b->hCounts[2 * j + l] - b->jobs[j].cap[l];
+                }
                 b->jobs[j].cap[l] = std::max(b->jobs[j].cap[l], b->hCounts[2 * j + l]);
                 dj[j].cap[l] = b->jobs[j].cap[l];
             }
+            b->lastOvfJobs += over;
             b->jobs[j].outOff = off;
             dj[j].outOff = off;
             off += (int64_t)dj[j].cap[0] + dj[j].cap[1];
@@ -1606,6 +1758,18 @@ extern "C" mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *r
     if (jobs) *jobs = (int64_t)b->jobs.size();
     if (recordCapacity) *recordCapacity = b->outTotal;
     if (records) *records = b->records;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_rerun_stats(const mp_batch *b, int32_t *reruns, int64_t *overflowJobs,
+                                           int64_t *missingRecords) {
+    if (!b) {
+        mp::set_error("mp_batch_rerun_stats: null batch");
+        return MP_EINVAL;
+    }
+    if (reruns) *reruns = b->lastReruns;
+    if (overflowJobs) *overflowJobs = b->lastOvfJobs;
+    if (missingRecords) *missingRecords = b->lastOvfMissing;
     return MP_OK;
 }
 
@@ -1697,6 +1861,8 @@ extern "C" mp_status mp_batch_run(mp_batch *b) {
         mp::set_error("mp_batch_run: null batch");
         return MP_EINVAL;
     }
+    b->lastReruns = 0;
+    b->lastOvfJobs = b->lastOvfMissing = 0;
     return batch_run_impl(b, false);
 }
 
@@ -1705,6 +1871,8 @@ extern "C" mp_status mp_batch_run_plain(mp_batch *b) {
         mp::set_error("mp_batch_run_plain: null batch");
         return MP_EINVAL;
     }
+    b->lastReruns = 0;
+    b->lastOvfJobs = b->lastOvfMissing = 0;
     return batch_run_impl(b, true);
 }
 

@@ -332,6 +332,13 @@ class Batch:
         check(lib().mp_batch_stats(self._h, C.byref(j), C.byref(c), C.byref(r)))
         return j.value, c.value, r.value
 
+    def rerun_stats(self):
+        """The last run's record-region re-runs: (re-runs, jobs over their
+        capacity, records beyond their regions)."""
+        n, j, m = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().mp_batch_rerun_stats(self._h, C.byref(n), C.byref(j), C.byref(m)))
+        return n.value, j.value, m.value
+
     def total_log_prob(self):
         """Sum of the pairs' log-likelihoods, accumulated in FP64 on the device."""
         v = C.c_double()

@@ -558,28 +558,31 @@ def test_n_symbols_parity(ph, params, rle):
             assert err <= TOL + 1e-7, (st, err)
 
 
-def test_window_jobs_equal_exact_chain(ph, params):
+@pytest.mark.parametrize("W", [10, 48], ids=["narrow", "wide"])
+def test_window_jobs_equal_exact_chain(ph, params, W):
     """Window jobs (each traceback window its own job, the forward restarted
     512 anti-diagonals early from a flat vector) against the reference's
     single forward chain (MP_BATCH_EXACT_CHAIN) and O_exact: 6 pairs of ~8 k
     anti-diagonals with a window every 1000 (restarts from the third window
-    on), both strands, ragged ends.  The two runs round differently (their
-    FP32 scale trajectories differ), so the bar is the FP32 level: posteriors
-    within 2e-5 of each other (each within 1e-4 of O_exact), log-likelihoods
-    (chained in FP64 across the windows) within 1e-3 nats; the device's FP64
-    batch total equals the sum of the pairs'."""
+    on), both strands, ragged ends; W = 10 runs on the split kernel, W = 48
+    (bands of 97 cells) on the wide-band kernel.  The two runs round
+    differently (their FP32 scale trajectories differ), so the bar is the
+    FP32 level: posteriors within 2e-5 of each other (each within 1e-4 of
+    O_exact), log-likelihoods (chained in FP64 across the windows) within
+    1e-3 nats; the device's FP64 batch total equals the sum of the pairs'."""
     from marginpolish_amd import crop, sim
     pp, op = params
-    w = sim.config0(n_reads=6, ref_len=6000, seed=23, W=10, strands=(1, 0))
+    w = sim.config0(n_reads=6, ref_len=6000, seed=23, W=W, strands=(1, 0))
     jobs = []
     for i, pr in enumerate(w.pairs):
         sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
         jobs.append((sx, sy, an, pr.strand, bool(i & 1), bool(i & 2)))
-    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=10,
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=W,
                     splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
-    say("case: window jobs vs exact chain, 6 pairs")
+    say("case: window jobs vs exact chain, 6 pairs, W=%d" % W)
     bw = ph.Batch(pp.machines, jobs, _bp(ph, opar)).run()
     be = ph.Batch(pp.machines, jobs, _bp(ph, opar), exact_chain=True).run()
+    assert bw.stats()[0] > 12 and be.stats()[0] == 6  # one job per window vs one per pair
     gw, ge = bw.fetch(), be.fetch()
     worst, drift = 0.0, 0.0
     for a, c in zip(gw, ge):
@@ -593,3 +596,57 @@ def test_window_jobs_equal_exact_chain(ph, params):
     for b, g in ((bw, gw), (be, ge)):
         tot = b.total_log_prob()
         assert abs(tot - math.fsum(r[3] for r in g)) <= 1e-9 * abs(tot)
+
+
+def _band_widths(an, lX, lY, e):
+    """Cells per anti-diagonal of the library's band (mp_band_construct, dynamic expansion)."""
+    import ctypes as C
+    from marginpolish_amd import _lib
+    a = np.ascontiguousarray(np.asarray(an, np.int64).reshape(-1, 3))
+    L = np.zeros(lX + lY + 1, np.int32)
+    R = np.zeros(lX + lY + 1, np.int32)
+    _lib.check(_lib.lib().mp_band_construct(a.ctypes.data_as(C.POINTER(C.c_int64)), len(a), lX, lY, e, 1,
+                                            L.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            R.ctypes.data_as(C.POINTER(C.c_int32))))
+    return (R - L) // 2 + 1
+
+
+def test_wide_pairs_with_narrow_windows(ph, params):
+    """Pairs whose band is wider than 64 cells only where the anchors leave a
+    gap (100-300 positions; the dynamic expansion spans it, as it spans the
+    low-confidence stretches of anchors taken from a POA graph), with a
+    traceback window every 1000 anti-diagonals: each window job takes its
+    own class, so the windows that fit 64 cells run on the split kernel
+    (through a narrow copy of the band) and the others on the wide-band
+    kernel.  Against the single-chain run (MP_BATCH_EXACT_CHAIN, the whole
+    pair on the wide-band kernel) and O_exact, at the bars of the window-job
+    test above."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(n_reads=6, ref_len=8000, seed=31, W=10, strands=(1, 0))
+    jobs = []
+    for i, pr in enumerate(w.pairs):
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        at, gap = 1500 + 700 * i, 100 + 40 * i
+        an = an[(an[:, 0] < at) | (an[:, 0] >= at + gap)]
+        wd = _band_widths(an, len(sx), len(sy), 10)
+        assert wd.max() > 64 and np.median(wd) <= 32, (i, wd.max(), np.median(wd))
+        jobs.append((sx, sy, an, pr.strand, bool(i & 1), bool(i & 2)))
+    opar = O.Params(threshold=0.01, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    say("case: wide pairs with narrow windows, 6 pairs")
+    bw = ph.Batch(pp.machines, jobs, _bp(ph, opar)).run()
+    be = ph.Batch(pp.machines, jobs, _bp(ph, opar), exact_chain=True).run()
+    assert bw.stats()[0] > 12 and be.stats()[0] == 6
+    gw, ge = bw.fetch(), be.fetch()
+    worst, drift = 0.0, 0.0
+    for a, c in zip(gw, ge):
+        for l in range(3):
+            err, _, only = compare(as_rows(a[l]), as_rows(c[l]), opar.threshold, 2e-5)
+            worst = max(worst, err)
+        drift = max(drift, abs(a[3] - c[3]))
+    say("wide pairs, window jobs vs exact chain: max posterior diff %.3g, max log-likelihood diff %.3g"
+        % (worst, drift))
+    assert worst <= 2e-5 and drift <= 1e-3
+    _check(gw, _oracle_jobs(op, jobs, opar), opar.threshold)
+    assert abs(bw.total_log_prob() - math.fsum(r[3] for r in gw)) <= 1e-9 * abs(bw.total_log_prob())
