@@ -250,6 +250,12 @@ mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *recordCapaci
  * that happened; overflowJobs: jobs that were over; missingRecords: the
  * records beyond their regions.  Outputs may be NULL. */
 mp_status mp_batch_rerun_stats(const mp_batch *b, int32_t *reruns, int64_t *overflowJobs, int64_t *missingRecords);
+/* Host time (ms) of mp_batch_create's phases (no counterpart in the
+ * reference): [0] checks, reference crop and split points, [1] symbol
+ * encoding and jobs, [2] bands, windows and classes, [3] window jobs,
+ * [4] device order and record regions, [5] slots, [6] upload and device
+ * buffers, [7] list buffers.  Writes min(n, 8) entries. */
+mp_status mp_batch_prep_ms(const mp_batch *b, double *ms, int32_t n);
 int64_t mp_batch_cells(const mp_batch *b);          /* band cells over all pairs (integer band geometry) */
 int64_t mp_batch_npairs(const mp_batch *b);
 /* Copy results to the host.  counts: npairs*3 list lengths; logProb: npairs

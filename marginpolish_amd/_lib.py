@@ -88,7 +88,7 @@ EXPORTS = [
     "mp_bubble_allele_supports", "mp_remove_overlaps", "mp_merge_chunks", "mp_poa_estimate_repeat_counts",
     "mp_poa_anchor_alignments", "mp_poa_consensus", "mp_poa_scores", "mp_poas_estimate_repeat_counts",
     "mp_batch_list_ms", "mp_batch_fetch_total_log_prob", "mp_batch_stats", "mp_batch_rerun_stats",
-    "mp_pairhmm_forward_log_prob_noalloc",
+    "mp_batch_prep_ms", "mp_pairhmm_forward_log_prob_noalloc",
 ]
 
 _lib = None
@@ -127,6 +127,7 @@ def lib():
         "mp_batch_fetch_total_log_prob": (C.c_int, [vp, P(dbl)]),
         "mp_batch_stats": (C.c_int, [vp, P(i64), P(i64), P(i64)]),
         "mp_batch_rerun_stats": (C.c_int, [vp, P(i32), P(i64), P(i64)]),
+        "mp_batch_prep_ms": (C.c_int, [vp, P(dbl), i32]),
         "mp_batch_cells": (i64, [vp]),
         "mp_batch_npairs": (i64, [vp]),
         "mp_batch_fetch_counts": (C.c_int, [vp, P(i64), P(dbl)]),

@@ -136,6 +136,7 @@ struct KParams {
     int32_t *ringExp;    // per 8-diagonal cell exponents of those values
     const float *tables;
     int32_t *queue;      // kNumClasses + 1 counters (the last for wide jobs)
+    const int32_t *jobMap;  // claimed positions -> job indices (a partial re-run), or null: position = job
     double *expect;      // per job: 9 expected transition counts + likelihood
     double *emitCount;   // per batch: expected emission counts by emission-table entry (kTabFloats)
     int32_t *error;      // set when a wavefront gives up waiting for its partner (never in a correct run)

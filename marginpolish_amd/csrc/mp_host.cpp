@@ -301,14 +301,19 @@ static inline bool box_diag(int64_t xay, int64_t xL, int64_t yL, int64_t xU, int
 
 bool build_band(const int64_t *a, int64_t n, int64_t lX, int64_t lY, int64_t expansion, bool dynamic, int32_t *L,
                 int32_t *R) {
+    return build_band(a, n, 0, 0, lX, lY, expansion, dynamic, L, R);
+}
+
+bool build_band(const int64_t *a, int64_t n, int64_t xo, int64_t yo, int64_t lX, int64_t lY, int64_t expansion,
+                bool dynamic, int32_t *L, int32_t *R) {
     const int64_t D = lX + lY;
     if (!box_diag(0, 0, 0, 0, 0, L[0], R[0])) return false;
     int64_t px = 0, py = 0, e = dynamic ? 0 : expansion, xay = 1;
     for (int64_t i = 0; i <= n && xay <= D; i++) {
         int64_t nx = lX, ny = lY;
         if (i < n) {
-            nx = a[3 * i] + 1;
-            ny = a[3 * i + 1] + 1;
+            nx = a[3 * i] - xo + 1;
+            ny = a[3 * i + 1] - yo + 1;
             if (dynamic) e = a[3 * i + 2];
         }
         const int64_t h = e / 2;
@@ -348,6 +353,11 @@ void build_windows(const int32_t *L, const int32_t *R, int64_t D, const mp_band_
 
 void split_points(const int64_t *a, int64_t n, int64_t lX, int64_t lY, int64_t maxArea, bool raggedL, bool raggedR,
                   std::vector<int64_t> &out) {
+    split_points(a, n, 0, lX, lY, maxArea, raggedL, raggedR, out);
+}
+
+void split_points(const int64_t *a, int64_t n, int64_t xo, int64_t lX, int64_t lY, int64_t maxArea, bool raggedL,
+                  bool raggedR, std::vector<int64_t> &out) {
     out.clear();
     const int64_t maxLen = (int64_t)std::sqrt((double)maxArea);
     int64_t x1 = 0, y1 = 0, x2 = 0, y2 = 0;
@@ -366,8 +376,8 @@ void split_points(const int64_t *a, int64_t n, int64_t lX, int64_t lY, int64_t m
         return true;
     };
     for (int64_t i = 0; i < n; i++) {
-        gap(a[3 * i], a[3 * i + 1], raggedL && i == 0);
-        x2 = a[3 * i] + 1;
+        gap(a[3 * i] - xo, a[3 * i + 1], raggedL && i == 0);
+        x2 = a[3 * i] - xo + 1;
         y2 = a[3 * i + 1] + 1;
     }
     if (!gap(lX, lY, raggedL && n == 0) || !raggedR) {

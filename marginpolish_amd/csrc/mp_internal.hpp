@@ -59,12 +59,19 @@ struct Window {
 // Returns false when a diagonal is invalid (bad anchors).
 bool build_band(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY, int64_t expansion, bool dynamic,
                 int32_t *L, int32_t *R);
+// The same with anchor coordinates read as (x - xo, y - yo): a sub-problem's
+// anchors taken in place from the pair's.
+bool build_band(const int64_t *anchors, int64_t n, int64_t xo, int64_t yo, int64_t lX, int64_t lY,
+                int64_t expansion, bool dynamic, int32_t *L, int32_t *R);
 // Traceback windows of getPosteriorProbsWithBanding (pairwiseAligner.c:715-783).
 void build_windows(const int32_t *L, const int32_t *R, int64_t D, const mp_band_params &p,
                    std::vector<Window> &out);
 // getSplitPoints (pairwiseAligner.c:899-926): (x1, y1, x2, y2) tuples.
 void split_points(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY, int64_t maxArea, bool raggedL,
                   bool raggedR, std::vector<int64_t> &out);
+// The same with anchor x read as x - xo (a cropped reference).
+void split_points(const int64_t *anchors, int64_t n, int64_t xo, int64_t lX, int64_t lY, int64_t maxArea,
+                  bool raggedL, bool raggedR, std::vector<int64_t> &out);
 // Anchor validity: strictly increasing, inside the sequences, even expansion.
 bool anchors_valid(const int64_t *anchors, int64_t n, int64_t lX, int64_t lY);
 

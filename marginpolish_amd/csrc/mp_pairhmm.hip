@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -69,6 +71,13 @@ struct HostJob {
     int32_t fstart = 0;  // forward start anti-diagonal (window jobs)
 };
 
+// A sub-problem's anchors, in place in the caller's array: n triples from a,
+// coordinates read as (x - xo, y - yo).
+struct JobAnc {
+    const int64_t *a = nullptr;
+    int64_t n = 0, xo = 0, yo = 0;
+};
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -120,6 +129,12 @@ struct mp_engine {
     // pinned host copies of a batch's three lists for the POA augmentation (grown on demand, reused)
     void *listHost[3] = {nullptr, nullptr, nullptr};
     size_t listHostBytes[3] = {0, 0, 0};
+    // pinned staging of a batch's symbol pool [0] and band entries [1]: the planner writes them
+    // there and they go to the device by DMA (grown on demand with headroom, reused); batch
+    // creation holds createMu while it uses them
+    void *upHost[2] = {nullptr, nullptr};
+    size_t upHostBytes[2] = {0, 0};
+    std::mutex createMu;
     // autotuner results: batch shape key -> (narrow workgroup threads, five-state waves per workgroup)
     std::map<uint64_t, std::pair<int32_t, int32_t>> tuned;
 };
@@ -147,6 +162,7 @@ struct mp_batch {
     DevBuf<int32_t> dBand;
     DevBuf<DevWin> dWin;
     DevBuf<int4> dOut;
+    DevBuf<int32_t> dJobMap;            // a partial re-run's job map
     DevBuf<int32_t> dCounts;
     DevBuf<int32_t> dWinEnd;
     DevBuf<int2> dLogTotal;
@@ -164,6 +180,10 @@ struct mp_batch {
     // the last run's exact-size re-runs, jobs over their record capacity and their missing records
     int32_t lastReruns = 0;
     int64_t lastOvfJobs = 0, lastOvfMissing = 0;
+    // host time of mp_batch_create's phases (ms): checks and split points, symbols and jobs,
+    // bands and windows, window jobs, device order and record regions, slots, upload and
+    // device buffers, list buffers
+    double prepMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DevBuf<int32_t> dRingWideIdx;       // 3-state wide kernel: row offsets, ringWideLen per wave
     int32_t modelStates = 3;
     hipGraphExec_t graph = nullptr;  // the run's resets and launches, captured once per parameter block
@@ -302,7 +322,7 @@ mp_status alloc_lists(mp_batch *b) {
 // sized by the largest job, tuple regions per job and list.
 mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *pairs, const mp_band_params *p,
                                  const std::vector<int64_t> &symOff, const std::vector<int64_t> &pairFirstSub,
-                                 std::vector<HostJob> &jobs, std::vector<std::vector<int64_t>> &jobAnchors) {
+                                 std::vector<HostJob> &jobs, const std::vector<JobAnc> &jobAnchors) {
     mp_engine *e = b->eng;
     const int64_t np = b->npairs, nj = (int64_t)jobs.size();
     b->ref = true;
@@ -333,8 +353,8 @@ mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *
                     const int64_t D = hj.lX + hj.lY;
                     L.resize(D + 1);
                     R.resize(D + 1);
-                    const std::vector<int64_t> &a = jobAnchors[j];
-                    if (!mp::build_band(a.data(), (int64_t)a.size() / 3, hj.lX, hj.lY, p->diagonalExpansion,
+                    const JobAnc &a = jobAnchors[j];
+                    if (!mp::build_band(a.a, a.n, a.xo, a.yo, hj.lX, hj.lY, p->diagonalExpansion,
                                         p->dynamicAnchorExpansion != 0, L.data(), R.data())) {
                         bad = 1;
                         continue;
@@ -618,13 +638,13 @@ inline int64_t window_fstart(size_t k, int64_t lo) {
 struct PlanScratch {
     std::vector<int32_t> L, R, XL;
     std::vector<mp::Window> w;
-    // prefix sums over the anti-diagonals of the band's width beyond the anchors'
-    // own 2 W + 1 cells (at most 16 per anti-diagonal): ex[d + 1] - ex[d]; and of
-    // the band's width: cw[d + 1] - cw[d]
-    std::vector<int64_t> ex, cw;
+    // per window: the band's width beyond the anchors' own 2 W + 1 cells (at most
+    // 16 per anti-diagonal) over its posterior anti-diagonals (lo, from], and the
+    // band cells of [lo, tb]
+    std::vector<int64_t> wex, wcells;
 };
-bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_band_params &bp, int32_t mode,
-              bool five, int32_t *bj, int64_t padEnd, PlanScratch &sc, std::vector<DevWin> &wj) {
+bool plan_job(HostJob &hj, const JobAnc &anc, const mp_band_params &bp, int32_t mode, bool five, int32_t *bj,
+              int64_t padEnd, PlanScratch &sc, std::vector<DevWin> &wj) {
     const mp_band_params *p = &bp;
     const bool split = use_split(mode);
     std::vector<int32_t> &L = sc.L, &R = sc.R, &XL = sc.XL;
@@ -633,53 +653,72 @@ bool plan_job(HostJob &hj, const int64_t *anchors, int64_t nAnchors, const mp_ba
     L.resize(D + 1);
     R.resize(D + 1);
     const bool dyn = mode != MP_MODE_FORWARD_ONLY && p->dynamicAnchorExpansion != 0;
-    if (!mp::build_band(anchors, nAnchors, hj.lX, hj.lY, p->diagonalExpansion, dyn, L.data(), R.data()))
+    if (!mp::build_band(anc.a, anc.n, anc.xo, anc.yo, hj.lX, hj.lY, p->diagonalExpansion, dyn, L.data(), R.data()))
         return false;
-    int32_t wmax = 0;
-    int64_t cells = 0;
-    sc.ex.resize(D + 2);
-    sc.cw.resize(D + 2);
-    sc.ex[0] = sc.cw[0] = 0;
-    const int32_t nominal = 2 * p->diagonalExpansion + 1;
-    for (int64_t d = 0; d <= D; d++) {
-        const int32_t wd = (R[d] - L[d]) / 2 + 1;
-        wmax = std::max(wmax, wd);
-        cells += wd;
-        sc.ex[d + 1] = sc.ex[d] + std::min(16, std::max(0, wd - nominal));
-        sc.cw[d + 1] = sc.cw[d] + wd;
-    }
-    hj.wmax = wmax;
-    hj.cells = D > 0 ? cells : 0;
-    hj.cls = five ? 3 : (wmax <= 16 ? 0 : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
-    const bool narrow = hj.cls < 3;
-    // xL = x of the band's first cell: (d + xmyL) / 2
-    XL.resize(D + 1);
-    for (int64_t d = 0; d <= D; d++) XL[d] = (int32_t)((d + L[d]) / 2);
-    if (hj.cls == 0) {
-        // rows of 16 x-aligned lanes: the cells of three consecutive
-        // anti-diagonals (and their neighbours) must span at most 16 x
-        auto xr = [&](int64_t d) { return XL[d] + (R[d] - L[d]) / 2; };
-        bool fits = true;
-        for (int64_t d = 0; d <= D && fits; d++) {
-            int64_t lo = XL[d] - 1, hi = xr(d);
-            for (int64_t e = d - 2; e < d; e++)
-                if (e >= 0) { lo = std::min<int64_t>(lo, XL[e]); hi = std::max<int64_t>(hi, xr(e)); }
-            int64_t lo2 = XL[d], hi2 = xr(d) + 1;
-            for (int64_t e = d + 1; e <= d + 2; e++)
-                if (e <= D) { lo2 = std::min<int64_t>(lo2, XL[e]); hi2 = std::max<int64_t>(hi2, xr(e)); }
-            fits = hi - lo <= 15 && hi2 - lo2 <= 15;
-        }
-        if (!fits) hj.cls = 1;
-    }
-    for (int64_t d = 0; d <= D; d++) {
-        const int32_t wd = (R[d] - L[d]) / 2 + 1;
-        bj[d] = (int32_t)((uint32_t)XL[d] << kBandShift) | (narrow ? wd - 1 : std::min(wd, 2048) - 1);
-    }
-    for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
-    if (narrow && hj.cls > 0) mark_slow(XL.data(), D, bj);
+    // traceback windows (they read only the band's extent)
     w.clear();
     if (D > 0 && mode == MP_MODE_FORWARD_ONLY) w.push_back(mp::Window{(int32_t)D, 0, (int32_t)D});
     else if (D > 0) mp::build_windows(L.data(), R.data(), D, *p, w);
+    const size_t nw = w.size();
+    sc.wex.assign(nw, 0);
+    sc.wcells.assign(nw, 0);
+    // One pass over the anti-diagonals: width, cells, xL = x of the band's first
+    // cell ((d + xmyL) / 2), the band entry (width - 1 in the low bits; the
+    // narrow kernels' flags are added below), per window the excess width over
+    // its posterior anti-diagonals (lo, from] (consecutive ranges) and the cells
+    // of [lo, tb] (a wide window's ring; an anti-diagonal is in at most two), and
+    // the x-alignment of rows of 16 lanes: the cells of three consecutive
+    // anti-diagonals, with the neighbour each side reads, must span at most 16 x
+    // (checked with a lag of two anti-diagonals).
+    XL.resize(D + 1);
+    int32_t wmax = 0;
+    int64_t cells = 0;
+    const int32_t nominal = 2 * p->diagonalExpansion + 1;
+    bool fits = true;
+    int32_t xlH[3] = {0, 0, 0}, xrH[3] = {0, 0, 0};  // XL and last x of d, d - 1, d - 2
+    size_t kp = 0, kr = 0;
+    for (int64_t d = 0; d <= D; d++) {
+        const int32_t wd = (R[d] - L[d]) / 2 + 1;
+        const int32_t xl = (int32_t)((d + L[d]) / 2), xr = xl + wd - 1;
+        wmax = std::max(wmax, wd);
+        cells += wd;
+        XL[d] = xl;
+        bj[d] = (int32_t)((uint32_t)xl << kBandShift) | (std::min(wd, 2048) - 1);
+        while (kp < nw && d > w[kp].from) kp++;
+        if (kp < nw && d > w[kp].lo) sc.wex[kp] += std::min(16, std::max(0, wd - nominal));
+        while (kr < nw && d > w[kr].tb) kr++;
+        for (size_t k = kr; k < nw && w[k].lo <= d; k++) sc.wcells[k] += wd;
+        xlH[2] = xlH[1]; xlH[1] = xlH[0]; xlH[0] = xl;
+        xrH[2] = xrH[1]; xrH[1] = xrH[0]; xrH[0] = xr;
+        if (fits) {
+            // forward reads of d: d-1, d-2 and the cell left of xL
+            int32_t lo = xl - 1, hi = xr;
+            for (int e = 1; e <= 2 && e <= d; e++) { lo = std::min(lo, xlH[e]); hi = std::max(hi, xrH[e]); }
+            // backward reads of d - 2: d - 1, d and the cell right of its last x
+            if (d >= 2) {
+                const int32_t lo2 = std::min(xlH[2], std::min(xlH[1], xl));
+                const int32_t hi2 = std::max(xrH[2] + 1, std::max(xrH[1], xr));
+                fits = hi2 - lo2 <= 15;
+            }
+            fits = fits && hi - lo <= 15;
+        }
+    }
+    if (fits) {  // the backward reads of the last two anti-diagonals (truncated at D)
+        for (int64_t d = std::max<int64_t>(0, D - 1); d <= D; d++) {
+            int32_t lo2 = XL[d], hi2 = XL[d] + (R[d] - L[d]) / 2 + 1;
+            for (int64_t e = d + 1; e <= D; e++) {
+                lo2 = std::min(lo2, XL[e]);
+                hi2 = std::max(hi2, XL[e] + (R[e] - L[e]) / 2);
+            }
+            fits = fits && hi2 - lo2 <= 15;
+        }
+    }
+    hj.wmax = wmax;
+    hj.cells = D > 0 ? cells : 0;
+    hj.cls = five ? 3 : (wmax <= 16 ? (fits ? 0 : 1) : (wmax <= 32 ? 1 : (wmax <= 64 ? 2 : 3)));
+    const bool narrow = hj.cls < 3;
+    for (int64_t d = D + 1; d < padEnd; d++) bj[d] = bj[D];
+    if (narrow && hj.cls > 0) mark_slow(XL.data(), D, bj);
     wj.clear();
     for (auto &x : w) wj.push_back(DevWin{x.tb, x.lo, x.from, x.tb - 1});
     if (split && narrow) choose_crossings(wj);
@@ -794,13 +833,35 @@ extern "C" void mp_engine_free(mp_engine *e) {
     }
     for (int l = 0; l < 3; l++)
         if (e->listHost[l]) (void)hipHostFree(e->listHost[l]);
+    for (int k = 0; k < 2; k++)
+        if (e->upHost[k]) (void)hipHostFree(e->upHost[k]);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->ev2) (void)hipEventDestroy(e->ev2);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
 extern "C" int32_t mp_engine_cu_count(const mp_engine *e) { return e ? e->cus : 0; }
+
+// The engine's pinned upload staging k, at least `bytes` long (grown with a
+// quarter of headroom); the caller holds e->createMu.
+static mp_status upload_stage(mp_engine *e, int k, size_t bytes, void **out) {
+    if (e->upHostBytes[k] < bytes) {
+        if (e->upHost[k]) (void)hipHostFree(e->upHost[k]);
+        e->upHost[k] = nullptr;
+        e->upHostBytes[k] = 0;
+        const size_t want = bytes + bytes / 4 + ((size_t)1 << 20);
+        if (hipHostMalloc(&e->upHost[k], want, hipHostMallocDefault) != hipSuccess) {
+            e->upHost[k] = nullptr;
+            mp::set_error("mp_batch_create: pinned host allocation of the upload staging failed");
+            return MP_ENOMEM;
+        }
+        e->upHostBytes[k] = want;
+    }
+    *out = e->upHost[k];
+    return MP_OK;
+}
 
 extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
                                      const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out) {
@@ -833,6 +894,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     }
     *out = nullptr;
     HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> createLock(e->createMu);
     mp_batch *b = new mp_batch();
     b->eng = e;
     b->mode = mode;
@@ -845,6 +907,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->pairFirstJob.assign(npairs, 0);
     std::vector<int64_t> pairFirstSub(npairs, 0);  // a pair's first sub-problem in `jobs`
     const bool split = use_split(mode);
+    auto tPhase = std::chrono::steady_clock::now();
+    auto mark = [&](int ph) {
+        const auto t = std::chrono::steady_clock::now();
+        b->prepMs[ph] += std::chrono::duration<double, std::milli>(t - tPhase).count();
+        tPhase = t;
+    };
 
     // ---- 1. encoded symbols, cropping, split points -> jobs (pair order) ----
     // Two parallel passes over the pairs: (A) checks, reference crop, split
@@ -899,9 +967,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
             pp.one[3] = lY;
             pp.nsp = 1;
         } else {
-            std::vector<int64_t> anc(pr.anchors, pr.anchors + 3 * pr.nAnchors);
-            for (int64_t a = 0; a < pr.nAnchors; a++) anc[3 * a] -= pp.first;
-            mp::split_points(anc.data(), pr.nAnchors, lX, lY, p->splitMatrixBiggerThanThis, pr.raggedLeft != 0,
+            mp::split_points(pr.anchors, pr.nAnchors, pp.first, lX, lY, p->splitMatrixBiggerThanThis,
+                             pr.raggedLeft != 0,
                              pr.raggedRight != 0, pp.more);
             pp.nsp = (int32_t)(pp.more.size() / 4);
             if (pp.nsp == 1) {
@@ -929,6 +996,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                       std::to_string(kMaxDiagonals - 1) + " anti-diagonals");
         return MP_EUNSUPPORTED;
     }
+    mark(0);
     // Symbol pool with padding at both ends (kernels read up to one row width
     // beyond a sequence without clamping).  Each sequence is preceded by an N
     // entry of its own kind: the cells x = 0 (y = 0) of the band read the symbol
@@ -949,13 +1017,21 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     }
     // every entry is written below (pairs in parallel, padding here): no value-initialisation pass
     const size_t nsym = (size_t)(symOff[npairs] + kSymPad);
-    std::unique_ptr<uint32_t[]> symBuf(new uint32_t[nsym]);
-    uint32_t *sym = symBuf.get();
+    uint32_t *sym = nullptr;
+    {
+        void *st = nullptr;
+        const mp_status ss = upload_stage(e, 0, nsym * sizeof(uint32_t), &st);
+        if (ss) {
+            delete b;
+            return ss;
+        }
+        sym = static_cast<uint32_t *>(st);
+    }
     for (int64_t i = 0; i < kSymPad; i++) sym[i] = enc_x(4, 0);
     for (size_t i = (size_t)symOff[npairs]; i < nsym; i++) sym[i] = enc_x(4, 0);
     sym[0] = kZeroSym;  // read by cells outside the band
     std::vector<HostJob> jobs((size_t)njobs);
-    std::vector<std::vector<int64_t>> jobAnchors((size_t)njobs);
+    std::vector<JobAnc> jobAnchors((size_t)njobs);
     parallel_for(npairs, [&](int64_t i) {
         const mp_pair &pr = pairs[i];
         const PairPrep &pp = prep[i];
@@ -972,15 +1048,9 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         int64_t j = 0;
         for (int64_t s = 0; s < ns; s++) {
             const int64_t x1 = spp[4 * s], y1 = spp[4 * s + 1], x2 = spp[4 * s + 2], y2 = spp[4 * s + 3];
-            std::vector<int64_t> &sub = jobAnchors[pairFirstSub[i] + s];
             const int64_t j0 = j;
             while (j < pr.nAnchors && (pr.anchors[3 * j] - pp.first) + pr.anchors[3 * j + 1] < x2 + y2) j++;
-            if (j > j0) sub.resize(3 * (size_t)(j - j0));
-            for (int64_t a = j0; a < j; a++) {
-                sub[3 * (a - j0)] = pr.anchors[3 * a] - pp.first - x1;
-                sub[3 * (a - j0) + 1] = pr.anchors[3 * a + 1] - y1;
-                sub[3 * (a - j0) + 2] = pr.anchors[3 * a + 2];
-            }
+            jobAnchors[pairFirstSub[i] + s] = JobAnc{pr.anchors + 3 * j0, j - j0, pp.first + x1, y1};
             HostJob &hj = jobs[pairFirstSub[i] + s];
             hj.pair = i;
             hj.xOff = x1;
@@ -994,6 +1064,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     });
 
+    mark(1);
     if (refArith) {
         mp_status st = create_reference_batch(b, m, pairs, p, symOff, pairFirstSub, jobs, jobAnchors);
         if (st) {
@@ -1015,8 +1086,16 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         return MP_EUNSUPPORTED;
     }
     // every entry of a job's region is written by its band pass below (anti-diagonals 0..D, then padding)
-    std::unique_ptr<int32_t[]> bandBuf(new int32_t[(size_t)bandOff[nj]]);
-    int32_t *band = bandBuf.get();
+    int32_t *band = nullptr;
+    {
+        void *st = nullptr;
+        const mp_status ss = upload_stage(e, 1, (size_t)std::max<int64_t>(bandOff[nj], 1) * sizeof(int32_t), &st);
+        if (ss) {
+            delete b;
+            return ss;
+        }
+        band = static_cast<int32_t *>(st);
+    }
     // each job's windows live in its thread's pool: (thread, offset, count) per job,
     // with each window's class and width maximum in parallel pools; a wide job whose
     // windows run as window jobs may keep a narrow copy of its band in the thread's
@@ -1051,9 +1130,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     if (j >= nj) break;
                     HostJob &hj = jobs[j];
                     hj.bandOff = bandOff[j];
-                    const std::vector<int64_t> &a = jobAnchors[j];
-                    if (!plan_job(hj, a.data(), (int64_t)a.size() / 3, *p, mode, five, band + bandOff[j],
-                                  bandOff[j + 1] - bandOff[j], sc, wj)) {
+                    if (!plan_job(hj, jobAnchors[j], *p, mode, five, band + bandOff[j], bandOff[j + 1] - bandOff[j],
+                                  sc, wj)) {
                         err = 1;
                         continue;
                     }
@@ -1061,14 +1139,10 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     pool.insert(pool.end(), wj.begin(), wj.end());
                     // the wide kernel's ring holds a window's forward values of [lo, tb], one row
                     // per anti-diagonal, plus one row lost where the ring wraps
-                    for (const DevWin &x : wj)
-                        rcPool[t].push_back(sc.cw[x.tb + 1] - sc.cw[x.lo] + hj.wmax);
+                    for (size_t k = 0; k < wj.size(); k++) rcPool[t].push_back(sc.wcells[k] + hj.wmax);
                     // excess band width over each window's posterior anti-diagonals (lo, from]
-                    for (const DevWin &x : wj)
-                        exPool[t].push_back(five ? 0
-                                                 : (int32_t)std::min<int64_t>(
-                                                       std::max<int64_t>(0, sc.ex[x.from + 1] - sc.ex[x.lo + 1]),
-                                                       INT32_MAX / 8));
+                    for (size_t k = 0; k < wj.size(); k++)
+                        exPool[t].push_back(five ? 0 : (int32_t)std::min<int64_t>(sc.wex[k], INT32_MAX / 8));
                     if (winJobs && hj.cls == 3 && wj.size() > 1) {
                         winRef[j].alt = window_classes(hj, sc, wj, bandOff[j + 1] - bandOff[j], clsPool[t], wmPool[t],
                                                        altPool[t]);
@@ -1105,6 +1179,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
     }
 
+    mark(2);
     // ---- 2b. window jobs (kWarmup): in the posterior modes a three-state sub-problem
     // with several traceback windows runs each window as a job of its own (narrow
     // ones on the split kernel, wide ones on the wide-band kernel) ----
@@ -1172,6 +1247,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     }
     const int64_t nxj = (int64_t)jobs.size();
 
+    mark(3);
     // ---- 3. device order: by class, then band area descending ----
     // (class ascending, cells descending, job index ascending): a stable LSD
     // radix sort of the jobs on (class, maxCells - cells)
@@ -1283,6 +1359,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         d.fstart = hj.fstart;
         b->jobs[q] = hj;
     });
+    mark(4);
     // ---- 3b. slots: per pair its sub-problems in split order, a sub-problem's window
     // jobs last window first (the order of the reference's lists) ----
     {
@@ -1308,6 +1385,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->ringLen = ringLen;
     b->outTotal = outOff;
 
+    mark(5);
     // ---- 4. upload ----
     hipStream_t s = e->stream;
     std::vector<float> tab;
@@ -1450,12 +1528,15 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         std::vector<int32_t> pairSlot((size_t)npairs + 1);
         for (int64_t i = 0; i < npairs; i++) pairSlot[i] = (int32_t)b->pairFirstJob[i];
         pairSlot[npairs] = (int32_t)b->order.size();
+        HIP_TRY(hipStreamSynchronize(s));
+        mark(6);
         if ((st = b->dPairSlot.upload(pairSlot, s)) || (st = alloc_lists(b))) {
             delete b;
             return st;
         }
     }
     HIP_TRY(hipStreamSynchronize(s));
+    mark(7);
     *out = b;
     return MP_OK;
 }
@@ -1688,7 +1769,90 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         drop_graph(b);
         return batch_run_impl(b, plain);
     }
-    // Overflowing record regions: size them exactly and run the batch again.
+    // Jobs that wrote more records than their regions hold run again alone, into
+    // regions of their exact size appended to the record buffer; every other
+    // job's records stay where they are.
+    if (ovf && b->lastReruns == 0) {
+        b->hCounts.resize((size_t)nj * 2);
+        HIP_TRY(hipMemcpy(b->hCounts.data(), b->dCounts.p, sizeof(int32_t) * 2 * nj, hipMemcpyDeviceToHost));
+        std::vector<DevJob> dj(nj);
+        HIP_TRY(hipMemcpy(dj.data(), b->dJobs.p, sizeof(DevJob) * nj, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> over((size_t)nj, 0);
+        int64_t add = 0;
+        b->lastReruns++;
+        for (int32_t j = 0; j < nj; j++) {
+            for (int l = 0; l < 2; l++)
+                if (b->hCounts[2 * j + l] > b->jobs[j].cap[l]) {
+                    over[j] = 1;
+                    b->lastOvfMissing += b->hCounts[2 * j + l] - b->jobs[j].cap[l];
+                }
+            if (!over[j]) continue;
+            b->lastOvfJobs++;
+            for (int l = 0; l < 2; l++) b->jobs[j].cap[l] = std::max(b->jobs[j].cap[l], b->hCounts[2 * j + l]);
+            add += (int64_t)b->jobs[j].cap[0] + b->jobs[j].cap[1];
+        }
+        const int64_t oldTotal = b->outTotal;
+        {
+            DevBuf<int4> grown;
+            if ((st = grown.alloc((size_t)(oldTotal + add)))) return st;
+            HIP_TRY(hipMemcpyAsync(grown.p, b->dOut.p, sizeof(int4) * (size_t)oldTotal, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            std::swap(grown.p, b->dOut.p);
+            std::swap(grown.n, b->dOut.n);
+        }
+        int64_t off = oldTotal;
+        for (int32_t j = 0; j < nj; j++) {
+            if (!over[j]) continue;
+            b->jobs[j].outOff = off;
+            dj[j].outOff = off;
+            dj[j].cap[0] = b->jobs[j].cap[0];
+            dj[j].cap[1] = b->jobs[j].cap[1];
+            off += (int64_t)dj[j].cap[0] + dj[j].cap[1];
+        }
+        HIP_TRY(hipMemcpy(b->dJobs.p, dj.data(), sizeof(DevJob) * nj, hipMemcpyHostToDevice));
+        b->kp.out = b->dOut.p;
+        b->outTotal = oldTotal + add;
+        if ((st = alloc_lists(b))) return st;
+        drop_graph(b);
+        // the over-capacity jobs of each kernel, in device order, as the launches' job map
+        std::vector<int32_t> map;
+        KParams kr = b->kp;
+        for (int c = 0; c < kNumClasses; c++) {
+            kr.jobBegin[c] = (int32_t)map.size();
+            for (int32_t q = b->classBegin[c]; q < b->classEnd[c]; q++)
+                if (over[q]) map.push_back(q);
+            kr.jobEnd[c] = (int32_t)map.size();
+        }
+        kr.wideBegin = (int32_t)map.size();
+        for (int32_t q = b->wideBegin; q < b->wideEnd; q++)
+            if (over[q]) map.push_back(q);
+        kr.wideEnd = (int32_t)map.size();
+        if ((st = b->dJobMap.upload(map, s))) return st;
+        kr.jobMap = b->dJobMap.p;
+        float msRe = 0.f, msReLists = 0.f;
+        HIP_TRY(hipEventRecord(e->ev0, s));
+        if ((st = reset_i32(b, b->dError.p, 1)) || (st = reset_i32(b, b->dQueue.p, kNumClasses + 1))) return st;
+        const bool anyNarrow = kr.jobEnd[0] > kr.jobBegin[0] || kr.jobEnd[1] > kr.jobBegin[1] ||
+                               kr.jobEnd[2] > kr.jobBegin[2];
+        if (anyNarrow && (st = launch_narrow(b, b->narrowBlocks, b->narrowThreads, kr))) return st;
+        if (kr.wideEnd > kr.wideBegin && (st = launch_wide(b, b->wideWaves, kr))) return st;
+        HIP_TRY(hipEventRecord(e->ev1, s));
+        if ((st = enqueue_lists(b))) return st;
+        HIP_TRY(hipEventRecord(e->ev2, s));
+        HIP_TRY(hipEventSynchronize(e->ev2));
+        HIP_TRY(hipEventElapsedTime(&msRe, e->ev0, e->ev1));
+        HIP_TRY(hipEventElapsedTime(&msReLists, e->ev1, e->ev2));
+        b->kernelMs += msRe;
+        b->listMs += msReLists;
+        HIP_TRY(hipMemcpy(&kerr, b->dError.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (kerr) {
+            mp::set_error("pair-HMM kernel: forward/backward sweeps lost step (code " + std::to_string(kerr) + ")");
+            return MP_EHIP;
+        }
+        HIP_TRY(hipMemcpy(&ovf, b->dOverflow.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    // Still overflowing (not expected after the partial run): size every region exactly
+    // and run the whole batch again.
     if (ovf) {
         b->hCounts.resize((size_t)nj * 2);
         HIP_TRY(hipMemcpy(b->hCounts.data(), b->dCounts.p, sizeof(int32_t) * 2 * nj, hipMemcpyDeviceToHost));
@@ -1758,6 +1922,15 @@ extern "C" mp_status mp_batch_stats(const mp_batch *b, int64_t *jobs, int64_t *r
     if (jobs) *jobs = (int64_t)b->jobs.size();
     if (recordCapacity) *recordCapacity = b->outTotal;
     if (records) *records = b->records;
+    return MP_OK;
+}
+
+extern "C" mp_status mp_batch_prep_ms(const mp_batch *b, double *ms, int32_t n) {
+    if (!b || (n > 0 && !ms)) {
+        mp::set_error("mp_batch_prep_ms: null argument");
+        return MP_EINVAL;
+    }
+    for (int32_t i = 0; i < n && i < 8; i++) ms[i] = b->prepMs[i];
     return MP_OK;
 }
 
@@ -2206,7 +2379,8 @@ extern "C" mp_status mp_pairhmm_forward_log_prob_noalloc(mp_engine *e, const mp_
     hj.symY = baseY;
     hj.flags = strand | (pr.raggedLeft ? 2 : 0) | (pr.raggedRight ? 4 : 0);
     band.resize((size_t)padEnd);
-    if (!plan_job(hj, pr.anchors, pr.nAnchors, *p, MP_MODE_FORWARD_ONLY, false, band.data(), padEnd, sc, wj)) {
+    if (!plan_job(hj, JobAnc{pr.anchors, pr.nAnchors, 0, 0}, *p, MP_MODE_FORWARD_ONLY, false, band.data(), padEnd, sc,
+                  wj)) {
         mp::set_error("mp_pairhmm_forward_log_prob_noalloc: invalid band (anchors inconsistent with the sequences)");
         return MP_EANCHORS;
     }

@@ -131,6 +131,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int32_t *const gQueue = kp.queue + cls;
     int32_t *const gError = kp.error;
     const int gJobBegin = kp.jobBegin[cls], gJobEnd = kp.jobEnd[cls];
+    const int32_t *const gJobMap = kp.jobMap;
     const float sv00 = kp.startV[0][0], sv01 = kp.startV[0][1], sv02 = kp.startV[0][2];
     const float sv10 = kp.startV[1][0], sv11 = kp.startV[1][1], sv12 = kp.startV[1][2];
     const float ev00 = kp.endV[0][0], ev01 = kp.endV[0][1], ev02 = kp.endV[0][2];
@@ -346,6 +347,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             jnew = rbcast<SEG>(jnew, rowBase, 0);
             job = gJobBegin + jnew;
             if (job >= gJobEnd) job = -1;
+            else if (gJobMap) job = gJobMap[job];
             if (k == 0) {
                 *jobIdP = job;
                 lds_release(jobSeqP, jn + 1);

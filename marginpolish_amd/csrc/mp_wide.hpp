@@ -98,8 +98,9 @@ __global__ __launch_bounds__(64 * kWideMaxGroupWaves) void pairhmm_wide_kernel(K
         int jn = 0;
         if (lane == 0) jn = atomicAdd(&kp.queue[3], 1);
         jn = __shfl(jn, 0, 64);
-        const int job = kp.wideBegin + jn;
-        if (job >= kp.wideEnd) break;
+        const int pos = kp.wideBegin + jn;
+        if (pos >= kp.wideEnd) break;
+        const int job = kp.jobMap ? kp.jobMap[pos] : pos;
         const DevJob jb = kp.jobs[job];
         const int D = jb.lX + jb.lY;
         const bool rgL = (jb.flags >> 1) & 1, rgR = (jb.flags >> 2) & 1;

@@ -157,8 +157,9 @@ __global__ __launch_bounds__(64 * kWide5MaxWaves) void pairhmm_wide5_kernel(KPar
             ctl[1] = 0;
         }
         __syncthreads();
-        const int job = kp.wideBegin + ctl[0];
-        if (job >= kp.wideEnd) break;
+        const int pos = kp.wideBegin + ctl[0];
+        if (pos >= kp.wideEnd) break;
+        const int job = kp.jobMap ? kp.jobMap[pos] : pos;
         const DevJob jb = kp.jobs[job];
         const int D = jb.lX + jb.lY;
         const int rStart = (jb.flags >> 1) & 1, rgR = (jb.flags >> 2) & 1;

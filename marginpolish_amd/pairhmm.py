@@ -332,6 +332,15 @@ class Batch:
         check(lib().mp_batch_stats(self._h, C.byref(j), C.byref(c), C.byref(r)))
         return j.value, c.value, r.value
 
+    def prep_ms(self):
+        """Host time (ms) of the batch's creation phases: checks and split
+        points, symbols and jobs, bands and windows, window jobs, device
+        order and record regions, slots, upload and device buffers, list
+        buffers."""
+        v = (C.c_double * 8)()
+        check(lib().mp_batch_prep_ms(self._h, v, 8))
+        return list(v)
+
     def rerun_stats(self):
         """The last run's record-region re-runs: (re-runs, jobs over their
         capacity, records beyond their regions)."""

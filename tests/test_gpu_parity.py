@@ -650,3 +650,34 @@ def test_wide_pairs_with_narrow_windows(ph, params):
     assert worst <= 2e-5 and drift <= 1e-3
     _check(gw, _oracle_jobs(op, jobs, opar), opar.threshold)
     assert abs(bw.total_log_prob() - math.fsum(r[3] for r in gw)) <= 1e-9 * abs(bw.total_log_prob())
+
+
+def test_overflow_reruns_only_the_jobs_over_capacity(ph, params):
+    """A threshold of 0.001 makes jobs write more records than their regions
+    (1.5 per posterior anti-diagonal) hold: the first run re-runs only those
+    jobs, into regions of their exact size appended to the record buffer.
+    Its lists and log-likelihoods equal the second run's (exact regions, no
+    re-run) bit for bit, and the oracle's."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(n_reads=8, ref_len=3000, seed=41, W=10, strands=(1, 0))
+    jobs = []
+    for i, pr in enumerate(w.pairs):
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand, bool(i & 1), bool(i & 2)))
+    opar = O.Params(threshold=0.001, minDiagsBetweenTraceBack=1000, traceBackDiagonals=40, diagonalExpansion=10,
+                    splitMatrixBiggerThanThis=250_000_000, dynamicAnchorExpansion=True)
+    say("case: record overflow, 8 pairs, threshold 0.001")
+    b = ph.Batch(pp.machines, jobs, _bp(ph, opar)).run()
+    r1 = b.rerun_stats()
+    g1 = b.fetch()
+    b.run()
+    r2 = b.rerun_stats()
+    g2 = b.fetch()
+    say("first run: %d re-run(s), %d jobs over capacity, %d records beyond; second run %s" % (r1 + (r2,)))
+    assert r1[0] == 1 and 1 <= r1[1] <= b.stats()[0] and r2 == (0, 0, 0)
+    for a, c in zip(g1, g2):
+        for l in range(3):
+            assert np.array_equal(a[l], c[l])
+        assert a[3] == c[3]
+    _check(g1, _oracle_jobs(op, jobs, opar), opar.threshold)
