@@ -54,6 +54,23 @@ __host__ __device__ inline uint32_t enc_y(uint32_t sym, int strand) {
 }
 constexpr uint32_t kZeroSym = (uint32_t)((kTabZeroE - kTabE) * 4 / 2) << 16;
 
+// A batch's symbol pool goes to the device as 16-bit raw entries, encoded there:
+// base min(sym & 255, 4) (bits 0-2), run length min(sym >> 8, 50) (bits 3-8),
+// side (bit 9: 0 reference x, 1 read y), strand (bit 10); kRawZero marks the
+// zero-emission entry read by cells outside the band.
+constexpr uint16_t kRawZero = 0xffff;
+__host__ __device__ inline uint16_t raw_sym(uint32_t sym, int y, int strand) {
+    const uint32_t base = (sym & 255u) < 4u ? (sym & 255u) : 4u;
+    const uint32_t rl = (sym >> 8) < 50u ? (sym >> 8) : 50u;
+    return (uint16_t)(base | (rl << 3) | ((uint32_t)y << 9) | ((uint32_t)strand << 10));
+}
+__host__ __device__ inline uint32_t enc_raw(uint16_t v) {
+    if (v == kRawZero) return kZeroSym;
+    const uint32_t s = (v & 7u) | (((uint32_t)v >> 3 & 63u) << 8);
+    const int strand = (v >> 10) & 1;
+    return ((v >> 9) & 1) ? enc_y(s, strand) : enc_x(s, strand);
+}
+
 struct Emis {
     float eX, eY, eM;
 };

@@ -844,6 +844,14 @@ extern "C" void mp_engine_free(mp_engine *e) {
 
 extern "C" int32_t mp_engine_cu_count(const mp_engine *e) { return e ? e->cus : 0; }
 
+// A batch's symbol pool from its raw entries (raw_sym) to the emission-table
+// offsets the kernels read (enc_x / enc_y): rleString_constructSymbolString's
+// symbols (stateMachine.c:124-146) as table addresses, on the device.
+__global__ __launch_bounds__(256) void encode_symbols(const uint16_t *raw, uint32_t *sym, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        sym[i] = enc_raw(raw[i]);
+}
+
 // The engine's pinned upload staging k, at least `bytes` long (grown with a
 // quarter of headroom); the caller holds e->createMu.
 static mp_status upload_stage(mp_engine *e, int k, size_t bytes, void **out) {
@@ -1017,32 +1025,32 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     }
     // every entry is written below (pairs in parallel, padding here): no value-initialisation pass
     const size_t nsym = (size_t)(symOff[npairs] + kSymPad);
-    uint32_t *sym = nullptr;
+    uint16_t *sym = nullptr;  // raw entries, encoded on the device (encode_symbols)
     {
         void *st = nullptr;
-        const mp_status ss = upload_stage(e, 0, nsym * sizeof(uint32_t), &st);
+        const mp_status ss = upload_stage(e, 0, nsym * sizeof(uint16_t), &st);
         if (ss) {
             delete b;
             return ss;
         }
-        sym = static_cast<uint32_t *>(st);
+        sym = static_cast<uint16_t *>(st);
     }
-    for (int64_t i = 0; i < kSymPad; i++) sym[i] = enc_x(4, 0);
-    for (size_t i = (size_t)symOff[npairs]; i < nsym; i++) sym[i] = enc_x(4, 0);
-    sym[0] = kZeroSym;  // read by cells outside the band
+    for (int64_t i = 0; i < kSymPad; i++) sym[i] = raw_sym(4, 0, 0);
+    for (size_t i = (size_t)symOff[npairs]; i < nsym; i++) sym[i] = raw_sym(4, 0, 0);
+    sym[0] = kRawZero;  // read by cells outside the band
     std::vector<HostJob> jobs((size_t)njobs);
     std::vector<JobAnc> jobAnchors((size_t)njobs);
     parallel_for(npairs, [&](int64_t i) {
         const mp_pair &pr = pairs[i];
         const PairPrep &pp = prep[i];
         const int strand = pr.strand ? 1 : 0;
-        uint32_t *o = sym + symOff[i];
-        *o++ = enc_x(4, strand);
+        uint16_t *o = sym + symOff[i];
+        *o++ = raw_sym(4, 0, strand);
         const int64_t baseX = symOff[i] + 1;
-        for (int64_t x = pp.first; x < pp.end; x++) *o++ = enc_x(pr.sX[x], strand);
-        *o++ = enc_y(4, strand);
+        for (int64_t x = pp.first; x < pp.end; x++) *o++ = raw_sym(pr.sX[x], 0, strand);
+        *o++ = raw_sym(4, 1, strand);
         const int64_t baseY = baseX + (pp.end - pp.first) + 1;
-        for (int64_t y = 0; y < pr.lY; y++) *o++ = enc_y(pr.sY[y], strand);
+        for (int64_t y = 0; y < pr.lY; y++) *o++ = raw_sym(pr.sY[y], 1, strand);
         const int64_t ns = pp.nsp;
         const int64_t *spp = pp.sp();
         int64_t j = 0;
@@ -1395,7 +1403,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     b->modelStates = m->mach[1].nStates;
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
-    if ((st = b->dJobs.upload(dj, s)) || (st = b->dSym.upload(sym, nsym, s)) ||
+    DevBuf<uint16_t> rawUp;  // the raw pool on the device until encode_symbols has run (freed on return)
+    if ((st = b->dJobs.upload(dj, s)) || (st = rawUp.upload(sym, nsym, s)) || (st = b->dSym.alloc(nsym)) ||
         (st = b->dBand.alloc((size_t)bandTotal)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
@@ -1409,6 +1418,15 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
         delete b;
         return st;
+    }
+    {
+        const unsigned g = (unsigned)std::min<size_t>((nsym + 255) / 256, (size_t)e->cus * 16);
+        hipLaunchKernelGGL(encode_symbols, dim3(std::max(g, 1u)), dim3(256), 0, s, rawUp.p, b->dSym.p, (int64_t)nsym);
+        if (hipGetLastError() != hipSuccess) {
+            delete b;
+            mp::set_error("mp_batch_create: symbol encoding launch failed");
+            return MP_EHIP;
+        }
     }
     {
         // the jobs' bands, then the narrow band copies of the threads' pools

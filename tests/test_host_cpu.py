@@ -239,3 +239,23 @@ def test_fit_sampler_and_gradient_plumbing():
     logT = torch.zeros(3, 3, dtype=torch.float64, requires_grad=True)
     fit.log_likelihood(logT, Quadratic()).backward()
     assert np.array_equal(logT.grad.numpy(), np.arange(9.0).reshape(3, 3))
+
+
+def test_device_symbol_encoding_matches_host_encoding(tmp_path):
+    """The batch's symbol pool goes to the device as 16-bit raw entries and is
+    encoded there: raw_sym then enc_raw equals enc_x / enc_y for every symbol,
+    side and strand (compiled for the host with hipcc)."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    exe = str(tmp_path / "symbol_encoding_check")
+    subprocess.check_call([hipcc, "-O2", "-std=c++17", "-I", os.path.join(root, "include"), "-I",
+                           os.path.join(root, "marginpolish_amd", "csrc"), "-o", exe,
+                           os.path.join(here, "symbol_encoding_check.cpp")], stderr=subprocess.DEVNULL)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    assert "mismatches 0" in out.stdout
