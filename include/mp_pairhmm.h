@@ -197,7 +197,7 @@ int32_t mp_engine_cu_count(const mp_engine *e);
  *      list length; when it exceeds *cap nothing is truncated silently: the
  *      call returns MP_OK with counts set and the arrays filled only up to cap,
  *      so the caller can retry with larger arrays.  logProb receives the
- *      forward log-likelihood of the last window. */
+ *      pair's forward log-likelihood (the sum over its split sub-problems). */
 mp_status mp_pairhmm_posteriors(mp_engine *e, const mp_model *m, const mp_pair *pair, const mp_band_params *p,
                                 int32_t mode, mp_aligned_pair *matches, int64_t matchCap, int64_t *matchCount,
                                 mp_aligned_pair *gapX, int64_t gapXCap, int64_t *gapXCount,

@@ -259,3 +259,40 @@ def test_device_symbol_encoding_matches_host_encoding(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
     assert "mismatches 0" in out.stdout
+
+
+def test_c_caller_compiles_against_the_headers(tmp_path):
+    """A C program compiled with gcc against include/*.h and linked with the
+    library: the ABI structs' sizes and offsets equal the ctypes mirrors, and
+    its host calls give the Python bindings' results."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    libdir = os.path.join(ROOT, "marginpolish_amd")
+    exe = str(tmp_path / "c_abi_check")
+    subprocess.check_call([gcc, "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-o", exe,
+                           os.path.join(here, "c_abi_check.c"), "-L", libdir, "-lmp_pairhmm",
+                           "-Wl,-rpath," + libdir])
+    out = subprocess.run([exe, PARAMS], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines()}
+
+    def layout(cls):
+        return [C.sizeof(cls)] + [getattr(cls, f).offset for f, _ in cls._fields_]
+    assert [int(v) for v in rows["mp_pair"]] == layout(_lib.Pair)
+    assert [int(v) for v in rows["mp_band_params"]] == layout(_lib.BandParams)
+    assert [int(v) for v in rows["mp_aligned_pair"]] == layout(_lib.AlignedPair)
+    assert [int(v) for v in rows["mp_rle_string"]] == layout(_lib.RleString)
+    assert int(rows["mp_poa_sizes"][0]) == C.sizeof(_lib.PoaSizes)
+    p = ph.PolishParams(PARAMS).p
+    assert [float(rows["params"][0])] + [int(v) for v in rows["params"][1:]] == [
+        p.threshold, p.minDiagsBetweenTraceBack, p.traceBackDiagonals, p.diagonalExpansion,
+        p.splitMatrixBiggerThanThis, int(p.dynamicAnchorExpansion)]
+    L, R = _band([(1, 0, 0), (2, 1, 0), (3, 3, 0)], 6, 5, 2, False)
+    assert rows["band"] == ["%d:%d" % (l, r) for l, r in zip(L, R)]
+    sp = O.split_points([(10, 10, 0), (11, 11, 0), (5000, 5000, 0), (5001, 5001, 0)], 6000, 6000, 1000000)
+    assert int(rows["splits"][0]) == len(sp) >= 2
+    assert [int(v) for v in rows["splits"][1:]] == [int(v) for t in sp for v in t]
