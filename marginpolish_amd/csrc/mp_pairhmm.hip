@@ -219,6 +219,7 @@ struct mp_batch {
     std::vector<mpk::RefJob> refJobs;
     mpk::RefParams rp{};
     int32_t refBlocks = 0;
+    int32_t refThreads = 256;  // workgroup size of the reference-arithmetic kernel (256 or 512)
     DevBuf<mpk::RefJob> dRefJobs;
     DevBuf<uint16_t> dRawSym;
     DevBuf<int2> dLR;
@@ -416,6 +417,7 @@ mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *
         b->cells += cells[j];
     }
     b->outTotal = outOff;
+    b->refThreads = maxW > 512 ? 512 : 256;
     {
         std::vector<int32_t> slots, slotK;
         for (int64_t i = 0; i < np; i++) {
@@ -1573,12 +1575,15 @@ static mp_status enqueue_run(mp_batch *b) {
     mp_status st;
     if (b->ref) {
         if ((st = reset_i32(b, b->dError.p, 1)) || (st = reset_i32(b, b->dQueue.p, 1))) return st;
-        if (b->rp.poly)
-            hipLaunchKernelGGL(mpk::pairhmm_ref_kernel<true>, dim3(b->refBlocks), dim3(mpk::kRefThreads), 0,
-                               b->eng->stream, b->rp);
+        const dim3 grid(b->refBlocks), block(b->refThreads);
+        if (b->rp.poly && b->refThreads == 512)
+            hipLaunchKernelGGL((mpk::pairhmm_ref_kernel<true, 512>), grid, block, 0, b->eng->stream, b->rp);
+        else if (b->rp.poly)
+            hipLaunchKernelGGL((mpk::pairhmm_ref_kernel<true, 256>), grid, block, 0, b->eng->stream, b->rp);
+        else if (b->refThreads == 512)
+            hipLaunchKernelGGL((mpk::pairhmm_ref_kernel<false, 512>), grid, block, 0, b->eng->stream, b->rp);
         else
-            hipLaunchKernelGGL(mpk::pairhmm_ref_kernel<false>, dim3(b->refBlocks), dim3(mpk::kRefThreads), 0,
-                               b->eng->stream, b->rp);
+            hipLaunchKernelGGL((mpk::pairhmm_ref_kernel<false, 256>), grid, block, 0, b->eng->stream, b->rp);
         HIP_TRY(hipGetLastError());
         return MP_OK;
     }

@@ -81,7 +81,6 @@ struct RefParams {
     int32_t *error;
 };
 
-constexpr int kRefThreads = 256;
 constexpr double kLogZero = -__builtin_inf();
 
 // The reference's lookup (pairwiseAligner.c:279-299): float-suffixed literals
@@ -148,11 +147,14 @@ struct RefEmit {
     }
 };
 
-template <bool POLY>
-__global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) {
+// NT threads per workgroup, one job at a time: 256, or 512 for batches with bands
+// wider than 512 cells (the stitch rectangles), whose anti-diagonals take most of
+// the time (1024 would cap the kernel at 128 registers and spill).
+template <bool POLY, int NT>
+__global__ __launch_bounds__(NT) void pairhmm_ref_kernel(RefParams rp) {
     __shared__ int sJob;
     __shared__ double sTotal;
-    __shared__ int sScan[3][kRefThreads / 64];
+    __shared__ int sScan[3][NT / 64];
     __shared__ int sCount[3];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
@@ -183,12 +185,12 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
         // cell offsets of the anti-diagonals (F storage)
         {
             int64_t carry = 0;
-            __shared__ int64_t sh64[kRefThreads / 64];
-            for (int64_t d0 = 0; d0 <= D; d0 += kRefThreads) {
+            __shared__ int64_t sh64[NT / 64];
+            for (int64_t d0 = 0; d0 <= D; d0 += NT) {
                 const int64_t d = d0 + tid;
                 const int64_t v = d <= D ? width(d) : 0;
                 int64_t tot;
-                const int64_t pre = block_excl_scan<int64_t, kRefThreads>(v, sh64, &tot);
+                const int64_t pre = block_excl_scan<int64_t, NT>(v, sh64, &tot);
                 if (d <= D) offs[d] = carry + pre;
                 carry += tot;
             }
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
         for (int64_t xay = 1; xay <= D; xay++) {
             // ---- forward anti-diagonal xay (cellCalculate with doTransitionForward) ----
             const int64_t w = width(xay), L0 = lr[xay].x;
-            for (int64_t i = tid; i < w; i += kRefThreads) {
+            for (int64_t i = tid; i < w; i += NT) {
                 const int64_t xmy = L0 + 2 * i, x = (xay + xmy) / 2, y = (xay - xmy) / 2;
                 double *cur = F + 3 * (offs[xay] + i);
                 double cM = kLogZero, cX = kLogZero, cY = kLogZero;
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                 const bool rg = atEnd && ((J.flags >> 2) & 1);
                 const double e0 = rg ? (rp.model.trans[3] + rp.model.trans[4]) / 2.0 : T_MM;
                 const double e1 = rg ? T_XX : T_XM, e2 = rg ? T_YY : T_YM;
-                for (int64_t i = tid; i < w; i += kRefThreads) {
+                for (int64_t i = tid; i < w; i += NT) {
                     double *b = Bc(xay, L0 + 2 * i);
                     b[0] = e0;
                     b[1] = e1;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                 if (d2 < xay) {
                     // backward values of d2: the pushes of d2 + 2 (match) and of d2 + 1 (gap Y of
                     // (x, y+1), then gap X of (x+1, y)), in the reference's order
-                    for (int64_t i = tid; i < w2; i += kRefThreads) {
+                    for (int64_t i = tid; i < w2; i += NT) {
                         const int64_t xmy = L2 + 2 * i, x = (d2 + xmy) / 2, y = (d2 - xmy) / 2;
                         double bM = kLogZero, bX = kLogZero, bY = kLogZero;
                         if (d2 + 2 <= xay && xmy >= lr[d2 + 2].x && xmy <= lr[d2 + 2].y) {
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                 if (d2 > from) continue;
                 if (count++ % 10 == 0) {
                     // diagonalCalculationTotalProbability: the cut at d2 plus the matches through it
-                    for (int64_t i = tid; i < w2; i += kRefThreads) {
+                    for (int64_t i = tid; i < w2; i += NT) {
                         const double *f = F + 3 * (offs[d2] + i), *b = Bc(d2, L2 + 2 * i);
                         double t = f[0] + b[0];
                         t = ref_logadd<POLY>(t, f[1] + b[1]);
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                     }
                     const bool through = d2 + 1 <= xay && d2 + 1 <= D;
                     const int64_t w1 = through ? width(d2 + 1) : 0, L1 = through ? lr[d2 + 1].x : 0;
-                    for (int64_t i = tid; i < w1; i += kRefThreads) {
+                    for (int64_t i = tid; i < w1; i += NT) {
                         const int64_t xmy = L1 + 2 * i, x = (d2 + 1 + xmy) / 2, y = (d2 + 1 - xmy) / 2;
                         const double *mid = Fc(d2 - 1, xmy);
                         double mt = kLogZero;
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                     total = sTotal;
                 }
                 // ---- posteriors of d2 (addPosteriorProb), cells by ascending x - y ----
-                for (int64_t i0 = 0; i0 < w2; i0 += kRefThreads) {
+                for (int64_t i0 = 0; i0 < w2; i0 += NT) {
                     const int64_t i = i0 + tid;
                     int64_t wgt[3] = {0, 0, 0};
                     bool on[3] = {false, false, false};
@@ -364,14 +366,14 @@ __global__ __launch_bounds__(kRefThreads) void pairhmm_ref_kernel(RefParams rp) 
                             }
                         }
                     }
-                    for (int l = 0; l < 3; l++) {
+                    for (int l = 0; l < (rp.withIndels ? 3 : 1); l++) {  // match-only batches: one list
                         const uint64_t m = __ballot(on[l]);
                         const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                         if (lane == 0) sScan[l][wv] = __popcll(m);
                         __syncthreads();
                         int before = 0, tot = 0;
-                        for (int k = 0; k < kRefThreads / 64; k++) {
+                        for (int k = 0; k < NT / 64; k++) {
                             if (k < wv) before += sScan[l][k];
                             tot += sScan[l][k];
                         }
