@@ -2,14 +2,65 @@
 // batch planner and the HIP driver.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "mp_pairhmm.h"
 #include "mp_poa.h"
 
 namespace mp {
+
+// Runs f(t) for t = 0 .. n-1 on n host threads (t = 0 on the calling thread)
+// and joins every thread it started before returning.  An exception thrown by
+// any f (std::bad_alloc, say), or a thread that cannot be started, is caught
+// where it happens and the first one is rethrown here once all threads have
+// ended, so no joinable std::thread is ever destroyed.  Work handed out from a
+// shared counter is still finished by the threads that run; a worker that
+// waits for others must also stop when `failed()` turns true.
+class ThreadPoolRun {
+public:
+    template <class F> void run(int n, F &&f) {
+        std::vector<std::thread> th;
+        auto guarded = [&](int t) {
+            try {
+                f(t);
+            } catch (...) {
+                record(std::current_exception());
+            }
+        };
+        try {
+            if (n > 1) th.reserve((size_t)n - 1);
+            for (int t = 1; t < n; t++) th.emplace_back(guarded, t);
+        } catch (...) {
+            record(std::current_exception());
+        }
+        guarded(0);
+        for (auto &x : th) x.join();
+        if (first_) std::rethrow_exception(first_);
+    }
+    bool failed() const { return failed_.load(std::memory_order_relaxed); }
+
+private:
+    void record(std::exception_ptr e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!first_) first_ = e;
+        failed_.store(true);
+    }
+    std::mutex mu_;
+    std::exception_ptr first_;
+    std::atomic<bool> failed_{false};
+};
+
+template <class F> void parallel_run(int n, F &&f) {
+    ThreadPoolRun pool;
+    pool.run(n, std::forward<F>(f));
+}
 
 constexpr int kMaxRepeat = 51;            // MAXIMUM_REPEAT_LENGTH, margin.h:123
 constexpr int kRepeatEntries = 5 * kMaxRepeat * kMaxRepeat;

@@ -345,9 +345,7 @@ mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *
     {
         const int nth = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
         std::atomic<int64_t> next{0};
-        std::vector<std::thread> th;
-        for (int t = 0; t < nth; t++)
-            th.emplace_back([&]() {
+        mp::parallel_run(nth, [&](int) {
                 std::vector<int32_t> L, R;
                 for (int64_t j; (j = next.fetch_add(1)) < nj;) {
                     const HostJob &hj = jobs[j];
@@ -372,7 +370,6 @@ mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *
                     wmax[j] = wm;
                 }
             });
-        for (auto &t : th) t.join();
     }
     if (bad) {
         mp::set_error("invalid band (anchors inconsistent with the sequences)");
@@ -873,8 +870,14 @@ static mp_status upload_stage(mp_engine *e, int k, size_t bytes, void **out) {
     return MP_OK;
 }
 
-extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
-                                     const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out) {
+static void discard_batch(mp_batch *b, mp_batch **inFlight) {
+    *inFlight = nullptr;
+    delete b;
+}
+
+static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
+                               const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out,
+                               mp_batch **inFlight) {
     if (!e || !m || !p || !out || (npairs > 0 && !pairs) || npairs < 0) {
         mp::set_error("mp_batch_create: null or negative argument");
         return MP_EINVAL;
@@ -906,6 +909,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     HIP_TRY(hipSetDevice(e->device));
     std::lock_guard<std::mutex> createLock(e->createMu);
     mp_batch *b = new mp_batch();
+    *inFlight = b;
     b->eng = e;
     b->mode = mode;
     b->flags = flags;
@@ -931,16 +935,13 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     const int nthreads = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     auto parallel_for = [&](int64_t n, const std::function<void(int64_t)> &fn) {
         std::atomic<int64_t> next{0};
-        std::vector<std::thread> th;
-        for (int t = 0; t < nthreads; t++)
-            th.emplace_back([&]() {
-                for (;;) {
-                    const int64_t i0 = next.fetch_add(16);
-                    if (i0 >= n) break;
-                    for (int64_t i = i0; i < std::min(n, i0 + 16); i++) fn(i);
-                }
-            });
-        for (auto &t : th) t.join();
+        mp::parallel_run(nthreads, [&](int) {
+            for (;;) {
+                const int64_t i0 = next.fetch_add(16);
+                if (i0 >= n) break;
+                for (int64_t i = i0; i < std::min(n, i0 + 16); i++) fn(i);
+            }
+        });
     };
     struct PairPrep {
         int64_t first = 0, end = 0;
@@ -992,7 +993,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     });
     for (int64_t i = 0; i < npairs; i++) {
         if (!prep[i].err) continue;
-        delete b;
+        discard_batch(b, inFlight);
         if (prep[i].err == 1) {
             mp::set_error("pair " + std::to_string(i) + ": bad sequence or anchor arguments");
             return MP_EINVAL;
@@ -1021,7 +1022,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         njobs += prep[i].nsp;
     }
     if (symOff[npairs] + kSymPad >= ((int64_t)1 << 30)) {
-        delete b;
+        discard_batch(b, inFlight);
         mp::set_error("batch too large: more than 2^30 symbols (split it into several batches)");
         return MP_EUNSUPPORTED;
     }
@@ -1032,7 +1033,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         void *st = nullptr;
         const mp_status ss = upload_stage(e, 0, nsym * sizeof(uint16_t), &st);
         if (ss) {
-            delete b;
+            discard_batch(b, inFlight);
             return ss;
         }
         sym = static_cast<uint16_t *>(st);
@@ -1078,7 +1079,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     if (refArith) {
         mp_status st = create_reference_batch(b, m, pairs, p, symOff, pairFirstSub, jobs, jobAnchors);
         if (st) {
-            delete b;
+            discard_batch(b, inFlight);
             return st;
         }
         *out = b;
@@ -1091,7 +1092,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     // Each job's band padded to whole cells of kK anti-diagonals plus one extra cell.
     for (int64_t j = 0; j < nj; j++) bandOff[j + 1] = bandOff[j] + ((jobs[j].lX + jobs[j].lY) / kK + 2) * kK;
     if (bandOff[nj] >= ((int64_t)1 << 30)) {
-        delete b;
+        discard_batch(b, inFlight);
         mp::set_error("batch too large: more than 2^30 anti-diagonals (split it into several batches)");
         return MP_EUNSUPPORTED;
     }
@@ -1101,7 +1102,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         void *st = nullptr;
         const mp_status ss = upload_stage(e, 1, (size_t)std::max<int64_t>(bandOff[nj], 1) * sizeof(int32_t), &st);
         if (ss) {
-            delete b;
+            discard_batch(b, inFlight);
             return ss;
         }
         band = static_cast<int32_t *>(st);
@@ -1124,11 +1125,8 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     const bool winJobs = split && (flags & MP_BATCH_EXACT_CHAIN) == 0 && !five;
     std::atomic<int> err{0};
     {
-        const int nth = nthW;
-        std::vector<std::thread> th;
         std::atomic<int64_t> next{0};
-        for (int t = 0; t < nth; t++)
-            th.emplace_back([&, t]() {
+        mp::parallel_run(nthW, [&](int t) {
                 PlanScratch sc;
                 std::vector<DevWin> wj;
                 std::vector<DevWin> &pool = winPool[t];
@@ -1162,26 +1160,25 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                     }
                 }
             });
-        for (auto &t : th) t.join();
     }
     // the narrow band copies follow the jobs' bands in the device band array
     std::vector<int64_t> altBase(nthW + 1, bandOff[nj]);
     for (int t = 0; t < nthW; t++) altBase[t + 1] = altBase[t] + (int64_t)altPool[t].size();
     const int64_t bandTotal = altBase[nthW];
     if (bandTotal >= ((int64_t)1 << 30)) {
-        delete b;
+        discard_batch(b, inFlight);
         mp::set_error("batch too large: more than 2^30 anti-diagonals (split it into several batches)");
         return MP_EUNSUPPORTED;
     }
     if (err) {
-        delete b;
+        discard_batch(b, inFlight);
         mp::set_error("invalid band (anchors inconsistent with the sequences)");
         return MP_EANCHORS;
     }
     const int32_t wideLimit = 64 * (fp64 ? kWide5MaxC64 : (five ? kWide5MaxC32 : kWideMaxC));
     for (int64_t j = 0; j < nj; j++) {
         if (jobs[j].wmax > wideLimit) {
-            delete b;
+            discard_batch(b, inFlight);
             mp::set_error("pair " + std::to_string(jobs[j].pair) + ": band width " + std::to_string(jobs[j].wmax) +
                           " exceeds the " + std::to_string(wideLimit) + "-cell kernel limit" +
                           (fp64 ? " of FP64 batches" : (five ? " of the five-state machine" : "")));
@@ -1204,7 +1201,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         }
         subFirstX[nj] = nx;
         if (nx >= ((int64_t)1 << 30)) {
-            delete b;
+            discard_batch(b, inFlight);
             mp::set_error("batch too large: more than 2^30 jobs (split it into several batches)");
             return MP_EUNSUPPORTED;
         }
@@ -1418,14 +1415,14 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         (st = b->dQueue.alloc(kNumClasses + 1)) || (st = b->dError.alloc(1)) ||
         (st = b->dExpect.alloc(mode == MP_MODE_EXPECTATIONS ? std::max<size_t>(10 * dj.size(), 1) : 1)) ||
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
-        delete b;
+        discard_batch(b, inFlight);
         return st;
     }
     {
         const unsigned g = (unsigned)std::min<size_t>((nsym + 255) / 256, (size_t)e->cus * 16);
         hipLaunchKernelGGL(encode_symbols, dim3(std::max(g, 1u)), dim3(256), 0, s, rawUp.p, b->dSym.p, (int64_t)nsym);
         if (hipGetLastError() != hipSuccess) {
-            delete b;
+            discard_batch(b, inFlight);
             mp::set_error("mp_batch_create: symbol encoding launch failed");
             return MP_EHIP;
         }
@@ -1440,7 +1437,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
                 he = hipMemcpyAsync(b->dBand.p + altBase[t], altPool[t].data(), sizeof(int32_t) * altPool[t].size(),
                                     hipMemcpyHostToDevice, s);
         if (he != hipSuccess) {
-            delete b;
+            discard_batch(b, inFlight);
             mp::set_error(std::string("band upload: ") + hipGetErrorString(he));
             return MP_EHIP;
         }
@@ -1475,12 +1472,12 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         if (split) {
             if ((st = b->dRing.alloc(waves * (size_t)(ringLen / kK) * (kRingCellBytes / 4))) ||
                 (st = b->dRingExp.alloc(waves * (size_t)(ringLen / kK) * 4))) {
-                delete b;
+                discard_batch(b, inFlight);
                 return st;
             }
         } else {
             if ((st = b->dRing.alloc(waves * (size_t)ringLen * 64 * 4))) {
-                delete b;
+                discard_batch(b, inFlight);
                 return st;
             }
         }
@@ -1507,7 +1504,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         const size_t ringBytes = five ? perWave : (size_t)std::max<int64_t>(b->ringWideCells, 1) * perCell;
         if ((st = b->dRingWide.alloc((size_t)b->wideWaves * ringBytes / sizeof(float))) ||
             (!five && (st = b->dRingWideIdx.alloc((size_t)b->wideWaves * std::max(b->ringWideLen, 1))))) {
-            delete b;
+            discard_batch(b, inFlight);
             return st;
         }
     }
@@ -1551,7 +1548,7 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
         HIP_TRY(hipStreamSynchronize(s));
         mark(6);
         if ((st = b->dPairSlot.upload(pairSlot, s)) || (st = alloc_lists(b))) {
-            delete b;
+            discard_batch(b, inFlight);
             return st;
         }
     }
@@ -1559,6 +1556,26 @@ extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_p
     mark(7);
     *out = b;
     return MP_OK;
+}
+
+// A batch that batch_create leaves behind on an early HIP error or an
+// exception (host memory exhausted) is freed here; the error paths that
+// free the batch themselves clear `inFlight` first.
+extern "C" mp_status mp_batch_create(mp_engine *e, const mp_model *m, const mp_pair *pairs, int64_t npairs,
+                                     const mp_band_params *p, int32_t mode, int32_t flags, mp_batch **out) {
+    mp_batch *inFlight = nullptr;
+    mp_status st;
+    try {
+        st = batch_create(e, m, pairs, npairs, p, mode, flags, out, &inFlight);
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_batch_create: out of host memory");
+        st = MP_ENOMEM;
+    }
+    if (st != MP_OK && inFlight) {
+        delete inFlight;
+        if (out) *out = nullptr;
+    }
+    return st;
 }
 
 static void drop_graph(mp_batch *b) {
@@ -2137,12 +2154,9 @@ static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t
         const size_t off = k * chunk, len = std::min(chunk, bytes - off);
         const char *sb = reinterpret_cast<const char *>(e->stage[k & 1]);
         const size_t part = (len + nth - 1) / nth;
-        std::vector<std::thread> th;
-        for (int t = 1; t < nth; t++)
-            if ((size_t)t * part < len)
-                th.emplace_back([=]() { std::memcpy(dst + off + t * part, sb + t * part, std::min(part, len - t * part)); });
-        std::memcpy(dst + off, sb, std::min(part, len));
-        for (auto &x : th) x.join();
+        mp::parallel_run(nth, [&](int t) {
+            if ((size_t)t * part < len) std::memcpy(dst + off + t * part, sb + t * part, std::min(part, len - t * part));
+        });
     };
     for (size_t k = 0; k < nchunk; k++) {
         const size_t off = k * chunk, len = std::min(chunk, bytes - off);

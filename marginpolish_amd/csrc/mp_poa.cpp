@@ -651,9 +651,11 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
         }
         done.fetch_add(1);
     };
-    auto worker = [&]() {
+    ThreadPoolRun pool;
+    auto worker = [&](int) {
         Scratch S;
         for (;;) {
+            if (pool.failed()) break;  // another worker ran out of memory: stop, join, report
             std::pair<int64_t, int> task{-1, 0};
             {
                 std::lock_guard<std::mutex> lk(readyMu);
@@ -704,21 +706,15 @@ mp_status poa_augment_pairs(mp_poa *const *poas, int64_t nPoas, const int64_t *p
                     }
                 }
         };
-        std::vector<std::thread> th;
-        for (int t = 1; t < threads; t++) th.emplace_back(checker);
-        checker();
-        for (auto &t : th) t.join();
+        parallel_run(threads, [&](int) { checker(); });
         if (badPair.load() != INT64_MAX) {
             set_error("mp_batch_poa_augment: pair " + std::to_string(badPair.load()) +
                       " has list coordinates outside its graph's reference or its read");
             return MP_EINVAL;
         }
-        th.clear();
-        for (int t = 1; t < threads; t++) th.emplace_back(worker);
-        worker();
-        for (auto &t : th) t.join();
+        pool.run(threads, worker);
     } catch (const std::bad_alloc &) {
-        set_error("mp_batch_poa_augment: out of host memory");
+        set_error("mp_batch_poa_augment: out of host memory (the graphs may be partly augmented)");
         return MP_ENOMEM;
     }
     return MP_OK;
@@ -999,10 +995,7 @@ extern "C" mp_status mp_poa_estimate_repeat_counts(const mp_poa *poa, const mp_m
             for (int64_t k0; (k0 = next.fetch_add(1024)) < n;)
                 for (int64_t k = k0; k < std::min<int64_t>(n, k0 + 1024); k++) repeatCounts[k - 1] = node_count(k);
         };
-        std::vector<std::thread> th;
-        for (int t = 1; t < threads; t++) th.emplace_back(work);
-        work();
-        for (auto &t : th) t.join();
+        mp::parallel_run(threads, [&](int) { work(); });
     } catch (const std::bad_alloc &) {
         mp::set_error("mp_poa_estimate_repeat_counts: out of host memory");
         return MP_ENOMEM;
@@ -1077,10 +1070,7 @@ extern "C" mp_status mp_poa_anchor_alignments(const mp_poa *poa, const int64_t *
         auto work = [&]() {
             for (int64_t r; (r = next.fetch_add(1)) < nReads;) per_read(r);
         };
-        std::vector<std::thread> th;
-        for (int t = 1; t < nth; t++) th.emplace_back(work);
-        work();
-        for (auto &t : th) t.join();
+        mp::parallel_run(nth, [&](int) { work(); });
         int64_t total = 0;
         for (int64_t r = 0; r < nReads; r++) {
             counts[r] = (int64_t)out[r].size() / 3;
@@ -1371,10 +1361,12 @@ extern "C" mp_status mp_poas_estimate_repeat_counts(const mp_poa *const *poas, i
             }
         }
     };
-    std::vector<std::thread> th;
-    for (int t = 1; t < threads; t++) th.emplace_back(work);
-    work();
-    for (auto &t : th) t.join();
+    try {
+        mp::parallel_run(threads, [&](int) { work(); });
+    } catch (const std::bad_alloc &) {
+        mp::set_error("mp_poas_estimate_repeat_counts: out of host memory");
+        return MP_ENOMEM;
+    }
     if (badGraph.load() >= 0) {
         mp::set_error("graph " + std::to_string(badGraph.load()) + ": " + errs[badGraph.load()]);
         return sts[badGraph.load()];
