@@ -341,10 +341,19 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int idle = 0;
     for (;;) {
         // ---------------- per-row protocol ----------------
-        if (phase == S_FETCH) {  // forward rows only
-            int jnew = 0;
-            if (k == 0) jnew = atomicAdd(gQueue, 1);
-            jnew = rbcast<SEG>(jnew, rowBase, 0);
+        // Forward rows take jobs as a gang: once no row of the wavefront holds
+        // a job, every fetching row takes the next job of the queue in one
+        // claim.  Neighbouring jobs of the queue have about the same band area,
+        // so the rows of a wavefront reach their crossings (where the posterior
+        // steps begin) at about the same macro step, and the wavefront runs the
+        // cheaper body without posteriors for the first half of its jobs.
+        const uint64_t fetchRows = __ballot(phase == S_FETCH && k == 0);
+        const bool gangReady = __all(phase == S_FETCH || phase == S_EXIT);
+        if (phase == S_FETCH && gangReady) {  // forward rows only
+            int claim = 0;
+            if (lane == __builtin_ctzll(fetchRows)) claim = atomicAdd(gQueue, __popcll(fetchRows));
+            claim = __builtin_amdgcn_readfirstlane(claim);
+            const int jnew = claim + __popcll(fetchRows & ((1ull << rowBase) - 1ull));
             job = gJobBegin + jnew;
             if (job >= gJobEnd) job = -1;
             else if (gJobMap) job = gJobMap[job];
