@@ -106,11 +106,22 @@ __global__ __launch_bounds__(1024) void slot_record_scan(const int32_t *slotJob,
     if (__any(ovf) && (threadIdx.x & 63) == 0) atomicOr(overflow, 1);
 }
 
-__global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobLists *jl, const int32_t *counts,
-                                                     const int64_t *slotBase, const int32_t *winEnd,
-                                                     const int *band, const int32_t *skip, const int32_t *error,
-                                                     float thr, int withIndels, int nseg, int2 *xy, int4 *wf) {
-    if (*skip || *error) return;
+// Records of one job side are ordered in tiles of kOrderTile: the tile's
+// record codes, with kOrderHalo records either side, are staged in LDS, so the
+// scans for the run of records on the same anti-diagonal (and the cell rank in
+// it) read LDS; a run reaching beyond the halo (bands wider than kOrderHalo
+// cells) reads the rest from global memory.
+constexpr int kOrderTile = 256;
+constexpr int kOrderHalo = 64;
+
+__global__ __launch_bounds__(kOrderTile) void order_records(const int4 *rec, const JobLists *jl,
+                                                            const int32_t *counts, const int64_t *slotBase,
+                                                            const int32_t *winEnd, const int *band,
+                                                            const int32_t *skip, const int32_t *error, float thr,
+                                                            int withIndels, int nseg, int2 *xy, int4 *wf,
+                                                            uint8_t *flg) {
+    __shared__ uint32_t code[kOrderTile + 2 * kOrderHalo];
+    if (*skip || *error) return;  // (grid-uniform)
     for (int seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
         const JobLists J = jl[seg >> 1];
         const int side = seg & 1;  // 0 forward, 1 backward
@@ -119,7 +130,19 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
         const int4 *r = rec + (side ? J.outB : J.outF);
         const int32_t *we = winEnd + 2 * J.winOff;
         const int64_t base = slotBase[J.slot];
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        for (int t0 = 0; t0 < n; t0 += kOrderTile) {
+            __syncthreads();  // the previous tile's LDS reads are done
+            for (int i = threadIdx.x; i < kOrderTile + 2 * kOrderHalo; i += kOrderTile) {
+                const int gi = t0 - kOrderHalo + i;
+                code[i] = gi >= 0 && gi < n ? (uint32_t)r[gi].w : 0xffffffffu;
+            }
+            __syncthreads();
+            const int t = t0 + (int)threadIdx.x;
+            if (t >= n) continue;
+            auto codeAt = [&](int i) -> uint32_t {
+                const int li = i - t0 + kOrderHalo;
+                return li >= 0 && li < kOrderTile + 2 * kOrderHalo ? code[li] : (uint32_t)r[i].w;
+            };
             int w = 0;
             while (w + 1 < J.nWin && t >= we[2 * w + side]) w++;
             const int fBeg = w ? we[2 * (w - 1)] : 0, fEnd = we[2 * w];
@@ -130,10 +153,10 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
             const int4 me = r[t];
             const uint32_t d = (uint32_t)me.w >> 11, cell = (uint32_t)me.w & 2047u;
             int u = t, v = t + 1;
-            while (u > beg && ((uint32_t)r[u - 1].w >> 11) == d) u--;
-            while (v < end && ((uint32_t)r[v].w >> 11) == d) v++;
+            while (u > beg && (codeAt(u - 1) >> 11) == d) u--;
+            while (v < end && (codeAt(v) >> 11) == d) v++;
             int rank = 0;
-            for (int i = u; i < v; i++) rank += ((uint32_t)r[i].w & 2047u) > cell;
+            for (int i = u; i < v; i++) rank += (codeAt(i) & 2047u) > cell;
             // backward blocks are stored anti-diagonals descending: walk them from the end
             const int64_t pos = side ? (int64_t)(bEnd - v) + rank : (int64_t)(bEnd - bBeg) + (u - fBeg) + rank;
             const int64_t g = base + off + pos;
@@ -144,6 +167,7 @@ __global__ __launch_bounds__(256) void order_records(const int4 *rec, const JobL
             xy[g] = make_int2(x + J.xAdd, y + J.yAdd);
             wf[g] = make_int4((int)floorf(fminf(pM, 1.f) * 1e7f), (int)floorf(fminf(pX, 1.f) * 1e7f),
                               (int)floorf(fminf(pY, 1.f) * 1e7f), flags);
+            flg[g] = (uint8_t)flags;
         }
     }
 }
@@ -158,7 +182,7 @@ __device__ __forceinline__ int prefix_in_wave(uint64_t m) {
 }
 
 // Per tile: the number of tuples of each list.
-__global__ __launch_bounds__(kScanThreads) void list_flag_sums(const int4 *wf, const int64_t *nPtr,
+__global__ __launch_bounds__(kScanThreads) void list_flag_sums(const uint8_t *flg, const int64_t *nPtr,
                                                                const int32_t *skip, int4 *tileSums) {
     __shared__ int sh[3][kScanThreads / 64];
     const int64_t n = *skip ? 0 : *nPtr;
@@ -169,7 +193,7 @@ __global__ __launch_bounds__(kScanThreads) void list_flag_sums(const int4 *wf, c
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
         const int64_t g = tile0 + i * kScanThreads + threadIdx.x;
-        const int f = g < n ? wf[g].w : 0;
+        const int f = g < n ? flg[g] : 0;
         c0 += __popcll(ballot_bits(f & 1));
         c1 += __popcll(ballot_bits(f & 2));
         c2 += __popcll(ballot_bits(f & 4));
@@ -284,7 +308,7 @@ __global__ __launch_bounds__(kScanThreads) void list_scan_emit(const int2 *xy, c
 // = list length.  One wavefront per pair: the tile's offset plus the list
 // flags of the tile's records before the pair's first record.
 __global__ __launch_bounds__(256) void pair_list_starts(const int32_t *pairSlot, int64_t np, const int64_t *slotBase,
-                                                        int32_t nslots, const int4 *wf, const int4 *tileOff,
+                                                        int32_t nslots, const uint8_t *flg, const int4 *tileOff,
                                                         const int4 *totals, const int32_t *skip, int64_t *starts) {
     const int lane = threadIdx.x & 63;
     const int64_t n = *skip ? 0 : slotBase[nslots];
@@ -298,7 +322,7 @@ __global__ __launch_bounds__(256) void pair_list_starts(const int32_t *pairSlot,
             const int64_t t = g / kScanTile, t0 = t * kScanTile;
             int c0 = 0, c1 = 0, c2 = 0;
             for (int64_t k = t0 + lane; k < g; k += 64) {
-                const int f = wf[k].w;
+                const int f = flg[k];
                 c0 += f & 1;
                 c1 += (f >> 1) & 1;
                 c2 += (f >> 2) & 1;

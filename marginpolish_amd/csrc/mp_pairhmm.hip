@@ -205,6 +205,7 @@ struct mp_batch {
     DevBuf<int32_t> dSlotJob, dSlotK, dPairSlot, dOverflow;
     DevBuf<int64_t> dSlotBase, dStarts;
     DevBuf<int2> dXY;
+    DevBuf<uint8_t> dFlg;  // list flags of the ordered records (one byte each, for the scans)
     DevBuf<int4> dWF, dTileSums, dListTotals;
     int64_t listCap[3] = {0, 0, 0};     // tuples the device lists hold (grown when a run needs more)
     int64_t records = 0;                // posterior records of the last run
@@ -302,7 +303,7 @@ mp_status alloc_lists(mp_batch *b) {
     if (!lists_mode(b)) return MP_OK;
     const int64_t cap = std::max<int64_t>(b->outTotal, 1);
     b->nTiles = (cap + mpk::kScanTile - 1) / mpk::kScanTile;
-    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) ||
+    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) || (st = b->dFlg.alloc((size_t)cap)) ||
         (st = b->dTileSums.alloc((size_t)b->nTiles + 1)) || (st = b->dListTotals.alloc(1)))
         return st;
     // first guess of the list lengths (a record holds at most one tuple of each list; the
@@ -1654,9 +1655,9 @@ static mp_status enqueue_lists(mp_batch *b) {
         hipLaunchKernelGGL(mpk::order_records, dim3((unsigned)std::max(1, std::min(2 * nj, 65535))), dim3(256), 0, s,
                            b->dOut.p, b->dJL.p, b->dCounts.p, b->dSlotBase.p, b->dWinEnd.p, b->dBand.p,
                            b->dOverflow.p, b->dError.p, b->kp.threshold, b->kp.withIndels, 2 * nj, b->dXY.p,
-                           b->dWF.p);
+                           b->dWF.p, b->dFlg.p);
         const int64_t *nPtr = b->dSlotBase.p + ns;
-        hipLaunchKernelGGL(mpk::list_flag_sums, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dWF.p,
+        hipLaunchKernelGGL(mpk::list_flag_sums, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dFlg.p,
                            nPtr, b->dOverflow.p, b->dTileSums.p);
         hipLaunchKernelGGL(mpk::list_block_offsets, dim3(1), dim3(1024), 0, s, b->dTileSums.p, nPtr, b->dOverflow.p,
                            b->dListTotals.p);
@@ -1668,7 +1669,7 @@ static mp_status enqueue_lists(mp_batch *b) {
                            b->dList[2].p);
         const unsigned gs = (unsigned)std::min<int64_t>((np + 1 + 3) / 4, 65535);  // one wavefront per pair
         hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase.p, ns,
-                           b->dWF.p, b->dTileSums.p, b->dListTotals.p, b->dOverflow.p, b->dStarts.p);
+                           b->dFlg.p, b->dTileSums.p, b->dListTotals.p, b->dOverflow.p, b->dStarts.p);
         HIP_TRY(hipGetLastError());
     }
     if (np > 0) {
