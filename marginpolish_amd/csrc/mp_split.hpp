@@ -16,10 +16,19 @@
 // match move cx-1 -> cx+1 or cx -> cx+2; in_F are the forward inputs gathered
 // from cx and cx-1, e . B the backward sends of cx+1 and cx+2).  In exact
 // arithmetic this equals the reference's window total.  After the crossing
-// the forward row emits posteriors for (cx, from] reading the backward values
-// its partner left in HBM, and the backward row emits (lo, cx] reading the
-// forward values; both continue their recursions.  A window's critical path
-// is one sweep instead of two.
+// the forward row emits posteriors for (cx, from] and the backward row for
+// (lo, cx]; both continue their recursions.  A window's critical path is one
+// sweep instead of two.
+//
+// Checkpoints instead of stored values: before the crossing each sweep
+// stores, for every 8-diagonal cell it enters, its state on entry (the sends
+// of the two diagonals before the cell: 16 bytes per lane) in an HBM ring.
+// After the crossing a row rebuilds its partner's values of the current cell
+// from that checkpoint, running the partner's recursion over the cell with the
+// same operations, rescale points and start vectors as the partner sweep, so
+// the rebuilt values are the partner's values bit for bit; the posterior steps
+// then read them from registers.  HBM carries 2 bytes per cell and lane each
+// way instead of 12.
 //
 // Anti-diagonals are walked in grid-aligned macro steps of 8 (one "cell" of
 // the grid d = 8g..8g+7); every per-step operand (band entry, encoded
@@ -54,10 +63,10 @@ struct SplitShared {
 constexpr int kSplitLdsBytes =
     ((kTabFloats * 4 + (int)sizeof(SplitShared)) > kTabReach ? (kTabFloats * 4 + (int)sizeof(SplitShared))
                                                               : kTabReach);
-// Ring geometry: [wave slot][cell][8 anti-diagonals][64 lanes] of 3 floats, so
-// that one step's 64 lanes store and load 768 contiguous bytes.
-constexpr int kRingStepBytes = 64 * 12;              // 768: one anti-diagonal, lane-major
-constexpr int kRingCellBytes = kK * kRingStepBytes;  // 6144
+// Checkpoint ring: [wave slot][cell][64 lanes] of 4 floats (the sweep's sends
+// on entering the cell), so that one cell's 64 lanes store and load 1024
+// contiguous bytes.
+constexpr int kRingCellBytes = 64 * 16;
 
 enum { S_FETCH = 0, S_RUN = 1, S_XWAIT = 2, S_JWAIT = 3, S_EXIT = 4 };
 
@@ -149,7 +158,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     const char *peerRing = reinterpret_cast<const char *>(kp.ring) + (slot ^ 1) * (size_t)RC * kRingCellBytes;
     int *myExp = kp.ringExp + slot * (size_t)RC * 4;
     const int *peerExp = kp.ringExp + (slot ^ 1) * (size_t)RC * 4;
-    const uint32_t laneRing = (uint32_t)lane * 12u;
+    const uint32_t laneRing = (uint32_t)lane * 16u;
     // masked ring stores go through a buffer resource (out-of-range offsets are dropped)
     const __amdgpu_buffer_rsrc_t ringRs = __builtin_amdgcn_make_buffer_rsrc(myRing, 0, RC * kRingCellBytes, kBufWord3);
     const char *symB = reinterpret_cast<const char *>(kp.sym);
@@ -174,6 +183,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int64_t outBase = 0;
     int tb = 0, lo = 0, from = 0, cx = 0, postFrom = -1;
     int capD = -1;  // restarted window job: the previous window's traceback anti-diagonal (chain value)
+    int fst = 0;    // the job's forward start anti-diagonal (a cell boundary)
     int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, cpos = 0;
     bool post = false, init = false, reload = true;
     float totalInv = 0.f;
@@ -182,7 +192,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int bm1 = kEmptyBand, bm2 = kEmptyBand;  // band entries of the previous two processed diagonals
     int bC[K], bN[K];
     uint32_t sxC[K], syC[K];
-    F3 rC[K];
+    F3 rC[K];                   // the partner's values of the current cell, rebuilt from its checkpoint
+    float4 ckC = make_float4(0.f, 0.f, 0.f, 0.f), ckN = ckC;  // partner checkpoints: current and next cell
     int eC = 0;
 #pragma unroll
     for (int j = 0; j < K; j++) {
@@ -217,10 +228,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             sy[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
         }
     };
-    auto peerCell = [&](int cp, F3 *r) {
-        const char *p = peerRing + ((uint32_t)cp * kRingCellBytes + laneRing);
-#pragma unroll
-        for (int j = 0; j < K; j++) r[j] = *reinterpret_cast<const F3 *>(p + j * kRingStepBytes);
+    auto peerCkpt = [&](int cp) {
+        return *reinterpret_cast<const float4 *>(peerRing + ((uint32_t)cp * kRingCellBytes + laneRing));
     };
     auto resetExchange = [&]() {
         oX1 = oY1 = oM1 = oM2 = 0.f;
@@ -259,6 +268,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         jflags = jb.flags;
         cap = bwd ? jb.cap[1] : jb.cap[0];
         outBase = jb.outOff + (bwd ? jb.cap[0] : 0);
+        fst = jb.fstart;
         cnt = 0;
         win = 0;
         loadWin();
@@ -427,7 +437,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             bandCell(cell, bC);
             bandCell(cN, bN);
             symCell(cell, bC, sxC, syC);
-            peerCell(cpos, rC);
+            ckC = peerCkpt(post ? cpos : 0);
             eC = peerExp[cpos * 4 + row];
             reload = false;
         }
@@ -435,25 +445,19 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         int bN2[K];
         bandCell(cN2, bN2);
         const int eN = peerExp[cposN * 4 + row];
-        // rolling prefetch: after step j consumes its symbols and partner values,
-        // the same registers are reloaded for step j of the next cell
-        const uint32_t peerN = (uint32_t)cposN * kRingCellBytes + laneRing;
+        // (rows not emitting posteriors read one fixed checkpoint slot, a cache hit)
+        ckN = peerCkpt(post ? cposN : 0);
+        // rolling prefetch: after step j consumes its symbols, the same registers
+        // are reloaded for step j of the next cell
         // Loads are never predicated (a predicated load would be copied into its
         // register with a wait for every load in flight): cells outside the next
-        // band read the symbol pool's first entry, and rows not emitting
-        // posteriors read one fixed ring slot, both cache hits.
-        // Partner values are only fetched in bodies where some row emits: a row
-        // starts emitting after its crossing, which reloads the whole cell.
-        auto prefetchStep = [&](auto J, auto POSTc) {
+        // band read the symbol pool's first entry, a cache hit.
+        auto prefetchStep = [&](auto J, auto) {
             constexpr int j = decltype(J)::value;
             uint32_t xa, ya;
             symAddr(cN * K + j, bN[j], xa, ya);
             sxC[j] = *reinterpret_cast<const uint32_t *>(symB + xa);
             syC[j] = *reinterpret_cast<const uint32_t *>(symB + ya);
-            if constexpr (decltype(POSTc)::value) {
-                const uint32_t pa = ((xa != 0u) & post) ? peerN + j * kRingStepBytes : 0u;
-                rC[j] = *reinterpret_cast<const F3 *>(peerRing + pa);
-            }
         };
 
         const bool slowRow = (bC[0] & (bwd ? kSlowB : kSlowF)) != 0;
@@ -467,6 +471,146 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         const uint32_t cbase = (uint32_t)cpos * kRingCellBytes + laneRing;
         int *expSlot = myExp + cpos * 4 + row;
         bool reached = false;
+
+        // This sweep's checkpoint on entering the cell: read by the partner after
+        // the crossing (and, going forward, by the next window's backward row
+        // above `from`; a window job's warm-up at or below lo is never read).
+        {
+            const bool enter = bwd ? (d == gs + K - 1) & !post
+                                   : (d == gs) & (!post | (gs + K - 1 > from)) & (gs + K - 1 > lo);
+            buf_store4(ringRs, enter ? cbase : kBufOff, oX1, oY1, oM1, oM2);
+        }
+
+        // The partner's values of this cell, rebuilt from its checkpoint for the
+        // rows emitting posteriors: the partner's own recursion, rescale points
+        // (the forward at the cell's first diagonal, the backward at its last
+        // unless the sweep starts in the cell) and start vectors, so the values
+        // are the ones the partner computed.  FR: every emitting row needs the
+        // whole cell and no sweep starts in it.
+        auto rebuildBackward = [&](auto FRc) {  // forward rows: the backward sweep of this window
+            constexpr bool FR = decltype(FRc)::value;
+            const bool initCell = tb <= gs + K - 1;  // the backward sweep starts at tb, in this cell
+            float uX1 = ckC.x, uY1 = ckC.y, uM1 = ckC.z, uM2 = ckC.w;
+            int ub1 = bN[0], ub2 = bN[1];
+            if (!FR && initCell) {
+                uX1 = uY1 = uM1 = uM2 = 0.f;
+                ub1 = ub2 = kEmptyBand;
+            }
+            const int hiB = min(gs + K - 1, tb), loB = max(gs, d);
+            const bool rg = tb == D && ((jflags >> 2) & 1);
+            const bool slowB = !XAL && __any(post && (bC[0] & kSlowB) != 0);
+            static_for<0, K>([&](auto J) {
+                constexpr int j = K - 1 - decltype(J)::value;
+                const int dj = gs + j;
+                if (!FR && !(post && dj <= hiB && dj >= loB)) return;
+                const int b = bC[j];
+                const int xL = band_xL(b);
+                const int kx = cellOf(xL);
+                const bool act = kx <= band_wm1_narrow(b);
+                const Emis e = emissions(tab, sxC[j], syC[j]);
+                float inX, inY, inM;
+                if constexpr (XAL) {
+                    inX = dpp<kRowRor15, false>(uX1);
+                    inY = uY1;
+                    inM = dpp<kRowRor15, false>(uM2);
+                } else if (!slowB) {
+                    bwd_in_fast<SEG>(xL, band_xL(ub1), band_xL(ub2), uX1, uY1, uM2, k, inX, inY, inM);
+                } else {
+                    bwd_in_slow(xL, ub1, ub2, uX1, uY1, uM2, act, rowBase, k, inX, inY, inM);
+                }
+                float q0 = fmaf(T02, inY, fmaf(T01, inX, T00 * inM));
+                float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
+                float q2 = fmaf(T22, inY, fmaf(T21, inX, T20 * inM));
+                if (!FR && initCell && dj == tb) {
+                    q0 = rg ? ev10 : ev00;
+                    q1 = rg ? ev11 : ev01;
+                    q2 = rg ? ev12 : ev02;
+                }
+                if (j == K - 1 && (FR || !initCell)) {
+                    const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
+                    const int e2 = __builtin_amdgcn_frexp_expf(m);
+                    q0 = ldexpf(q0, -e2);
+                    q1 = ldexpf(q1, -e2);
+                    q2 = ldexpf(q2, -e2);
+                    uM1 = ldexpf(uM1, -e2);
+                }
+                rC[j] = F3{q0, q1, q2};
+                uM2 = uM1;
+                uX1 = e.eX * q1;
+                uY1 = e.eY * q2;
+                uM1 = e.eM * q0;
+                ub2 = ub1;
+                ub1 = b;
+            });
+        };
+        auto rebuildForward = [&](auto FRc) {  // backward rows: the job's forward sweep
+            constexpr bool FR = decltype(FRc)::value;
+            const bool initCell = fst == gs;  // the forward sweep starts at this cell's first diagonal
+            float uX1 = ckC.x, uY1 = ckC.y, uM1 = ckC.z, uM2 = ckC.w;
+            int ub1 = bN[K - 1], ub2 = bN[K - 2];
+            if (!FR && initCell) {
+                uX1 = uY1 = uM1 = uM2 = 0.f;
+                ub1 = ub2 = kEmptyBand;
+            }
+            const int hiF = min(gs + K - 1, d);
+            const bool restart = (jflags & kJobRestart) != 0, rg = (jflags >> 1) & 1;
+            const bool slowF = !XAL && __any(post && (bC[0] & kSlowF) != 0);
+            static_for<0, K>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const int dj = gs + j;
+                if (!FR && !(post && dj <= hiF)) return;
+                const int b = bC[j];
+                const int xL = band_xL(b);
+                const int kx = cellOf(xL);
+                const bool act = kx <= band_wm1_narrow(b);
+                const Emis e = emissions(tab, sxC[j], syC[j]);
+                float inX, inY, inM;
+                if constexpr (XAL) {
+                    inX = dpp<kRowRor1, false>(uX1);
+                    inY = uY1;
+                    inM = dpp<kRowRor1, false>(uM2);
+                } else if (!slowF) {
+                    fwd_in_fast<SEG>(xL, band_xL(ub1), band_xL(ub2), uX1, uY1, uM2, k, inX, inY, inM);
+                } else {
+                    fwd_in_slow(xL, ub1, ub2, uX1, uY1, uM2, act, rowBase, k, inX, inY, inM);
+                }
+                float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
+                if (!FR && j == 0 && initCell) {
+                    p0 = act ? (restart ? 1.f : (rg ? sv10 : sv00)) : 0.f;
+                    p1 = act ? (restart ? 1.f : (rg ? sv11 : sv01)) : 0.f;
+                    p2 = act ? (restart ? 1.f : (rg ? sv12 : sv02)) : 0.f;
+                }
+                if (j == 0) {
+                    const float m = rmaxpos<SEG>(fmaxf(p0, fmaxf(p1, p2)));
+                    const int e2 = __builtin_amdgcn_frexp_expf(m);
+                    p0 = ldexpf(p0, -e2);
+                    p1 = ldexpf(p1, -e2);
+                    p2 = ldexpf(p2, -e2);
+                    uM1 = ldexpf(uM1, -e2);
+                }
+                rC[j] = F3{p0, p1, p2};
+                const float q0 = fmaf(T20, p2, fmaf(T10, p1, T00 * p0));
+                const float q1 = fmaf(T21, p2, fmaf(T11, p1, T01 * p0));
+                const float q2 = fmaf(T22, p2, fmaf(T12, p1, T02 * p0));
+                uM2 = uM1;
+                uX1 = q1;
+                uY1 = q2;
+                uM1 = q0;
+                ub2 = ub1;
+                ub1 = b;
+            });
+        };
+        if (anyPost) {
+            using T_ = std::true_type;
+            using F_ = std::false_type;
+            if (!bwd) {
+                if (__all(!post || ((d == gs) & (tb > gs + K - 1)))) rebuildBackward(T_{});
+                else rebuildBackward(F_{});
+            } else {
+                if (__all(!post || ((d == gs + K - 1) & (fst != gs)))) rebuildForward(T_{});
+                else rebuildForward(F_{});
+            }
+        }
 
         // One step.  FULL: every running row processes all 8 anti-diagonals with
         // fast shifts (no per-step conditions); POST: some row emits posteriors.
@@ -511,13 +655,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 S += e2;
                 *expSlot = S;
             }
-            // kept for the backward row (before the crossing) and for the next
-            // window's lower posterior diagonals (above `from`); a window job's
-            // warm-up (at or below lo) is never read
-            // (whole rows: a full 192-byte row segment makes whole-line writes;
-            // lanes outside the band store values that are never read)
-            buf_store3(ringRs, ((!post | (dj > from)) & (dj > lo)) ? cbase + j * kRingStepBytes : kBufOff, p0, p1,
-                       p2);
             if constexpr (!FULL) {
                 // chain value of a restarted window job: F . endV at the previous window's tb
                 const bool capHere = dj == capD;
@@ -599,7 +736,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // the cell's exponent is final after its first processed diagonal
             if (FULL ? j == K - 1 : dj == d) *expSlot = S;
-            buf_store3(ringRs, !post ? cbase + j * kRingStepBytes : kBufOff, q0, q1, q2);
             if constexpr (POST) {
                 if (!FULL || j == K - 1) sc = ldexpf(totalInv, eC + S - Stot);
                 const F3 pf = rC[j];
@@ -648,6 +784,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             bN[j] = bN2[j];
         }
         eC = eN;
+        ckC = ckN;
 
         if (reached) {
             if (!post) {
