@@ -190,15 +190,10 @@ struct KParams {
 constexpr int kBufWord3 = 0x00020000;
 constexpr int kCachePolicyNT = 2;           // gfx94x/gfx950 cache-policy bit "nt"
 constexpr uint32_t kBufOff = 0xfffffff0u;  // a raw offset beyond every extent used here
-typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // (non-temporal: ring values are read back by the partner sweep much later)
-__device__ __forceinline__ void buf_store3(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c) {
-    const u32x3 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
-    __builtin_amdgcn_raw_buffer_store_b96(v, r, (int)off, 0, kCachePolicyNT);
-}
 __device__ __forceinline__ void buf_store4(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c,
                                            float d) {
     const u32x4 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d)};

@@ -59,6 +59,7 @@ struct SplitShared {
     int jobId[kNumClasses][kMaxPairs][4];        // job published by the forward row
     int jobSeq[kNumClasses][kMaxPairs][4];       // jobs published
     int doneB[kNumClasses][kMaxPairs][4];        // jobs finished by the backward row
+    float vec[2][2][3];                          // [start / end][plain / ragged][state]: the boundary vectors
 };
 constexpr int kSplitLdsBytes =
     ((kTabFloats * 4 + (int)sizeof(SplitShared)) > kTabReach ? (kTabFloats * 4 + (int)sizeof(SplitShared))
@@ -141,10 +142,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int32_t *const gError = kp.error;
     const int gJobBegin = kp.jobBegin[cls], gJobEnd = kp.jobEnd[cls];
     const int32_t *const gJobMap = kp.jobMap;
-    const float sv00 = kp.startV[0][0], sv01 = kp.startV[0][1], sv02 = kp.startV[0][2];
-    const float sv10 = kp.startV[1][0], sv11 = kp.startV[1][1], sv12 = kp.startV[1][2];
-    const float ev00 = kp.endV[0][0], ev01 = kp.endV[0][1], ev02 = kp.endV[0][2];
-    const float ev10 = kp.endV[1][0], ev11 = kp.endV[1][1], ev12 = kp.endV[1][2];
+    // (the boundary vectors, used at sweep starts only, are read from LDS)
+    const float(*const sv)[3] = sh->vec[0];
+    const float(*const ev)[3] = sh->vec[1];
     const int lane = threadIdx.x & 63;
     const int k = lane & (SEG - 1);
     const int row = lane / SEG;
@@ -252,9 +252,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         reload = true;
         resetExchange();
         const bool rg = tb == D && ((jflags >> 2) & 1);
-        iv0 = rg ? ev10 : ev00;
-        iv1 = rg ? ev11 : ev01;
-        iv2 = rg ? ev12 : ev02;
+        iv0 = ev[rg][0];
+        iv1 = ev[rg][1];
+        iv2 = ev[rg][2];
     };
     auto setupJob = [&]() {
         const DevJob jb = gJobs[job];
@@ -284,9 +284,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             const bool restart = (jflags & kJobRestart) != 0;
             const bool rg = (jflags >> 1) & 1;
             // a restarted window job starts from a flat vector (kWarmup before lo)
-            iv0 = restart ? 1.f : (rg ? sv10 : sv00);
-            iv1 = restart ? 1.f : (rg ? sv11 : sv01);
-            iv2 = restart ? 1.f : (rg ? sv12 : sv02);
+            iv0 = restart ? 1.f : sv[rg][0];
+            iv1 = restart ? 1.f : sv[rg][1];
+            iv2 = restart ? 1.f : sv[rg][2];
             capD = restart ? lo + gTbd + 1 : -1;
         } else {
             startBwdWindow();
@@ -502,12 +502,12 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             static_for<0, K>([&](auto J) {
                 constexpr int j = K - 1 - decltype(J)::value;
                 const int dj = gs + j;
+                const Emis e = emissions(tab, sxC[j], syC[j]);
                 if (!FR && !(post && dj <= hiB && dj >= loB)) return;
                 const int b = bC[j];
                 const int xL = band_xL(b);
                 const int kx = cellOf(xL);
                 const bool act = kx <= band_wm1_narrow(b);
-                const Emis e = emissions(tab, sxC[j], syC[j]);
                 float inX, inY, inM;
                 if constexpr (XAL) {
                     inX = dpp<kRowRor15, false>(uX1);
@@ -522,9 +522,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
                 float q2 = fmaf(T22, inY, fmaf(T21, inX, T20 * inM));
                 if (!FR && initCell && dj == tb) {
-                    q0 = rg ? ev10 : ev00;
-                    q1 = rg ? ev11 : ev01;
-                    q2 = rg ? ev12 : ev02;
+                    q0 = ev[rg][0];
+                    q1 = ev[rg][1];
+                    q2 = ev[rg][2];
                 }
                 if (j == K - 1 && (FR || !initCell)) {
                     const float m = rmaxpos<SEG>(fmaxf(q0, fmaxf(q1, q2)));
@@ -558,12 +558,12 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             static_for<0, K>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 const int dj = gs + j;
+                const Emis e = emissions(tab, sxC[j], syC[j]);
                 if (!FR && !(post && dj <= hiF)) return;
                 const int b = bC[j];
                 const int xL = band_xL(b);
                 const int kx = cellOf(xL);
                 const bool act = kx <= band_wm1_narrow(b);
-                const Emis e = emissions(tab, sxC[j], syC[j]);
                 float inX, inY, inM;
                 if constexpr (XAL) {
                     inX = dpp<kRowRor1, false>(uX1);
@@ -576,9 +576,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 }
                 float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
                 if (!FR && j == 0 && initCell) {
-                    p0 = act ? (restart ? 1.f : (rg ? sv10 : sv00)) : 0.f;
-                    p1 = act ? (restart ? 1.f : (rg ? sv11 : sv01)) : 0.f;
-                    p2 = act ? (restart ? 1.f : (rg ? sv12 : sv02)) : 0.f;
+                    p0 = act ? (restart ? 1.f : sv[rg][0]) : 0.f;
+                    p1 = act ? (restart ? 1.f : sv[rg][1]) : 0.f;
+                    p2 = act ? (restart ? 1.f : sv[rg][2]) : 0.f;
                 }
                 if (j == 0) {
                     const float m = rmaxpos<SEG>(fmaxf(p0, fmaxf(p1, p2)));
@@ -626,6 +626,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             const int xL = band_xL(b);
             const int kx = cellOf(xL);
             const bool act = kx <= band_wm1_narrow(b);
+            // (bodies with posteriors follow a rebuild, which looked the match emissions up)
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
             if constexpr (XAL) {
@@ -659,7 +660,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 // chain value of a restarted window job: F . endV at the previous window's tb
                 const bool capHere = dj == capD;
                 if (__any(capHere)) {
-                    const float u = rsum<SEG>(fmaf(ev02, p2, fmaf(ev01, p1, ev00 * p0)));
+                    const float u = rsum<SEG>(fmaf(ev[0][2], p2, fmaf(ev[0][1], p1, ev[0][0] * p0)));
                     if (capHere && k == 0) gChain[job] = make_int2(__float_as_int(u), S);
                 }
             }
@@ -704,6 +705,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             const int xL = band_xL(b);
             const int kx = cellOf(xL);
             const bool act = kx <= band_wm1_narrow(b);
+            // (bodies with posteriors follow a rebuild, which looked the match emissions up)
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
             if constexpr (XAL) {
@@ -822,8 +824,12 @@ __global__ __launch_bounds__(512) void pairhmm_split_kernel(KParams kp) {
     for (int i = threadIdx.x; i < kTabFloats; i += blockDim.x) lds[i] = kp.tables[i];
     SplitShared *sh = reinterpret_cast<SplitShared *>(lds + kTabFloats);
     int *z = reinterpret_cast<int *>(sh->xflag);
-    const int nz = (int)((sizeof(SplitShared) - offsetof(SplitShared, xflag)) / sizeof(int));
+    const int nz = (int)((offsetof(SplitShared, vec) - offsetof(SplitShared, xflag)) / sizeof(int));
     for (int i = threadIdx.x; i < nz; i += blockDim.x) z[i] = 0;
+    if (threadIdx.x < 6) {
+        sh->vec[0][threadIdx.x / 3][threadIdx.x % 3] = kp.startV[threadIdx.x / 3][threadIdx.x % 3];
+        sh->vec[1][threadIdx.x / 3][threadIdx.x % 3] = kp.endV[threadIdx.x / 3][threadIdx.x % 3];
+    }
     __syncthreads();
     split_class<64>(kp, lds, sh, 2);
     if (__hip_atomic_load(kp.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
