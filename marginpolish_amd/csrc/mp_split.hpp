@@ -178,16 +178,15 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
 
     int phase = bwd ? S_JWAIT : S_FETCH;
     int jn = 0, xn = 0;
-    int job = -1, D = 0, lastCell = 0, nWin = 0, winOff = 0, jflags = 0, cap = 0, win = 0;
+    int job = -1, D = 0, nWin = 0, winOff = 0, jflags = 0, cap = 0, win = 0;
     uint32_t xrow = 0, yrow = 0, bandRow = 0;  // byte offsets: x/y symbol bases, the job's band
     int64_t outBase = 0;
-    int tb = 0, lo = 0, from = 0, cx = 0, postFrom = -1;
+    int tb = 0, lo = 0, from = 0, cx = 0;
     int capD = -1;  // restarted window job: the previous window's traceback anti-diagonal (chain value)
     int fst = 0;    // the job's forward start anti-diagonal (a cell boundary)
     int d = 0, gs = 0, S = 0, Stot = 0, cnt = 0, cpos = 0;
     bool post = false, init = false, reload = true;
     float totalInv = 0.f;
-    float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
     float oX1 = 0.f, oY1 = 0.f, oM1 = 0.f, oM2 = 0.f;
     int bm1 = kEmptyBand, bm2 = kEmptyBand;  // band entries of the previous two processed diagonals
     int bC[K], bN[K];
@@ -248,13 +247,8 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         S = 0;
         init = true;
         post = false;
-        postFrom = -1;
         reload = true;
         resetExchange();
-        const bool rg = tb == D && ((jflags >> 2) & 1);
-        iv0 = ev[rg][0];
-        iv1 = ev[rg][1];
-        iv2 = ev[rg][2];
     };
     auto setupJob = [&]() {
         const DevJob jb = gJobs[job];
@@ -262,7 +256,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         yrow = (uint32_t)(jb.symY - 1) * 4u;
         bandRow = (uint32_t)jb.bandOff * 4u;
         D = jb.lX + jb.lY;
-        lastCell = D / K + 1;  // bands are padded by one extra cell
         winOff = jb.winOff;
         nWin = jb.nWin;
         jflags = jb.flags;
@@ -278,15 +271,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             S = 0;
             init = true;
             post = false;
-            postFrom = -1;
             reload = true;
             resetExchange();
             const bool restart = (jflags & kJobRestart) != 0;
-            const bool rg = (jflags >> 1) & 1;
-            // a restarted window job starts from a flat vector (kWarmup before lo)
-            iv0 = restart ? 1.f : sv[rg][0];
-            iv1 = restart ? 1.f : sv[rg][1];
-            iv2 = restart ? 1.f : sv[rg][2];
             capD = restart ? lo + gTbd + 1 : -1;
         } else {
             startBwdWindow();
@@ -395,7 +382,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 if (!bwd && win + 1 == nWin && k == 0) gLogTotal[job] = make_int2(__float_as_int(t), Stot);
                 xn += 1;
                 post = true;
-                postFrom = from;
                 reload = true;
                 phase = S_RUN;
                 if (!bwd) {
@@ -418,18 +404,29 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             continue;
         }
         idle = 0;
+        const bool allRunning = __all(running);
         if (!running) continue;
+        // While every row of the wavefront runs, macro steps follow each other in
+        // this inner loop, their state in the same registers, until some row
+        // reaches its stop diagonal and the row protocol takes over; with a row
+        // waiting, the protocol runs after every macro step.
+        bool reached = false;
+        do {
         const int stop = bwd ? (post ? lo + 1 : cx + 1) : (post ? tb : cx);
         // a macro step that cannot reach the row's stop diagonal means a broken
         // invariant: report it instead of walking off the band
-        if (bwd ? (gs + K - 1 < stop || gs < 0) : (gs > stop)) {
-            atomicExch(gError, 2);
-            phase = S_EXIT;
-            continue;
+        const bool broken = bwd ? (gs + K - 1 < stop || gs < 0) : (gs > stop);
+        if (__any(broken)) {
+            if (broken) {
+                atomicExch(gError, 2);
+                phase = S_EXIT;
+            }
+            break;
         }
 
         // ---------------- one macro step (8 anti-diagonals of one grid cell) ----------------
         const int cell = gs / K;
+        const int lastCell = D / K + 1;  // bands are padded by one extra cell
         const int cN = bwd ? max(cell - 1, 0) : min(cell + 1, lastCell);
         const int cN2 = bwd ? max(cell - 2, 0) : min(cell + 2, lastCell);
         if (reload) {
@@ -470,7 +467,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         const bool slowWave = __any(slowRow);
         const uint32_t cbase = (uint32_t)cpos * kRingCellBytes + laneRing;
         int *expSlot = myExp + cpos * 4 + row;
-        bool reached = false;
 
         // This sweep's checkpoint on entering the cell: read by the partner after
         // the crossing (and, going forward, by the next window's backward row
@@ -641,9 +637,11 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             float p0 = e.eM * inM, p1 = e.eX * inX, p2 = e.eY * inY;
             // (cells outside the band have zero emissions, hence p = 0)
             if (!FULL && j == 0 && init) {
-                p0 = act ? iv0 : 0.f;
-                p1 = act ? iv1 : 0.f;
-                p2 = act ? iv2 : 0.f;
+                // the start vector; a restarted window job starts from a flat vector (kWarmup before lo)
+                const bool restart = (jflags & kJobRestart) != 0, rg = (jflags >> 1) & 1;
+                p0 = act ? (restart ? 1.f : sv[rg][0]) : 0.f;
+                p1 = act ? (restart ? 1.f : sv[rg][1]) : 0.f;
+                p2 = act ? (restart ? 1.f : sv[rg][2]) : 0.f;
             }
             if (j == 0 && (FULL || dj == d)) {
                 // exact power-of-two rescale at the start of every grid cell
@@ -669,7 +667,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const F3 pb = rC[j];
                 const float pM = p0 * pb.x * sc, pX = p1 * pb.y * sc, pY = p2 * pb.z * sc;
                 const float pXY = __uint_as_float(max(__float_as_uint(pX), __float_as_uint(pY)));
-                const bool emit = act & (dj <= postFrom) & ((pM >= thr) | (pXY >= thrXY));
+                const bool emit = act & post & (dj <= from) & ((pM >= thr) | (pXY >= thrXY));
                 const uint64_t m = __ballot(emit);
                 const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -722,9 +720,11 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             float q1 = fmaf(T12, inY, fmaf(T11, inX, T10 * inM));
             float q2 = fmaf(T22, inY, fmaf(T21, inX, T20 * inM));
             if (!FULL && init && dj == d) {
-                q0 = iv0;
-                q1 = iv1;
-                q2 = iv2;
+                // the end vector (the ragged one at the matrix end of a ragged-right job)
+                const bool rg = tb == D && ((jflags >> 2) & 1);
+                q0 = ev[rg][0];
+                q1 = ev[rg][1];
+                q2 = ev[rg][2];
             }
             // (cells outside the band keep q but send e . q = 0; their stored q is never read)
             if (j == K - 1 && (FULL || (dj == d && !init))) {
@@ -743,7 +743,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const F3 pf = rC[j];
                 const float pM = pf.x * q0 * sc, pX = pf.y * q1 * sc, pY = pf.z * q2 * sc;
                 const float pXY = __uint_as_float(max(__float_as_uint(pX), __float_as_uint(pY)));
-                const bool emit = act & (dj <= postFrom) & ((pM >= thr) | (pXY >= thrXY));
+                const bool emit = act & post & (dj <= from) & ((pM >= thr) | (pXY >= thrXY));
                 const uint64_t m = __ballot(emit);
                 const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -787,6 +787,12 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         }
         eC = eN;
         ckC = ckN;
+        if (!reached) {
+            gs += bwd ? -K : K;
+            d = bwd ? gs + K - 1 : gs;
+            cpos = cposN;
+        }
+        } while (allRunning && !__any(reached));
 
         if (reached) {
             if (!post) {
@@ -802,7 +808,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                     const int next = tb + 1;
                     loadWin();
                     post = false;
-                    postFrom = -1;
                     d = next;
                     gs = d & ~(K - 1);
                     reload = true;
@@ -811,10 +816,6 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             } else {
                 bwdWindowEnd();
             }
-        } else {
-            gs += bwd ? -K : K;
-            d = bwd ? gs + K - 1 : gs;
-            cpos = cposN;
         }
     }
 }
