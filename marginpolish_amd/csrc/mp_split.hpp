@@ -32,9 +32,11 @@
 //
 // Anti-diagonals are walked in grid-aligned macro steps of 8 (one "cell" of
 // the grid d = 8g..8g+7); every per-step operand (band entry, encoded
-// symbols, partner values) is a register loaded one macro step ahead (the
-// band two).  Macro steps that cover all 8 anti-diagonals of every running
-// row with one-lane neighbour shifts (the common case) run a branch-free body.
+// symbols) and the partner's checkpoint are registers loaded one macro step
+// ahead (the band two).  Macro steps that cover all 8 anti-diagonals of every
+// running row with one-lane neighbour shifts (the common case) run a
+// branch-free body; while every row runs they follow each other in an inner
+// loop.
 //
 // Rows of 16 lanes (the common case) hold cell x of an anti-diagonal on lane
 // x mod 16 ("x-aligned"); when every three consecutive anti-diagonals of the
@@ -506,9 +508,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const bool act = kx <= band_wm1_narrow(b);
                 float inX, inY, inM;
                 if constexpr (XAL) {
-                    inX = dpp<kRowRor15, false>(uX1);
+                    inX = dpp<kRowRor15, true>(uX1);
                     inY = uY1;
-                    inM = dpp<kRowRor15, false>(uM2);
+                    inM = dpp<kRowRor15, true>(uM2);
                 } else if (!slowB) {
                     bwd_in_fast<SEG>(xL, band_xL(ub1), band_xL(ub2), uX1, uY1, uM2, k, inX, inY, inM);
                 } else {
@@ -562,9 +564,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                 const bool act = kx <= band_wm1_narrow(b);
                 float inX, inY, inM;
                 if constexpr (XAL) {
-                    inX = dpp<kRowRor1, false>(uX1);
+                    inX = dpp<kRowRor1, true>(uX1);
                     inY = uY1;
-                    inM = dpp<kRowRor1, false>(uM2);
+                    inM = dpp<kRowRor1, true>(uM2);
                 } else if (!slowF) {
                     fwd_in_fast<SEG>(xL, band_xL(ub1), band_xL(ub2), uX1, uY1, uM2, k, inX, inY, inM);
                 } else {
@@ -626,9 +628,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
             if constexpr (XAL) {
-                inX = dpp<kRowRor1, false>(oX1);
+                inX = dpp<kRowRor1, true>(oX1);
                 inY = oY1;
-                inM = dpp<kRowRor1, false>(oM2);
+                inM = dpp<kRowRor1, true>(oM2);
             } else if (FULL || !slowWave) {
                 fwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
             } else {
@@ -707,9 +709,9 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             const Emis e = emissions(tab, sxC[j], syC[j]);
             float inX, inY, inM;
             if constexpr (XAL) {
-                inX = dpp<kRowRor15, false>(oX1);
+                inX = dpp<kRowRor15, true>(oX1);
                 inY = oY1;
-                inM = dpp<kRowRor15, false>(oM2);
+                inM = dpp<kRowRor15, true>(oM2);
             } else if (FULL || !slowWave) {
                 bwd_in_fast<SEG>(xL, band_xL(bm1), band_xL(bm2), oX1, oY1, oM2, k, inX, inY, inM);
             } else {
