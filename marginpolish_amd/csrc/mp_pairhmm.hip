@@ -123,7 +123,7 @@ struct mp_engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     // pinned double buffer for large device-to-host result copies (allocated at first use)
-    static constexpr size_t kStageBytes = (size_t)64 << 20;
+    static constexpr size_t kStageBytes = (size_t)128 << 20;
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stageEv[2] = {nullptr, nullptr};
     // pinned host copies of a batch's three lists for the POA augmentation (grown on demand, reused)
@@ -2150,7 +2150,7 @@ static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t
     char *dst = reinterpret_cast<char *>(out);
     const size_t chunk = mp_engine::kStageBytes;
     const size_t nchunk = (bytes + chunk - 1) / chunk;
-    const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     auto drain = [&](size_t k) {  // copy chunk k out of its staging buffer (after its DMA)
         const size_t off = k * chunk, len = std::min(chunk, bytes - off);
         const char *sb = reinterpret_cast<const char *>(e->stage[k & 1]);

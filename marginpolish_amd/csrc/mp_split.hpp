@@ -193,14 +193,12 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     int bm1 = kEmptyBand, bm2 = kEmptyBand;  // band entries of the previous two processed diagonals
     int bC[K], bN[K];
     uint32_t sxC[K], syC[K];
-    F3 rC[K];                   // the partner's values of the current cell, rebuilt from its checkpoint
     float4 ckC = make_float4(0.f, 0.f, 0.f, 0.f), ckN = ckC;  // partner checkpoints: current and next cell
     int eC = 0;
 #pragma unroll
     for (int j = 0; j < K; j++) {
         bC[j] = bN[j] = 0;
         sxC[j] = syC[j] = 0u;
-        rC[j] = F3{0.f, 0.f, 0.f};
     }
 
     // cell index of this lane on an anti-diagonal starting at xL
@@ -414,6 +412,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
         // waiting, the protocol runs after every macro step.
         bool reached = false;
         do {
+        F3 rC[K];  // the partner's values of this cell, rebuilt from its checkpoint when some row emits
         const int stop = bwd ? (post ? lo + 1 : cx + 1) : (post ? tb : cx);
         // a macro step that cannot reach the row's stop diagonal means a broken
         // invariant: report it instead of walking off the band
@@ -664,7 +663,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
                     if (capHere && k == 0) gChain[job] = make_int2(__float_as_int(u), S);
                 }
             }
-            if constexpr (POST) {
+            if constexpr (POST) if (FULL || anyPost) {  // (the general body also runs without a rebuild)
                 if (!FULL || j == 0) sc = ldexpf(totalInv, S + eC - Stot);
                 const F3 pb = rC[j];
                 const float pM = p0 * pb.x * sc, pX = p1 * pb.y * sc, pY = p2 * pb.z * sc;
@@ -740,7 +739,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
             }
             // the cell's exponent is final after its first processed diagonal
             if (FULL ? j == K - 1 : dj == d) *expSlot = S;
-            if constexpr (POST) {
+            if constexpr (POST) if (FULL || anyPost) {  // (the general body also runs without a rebuild)
                 if (!FULL || j == K - 1) sc = ldexpf(totalInv, eC + S - Stot);
                 const F3 pf = rC[j];
                 const float pM = pf.x * q0 * sc, pX = pf.y * q1 * sc, pY = pf.z * q2 * sc;

@@ -361,18 +361,32 @@ class Batch:
                                           lp.ctypes.data_as(C.POINTER(C.c_double))))
         return c[: self.npairs], lp[: self.npairs]
 
-    def fetch(self):
-        """Per pair: (matches, gapX, gapY, logProb)."""
+    def fetch_arrays(self):
+        """The three lists of every pair at once, as the C ABI hands them over:
+        (lists, starts, logProb), lists[l] the structured array of list l
+        (matches, gapX, gapY) in pair order, pair i's entries
+        lists[l][starts[l][i]:starts[l][i + 1]]."""
         counts, lp = self.counts()
-        lists = []
+        lists, starts = [], []
         for li in range(3):
             tot = int(counts[:, li].sum())
             buf = np.empty(max(tot, 1), dtype=ALIGNED_PAIR_DTYPE)
             if tot:
                 check(lib().mp_batch_fetch_list(self._h, li, buf.ctypes.data_as(C.POINTER(_lib.AlignedPair))))
-            ends = np.cumsum(counts[:, li]).tolist()
-            lists.append([buf[a:e] for a, e in zip([0] + ends[:-1], ends)] if self.npairs else [])
-        return [(lists[0][i], lists[1][i], lists[2][i], float(lp[i])) for i in range(self.npairs)]
+            st = np.zeros(self.npairs + 1, dtype=np.int64)
+            np.cumsum(counts[:, li], out=st[1:])
+            lists.append(buf[:tot])
+            starts.append(st)
+        return lists, starts, lp
+
+    def fetch(self):
+        """Per pair: (matches, gapX, gapY, logProb)."""
+        lists, starts, lp = self.fetch_arrays()
+        per = []
+        for li in range(3):
+            st = starts[li].tolist()
+            per.append([lists[li][a:e] for a, e in zip(st[:-1], st[1:])])
+        return [(per[0][i], per[1][i], per[2][i], float(lp[i])) for i in range(self.npairs)]
 
     def set_model(self, machines):
         """Re-read a model's transitions and emission tables (same pairs)."""
