@@ -126,7 +126,9 @@ __device__ __forceinline__ void bwd_in_slow(int xL, int b1, int b2, float oX1, f
     inM = gather(oM2, k + xL - xL2 + 1, band_wm1_narrow(b2), act, rowBase, k);
 }
 
-template <int SEG>
+// BWD: the wavefront's role (compiled separately, so each role's code holds
+// only its own sweep, rebuild and posterior bodies).
+template <int SEG, bool BWD>
 __device__ __forceinline__ void split_class(const KParams &kp, const float *tab, SplitShared *sh, int cls) {
     constexpr int K = kK;
     constexpr bool XAL = SEG == 16;  // x-aligned lanes
@@ -153,7 +155,7 @@ __device__ __forceinline__ void split_class(const KParams &kp, const float *tab,
     const int rowBase = lane & ~(SEG - 1);
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int pr = wib >> 1;
-    const bool bwd = (wib & 1) != 0;  // wave-uniform role
+    constexpr bool bwd = BWD;
     const size_t slot = (size_t)blockIdx.x * (blockDim.x >> 6) + wib;
     const int RC = kp.ringLen / K;
     char *myRing = reinterpret_cast<char *>(kp.ring) + slot * (size_t)RC * kRingCellBytes;
@@ -833,11 +835,16 @@ __global__ __launch_bounds__(512) void pairhmm_split_kernel(KParams kp) {
         sh->vec[1][threadIdx.x / 3][threadIdx.x % 3] = kp.endV[threadIdx.x / 3][threadIdx.x % 3];
     }
     __syncthreads();
-    split_class<64>(kp, lds, sh, 2);
+    // wavefronts 2p and 2p+1 of a workgroup are the forward and backward rows of partner pair p
+    const bool bwd = ((threadIdx.x >> 6) & 1) != 0;
+    if (bwd) split_class<64, true>(kp, lds, sh, 2);
+    else split_class<64, false>(kp, lds, sh, 2);
     if (__hip_atomic_load(kp.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    split_class<32>(kp, lds, sh, 1);
+    if (bwd) split_class<32, true>(kp, lds, sh, 1);
+    else split_class<32, false>(kp, lds, sh, 1);
     if (__hip_atomic_load(kp.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    split_class<16>(kp, lds, sh, 0);
+    if (bwd) split_class<16, true>(kp, lds, sh, 0);
+    else split_class<16, false>(kp, lds, sh, 0);
 }
 
 }  // namespace mpk
