@@ -204,13 +204,17 @@ struct mp_batch {
     DevBuf<mpk::JobLists> dJL;
     DevBuf<int32_t> dSlotJob, dSlotK, dPairSlot, dOverflow;
     DevBuf<int64_t> dSlotBase, dStarts;
-    DevBuf<int2> dXY;
-    DevBuf<uint8_t> dFlg;  // list flags of the ordered records (one byte each, for the scans)
-    DevBuf<int4> dWF, dTileSums, dListTotals;
+    DevBuf<int4> dSegCnt;     // per window and side: records of each list
+    DevBuf<int64_t> dSegBase;  // per window and side: first position in each list
+    // tiles of mpk::kBigTile records over the one-window job sides (by record capacity)
+    DevBuf<int32_t> dSegTile;
+    DevBuf<int4> dTileMeta, dTileCnt, dTilePre;
+    int64_t nListTiles = 0;
+    bool multiWin = false;  // some job has several windows (exact-chain batches)
+    DevBuf<int4> dListTotals;
     int64_t listCap[3] = {0, 0, 0};     // tuples the device lists hold (grown when a run needs more)
     int64_t records = 0;                // posterior records of the last run
     DevBuf<double> dPairLL, dBatchLL;
-    int64_t nTiles = 0;
     DevBuf<mpk::ListTuple> dList[3];    // the lists on the device (copied out by fetch)
     std::vector<int64_t> listStart[3];  // per pair, npairs + 1
     std::vector<double> pairLogProb;
@@ -302,10 +306,42 @@ mp_status alloc_lists(mp_batch *b) {
     if ((st = b->dJL.upload(b->jlHost, s))) return st;
     if (!lists_mode(b)) return MP_OK;
     const int64_t cap = std::max<int64_t>(b->outTotal, 1);
-    b->nTiles = (cap + mpk::kScanTile - 1) / mpk::kScanTile;
-    if ((st = b->dXY.alloc((size_t)cap)) || (st = b->dWF.alloc((size_t)cap)) || (st = b->dFlg.alloc((size_t)cap)) ||
-        (st = b->dTileSums.alloc((size_t)b->nTiles + 1)) || (st = b->dListTotals.alloc(1)))
+    const size_t nseg = 2 * std::max<size_t>(b->winHost.size(), 1);
+    if ((st = b->dSegCnt.alloc(nseg)) || (st = b->dSegBase.alloc(3 * nseg)) ||
+        (st = b->dSlotBase3.alloc(3 * (b->order.size() + 1))) || (st = b->dListTotals.alloc(1)))
         return st;
+    // (host arrays kept to the stream synchronisation below: the uploads are asynchronous)
+    std::vector<int32_t> segTile((size_t)std::max<int64_t>(2 * nj, 1), 0);
+    std::vector<int4> tileMeta;
+    {
+        int64_t nt = 0;
+        b->multiWin = false;
+        for (int64_t q = 0; q < nj; q++) {
+            const mpk::JobLists &J = b->jlHost[q];
+            b->multiWin |= J.nWin > 1;
+            for (int side = 0; side < 2; side++) {
+                segTile[2 * q + side] = (int32_t)nt;
+                if (J.nWin == 1) nt += ((int64_t)(side ? J.capB : J.capF) + mpk::kBigTile - 1) / mpk::kBigTile;
+                if (nt >= INT32_MAX) {
+                    mp::set_error("batch too large: more than 2^31 list tiles (split it into several batches)");
+                    return MP_EUNSUPPORTED;
+                }
+            }
+        }
+        tileMeta.assign((size_t)std::max<int64_t>(nt, 1), make_int4(0, 0, 0, 0));
+        for (int64_t q = 0; q < nj; q++)
+            for (int side = 0; side < 2; side++) {
+                const mpk::JobLists &J = b->jlHost[q];
+                const int64_t k0 = segTile[2 * q + side], k1 = 2 * q + side + 1 < 2 * nj ? segTile[2 * q + side + 1] : nt;
+                for (int64_t k = k0; k < k1; k++)
+                    tileMeta[k] = make_int4((int)(2 * q + side), (int)((k - k0) * mpk::kBigTile), 2 * J.winOff + side, 0);
+            }
+        b->nListTiles = nt;
+        if ((st = b->dTileMeta.upload(tileMeta, s)) || (st = b->dSegTile.upload(segTile, s)) ||
+            (st = b->dTileCnt.alloc((size_t)std::max<int64_t>(nt, 1))) ||
+            (st = b->dTilePre.alloc((size_t)std::max<int64_t>(nt, 1))))
+            return st;
+    }
     // first guess of the list lengths (a record holds at most one tuple of each list; the
     // capacities count two records per posterior anti-diagonal and side): half the records
     for (int l = 0; l < 3; l++) {
@@ -1652,24 +1688,38 @@ static mp_status enqueue_lists(mp_batch *b) {
         if ((st = reset_i32(b, b->dOverflow.p, 1))) return st;
         hipLaunchKernelGGL(mpk::slot_record_scan, dim3(1), dim3(1024), 0, s, b->dSlotJob.p, ns, b->dCounts.p,
                            b->dJL.p, b->dError.p, b->dSlotBase.p, b->dOverflow.p);
-        hipLaunchKernelGGL(mpk::order_records, dim3((unsigned)std::max(1, std::min(2 * nj, 65535))), dim3(256), 0, s,
-                           b->dOut.p, b->dJL.p, b->dCounts.p, b->dSlotBase.p, b->dWinEnd.p, b->dBand.p,
-                           b->dOverflow.p, b->dError.p, b->kp.threshold, b->kp.withIndels, 2 * nj, b->dXY.p,
-                           b->dWF.p, b->dFlg.p);
-        const int64_t *nPtr = b->dSlotBase.p + ns;
-        hipLaunchKernelGGL(mpk::list_flag_sums, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dFlg.p,
-                           nPtr, b->dOverflow.p, b->dTileSums.p);
-        hipLaunchKernelGGL(mpk::list_block_offsets, dim3(1), dim3(1024), 0, s, b->dTileSums.p, nPtr, b->dOverflow.p,
-                           b->dListTotals.p);
+        const dim3 gseg((unsigned)std::max(1, std::min(2 * nj, 65535)));
+        const int64_t nt = b->nListTiles;
+        const dim3 gtile((unsigned)std::max<int64_t>(1, std::min<int64_t>(nt, (int64_t)1 << 22)));
+        if (nt > 0)
+            hipLaunchKernelGGL(mpk::tile_counts, gtile, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
+                               b->dWinEnd.p, b->dBand.p, b->dTileMeta.p, nt, b->dOverflow.p, b->dError.p,
+                               b->kp.threshold, b->kp.withIndels, b->dTileCnt.p);
+        hipLaunchKernelGGL(mpk::side_tile_scan, dim3((unsigned)std::max(1, std::min((2 * nj + 255) / 256, 65535))),
+                           dim3(256), 0, s, b->dJL.p, b->dCounts.p, b->dWinEnd.p, b->dSegTile.p, 2 * nj, b->dTileCnt.p,
+                           b->dOverflow.p, b->dError.p, b->dTilePre.p, b->dSegCnt.p);
+        if (b->multiWin)
+            hipLaunchKernelGGL(mpk::window_counts, gseg, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
+                               b->dWinEnd.p, b->dBand.p, b->dOverflow.p, b->dError.p, b->kp.threshold,
+                               b->kp.withIndels, 2 * nj, b->dSegCnt.p);
+        hipLaunchKernelGGL(mpk::slot_list_scan, dim3(1), dim3(1024), 0, s, b->dSlotJob.p, ns, b->dJL.p, b->dSegCnt.p,
+                           b->dOverflow.p, b->dError.p, b->dSlotBase3.p, b->dSegBase.p);
         const int4 cap = make_int4((int)std::min<int64_t>(b->listCap[0], INT32_MAX),
                                    (int)std::min<int64_t>(b->listCap[1], INT32_MAX),
                                    (int)std::min<int64_t>(b->listCap[2], INT32_MAX), 0);
-        hipLaunchKernelGGL(mpk::list_scan_emit, dim3((unsigned)b->nTiles), dim3(mpk::kScanThreads), 0, s, b->dXY.p,
-                           b->dWF.p, nPtr, b->dOverflow.p, b->dTileSums.p, cap, b->dList[0].p, b->dList[1].p,
-                           b->dList[2].p);
-        const unsigned gs = (unsigned)std::min<int64_t>((np + 1 + 3) / 4, 65535);  // one wavefront per pair
-        hipLaunchKernelGGL(mpk::pair_list_starts, dim3(gs), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase.p, ns,
-                           b->dFlg.p, b->dTileSums.p, b->dListTotals.p, b->dOverflow.p, b->dStarts.p);
+        if (nt > 0)
+            hipLaunchKernelGGL(mpk::emit_tiles, gtile, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
+                               b->dWinEnd.p, b->dBand.p, b->dTileMeta.p, nt, b->dTilePre.p,
+                               b->dSegCnt.p, b->dSegBase.p, b->dOverflow.p, b->dError.p, b->kp.threshold,
+                               b->kp.withIndels, cap, b->dList[0].p, b->dList[1].p, b->dList[2].p);
+        if (b->multiWin)
+            hipLaunchKernelGGL(mpk::emit_windows, gseg, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
+                               b->dWinEnd.p, b->dBand.p, b->dSegCnt.p, b->dSegBase.p, b->dOverflow.p, b->dError.p,
+                               b->kp.threshold, b->kp.withIndels, 2 * nj, cap, b->dList[0].p, b->dList[1].p,
+                               b->dList[2].p);
+        const unsigned gp = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
+        hipLaunchKernelGGL(mpk::ref_pair_starts, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase3.p,
+                           b->dOverflow.p, b->dStarts.p, b->dListTotals.p);
         HIP_TRY(hipGetLastError());
     }
     if (np > 0) {
