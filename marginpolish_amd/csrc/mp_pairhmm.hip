@@ -970,13 +970,18 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
     // points and sizes; (B) after prefix sums, symbol encoding into the pool and
     // the jobs with their anchors.  The first failing pair (in order) is reported.
     const int nthreads = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    // items claimed in chunks of n / (64 threads) (16 to 4096): few contended claims on
+    // hundreds of thousands of small items, balance on thousands of large ones; no more
+    // threads than chunks
     auto parallel_for = [&](int64_t n, const std::function<void(int64_t)> &fn) {
+        const int64_t chunk = std::min<int64_t>(4096, std::max<int64_t>(16, n / (64 * (int64_t)nthreads)));
+        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, (n + chunk - 1) / chunk));
         std::atomic<int64_t> next{0};
-        mp::parallel_run(nthreads, [&](int) {
+        mp::parallel_run(nt, [&](int) {
             for (;;) {
-                const int64_t i0 = next.fetch_add(16);
+                const int64_t i0 = next.fetch_add(chunk);
                 if (i0 >= n) break;
-                for (int64_t i = i0; i < std::min(n, i0 + 16); i++) fn(i);
+                for (int64_t i = i0; i < std::min(n, i0 + chunk); i++) fn(i);
             }
         });
     };
