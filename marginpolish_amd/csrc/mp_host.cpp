@@ -320,8 +320,10 @@ bool build_band(const int64_t *a, int64_t n, int64_t xo, int64_t yo, int64_t lX,
         const int64_t xL = clampz(px - h, lX), yL = clampz(ny + h, lY);
         const int64_t xU = clampz(nx + h, lX), yU = clampz(py - h, lY);
         const int64_t last = i < n ? std::min(nx + ny, D) : D;
-        for (; xay <= last; xay++)
-            if (!box_diag(xay, xL, yL, xU, yU, L[xay], R[xay])) return false;
+        // (no exit inside the loop, so it vectorises: an empty anti-diagonal fails the segment)
+        bool ok = true;
+        for (; xay <= last; xay++) ok &= box_diag(xay, xL, yL, xU, yU, L[xay], R[xay]);
+        if (!ok) return false;
         px = nx;
         py = ny;
     }

@@ -441,9 +441,6 @@ __global__ __launch_bounds__(kTile) void emit_tiles(const int4 *rec, const JobLi
     __shared__ uint64_t cnt[NC];     // per chunk: list counts, then their exclusive prefix
     if (*skip || *error) return;     // (grid-uniform)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    auto pk = [](int f) -> uint64_t {
-        return (uint64_t)(f & 1) | ((uint64_t)((f >> 1) & 1) << 21) | ((uint64_t)(f >> 2) << 42);
-    };
     auto fld = [](uint64_t v, int l) -> int { return (int)((v >> (21 * l)) & 0x1fffff); };
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const int4 m = tileMeta[k];

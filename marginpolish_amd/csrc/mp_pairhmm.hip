@@ -672,7 +672,7 @@ inline int64_t window_fstart(size_t k, int64_t lo) {
 // need: the planner's per-job step (pairwiseAligner.c:120-226, 715-783).
 // Returns false when the anchors do not fit the sequences.
 struct PlanScratch {
-    std::vector<int32_t> L, R, XL;
+    std::vector<int32_t> L, R, XL, WD;  // band extent, first x and width per anti-diagonal
     std::vector<mp::Window> w;
     // per window: the band's width beyond the anchors' own 2 W + 1 cells (at most
     // 16 per anti-diagonal) over its posterior anti-diagonals (lo, from], and the
@@ -698,55 +698,82 @@ bool plan_job(HostJob &hj, const JobAnc &anc, const mp_band_params &bp, int32_t 
     const size_t nw = w.size();
     sc.wex.assign(nw, 0);
     sc.wcells.assign(nw, 0);
-    // One pass over the anti-diagonals: width, cells, xL = x of the band's first
-    // cell ((d + xmyL) / 2), the band entry (width - 1 in the low bits; the
-    // narrow kernels' flags are added below), per window the excess width over
-    // its posterior anti-diagonals (lo, from] (consecutive ranges) and the cells
-    // of [lo, tb] (a wide window's ring; an anti-diagonal is in at most two), and
-    // the x-alignment of rows of 16 lanes: the cells of three consecutive
-    // anti-diagonals, with the neighbour each side reads, must span at most 16 x
-    // (checked with a lag of two anti-diagonals).
+    // Passes over the anti-diagonals (each a simple loop the compiler vectorises):
+    // width, cells, xL = x of the band's first cell ((d + xmyL) / 2) and the band
+    // entry (width - 1 in the low bits; the narrow kernels' flags are added below);
+    // the x-alignment of rows of 16 lanes (the cells of three consecutive
+    // anti-diagonals, with the neighbour each side reads, must span at most 16 x);
+    // per window the excess width over its posterior anti-diagonals (lo, from]
+    // (consecutive ranges) and the cells of [lo, tb] (a wide window's ring).
     XL.resize(D + 1);
+    sc.WD.resize(D + 1);
+    int32_t *xlA = XL.data(), *wdA = sc.WD.data();
+    const int32_t *Lp = L.data(), *Rp = R.data();
     int32_t wmax = 0;
     int64_t cells = 0;
     const int32_t nominal = 2 * p->diagonalExpansion + 1;
-    bool fits = true;
-    int32_t xlH[3] = {0, 0, 0}, xrH[3] = {0, 0, 0};  // XL and last x of d, d - 1, d - 2
-    size_t kp = 0, kr = 0;
     for (int64_t d = 0; d <= D; d++) {
-        const int32_t wd = (R[d] - L[d]) / 2 + 1;
-        const int32_t xl = (int32_t)((d + L[d]) / 2), xr = xl + wd - 1;
+        const int32_t wd = (Rp[d] - Lp[d]) / 2 + 1;
+        const int32_t xl = (int32_t)((d + Lp[d]) / 2);
         wmax = std::max(wmax, wd);
         cells += wd;
-        XL[d] = xl;
+        xlA[d] = xl;
+        wdA[d] = wd;
         bj[d] = (int32_t)((uint32_t)xl << kBandShift) | (std::min(wd, 2048) - 1);
-        while (kp < nw && d > w[kp].from) kp++;
-        if (kp < nw && d > w[kp].lo) sc.wex[kp] += std::min(16, std::max(0, wd - nominal));
-        while (kr < nw && d > w[kr].tb) kr++;
-        for (size_t k = kr; k < nw && w[k].lo <= d; k++) sc.wcells[k] += wd;
-        xlH[2] = xlH[1]; xlH[1] = xlH[0]; xlH[0] = xl;
-        xrH[2] = xrH[1]; xrH[1] = xrH[0]; xrH[0] = xr;
-        if (fits) {
-            // forward reads of d: d-1, d-2 and the cell left of xL
-            int32_t lo = xl - 1, hi = xr;
-            for (int e = 1; e <= 2 && e <= d; e++) { lo = std::min(lo, xlH[e]); hi = std::max(hi, xrH[e]); }
-            // backward reads of d - 2: d - 1, d and the cell right of its last x
-            if (d >= 2) {
-                const int32_t lo2 = std::min(xlH[2], std::min(xlH[1], xl));
-                const int32_t hi2 = std::max(xrH[2] + 1, std::max(xrH[1], xr));
-                fits = hi2 - lo2 <= 15;
-            }
-            fits = fits && hi - lo <= 15;
-        }
     }
+    // forward reads of d: d - 1, d - 2 and the cell left of xL; backward reads of d - 2:
+    // d - 1, d and the cell right of its last x
+    int bad = 0;
+    for (int64_t d = 0; d <= D; d++) {
+        const int32_t xl = xlA[d], xr = xl + wdA[d] - 1;
+        int32_t lo = xl - 1, hi = xr;
+        if (d >= 1) {
+            lo = std::min(lo, xlA[d - 1]);
+            hi = std::max(hi, xlA[d - 1] + wdA[d - 1] - 1);
+        }
+        if (d >= 2) {
+            lo = std::min(lo, xlA[d - 2]);
+            hi = std::max(hi, xlA[d - 2] + wdA[d - 2] - 1);
+            const int32_t lo2 = std::min(xlA[d - 2], std::min(xlA[d - 1], xl));
+            const int32_t hi2 = std::max(xlA[d - 2] + wdA[d - 2], std::max(xlA[d - 1] + wdA[d - 1] - 1, xr));
+            bad |= hi2 - lo2 > 15;
+        }
+        bad |= hi - lo > 15;
+    }
+    bool fits = !bad;
     if (fits) {  // the backward reads of the last two anti-diagonals (truncated at D)
         for (int64_t d = std::max<int64_t>(0, D - 1); d <= D; d++) {
-            int32_t lo2 = XL[d], hi2 = XL[d] + (R[d] - L[d]) / 2 + 1;
+            int32_t lo2 = xlA[d], hi2 = xlA[d] + wdA[d];
             for (int64_t e = d + 1; e <= D; e++) {
-                lo2 = std::min(lo2, XL[e]);
-                hi2 = std::max(hi2, XL[e] + (R[e] - L[e]) / 2);
+                lo2 = std::min(lo2, xlA[e]);
+                hi2 = std::max(hi2, xlA[e] + wdA[e] - 1);
             }
             fits = fits && hi2 - lo2 <= 15;
+        }
+    }
+    // anti-diagonal d joins the excess of the first window whose from >= d when d > its
+    // lo, and the cells of every window whose [lo, tb] holds it: with lo, tb and from
+    // non-decreasing over the windows, sums over ranges
+    bool mono = true;
+    for (size_t k = 1; k < nw; k++)
+        mono &= w[k].lo >= w[k - 1].lo && w[k].tb >= w[k - 1].tb && w[k].from >= w[k - 1].from;
+    if (mono) {
+        for (size_t k = 0; k < nw; k++) {
+            const int64_t a = std::max<int64_t>(w[k].lo, k ? w[k - 1].from : -1), e = std::min<int64_t>(w[k].from, D);
+            int64_t ex = 0;
+            for (int64_t d = a + 1; d <= e; d++) ex += std::min(16, std::max(0, wdA[d] - nominal));
+            sc.wex[k] = ex;
+            int64_t c = 0;
+            for (int64_t d = std::max<int64_t>(0, w[k].lo), f = std::min<int64_t>(w[k].tb, D); d <= f; d++) c += wdA[d];
+            sc.wcells[k] = c;
+        }
+    } else {
+        size_t kp = 0, kr = 0;
+        for (int64_t d = 0; d <= D; d++) {
+            while (kp < nw && d > w[kp].from) kp++;
+            if (kp < nw && d > w[kp].lo) sc.wex[kp] += std::min(16, std::max(0, wdA[d] - nominal));
+            while (kr < nw && d > w[kr].tb) kr++;
+            for (size_t k = kr; k < nw && w[k].lo <= d; k++) sc.wcells[k] += wdA[d];
         }
     }
     hj.wmax = wmax;
@@ -1168,14 +1195,17 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
     std::atomic<int> err{0};
     {
         std::atomic<int64_t> next{0};
+        // jobs claimed in runs of nj / (256 threads), 1 to 32 (one shared counter): long
+        // sub-problems balance, hundreds of thousands of short ones claim rarely
+        const int64_t runLen = std::min<int64_t>(32, std::max<int64_t>(1, nj / (256 * (int64_t)nthW)));
         mp::parallel_run(nthW, [&](int t) {
                 PlanScratch sc;
                 std::vector<DevWin> wj;
                 std::vector<DevWin> &pool = winPool[t];
                 for (int64_t j = 0, jEnd = 0;; j++) {
-                    if (j >= jEnd) {  // claim jobs in runs of 32 (one shared counter)
-                        j = next.fetch_add(32);
-                        jEnd = std::min<int64_t>(j + 32, nj);
+                    if (j >= jEnd) {
+                        j = next.fetch_add(runLen);
+                        jEnd = std::min<int64_t>(j + runLen, nj);
                     }
                     if (j >= nj) break;
                     HostJob &hj = jobs[j];
