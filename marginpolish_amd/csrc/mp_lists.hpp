@@ -50,6 +50,18 @@ struct ListTuple {          // the C ABI's mp_aligned_pair
     int32_t x, y;
 };
 
+// A list entry on the device: with pk > 0 one 64-bit word, weight (below 2^24)
+// in bits 0-23, x + 1 in the next pk bits, y + 1 above (coordinates start at
+// -1; the batch chose pk from its longest reference and read so that both
+// fit); with pk == 0 a ListTuple.
+__device__ __forceinline__ void put_tuple(ListTuple *out, int64_t pos, int pk, int w, int x, int y) {
+    if (pk)
+        reinterpret_cast<uint64_t *>(out)[pos] = (uint64_t)(uint32_t)w | ((uint64_t)(uint32_t)(x + 1) << 24) |
+                                                 ((uint64_t)(uint32_t)(y + 1) << (24 + pk));
+    else
+        *reinterpret_cast<int4 *>(out + pos) = make_int4(w, w >> 31, x, y);
+}
+
 // Exclusive scan over a workgroup of 64-lane waves; returns this thread's
 // prefix and writes the workgroup total to *total (every thread).
 template <class T, int NT>
@@ -172,7 +184,7 @@ __device__ __forceinline__ int field3(int v, int l) { return (v >> (10 * l)) & 1
 template <class CodeAt, class FlagsAt>
 __device__ __forceinline__ void emit_record(int t, int4 me, int x, int myF, int beg, int end, int side, const int pt[3],
                                             int4 T, const int64_t base[3], const JobLists &J, const int *bandJ,
-                                            CodeAt codeAt, FlagsAt flagsAt, int4 cap, ListTuple *out0,
+                                            CodeAt codeAt, FlagsAt flagsAt, int4 cap, int pk, ListTuple *out0,
                                             ListTuple *out1, ListTuple *out2) {
     const uint32_t d = (uint32_t)me.w >> 11, cell = (uint32_t)me.w & 2047u;
     // over the run of this anti-diagonal: list records before / after this one in storage,
@@ -208,10 +220,7 @@ __device__ __forceinline__ void emit_record(int t, int4 me, int x, int myF, int 
         // backward: list records after the run, then the run by cell descending
         const int inSeg = side ? segTot[l] - (pt[l] + 1 + af) + lg : pt[l] - bf + lg;
         const int64_t pos = base[l] + inSeg;
-        if (pos < capL[l]) {
-            const int wgt = (int)floorf(fminf(p[l], 1.f) * 1e7f);  // the int64 weight as two words
-            *reinterpret_cast<int4 *>(out[l] + pos) = make_int4(wgt, wgt >> 31, x + J.xAdd, y + J.yAdd);
-        }
+        if (pos < capL[l]) put_tuple(out[l], pos, pk, (int)floorf(fminf(p[l], 1.f) * 1e7f), x + J.xAdd, y + J.yAdd);
     }
 }
 
@@ -431,7 +440,7 @@ __global__ __launch_bounds__(kTile) void emit_tiles(const int4 *rec, const JobLi
                                                     const int32_t *winEnd, const int *band, const int4 *tileMeta,
                                                     int64_t ntiles, const int4 *tilePre, const int4 *segCnt,
                                                     const int64_t *segBase, const int32_t *skip, const int32_t *error,
-                                                    float thr, int withIndels, int4 cap, ListTuple *out0,
+                                                    float thr, int withIndels, int4 cap, int pk, ListTuple *out0,
                                                     ListTuple *out1, ListTuple *out2) {
     constexpr int NS = kBigTile + 2 * kHalo;
     constexpr int NC = NS / 64;  // chunks of staged records: left halo, kPer rounds x waves, right halo
@@ -557,7 +566,7 @@ __global__ __launch_bounds__(kTile) void emit_tiles(const int4 *rec, const JobLi
                     return li >= 0 && li < NS ? code[li] : (uint32_t)r[i2].w;
                 };
                 auto flagsAt = [&](int i2) -> int { return record_flags(r[i2], bandJ, thr, withIndels); };
-                emit_record(t, me[i], xs[i], f[i], 0, end, side, pt, T, base, J, bandJ, codeAt, flagsAt, cap, out0,
+                emit_record(t, me[i], xs[i], f[i], 0, end, side, pt, T, base, J, bandJ, codeAt, flagsAt, cap, pk, out0,
                             out1, out2);
                 continue;
             }
@@ -579,10 +588,8 @@ __global__ __launch_bounds__(kTile) void emit_tiles(const int4 *rec, const JobLi
                 const int inSeg = side ? segTot[l] - (pt[l] + fld(sV - sT, l)) + fld(larger, l)
                                        : pt[l] - fld(sT - sU, l) + fld(larger, l);
                 const int64_t pos = base[l] + inSeg;
-                if (pos < capL[l]) {
-                    const int wgt = (int)floorf(fminf(p[l], 1.f) * 1e7f);  // the int64 weight as two words
-                    *reinterpret_cast<int4 *>(out[l] + pos) = make_int4(wgt, wgt >> 31, xs[i] + J.xAdd, y0 + J.yAdd);
-                }
+                if (pos < capL[l])
+                    put_tuple(out[l], pos, pk, (int)floorf(fminf(p[l], 1.f) * 1e7f), xs[i] + J.xAdd, y0 + J.yAdd);
             }
         }
     }
@@ -595,7 +602,8 @@ __global__ __launch_bounds__(kTile) void emit_windows(const int4 *rec, const Job
                                                       const int32_t *winEnd, const int *band, const int4 *segCnt,
                                                       const int64_t *segBase, const int32_t *skip,
                                                       const int32_t *error, float thr, int withIndels, int nseg,
-                                                      int4 cap, ListTuple *out0, ListTuple *out1, ListTuple *out2) {
+                                                      int4 cap, int pk, ListTuple *out0, ListTuple *out1,
+                                                      ListTuple *out2) {
     constexpr int NS = kTile + 2 * kHalo;
     __shared__ uint32_t code[NS];
     __shared__ uint8_t fl[NS];
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(kTile) void emit_windows(const int4 *rec, const Job
                     };
                     const int4 me = r[t];
                     const int x = band_xL(bandJ[(uint32_t)me.w >> 11]) + (int)((uint32_t)me.w & 2047u);
-                    emit_record(t, me, x, myF, beg, end, side, pt, T, base, J, bandJ, codeAt, flagsAt, cap, out0,
+                    emit_record(t, me, x, myF, beg, end, side, pt, T, base, J, bandJ, codeAt, flagsAt, cap, pk, out0,
                                 out1, out2);
                 }
                 for (int l = 0; l < 3; l++) carry[l] += field3(tileTot, l);

@@ -213,6 +213,9 @@ struct mp_batch {
     bool multiWin = false;  // some job has several windows (exact-chain batches)
     DevBuf<int4> dListTotals;
     int64_t listCap[3] = {0, 0, 0};     // tuples the device lists hold (grown when a run needs more)
+    // bits of x in a device list entry (mpk::put_tuple): 64-bit entries when the batch's
+    // longest reference and read fit beside the 24-bit weight, else 0 (16-byte entries)
+    int32_t packBits = 0;
     int64_t records = 0;                // posterior records of the last run
     DevBuf<double> dPairLL, dBatchLL;
     DevBuf<mpk::ListTuple> dList[3];    // the lists on the device (copied out by fetch)
@@ -274,6 +277,14 @@ mp_status reset_i32(mp_batch *b, int32_t *q, int n) {
 }
 
 bool lists_mode(const mp_batch *b) { return b->mode == MP_MODE_POSTERIORS || b->mode == MP_MODE_MATCH_ONLY; }
+
+// Device list l with room for `tuples` entries (8 or 16 bytes each).
+mp_status alloc_list(mp_batch *b, int l, int64_t tuples) {
+    const int64_t units = b->packBits ? (tuples + 1) / 2 : tuples;  // in 16-byte units
+    mp_status st = b->dList[l].alloc((size_t)std::max<int64_t>(units, 1));
+    if (!st) b->listCap[l] = tuples;
+    return st;
+}
 
 // Per-job list metadata (record regions, capacities, slots) and the device
 // buffers of the list assembly, sized by the record capacity b->outTotal.
@@ -346,10 +357,7 @@ mp_status alloc_lists(mp_batch *b) {
     // capacities count two records per posterior anti-diagonal and side): half the records
     for (int l = 0; l < 3; l++) {
         const int64_t want = std::max<int64_t>(cap / 2, 1024);
-        if (b->listCap[l] < want) {
-            if ((st = b->dList[l].alloc((size_t)want))) return st;
-            b->listCap[l] = want;
-        }
+        if (b->listCap[l] < want && (st = alloc_list(b, l, want))) return st;
     }
     HIP_TRY(hipStreamSynchronize(s));
     return MP_OK;
@@ -501,9 +509,7 @@ mp_status create_reference_batch(mp_batch *b, const mp_model *m, const mp_pair *
     const bool rle = m->mach[1].rle != 0;
     if (rle && (st = b->dRepeat.upload(m->repeatLog10.data(), (size_t)4 * 51 * 51, s))) return st;
     for (int l = 0; l < 3; l++) {
-        const int64_t want = std::max<int64_t>(sumD / 2, 1024);
-        if ((st = b->dList[l].alloc((size_t)want))) return st;
-        b->listCap[l] = want;
+        if ((st = alloc_list(b, l, std::max<int64_t>(sumD / 2, 1024)))) return st;
     }
     mpk::RefParams &rp = b->rp;
     std::memset(&rp, 0, sizeof(rp));
@@ -978,6 +984,22 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
     b->mode = mode;
     b->flags = flags;
     b->npairs = npairs;
+    {
+        // 64-bit list entries when x + 1 < 2^bx and y + 1 < 2^by for every pair (list
+        // coordinates run from -1 to the length - 1) and 24 + bx + by <= 64
+        int64_t mx = 1, my = 1;
+        for (int64_t i = 0; i < npairs; i++) {
+            mx = std::max(mx, pairs[i].lX + 1);
+            my = std::max(my, pairs[i].lY + 1);
+        }
+        auto bits = [](int64_t v) {  // bits of the values 0 .. v - 1
+            int n = 1;
+            while (n < 62 && ((int64_t)1 << n) < v) n++;
+            return n;
+        };
+        const int bx = bits(mx), by = bits(my);
+        b->packBits = 24 + bx + by <= 64 ? bx : 0;
+    }
     b->pairCrop.assign(npairs, 0);
     b->pairStrand.resize(npairs);
     for (int64_t i = 0; i < npairs; i++) b->pairStrand[i] = pairs[i].strand ? 1 : 0;
@@ -1707,7 +1729,7 @@ static mp_status enqueue_lists(mp_batch *b) {
                                    (int)std::min<int64_t>(b->listCap[2], INT32_MAX), 0);
         hipLaunchKernelGGL(mpk::ref_lists_copy, dim3((unsigned)std::max(1, std::min(ns, 65535))), dim3(256), 0, s,
                            b->dSlotJob.p, ns, b->dCounts.p, b->dRefJobs.p, b->dRaw.p, b->dSlotBase3.p, b->dOverflow.p,
-                           cap, b->dList[0].p, b->dList[1].p, b->dList[2].p);
+                           cap, b->packBits, b->dList[0].p, b->dList[1].p, b->dList[2].p);
         hipLaunchKernelGGL(mpk::ref_pair_starts, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase3.p,
                            b->dOverflow.p, b->dStarts.p, b->dListTotals.p);
         if (np > 0) {
@@ -1746,12 +1768,12 @@ static mp_status enqueue_lists(mp_batch *b) {
             hipLaunchKernelGGL(mpk::emit_tiles, gtile, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
                                b->dWinEnd.p, b->dBand.p, b->dTileMeta.p, nt, b->dTilePre.p,
                                b->dSegCnt.p, b->dSegBase.p, b->dOverflow.p, b->dError.p, b->kp.threshold,
-                               b->kp.withIndels, cap, b->dList[0].p, b->dList[1].p, b->dList[2].p);
+                               b->kp.withIndels, cap, b->packBits, b->dList[0].p, b->dList[1].p, b->dList[2].p);
         if (b->multiWin)
             hipLaunchKernelGGL(mpk::emit_windows, gseg, dim3(mpk::kTile), 0, s, b->dOut.p, b->dJL.p, b->dCounts.p,
                                b->dWinEnd.p, b->dBand.p, b->dSegCnt.p, b->dSegBase.p, b->dOverflow.p, b->dError.p,
-                               b->kp.threshold, b->kp.withIndels, 2 * nj, cap, b->dList[0].p, b->dList[1].p,
-                               b->dList[2].p);
+                               b->kp.threshold, b->kp.withIndels, 2 * nj, cap, b->packBits, b->dList[0].p,
+                               b->dList[1].p, b->dList[2].p);
         const unsigned gp = (unsigned)std::min<int64_t>((np + 256) / 256, 65535);
         hipLaunchKernelGGL(mpk::ref_pair_starts, dim3(gp), dim3(256), 0, s, b->dPairSlot.p, np, b->dSlotBase3.p,
                            b->dOverflow.p, b->dStarts.p, b->dListTotals.p);
@@ -2024,9 +2046,7 @@ static mp_status batch_run_impl(mp_batch *b, bool plain) {
         bool grow = false;
         for (int l = 0; l < 3; l++)
             if (need[l] > b->listCap[l]) {
-                const int64_t want = need[l] + need[l] / 16 + 1024;
-                if ((st = b->dList[l].alloc((size_t)want))) return st;
-                b->listCap[l] = want;
+                if ((st = alloc_list(b, l, need[l] + need[l] / 16 + 1024))) return st;
                 grow = true;
             }
         if (grow) {
@@ -2208,6 +2228,17 @@ static mp_status materialize(mp_batch *b) {
     return MP_OK;
 }
 
+// 64-bit device list entries (mpk::put_tuple, pk = bits of x + 1) to the C ABI's tuples.
+static void unpack_tuples(const uint64_t *src, mp_aligned_pair *dst, size_t n, int pk) {
+    const uint64_t mx = ((uint64_t)1 << pk) - 1;
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t v = src[i];
+        dst[i].weight = (int64_t)(v & 0xffffff);
+        dst[i].x = (int32_t)((v >> 24) & mx) - 1;
+        dst[i].y = (int32_t)(v >> (24 + pk)) - 1;
+    }
+}
+
 // First n tuples of list l (device) to host memory.
 static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t n) {
     static_assert(sizeof(mpk::ListTuple) == sizeof(mp_aligned_pair), "tuple layout");
@@ -2218,9 +2249,17 @@ static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t
     }
     HIP_TRY(hipSetDevice(b->eng->device));
     mp_engine *e = b->eng;
-    const size_t bytes = sizeof(mp_aligned_pair) * (size_t)n;
+    const int pk = b->packBits;
+    const size_t esz = pk ? sizeof(uint64_t) : sizeof(mp_aligned_pair);  // bytes per device entry
+    const size_t bytes = esz * (size_t)n;
     if (bytes < ((size_t)8 << 20)) {
-        HIP_TRY(hipMemcpy(out, b->dList[l].p, bytes, hipMemcpyDeviceToHost));
+        if (!pk) {
+            HIP_TRY(hipMemcpy(out, b->dList[l].p, bytes, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<uint64_t> tmp((size_t)n);
+            HIP_TRY(hipMemcpy(tmp.data(), b->dList[l].p, bytes, hipMemcpyDeviceToHost));
+            unpack_tuples(tmp.data(), out, (size_t)n, pk);
+        }
         return MP_OK;
     }
     // Large lists: DMA into a pinned double buffer in 64 MB chunks; while the
@@ -2236,9 +2275,19 @@ static mp_status copy_list(mp_batch *b, int32_t l, mp_aligned_pair *out, int64_t
     const size_t chunk = mp_engine::kStageBytes;
     const size_t nchunk = (bytes + chunk - 1) / chunk;
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    auto drain = [&](size_t k) {  // copy chunk k out of its staging buffer (after its DMA)
+    auto drain = [&](size_t k) {  // copy (or unpack) chunk k out of its staging buffer (after its DMA)
         const size_t off = k * chunk, len = std::min(chunk, bytes - off);
         const char *sb = reinterpret_cast<const char *>(e->stage[k & 1]);
+        if (pk) {
+            const size_t cnt = len / sizeof(uint64_t), part = (cnt + nth - 1) / nth;
+            const uint64_t *sv = reinterpret_cast<const uint64_t *>(sb);
+            mp::parallel_run(nth, [&](int t) {
+                if ((size_t)t * part < cnt)
+                    unpack_tuples(sv + t * part, out + off / sizeof(uint64_t) + t * part, std::min(part, cnt - t * part),
+                                  pk);
+            });
+            return;
+        }
         const size_t part = (len + nth - 1) / nth;
         mp::parallel_run(nth, [&](int t) {
             if ((size_t)t * part < len) std::memcpy(dst + off + t * part, sb + t * part, std::min(part, len - t * part));
@@ -2338,9 +2387,13 @@ extern "C" mp_status mp_batch_poa_augment(mp_batch *b, mp_poa *const *poas, int6
             e->listHostBytes[l] = want;
         }
         const int64_t n = b->listStart[l][b->npairs];
-        if (n > 0)
+        if (n > 0 && b->packBits) {  // 64-bit entries: staged and unpacked by host threads
+            mp_status cs = copy_list(b, l, static_cast<mp_aligned_pair *>(e->listHost[l]), n);
+            if (cs) return cs;
+        } else if (n > 0) {
             HIP_TRY(hipMemcpyAsync(e->listHost[l], b->dList[l].p, sizeof(mp_aligned_pair) * (size_t)n,
                                    hipMemcpyDeviceToHost, e->stream));
+        }
         lists[l] = static_cast<const mp_aligned_pair *>(e->listHost[l]);
         starts[l] = b->listStart[l].data();
     }

@@ -435,7 +435,7 @@ __global__ __launch_bounds__(1024) void ref_slot_scan(const int32_t *slotJob, in
 // dropped (the host grows that list and assembles again).
 __global__ __launch_bounds__(256) void ref_lists_copy(const int32_t *slotJob, int32_t nslots, const int32_t *counts,
                                                       const RefJob *jobs, const ListTuple *raw,
-                                                      const int64_t *slotBase3, const int32_t *skip, int4 cap,
+                                                      const int64_t *slotBase3, const int32_t *skip, int4 cap, int pk,
                                                       ListTuple *out0, ListTuple *out1, ListTuple *out2) {
     if (*skip) return;
     ListTuple *out[3] = {out0, out1, out2};
@@ -448,7 +448,10 @@ __global__ __launch_bounds__(256) void ref_lists_copy(const int32_t *slotJob, in
             const int32_t n = counts[3 * q + l];
             const int64_t base = slotBase3[3 * (int64_t)s + l];
             for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
-                if (base + i < capL[l]) out[l][base + i] = raw[J.outOff[l] + (n - 1 - i)];
+                if (base + i < capL[l]) {
+                    const ListTuple r = raw[J.outOff[l] + (n - 1 - i)];
+                    put_tuple(out[l], base + i, pk, (int)r.weight, r.x, r.y);
+                }
         }
     }
 }
