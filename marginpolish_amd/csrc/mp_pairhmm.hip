@@ -767,7 +767,7 @@ bool plan_job(HostJob &hj, const JobAnc &anc, const mp_band_params &bp, int32_t 
         for (size_t k = 0; k < nw; k++) {
             const int64_t a = std::max<int64_t>(w[k].lo, k ? w[k - 1].from : -1), e = std::min<int64_t>(w[k].from, D);
             int64_t ex = 0;
-            for (int64_t d = a + 1; d <= e; d++) ex += std::min(16, std::max(0, wdA[d] - nominal));
+            for (int64_t d = std::max<int64_t>(a + 1, 0); d <= e; d++) ex += std::min(16, std::max(0, wdA[d] - nominal));
             sc.wex[k] = ex;
             int64_t c = 0;
             for (int64_t d = std::max<int64_t>(0, w[k].lo), f = std::min<int64_t>(w[k].tb, D); d <= f; d++) c += wdA[d];
