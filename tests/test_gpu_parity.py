@@ -258,6 +258,44 @@ def test_config1_full_batch_properties(ph, params, config1):
         assert len(as_map(as_rows(m))) == len(m)
 
 
+def test_long_pair_lists_in_wide_entries(ph, params):
+    """A batch whose longest reference and read need more than the 40 bits of
+    a 64-bit list entry keeps 16-byte entries: the config 0 pairs' lists in
+    such a batch equal their lists in a batch of their own (64-bit entries),
+    and the long pair's reference coordinates reach past 2^25."""
+    from marginpolish_amd import crop, sim
+    pp, op = params
+    w = sim.config0(strands=(1, 0))
+    jobs = []
+    for pr in w.pairs[:4]:
+        sx, sy, an, _ = crop.crop_pair(pr.ref_sym, pr.read_sym, pr.anchors)
+        jobs.append((sx, sy, an, pr.strand))
+    # the long pair: a 64 Mb reference, a 40 kb read copied from near its end, anchors on
+    # that diagonal (27 + 16 bits of coordinates; one sub-problem of ~80 k anti-diagonals)
+    rng = np.random.default_rng(3)
+    lX, lY = (1 << 26) + 5, 40000
+    xs = lX - lY - 1000
+    ref = (rng.integers(0, 4, lX) | (1 << 8)).astype(np.uint16)
+    read = ref[xs:xs + lY].copy()
+    k = np.arange(50, lY - 50, 100, dtype=np.int64)
+    anchors = np.stack([xs + k, k, np.full_like(k, 20)], axis=1).reshape(-1)
+    say("case: 4 config0 pairs with a 64 Mb x 40 kb pair (16-byte list entries) vs alone")
+    big = ph.Batch(pp.machines, jobs + [(ref, read, anchors, 1)], pp.p, crop_reference=True)
+    big.run()
+    gb = big.fetch()
+    alone = ph.Batch(pp.machines, jobs, pp.p, crop_reference=True)
+    alone.run()
+    ga = alone.fetch()
+    for (m1, x1, y1, l1), (m2, x2, y2, l2) in zip(gb[:4], ga):
+        for a1, a2 in ((m1, m2), (x1, x2), (y1, y2)):
+            assert np.array_equal(a1, a2)
+        assert l1 == l2
+    m, gx, gy, lp = gb[4]
+    assert len(m) > lY // 2 and np.isfinite(lp)
+    assert m["x"].min() >= xs - 1 and m["x"].max() >= (1 << 25) and m["x"].max() < lX and m["y"].max() < lY
+    assert np.all(m["weight"] > 0) and np.all(m["weight"] <= 10 ** 7)
+
+
 def test_too_wide_band_rejected_loudly(ph):
     m = ph.StateMachines.default_nucleotide()
     p = ph.PairwiseAlignmentParameters()
