@@ -942,6 +942,8 @@ static mp_status upload_stage(mp_engine *e, int k, size_t bytes, void **out) {
 
 static void discard_batch(mp_batch *b, mp_batch **inFlight) {
     *inFlight = nullptr;
+    // (the symbol upload may still read the engine's staging and write the batch's buffers)
+    if (b->eng && b->eng->stream) (void)hipStreamSynchronize(b->eng->stream);
     delete b;
 }
 
@@ -1175,6 +1177,23 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
         }
         *out = b;
         return MP_OK;
+    }
+    // the symbol pool is final: its DMA and the device encoding run while the bands are planned
+    hipStream_t s = e->stream;
+    DevBuf<uint16_t> rawUp;  // the raw pool on the device until encode_symbols has run (freed on return)
+    {
+        mp_status st;
+        if ((st = rawUp.upload(sym, nsym, s)) || (st = b->dSym.alloc(nsym))) {
+            discard_batch(b, inFlight);
+            return st;
+        }
+        const unsigned g = (unsigned)std::min<size_t>((nsym + 255) / 256, (size_t)e->cus * 16);
+        hipLaunchKernelGGL(encode_symbols, dim3(std::max(g, 1u)), dim3(256), 0, s, rawUp.p, b->dSym.p, (int64_t)nsym);
+        if (hipGetLastError() != hipSuccess) {
+            discard_batch(b, inFlight);
+            mp::set_error("mp_batch_create: symbol encoding launch failed");
+            return MP_EHIP;
+        }
     }
 
     // ---- 2. bands, windows, classes (parallel over jobs) ----
@@ -1488,7 +1507,6 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
 
     mark(5);
     // ---- 4. upload ----
-    hipStream_t s = e->stream;
     std::vector<float> tab;
     KParams &kp = b->kp;
     std::memset(&kp, 0, sizeof(kp));
@@ -1496,9 +1514,7 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
     b->modelStates = m->mach[1].nStates;
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
-    DevBuf<uint16_t> rawUp;  // the raw pool on the device until encode_symbols has run (freed on return)
-    if ((st = b->dJobs.upload(dj, s)) || (st = rawUp.upload(sym, nsym, s)) || (st = b->dSym.alloc(nsym)) ||
-        (st = b->dBand.alloc((size_t)bandTotal)) ||
+    if ((st = b->dJobs.upload(dj, s)) || (st = b->dBand.alloc((size_t)bandTotal)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
@@ -1511,15 +1527,6 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
         discard_batch(b, inFlight);
         return st;
-    }
-    {
-        const unsigned g = (unsigned)std::min<size_t>((nsym + 255) / 256, (size_t)e->cus * 16);
-        hipLaunchKernelGGL(encode_symbols, dim3(std::max(g, 1u)), dim3(256), 0, s, rawUp.p, b->dSym.p, (int64_t)nsym);
-        if (hipGetLastError() != hipSuccess) {
-            discard_batch(b, inFlight);
-            mp::set_error("mp_batch_create: symbol encoding launch failed");
-            return MP_EHIP;
-        }
     }
     {
         // the jobs' bands, then the narrow band copies of the threads' pools
