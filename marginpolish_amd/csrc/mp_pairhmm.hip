@@ -1298,6 +1298,26 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
             return MP_EUNSUPPORTED;
         }
     }
+    {
+        // the bands are final: the jobs' bands, then the narrow band copies of the threads'
+        // pools, go to the device while the rest is planned
+        hipError_t he = hipSuccess;
+        if (b->dBand.alloc((size_t)bandTotal)) {
+            discard_batch(b, inFlight);
+            return MP_ENOMEM;
+        }
+        if (bandOff[nj] > 0)
+            he = hipMemcpyAsync(b->dBand.p, band, sizeof(int32_t) * (size_t)bandOff[nj], hipMemcpyHostToDevice, s);
+        for (int t = 0; t < nthW && he == hipSuccess; t++)
+            if (!altPool[t].empty())
+                he = hipMemcpyAsync(b->dBand.p + altBase[t], altPool[t].data(), sizeof(int32_t) * altPool[t].size(),
+                                    hipMemcpyHostToDevice, s);
+        if (he != hipSuccess) {
+            discard_batch(b, inFlight);
+            mp::set_error(std::string("band upload: ") + hipGetErrorString(he));
+            return MP_EHIP;
+        }
+    }
 
     mark(2);
     // ---- 2b. window jobs (kWarmup): in the posterior modes a three-state sub-problem
@@ -1514,7 +1534,7 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
     b->modelStates = m->mach[1].nStates;
     const size_t nwin = std::max<size_t>(dw.size(), 1);
     mp_status st;
-    if ((st = b->dJobs.upload(dj, s)) || (st = b->dBand.alloc((size_t)bandTotal)) ||
+    if ((st = b->dJobs.upload(dj, s)) ||
         (st = b->dWin.upload(dw, s)) || (st = b->dTables.upload(tab, s)) ||
         (st = b->dOut.alloc(std::max<int64_t>(outOff, 1))) || (st = b->dCounts.alloc(std::max<size_t>(2 * dj.size(), 1))) ||
         (st = b->dWinEnd.alloc(2 * nwin)) || (st = b->dLogTotal.alloc(std::max<size_t>(dj.size(), 1))) ||
@@ -1527,21 +1547,6 @@ static mp_status batch_create(mp_engine *e, const mp_model *m, const mp_pair *pa
         (st = b->dEmit.alloc(mode == MP_MODE_EXPECTATIONS ? kTabFloats : 1))) {
         discard_batch(b, inFlight);
         return st;
-    }
-    {
-        // the jobs' bands, then the narrow band copies of the threads' pools
-        hipError_t he = bandOff[nj] > 0 ? hipMemcpyAsync(b->dBand.p, band, sizeof(int32_t) * (size_t)bandOff[nj],
-                                                         hipMemcpyHostToDevice, s)
-                                        : hipSuccess;
-        for (int t = 0; t < nthW && he == hipSuccess; t++)
-            if (!altPool[t].empty())
-                he = hipMemcpyAsync(b->dBand.p + altBase[t], altPool[t].data(), sizeof(int32_t) * altPool[t].size(),
-                                    hipMemcpyHostToDevice, s);
-        if (he != hipSuccess) {
-            discard_batch(b, inFlight);
-            mp::set_error(std::string("band upload: ") + hipGetErrorString(he));
-            return MP_EHIP;
-        }
     }
     // Rings: one slice per resident wavefront.
     const bool anyNarrow = b->classEnd[0] > b->classBegin[0] || b->classEnd[1] > b->classBegin[1] ||
