@@ -289,13 +289,11 @@ mp_status alloc_list(mp_batch *b, int l, int64_t tuples) {
 // Per-job list metadata (record regions, capacities, slots) and the device
 // buffers of the list assembly, sized by the record capacity b->outTotal.
 mp_status alloc_lists(mp_batch *b) {
-    const int64_t nj = (int64_t)b->jobs.size(), np = b->npairs, ns = (int64_t)b->order.size();
+    if (!lists_mode(b)) return MP_OK;  // (no records: forward-only and expectation batches)
+    const int64_t nj = (int64_t)b->jobs.size(), ns = (int64_t)b->order.size();
     std::vector<int32_t> jobSlot((size_t)std::max<int64_t>(nj, 1), -1);
     for (int64_t sl = 0; sl < ns; sl++)
         if (b->order[sl] >= 0) jobSlot[b->order[sl]] = (int32_t)sl;
-    std::vector<int32_t> slotPair((size_t)std::max<int64_t>(ns, 1), 0);
-    for (int64_t i = 0; i < np; i++)
-        for (int32_t k = 0; k < b->pairNJobs[i]; k++) slotPair[b->pairFirstJob[i] + k] = (int32_t)i;
     b->jlHost.resize((size_t)nj);
     for (int64_t q = 0; q < nj; q++) {
         const HostJob &hj = b->jobs[q];
@@ -315,7 +313,6 @@ mp_status alloc_lists(mp_batch *b) {
     hipStream_t s = b->eng->stream;
     mp_status st;
     if ((st = b->dJL.upload(b->jlHost, s))) return st;
-    if (!lists_mode(b)) return MP_OK;
     const int64_t cap = std::max<int64_t>(b->outTotal, 1);
     const size_t nseg = 2 * std::max<size_t>(b->winHost.size(), 1);
     if ((st = b->dSegCnt.alloc(nseg)) || (st = b->dSegBase.alloc(3 * nseg)) ||
